@@ -1,0 +1,1227 @@
+"""ORACLE — test infrastructure only.
+
+FP64 CPU restatement of the MOLSCAT scattering path (units, ITYPE=1 rotor
+basis and coupling, POTENL powers, HAMMAT, YINIT, FINDRM, the
+log-derivative propagators MDPROP/LDPROP/ODPROP, Y->K->S matching and the
+cross-section/scattering-length formulas), written from the reference's
+Fortran as a checker for the HIP engine.  Dense inversions go through the
+plain-C restatement in ``oracle/csrc/oracle_la.c`` (SYMINV inline path,
+Bunch-Kaufman DSYTF2 + DSYNEG + DSYTRI); eigen-decompositions and the one
+general solve use numpy's LAPACK (``eigh`` for DIAGVC/DIAGVL, ``solve``
+for DGESV), the same third-party dependency the reference links
+(source_code/Makefile:12, version unpinned).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+import this module.
+
+Every function cites the reference routine (file:line under source_code/)
+it restates.  Matrices are numpy arrays; index conventions are 0-based.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+from dataclasses import dataclass, field
+from functools import lru_cache
+from typing import Callable, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle_la.so")
+
+# ---------------------------------------------------------------------------
+# physical constants: source_code/physical_constants_module.f:7-88
+# ---------------------------------------------------------------------------
+PI_VALUE = 3.141592653589793
+PLANCK_SI = 6.62607015e-34
+HBAR_SI = PLANCK_SI / 2.0 / PI_VALUE
+C_SI = 2.99792458e8
+C_CM = C_SI * 1.0e2
+AMU_SI = 1.66053906892e-27
+ANGSTROM_SI = 1.0e-10
+BFCT = HBAR_SI / (4.0 * PI_VALUE * C_CM) / (ANGSTROM_SI) ** 2 / AMU_SI
+HZ_IN_INV_CM = 3.335640951e-11
+MHZ_IN_INV_CM = HZ_IN_INV_CM * 1.0e6
+BOHR_MAGNETON_INV_CM_PER_G = 0.46686447719 / 1.0e4
+
+
+# ---------------------------------------------------------------------------
+# C restatement of the inversion routines
+# ---------------------------------------------------------------------------
+def build_lib(force: bool = False) -> str:
+    """Compile oracle/csrc/oracle_la.c with gcc (oracle/Makefile)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle_la.so"])
+    return _LIB_PATH
+
+
+@lru_cache(maxsize=1)
+def _lib():
+    build_lib()
+    lib = ctypes.CDLL(_LIB_PATH)
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    lib.orc_syminv.argtypes = [dp, ctypes.c_int, ctypes.c_int]
+    lib.orc_syminv.restype = ctypes.c_int
+    lib.orc_syminv_inline.argtypes = [dp, ctypes.c_int, ctypes.c_int]
+    lib.orc_syminv_inline.restype = ctypes.c_int
+    lib.orc_dsytf2_l.argtypes = [dp, ctypes.c_int, ctypes.c_int, ip]
+    lib.orc_dsytf2_l.restype = ctypes.c_int
+    lib.orc_dsyneg_l.argtypes = [dp, ctypes.c_int, ctypes.c_int, ip]
+    lib.orc_dsyneg_l.restype = ctypes.c_int
+    lib.orc_dsytri_l.argtypes = [dp, ctypes.c_int, ctypes.c_int, ip]
+    lib.orc_dsytri_l.restype = ctypes.c_int
+    return lib
+
+
+def syminv(a: np.ndarray) -> Tuple[np.ndarray, int]:
+    """SYMINV (source_code/syminv.f:1-97 with SYMINL source_code/syminl-alloc.f).
+
+    Takes a symmetric matrix (only the lower triangle is read), returns the
+    full symmetric inverse (lower triangle mirrored, as every caller on the
+    path uses only the lower triangle or mirrors it) and KOUNT, the number of
+    negative eigenvalues.  KOUNT = N+1 flags an exact zero pivot on the
+    inline path; a negative value is -INFO from DSYTRF/DSYTRI.
+    """
+    n = a.shape[0]
+    f = np.array(a, dtype=np.float64, order="F", copy=True)
+    k = _lib().orc_syminv(f.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n, n)
+    lo = np.tril(f)
+    full = lo + np.tril(f, -1).T
+    return full, int(k)
+
+
+def syminv_raw(a: np.ndarray) -> Tuple[np.ndarray, int]:
+    """SYMINV returning the raw column-major output (upper stale for N>30)."""
+    n = a.shape[0]
+    f = np.array(a, dtype=np.float64, order="F", copy=True)
+    k = _lib().orc_syminv(f.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n, n)
+    return f, int(k)
+
+
+# ---------------------------------------------------------------------------
+# ITYPE=1 atom + linear rigid rotor basis and coupling
+# ---------------------------------------------------------------------------
+def rotor_levels(jlevels: Sequence[int], be: float) -> np.ndarray:
+    """Level energies (cm-1): source_code/setbas.f:77-81 with ALPHAE=DE=0."""
+    return np.array([(be - 0.0 / 2.0) * float(j * (j + 1)) - 0.0 for j in jlevels])
+
+
+def itype1_channels(jtot: int, ib: int, jlevels: Sequence[int]) -> List[Tuple[int, int]]:
+    """Channel list (state index, L) of one symmetry block.
+
+    source_code/base.f:276-352 (unified ITYPE code, ITP=1): L runs from
+    LMIN to LMAX in the outer loop, pair states in input order inside;
+    block IB keeps channels with (L + j + JTOT) mod 2 == IB mod 2 and
+    |JTOT-L| <= j <= JTOT+L.  JMIN starts at 0 (source_code/base.f:512).
+    """
+    jmax = max(max(jlevels), 0)
+    jmin = min(min(jlevels), 0)
+    lmax = jtot + jmax
+    lmin = jtot - jmax
+    if lmin < 0:
+        lmin = max(jmin - jtot, 0)
+    ipty = ib - 2 * (ib // 2)
+    chans = []
+    for li in range(lmin, lmax + 1):
+        jk = abs(jtot - li)
+        jtop = jtot + li
+        for i, ji in enumerate(jlevels):
+            lpj = li + ji + jtot
+            if (lpj - 2 * (lpj // 2)) != ipty:
+                continue
+            if ji < jk or ji > jtop:
+                continue
+            chans.append((i, li))
+    return chans
+
+
+@lru_cache(maxsize=None)
+def _w3j000(a: int, b: int, c: int) -> float:
+    from sympy.physics.wigner import wigner_3j
+
+    return float(wigner_3j(a, b, c, 0, 0, 0))
+
+
+@lru_cache(maxsize=None)
+def _w6j(a: int, b: int, c: int, d: int, e: int, f: int) -> float:
+    from sympy.physics.wigner import wigner_6j
+
+    return float(wigner_6j(a, b, c, d, e, f))
+
+
+def itype1_coupling(jtot: int, chans, jlevels, lambdas: Sequence[int]) -> np.ndarray:
+    """Percival-Seaton coupling VL(NHAM, N(N+1)/2), lambda fastest.
+
+    source_code/couple.f:51-143: packed index runs over the upper triangle
+    column by column (row <= col); value
+    sqrt((2jc+1)(2jr+1)(2Lc+1)(2Lr+1)) (jc lam jr;000)(Lc lam Lr;000)
+    {jr Lr J; Lc jc lam} (-1)^(jc+jr+J) (J6J's {J1 J2 J3; L1 L2 L3} with
+    J1=jr, J2=Lr, J3=JTOT, L1=Lc, L2=jc, L3=lam, source_code/j6j.f:10-13).
+    The 3j/6j symbols are exact
+    (sympy) here, independent of the reference's recursions.
+    """
+    n = len(chans)
+    nham = len(lambdas)
+    vl = np.zeros((nham, n * (n + 1) // 2))
+    idx = 0
+    for icol in range(n):
+        jc = jlevels[chans[icol][0]]
+        lc = chans[icol][1]
+        for irow in range(icol + 1):
+            jr = jlevels[chans[irow][0]]
+            lr = chans[irow][1]
+            for il, lam in enumerate(lambdas):
+                t = _w3j000(jc, lam, jr)
+                if t == 0.0:
+                    continue
+                u = _w3j000(lc, lam, lr)
+                if u == 0.0:
+                    continue
+                s = _w6j(jr, lr, jtot, lc, jc, lam)
+                if s == 0.0:
+                    continue
+                v = math.sqrt((2 * jc + 1) * (2 * jr + 1) * (2 * lc + 1) * (2 * lr + 1)) * t * u * s
+                if (jc + jr + jtot) % 2 == 1:
+                    v = -v
+                vl[il, idx] = v
+            idx += 1
+    return vl
+
+
+def packed_to_full(vl_col: np.ndarray, n: int) -> np.ndarray:
+    """Unpack one VL block (upper triangle, column by column) into a full matrix."""
+    m = np.zeros((n, n))
+    iu = np.triu_indices(n)
+    # triu_indices is row-major over (row, col) pairs; build column order
+    rows, cols = [], []
+    for c in range(n):
+        for r in range(c + 1):
+            rows.append(r)
+            cols.append(c)
+    m[rows, cols] = vl_col
+    m[cols, rows] = vl_col
+    del iu
+    return m
+
+
+# ---------------------------------------------------------------------------
+# POTENL case 1 (powers and exponentials)
+# ---------------------------------------------------------------------------
+@dataclass
+class PowerPotential:
+    """POTENL case 1, source_code/potenl.f:160-200.
+
+    terms[lam_index] = list of (A, n) with n != 0 meaning A*R**n, or
+    (A, 0, E) meaning A*exp(E*R).  R is in the potential's length unit
+    (R_prog * RSCALE); the result is in EPSIL units.
+    """
+
+    terms: List[List[tuple]]
+    rm: float = 1.0
+    epsil: float = 1.0
+
+    def __call__(self, r: float) -> np.ndarray:
+        p = np.zeros(len(self.terms))
+        for i, tl in enumerate(self.terms):
+            tot = 0.0
+            for t in tl:
+                if t[1] != 0:
+                    rr = r
+                    if t[1] < 0 and abs(r) < 1.0e-8:
+                        rr = 1.0e-8
+                    tot += t[0] * rr ** t[1]
+                else:
+                    tot += t[0] * math.exp(t[2] * r)
+            p[i] = tot
+        return p
+
+
+def lj_rotor_potential(cvals: Sequence[float], rm: float, epsil: float) -> PowerPotential:
+    """V_lam = c_lam * eps * [(Rm/R)^12 - 2 (Rm/R)^6] as &POTL NTERM=2,
+    NPOWER=-12,-6, A=c,-2c (the form of examples/input/molscat-basic1.input)."""
+    return PowerPotential([[(c, -12), (-2.0 * c, -6)] for c in cvals], rm=rm, epsil=epsil)
+
+
+# ---------------------------------------------------------------------------
+# one (JTOT, IB) block and its W(R)
+# ---------------------------------------------------------------------------
+@dataclass
+class Block:
+    n: int
+    vl: np.ndarray  # (NHAM, N(N+1)/2)
+    eint: np.ndarray  # reduced units
+    cent: np.ndarray  # L(L+1)
+    lvals: np.ndarray
+    levidx: np.ndarray  # 0-based level index of each channel
+    potl: Callable[[float], np.ndarray]
+    rscale: float
+    ep2ru: float
+    cm2ru: float
+    vl_full: np.ndarray = field(default=None)  # (NHAM, N, N)
+
+    def __post_init__(self):
+        if self.vl_full is None:
+            self.vl_full = np.stack([packed_to_full(self.vl[k], self.n) for k in range(self.vl.shape[0])])
+
+    def pvec(self, r: float) -> np.ndarray:
+        """HAMMAT steps 1-6 (source_code/hammat.f:38-73) for NCONST=NRSQ=0."""
+        p = np.array(self.potl(r * self.rscale), dtype=np.float64)
+        p = np.where(np.abs(p) > 1.0e60, np.sign(p) * 1.0e60, p)
+        return self.ep2ru * p
+
+    def hammat(self, r: float, ered: float) -> Tuple[np.ndarray, np.ndarray]:
+        """HAMMAT: source_code/hammat.f:1-116 with SUMLAM source_code/sumlam.f:29-36.
+
+        Returns the full W(N,N) and DIAG(N) = diag(W).
+        """
+        rsq = min(1.0 / (r * r), 1.0e16)
+        p = self.pvec(r)
+        w = np.tensordot(p, self.vl_full, axes=(0, 0))
+        d = np.diag(w).copy()
+        d = d + rsq * self.cent
+        d = d + self.eint
+        d = d - ered
+        np.fill_diagonal(w, d)
+        return w, d.copy()
+
+
+def make_itype1_block(jtot, ib, jlevels, be, lambdas, potl: PowerPotential, ured, runit=None):
+    """BASE + COUPLE for ITYPE=1, plus the unit conversions of
+    source_code/mol.driver.f:734-735 and source_code/chckru.f (RUNIT unset
+    means RUNIT = RPUNIT = RM)."""
+    if runit is None:
+        runit = potl.rm
+    rscale = runit / potl.rm
+    cm2ru = ured * 1.0 * runit * runit / BFCT
+    ep2ru = potl.epsil * cm2ru
+    chans = itype1_channels(jtot, ib, jlevels)
+    n = len(chans)
+    elev = rotor_levels(jlevels, be)
+    eint = np.array([cm2ru * elev[c[0]] for c in chans])
+    lv = np.array([c[1] for c in chans])
+    cent = np.array([float(l * (l + 1)) for l in lv])
+    vl = itype1_coupling(jtot, chans, jlevels, lambdas)
+    return Block(n=n, vl=vl, eint=eint, cent=cent, lvals=lv, levidx=np.array([c[0] for c in chans]),
+                 potl=potl, rscale=rscale, ep2ru=ep2ru, cm2ru=cm2ru), runit
+
+
+# ---------------------------------------------------------------------------
+# start-up, grid and step control
+# ---------------------------------------------------------------------------
+def calck(ep: float, ered: float, eint: np.ndarray) -> float:
+    """CALCK: source_code/calck.f."""
+    c = 0.0
+    for e in eint:
+        if ep + ered - e <= 0.0:
+            continue
+        c = max(c, math.sqrt(ep + ered - e))
+    return c
+
+
+def drcalc(rstart: float, rstop: float, kstep: int, nstep: int, pow_: float) -> float:
+    """DRCALC: source_code/drcalc.f:23-38."""
+    powint = 1.0 - pow_
+    if powint != 0.0:
+        powinv = 1.0 / powint
+        zstart = rstart ** powint
+        zstop = rstop ** powint
+        dz = (zstop - zstart) / float(nstep)
+        rnow = (zstart + float(kstep + 1) * dz) ** powinv
+        rlast = (zstart + float(kstep) * dz) ** powinv
+        return rnow - rlast
+    dz = rstart * ((rstop / rstart) ** (1.0 / float(nstep)) - 1.0)
+    return dz * (1.0 + dz / rstart) ** kstep
+
+
+UNSET = -1.0e30  # stands for the reference's 'unset' sentinel
+
+
+def drset(rstart, rstop, step, tolhi, cay, drseg, iprop, pow_, dr_prev=None):
+    """DRSET: source_code/drset.f:30-86.  Returns (NSTEP, DR)."""
+    rsmall = min(rstart, rstop)
+    rlarge = max(rstart, rstop)
+    powint = 1.0 - pow_
+    prange = rlarge - rsmall
+    nstep = 0
+    dr = dr_prev
+    if iprop == 4:
+        if drseg != UNSET:
+            dr = drseg
+        return nstep, dr
+    elif step > 0.0:
+        drfit = math.acos(-1.0) / (cay * step)
+    elif drseg != UNSET:
+        drfit = abs(drseg)
+    else:
+        drfit = abs(dr_prev)
+    if tolhi == 0.0:
+        rratio = drfit / rsmall
+        if pow_ == 0.0:
+            nstep = max(int(round_half_away(prange / drfit)), 1)
+            dr = prange / float(nstep)
+        elif pow_ == 1.0:
+            nstep = int(round_half_away(math.log(rlarge / rsmall) / math.log(rratio + 1.0)))
+            dr = drcalc(rstart, rstop, 0, nstep, pow_)
+        else:
+            zratio = (rlarge / rsmall) ** powint
+            zrm1 = zratio - 1.0
+            nstep = int(round_half_away(zrm1 / ((rratio + 1.0) ** powint - 1.0)))
+            dr = drcalc(rstart, rstop, 0, nstep, pow_)
+    elif tolhi < 1.0:
+        dr = drfit
+    if rstop < rstart:
+        dr = -abs(dr)
+    return nstep, dr
+
+
+def round_half_away(x: float) -> float:
+    """Fortran NINT."""
+    return math.floor(x + 0.5) if x >= 0 else -math.floor(-x + 0.5)
+
+
+def findrm(blk: Block, rmin: float, ered: float, irmset: int, cdrive: str = "M") -> Tuple[float, float]:
+    """FINDRM: source_code/findrm.f (returns RSTART, RTURN).
+
+    DIAGVL (lowest adiabat) is numpy eigvalsh, ascending like DSYEVR.
+    """
+    n = blk.n
+
+    def lowest(r):
+        w, _ = blk.hammat(r, ered)
+        return np.linalg.eigvalsh(w)[0]
+
+    if cdrive != "M":
+        rturn = rmin
+        itry = 1
+    else:
+        rturn = 1.0e30
+        for i in range(n):
+            dif = ered - blk.eint[i]
+            if dif < 0.0:
+                continue
+            rcent = math.sqrt(blk.cent[i] / dif)
+            rcent = max(rcent, rmin)
+            rturn = min(rturn, rcent)
+        itry = 0
+        if rturn == rmin:
+            itry = 1
+    while True:  # label 10
+        itry += 1
+        rstart = rturn
+        if itry > 25:
+            return rmin, rturn
+        diag1 = lowest(rstart)
+        rturn = 0.999 * rstart
+        converged = False
+        retry = False
+        dv1 = 0.0
+        for _ in range(100):
+            v2 = lowest(rturn)
+            dv1 = (v2 - diag1) / (rturn - rstart)
+            diag1 = v2
+            if dv1 >= 0.0:
+                if itry == 1:
+                    rturn = rmin
+                    retry = True
+                    break
+                return rmin, 2.0 * rmin
+            rstart = rturn
+            v1 = v2
+            dr = -v1 / dv1
+            if dr < -0.3 * rturn:
+                dr = -0.3 * rturn
+            rturn = rturn + dr
+            if dr > 1.0e3:
+                break
+            if rturn <= 0.0:
+                break
+            if abs(dr / rturn) <= 1.0e-3:
+                converged = True
+                break
+        if retry:
+            continue
+        if not converged:
+            return rmin, rturn
+        break
+    # label 160
+    if irmset <= 0:
+        return rmin, rturn
+    nstep = 3 + irmset // 3
+    target = 2.303 * float(irmset)
+    dr = (1.5 * target / math.sqrt(abs(dv1)) / float(nstep)) ** (2.0 / 3.0)
+    rstart = rturn
+    phase = 0.0
+    while True:  # label 220
+        restart = False
+        xk2 = 0.0
+        for istep in range(1, nstep + 1):
+            rnext = rstart - dr
+            if rnext < 0.0:
+                return 0.0, rturn
+            d1 = lowest(rnext)
+            drnext = 0.0
+            if d1 <= 0.0:
+                raise NotImplementedError("FINDRM inner allowed region branch")
+            xk1 = math.sqrt(d1)
+            if istep == 1:
+                phase = (2.0 / 3.0) * xk1 * dr
+            else:
+                xkavg = 0.5 * (xk1 + xk2)
+                phase = phase + dr * xkavg
+            drnext = max(drnext, (target - phase) / xk1)
+            xk2 = xk1
+            rstart = rnext
+            if istep < nstep:
+                dr = drnext / float(nstep - istep)
+            if drnext <= 0.0:
+                break
+            if istep < nstep and dr > 0.5 * rstart and dr > 0.5 * rmin:
+                dr = 0.02 * rstart
+                restart = True
+                break
+        if not restart:
+            break
+    return rstart, rturn
+
+
+def yinit(blk: Block, ered: float, r: float, zout: bool = True,
+          bcyomn: Optional[float] = None, bcycmx: float = -1.0, wkb: bool = True,
+          adiab: bool = True) -> np.ndarray:
+    """YINIT: source_code/yinit.f:70-150 (ADIAMN/ADIAMX default .TRUE.,
+    WKBMN/WKBMX default .TRUE.)."""
+    w, _ = blk.hammat(r, ered)
+    n = blk.n
+    if adiab:
+        ev, evec = np.linalg.eigh(w)
+    else:
+        ev = np.diag(w).copy()
+    yd = np.zeros(n)
+    nopen = 0
+    for i in range(n):
+        wv = ev[i]
+        if wv > 0.0:
+            if zout:
+                yd[i] = math.sqrt(abs(wv)) if wkb else -1.0
+            else:
+                yd[i] = -math.sqrt(abs(wv)) if wkb else bcycmx
+        else:
+            if zout:
+                if bcyomn is None:
+                    raise RuntimeError("YINIT: locally open channel at RMIN and BCYOMN unset")
+                yd[i] = bcyomn
+            else:
+                yd[i] = bcyomn if bcyomn is not None else 0.0
+            if abs(wv) > 1.0e-10:
+                nopen += 1
+    if adiab:
+        return (evec * yd[None, :]) @ evec.T
+    return np.diag(yd)
+
+
+# ---------------------------------------------------------------------------
+# propagators
+# ---------------------------------------------------------------------------
+def _sym_lower(m: np.ndarray) -> np.ndarray:
+    lo = np.tril(m)
+    return lo + np.tril(m, -1).T
+
+
+def mdprop(y: np.ndarray, hammat: Callable[[float], Tuple[np.ndarray, np.ndarray]],
+           rstart: float, nstep: int, dr: float) -> Tuple[np.ndarray, int]:
+    """MDPROP (diabatic modified log-derivative, IPROP=6): source_code/mdprop.f:64-279.
+
+    ``hammat(r)`` returns (W, DIAG) at the propagation energy.  Returns the
+    final log-derivative matrix and NODES.
+    """
+    n = y.shape[0]
+    h = dr / 2.0
+    d1 = h * h / 3.0
+    d2 = 2.0 * d1
+    d4 = -d1 / 16.0
+    half = 0.5 * h
+    r = rstart
+    nsave = 0
+    nodes = 0
+    u, diag = hammat(r)
+    y = h * y + d1 * u
+    idx = np.arange(n)
+    for kstep in range(1, nstep + 1):
+        r = r + h
+        u, diag = hammat(r)
+        u = d4 * u
+        u[idx, idx] = 0.125
+        u, kount = syminv(u)
+        if kount > n or kount < 0:
+            raise RuntimeError(f"MDPROP: MATRIX INVERSION ERROR AT STEP K = {kstep}")
+        nsave += kount
+        u[idx, idx] -= 8.0
+        wref = diag
+        arg = half * np.sqrt(np.abs(wref))
+        y14 = np.empty(n)
+        y23 = np.empty(n)
+        neg = wref < 0.0
+        tn = np.tan(arg[neg])
+        y14[neg] = arg[neg] / tn - arg[neg] * tn
+        y23[neg] = arg[neg] / tn + arg[neg] * tn
+        pos = ~neg
+        th = np.tanh(arg[pos])
+        y14[pos] = arg[pos] / th + arg[pos] * th
+        y23[pos] = arg[pos] / th - arg[pos] * th
+        u[idx, idx] += 2.0 * y14
+        y14 = y14 - d1 * diag
+        y[idx, idx] += y14
+        y, kount = syminv(y)
+        if kount > n or kount < 0:
+            raise RuntimeError(f"MDPROP: MATRIX INVERSION ERROR AT STEP K = {kstep}")
+        nodes += kount
+        y = u - y23[:, None] * y * y23[None, :]
+        y, kount = syminv(y)
+        if kount > n or kount < 0:
+            raise RuntimeError(f"MDPROP: MATRIX INVERSION ERROR AT STEP K = {kstep}")
+        nodes += kount
+        if kstep == nstep:
+            d2 = d1
+        r = r + h
+        u, _ = hammat(r)
+        u = d2 * u
+        y = u - y23[:, None] * y * y23[None, :]
+        y[idx, idx] += y14
+    y = _sym_lower(y / h)
+    return y, nodes - nsave
+
+
+def ldprop(z: np.ndarray, hammat, rstart: float, nstep: int, dr: float) -> Tuple[np.ndarray, int]:
+    """LDPROP (Johnson log-derivative, IPROP=5): source_code/ldprop.f:24-147."""
+    n = z.shape[0]
+    h = dr / 2.0
+    d1 = h * h / 3.0
+    d2 = 2.0 * d1
+    d4 = -d1 / 16.0
+    r = rstart
+    nodes = 0
+    idx = np.arange(n)
+    u, _ = hammat(r)
+    z = h * z + d1 * u
+    z[idx, idx] += 1.0
+    for istep in range(1, nstep + 1):
+        r = r + h
+        u, _ = hammat(r)
+        u = d4 * u
+        u[idx, idx] += 0.125
+        u, ncu = syminv(u)
+        if ncu > n or ncu < 0:
+            raise RuntimeError("LDPROP: ERROR IN SYMINV")
+        z, ncz = syminv(z)
+        if ncz > n or ncz < 0:
+            raise RuntimeError("LDPROP: ERROR IN SYMINV")
+        nodes += ncz
+        z = u - z
+        z[idx, idx] -= 6.0
+        z, ncz = syminv(z)
+        if ncz > n or ncz < 0:
+            raise RuntimeError("LDPROP: ERROR IN SYMINV")
+        nodes += ncz - ncu
+        r = r + h
+        if istep == nstep:
+            d2 = d1
+        u, _ = hammat(r)
+        u = d2 * u
+        u[idx, idx] += 2.0
+        z = u - z
+    hi = 1.0 / h
+    z = _sym_lower(hi * z)
+    z[idx, idx] -= hi
+    return z, nodes
+
+
+def odprop(y0: float, blk: Block, ered: float, rstart: float, nstep: int, dr: float) -> Tuple[float, int]:
+    """ODPROP (single-channel LDMD): source_code/odprop.f (NCONST=NRSQ=0,
+    CONLEN=0, no scratch replay).  NODES is always 0 there."""
+    npt = nstep + 1
+    dr6 = dr / 6.0
+    h = dr / 2.0
+    u = np.zeros(npt)
+    w = np.zeros(npt)
+    eintep = blk.eint[0]
+    r = rstart
+    pts = []
+    for i in range(npt):
+        pts.append(blk.pvec(r))
+        r += dr
+    u[:] = eintep
+    for i in range(npt):
+        u[i] += float(np.dot(blk.vl[:, 0], pts[i]))
+    r = rstart
+    for i in range(npt):
+        rsq = min(1.0 / (r * r), 1.0e16)
+        u[i] += blk.cent[0] * rsq
+        r += dr
+    r = rstart + h
+    mids = []
+    for i in range(nstep):
+        mids.append(blk.pvec(r))
+        r += dr
+    w[:] = eintep
+    for i in range(nstep):
+        w[i] += float(np.dot(blk.vl[:, 0], mids[i]))
+    # P(:,NPT) keeps the end-point value in the reference (odprop.f loops to NPT)
+    w[npt - 1] += float(np.dot(blk.vl[:, 0], pts[npt - 1]))
+    r = rstart + h
+    for i in range(nstep):
+        rsq = min(1.0 / (r * r), 1.0e16)
+        w[i] += blk.cent[0] * rsq
+        r += dr
+    q = np.zeros(npt)
+    for i in range(1, npt):
+        q[i] = u[i] - w[i - 1]
+    qlast = q[npt - 1] * dr6
+    for i in range(nstep):
+        u[i] = (u[i] - w[i] + q[i]) * dr6
+    y = y0
+    for i in range(nstep):
+        wref = w[i] - ered
+        flam = 0.5 * math.sqrt(abs(wref))
+        if wref < 0.0:
+            tn = math.tan(flam * dr)
+            y1 = flam / tn - flam * tn
+            y2 = flam / tn + flam * tn
+        else:
+            tn = math.tanh(flam * dr)
+            y1 = flam / tn + flam * tn
+            y2 = flam / tn - flam * tn
+        y2 = y2 * y2
+        y = y1 - y2 / (y + (y1 + u[i]))
+    return y + qlast, 0
+
+
+# ---------------------------------------------------------------------------
+# Riccati-Bessel functions (Temme's methods), source_code/rbesjy.f, rbessk.f
+# ---------------------------------------------------------------------------
+_TEMME_B = (-0.283876542276024, -0.076852840844786, +0.001706305071096, +0.001271927136655,
+            +0.000076309597586, -0.000004971736704, -0.000000865920800, -0.000000033126120,
+            +0.000000001745136, +0.000000000242310, +0.000000000009161, -0.000000000000170)
+
+
+def _rgamma(x: float) -> Tuple[float, float, float]:
+    """1/Gamma(1-x) with odd/even parts (Temme 1975), source_code/rbesjy.f RGAMMA."""
+    x2 = x * x * 8.0
+    alfa = -0.000000000000001
+    beta = 0.0
+    for i in range(12, 1, -2):
+        beta = -(2 * alfa + beta)
+        alfa = -beta * x2 - alfa + _TEMME_B[i - 1]
+    even = (beta / 2.0 + alfa) * x2 - alfa + 0.921870293650453
+    alfa = -0.000000000000034
+    beta = 0.0
+    for i in range(11, 0, -2):
+        beta = -(2 * alfa + beta)
+        alfa = -beta * x2 - alfa + _TEMME_B[i - 1]
+    odd = 2 * (alfa + beta)
+    return odd * x + even, odd, even
+
+
+def _bessjy(v: float, x: float):
+    """BESSJY: source_code/rbesjy.f:52-283.  Returns (CJ,DJ,EJ,CY,DY,EY)."""
+    eps = 1.0e-15
+    maxit = 1000
+    pi = math.acos(-1.0)
+    xmin = 3.0
+    xmax = 5.0 - math.log10(eps)
+    na = int(v + 0.5)
+    a = v - na
+    if x < xmin:
+        b = x / 2.0
+        d = -math.log(b)
+        e = a * d
+        c = 1.0 / pi if abs(a) < eps else a / math.sin(a * pi)
+        s = 1.0 if abs(e) < eps else math.sinh(e) / e
+        e = math.exp(e)
+        rg, p, q = _rgamma(a)
+        g = e * rg
+        e = (e + 1.0 / e) / 2.0
+        f = 2 * c * (p * e + q * s * d)
+        e = a * a
+        p = g * c
+        q = 1.0 / g / pi
+        c = a * pi / 2.0
+        r = 1.0 if abs(c) < eps else math.sin(c) / c
+        r = pi * c * r * r
+        c = 1.0
+        d = -b * b
+        ya = f + r * q
+        ya1 = p
+        ok = False
+        for n in range(1, maxit + 1):
+            f = (f * n + p + q) / (n * n - e)
+            c = c * d / n
+            p = p / (n - a)
+            q = q / (n + a)
+            g = c * (f + r * q)
+            h = c * p - n * g
+            ya = ya + g
+            ya1 = ya1 + h
+            dl = abs(g) / (1.0 + abs(ya))
+            dl1 = abs(h) / (1.0 + abs(ya1))
+            if dl + dl1 < eps:
+                ok = True
+                break
+        if not ok:
+            raise RuntimeError("BESSJY 1")
+        f = -ya
+        g = -ya1 / b
+        # d, pa, qa etc. are not set on this branch; the J part below uses
+        # CF1 unless X >= max(XMIN,V), which cannot hold here.
+        dd = pa = qa = pa1 = qa1 = cc = ss = 0.0
+    elif x < xmax:
+        c = 0.25 - a * a
+        b = x + x
+        e = (x * math.cos(a * pi) / pi / eps) ** 2
+        p = 1.0
+        q = -x
+        r = 1.0 + x * x
+        s = r
+        ok = False
+        for n in range(2, maxit + 1):
+            d = (n - 1 + c / n) / s
+            p = (2 * n - p * d) / (n + 1)
+            q = (-b + q * d) / (n + 1)
+            s = p * p + q * q
+            r = r * s
+            if r * n * n > e:
+                ok = True
+                break
+        if not ok:
+            raise RuntimeError("BESSJY 2")
+        p = p / s
+        f = p
+        q = -q / s
+        g = q
+        for m in range(n, 0, -1):
+            r = (m + 1) * (2.0 - p) - 2.0
+            s = b + (m + 1) * q
+            d = (m - 1 + c / m) / (r * r + s * s)
+            p = d * r
+            q = d * s
+            e = f + 1.0
+            f = p * e - g * q
+            g = q * e + p * g
+        f = 1.0 + f
+        d = f * f + g * g
+        pa = f / d
+        qa = -g / d
+        d = a + 0.5 - p
+        q = q + x
+        pa1 = (pa * q - qa * d) / x
+        qa1 = (qa * q + pa * d) / x
+        b = x - pi * (a + 0.5) / 2.0
+        cc = math.cos(b)
+        ss = math.sin(b)
+        dd = math.sqrt(2.0 / x / pi)
+        f = dd * (pa * ss + qa * cc)
+        g = dd * (qa1 * ss - pa1 * cc)
+    else:
+        p = 0.0
+        q = 0.0
+        pa = qa = 0.0
+        for ia in range(2):
+            pa = p
+            qa = q
+            yy = 4.0 * (a + ia) ** 2
+            z = 8.0 * x
+            d = 0.0
+            w = -1.0
+            p = 1.0
+            q = 0.0
+            tp = 1.0
+            ok = False
+            for _k in range(maxit):
+                d = d + z
+                w = w + 2.0
+                tq = +tp * (yy - w * w) / d
+                q = q + tq
+                d = d + z
+                w = w + 2.0
+                tp = -tq * (yy - w * w) / d
+                p = p + tp
+                if abs(tp) + abs(tq) < eps:
+                    ok = True
+                    break
+            if not ok:
+                raise RuntimeError("BESSJY 3")
+            p = p - 0.5 * tp
+            q = q - 0.5 * tq
+        pa1 = p
+        qa1 = q
+        b = x - pi * (a + 0.5) / 2.0
+        cc = math.cos(b)
+        ss = math.sin(b)
+        dd = math.sqrt(2.0 / x / pi)
+        f = dd * (pa * ss + qa * cc)
+        g = dd * (qa1 * ss - pa1 * cc)
+    # upward recurrence for Y
+    p = 0.0
+    if na > 0:
+        yv = 2.0 / x
+        for n in range(1, na + 1):
+            h = yv * (a + n) * g - f
+            f = g
+            g = h
+            while abs(f) > 4.0:
+                p = p + 1.0
+                f = 0.0625 * f
+                g = 0.0625 * g
+    cy = f
+    dy = (v / x) * f - g
+    sy = math.sqrt(cy * cy + dy * dy)
+    cy = cy / sy
+    dy = dy / sy
+    ey = math.log(sy) + p * math.log(16.0)
+    vv = max(xmin, v)
+    if x >= vv:
+        f = dd * (pa * cc - qa * ss)
+        g = dd * (qa1 * cc + pa1 * ss)
+        if na > 0:
+            yv = 2.0 / x
+            for n in range(1, na + 1):
+                h = yv * (a + n) * g - f
+                f = g
+                g = h
+        cj = f
+        dj = (v / x) * f - g
+        sj = math.sqrt(cj * cj + dj * dj)
+        cj = cj / sj
+        dj = dj / sj
+        ej = math.log(sj)
+    else:
+        ap = 1.0
+        aa = v / x
+        bp = 0.0
+        bb = 1.0
+        f = 0.0
+        g = 0.0
+        yv = 2.0 / x
+        w = yv / pi
+        ok = False
+        for n in range(1, maxit + 1):
+            an = yv * (v + n) * aa - ap
+            ap = aa
+            aa = an
+            bn = yv * (v + n) * bb - bp
+            bp = bb
+            bb = bn
+            if abs(bb) > abs(aa):
+                ap = ap / bb
+                aa = aa / bb
+                bp = bp / bb
+                bb = 1.0
+                if abs(aa - f) < eps * abs(f):
+                    cj = w / (dy - cy * aa)
+                    dj = aa * cj
+                    ok = True
+                    break
+                f = aa
+            else:
+                bp = bp / aa
+                bb = bb / aa
+                ap = ap / aa
+                aa = 1.0
+                if abs(bb - g) < eps * abs(g):
+                    dj = w / (dy * bb - cy)
+                    cj = bb * dj
+                    ok = True
+                    break
+                g = bb
+        if not ok:
+            raise RuntimeError("BESSJY 4")
+        sj = math.sqrt(cj * cj + dj * dj)
+        cj = cj / sj
+        dj = dj / sj
+        ej = math.log(sj) - ey
+    return cj, dj, ej, cy, dy, ey
+
+
+def rbesjy(ell: float, x: float) -> Tuple[float, float, float, float]:
+    """Riccati-Bessel j, j', y, y' of fractional order: source_code/rbesjy.f:1-50."""
+    if x <= 0.0 or ell < -0.5:
+        raise RuntimeError("RBESJY 0")
+    v = ell + 0.5
+    cj, dj, ej, cy, dy, ey = _bessjy(v, x)
+    pi = math.acos(-1.0)
+    ex = 0.5 * math.log(pi * x / 2.0)
+    dj = dj + cj / (2.0 * x)
+    sj = math.sqrt(cj * cj + dj * dj)
+    cj = cj / sj
+    dj = dj / sj
+    ej = ej + math.log(sj) + ex
+    dy = dy + cy / (2.0 * x)
+    sy = math.sqrt(cy * cy + dy * dy)
+    cy = cy / sy
+    dy = dy / sy
+    ey = ey + math.log(sy) + ex
+    return cj * math.exp(ej), dj * math.exp(ej), cy * math.exp(ey), dy * math.exp(ey)
+
+
+def _mbessk(v: float, x: float):
+    """MBESSK: source_code/rbessk.f:41-148.  Returns (CK,DK,EK)."""
+    eps = 1.0e-15
+    maxit = 1000
+    pi = math.acos(-1.0)
+    xmin = 1.0
+    na = int(v + 0.5)
+    a = v - na
+    if x < xmin:
+        b = x / 2.0
+        d = -math.log(b)
+        e = a * d
+        c = a * pi
+        c = 1.0 if abs(c) < eps else c / math.sin(c)
+        s = 1.0 if abs(e) < eps else math.sinh(e) / e
+        e = math.exp(e)
+        rg, p, q = _rgamma(a)
+        g = e * rg
+        e = (e + 1.0 / e) / 2.0
+        f = c * (p * e + q * s * d)
+        e = a * a
+        p = 0.5 * g * c
+        q = 0.5 / g
+        c = 1.0
+        d = b * b
+        ak = f
+        ak1 = p
+        ok = False
+        for n in range(1, maxit + 1):
+            f = (f * n + p + q) / (n * n - e)
+            c = c * d / n
+            p = p / (n - a)
+            q = q / (n + a)
+            g = c * (p - n * f)
+            h = c * f
+            ak = ak + h
+            ak1 = ak1 + g
+            if h / ak + abs(g) / ak1 < eps:
+                ok = True
+                break
+        if not ok:
+            raise RuntimeError("MBESSK 1")
+        f = ak
+        g = ak1 / b
+        ex = 0.0
+    else:
+        c = 0.25 - a * a
+        g = 1.0
+        f = 0.0
+        e = x * math.cos(a * pi) / pi / eps
+        ok = False
+        for n in range(1, maxit + 1):
+            h = (2 * (n + x) * g - (n - 1 + c / n) * f) / (n + 1)
+            f = g
+            g = h
+            if h * n > e:
+                ok = True
+                break
+        if not ok:
+            raise RuntimeError("MBESSK 2")
+        p = f / g
+        q = p
+        b = x + x
+        e = b - 2.0
+        for m in range(n, 0, -1):
+            p = (m - 1 + c / m) / (e + (m + 1) * (2.0 - p))
+            q = p * (q + 1.0)
+        f = math.sqrt(pi / b) / (1.0 + q)
+        g = f * (a + x + 0.5 - p) / x
+        ex = x
+    p = 0.0
+    if na > 0:
+        yv = 2.0 / x
+        for n in range(1, na + 1):
+            h = yv * (a + n) * g + f
+            f = g
+            g = h
+            while abs(f) > 4.0:
+                p = p + 1.0
+                f = 0.0625 * f
+                g = 0.0625 * g
+    ck = f
+    dk = (v / x) * f - g
+    sk = math.sqrt(ck * ck + dk * dk)
+    ck = ck / sk
+    dk = dk / sk
+    ek = math.log(sk) + p * math.log(16.0) - ex
+    return ck, dk, ek
+
+
+def rbessk(ell: float, x: float) -> float:
+    """Ratio K'/K of the modified Riccati-Bessel function: source_code/rbessk.f:1-39."""
+    if x <= 0.0 or ell < -0.5:
+        raise RuntimeError("RBESSK 0")
+    v = ell + 0.5
+    ck, dk, ek = _mbessk(v, x)
+    dk = dk + ck / (2.0 * x)
+    return dk / ck
+
+
+# ---------------------------------------------------------------------------
+# matching
+# ---------------------------------------------------------------------------
+def chnsrt(eint: np.ndarray) -> np.ndarray:
+    """CHNSRT: source_code/chnsrt.f (exchange sort, 0-based NB)."""
+    n = len(eint)
+    nb = list(range(n))
+    for i in range(n - 1):
+        for j in range(i + 1, n):
+            if eint[nb[i]] <= eint[nb[j]]:
+                continue
+            if abs(eint[nb[i]] - eint[nb[j]]) < 1.0e-11:
+                continue
+            nb[i], nb[j] = nb[j], nb[i]
+    return np.array(nb)
+
+
+def gtwvec(ered: float, eint: np.ndarray) -> Tuple[np.ndarray, int]:
+    """GTWVEC: source_code/gtwvec.f:18-30."""
+    dif = ered - eint
+    wvec = np.sign(dif) * np.sqrt(np.abs(dif))
+    wvec = np.where(dif == 0.0, 0.0, wvec)
+    nopen = int(np.sum(dif > 0.0))
+    return wvec, nopen
+
+
+def ytok(nb, wvec, n, nopen, y, rval, cent) -> np.ndarray:
+    """YTOK + YTOKG + KSYM: source_code/ytok.f:21-48, ytokg.f:21-76, ksym.f."""
+    sj = np.zeros(n)
+    sjp = np.zeros(n)
+    sn = np.zeros(n)
+    snp = np.zeros(n)
+    for i in range(nopen):
+        nx = nb[i]
+        dw = wvec[nx]
+        darg = dw * rval
+        ell = math.sqrt(cent[nx] + 0.25) - 0.5
+        uj, ujp, un, unp = rbesjy(ell, darg)
+        rdw = math.sqrt(dw)
+        sj[nx] = uj / rdw
+        sjp[nx] = ujp * rdw
+        sn[nx] = un / rdw
+        snp[nx] = unp * rdw
+    for i in range(nopen, n):
+        nx = nb[i]
+        dw = abs(wvec[nx])
+        darg = dw * rval
+        ell = math.sqrt(cent[nx] + 0.25) - 0.5
+        ratio = rbessk(ell, darg)
+        sn[nx] = 1.0
+        snp[nx] = ratio * dw
+    ys = _sym_lower(y)  # DSYFIL('U') fills the upper from the lower triangle
+    yp = ys[np.ix_(nb, nb)]
+    t = yp[:, :nopen] * sj[nb[:nopen]][None, :]
+    for i in range(nopen):
+        t[i, i] -= sjp[nb[i]]
+    km = yp * sn[nb][None, :]
+    km[np.arange(n), np.arange(n)] -= snp[nb]
+    x = np.linalg.solve(km, t)
+    k = x[:nopen, :nopen].copy()
+    return 0.5 * (k + k.T)
+
+
+def ktos(k: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
+    """KTOS: source_code/ktos.f:20-40 (Im S = +2 (I+K^2)^-1 K, as coded)."""
+    nop = k.shape[0]
+    sr = 0.5 * (k @ k)
+    sr[np.arange(nop), np.arange(nop)] += 0.5
+    sr, _ = syminv(sr)
+    si = sr @ k
+    sr = sr.copy()
+    sr[np.arange(nop), np.arange(nop)] -= 1.0
+    return sr, si
+
+
+def calca(sr_ii: float, si_ii: float, wvec: float, ll: int, lrpow: int = 6) -> complex:
+    """CALCA: source_code/calca.f:42-59."""
+    smag = math.sqrt(sr_ii ** 2 + si_ii ** 2)
+    if smag > 1.0:
+        sr_ii /= smag
+        si_ii /= smag
+    if sr_ii <= -1.0:
+        return complex(1.0e30, 0.0)
+    m = min(lrpow - 2, ll + ll + 1)
+    return complex(1.0 - sr_ii, -si_ii) / (complex(1.0 + sr_ii, si_ii) * complex(0.0, wvec ** m))
+
+
+def sigma_block(sr, si, nb, nopen, levidx, jlevels, wvec, jtot, runit, nsig, wght=1.0):
+    """SPROC partial cross sections: source_code/sproc.f:380-430 with
+    DEGENF (ITYPE=1: 2j_i+1, source_code/base.f:1126-1129).  Returns
+    SIGMA(F, I) (0-based levels)."""
+    xj = float(2 * jtot + 1) * math.pi
+    if wght > 0.0:
+        xj *= wght
+    sig = np.zeros((nsig, nsig))
+    for icol in range(nopen):
+        lcol = levidx[nb[icol]]
+        for irow in range(nopen):
+            lrow = levidx[nb[irow]]
+            s_r = sr[irow, icol]
+            s_i = si[irow, icol]
+            smag = s_r ** 2 + s_i ** 2
+            if lrow < nsig and lcol < nsig:
+                dd = wvec[nb[irow]] ** 2 / runit ** 2
+                if irow == icol:
+                    smag = (1.0 - s_r) ** 2 + s_i ** 2
+                degen = float(2 * jlevels[lrow] + 1)
+                sig[lcol, lrow] += smag * xj / (degen * dd)
+    return sig
+
+
+# ---------------------------------------------------------------------------
+# SCCTRL-like single-segment control for one (JTOT, IB) block
+# ---------------------------------------------------------------------------
+@dataclass
+class Grid:
+    rstart: float
+    rstop: float
+    nstep: int
+    dr: float
+
+
+def ld_grid(blk: Block, rmin: float, rmax: float, eredmx: float, steps: float = -10.0,
+            drs: float = UNSET, irmset: int = 9) -> Tuple[Grid, float]:
+    """Start point (FINDRM) and constant-step grid (DRSET) for an LD segment,
+    as set up at the first energy: source_code/mol.driver.f:1490-1550 and
+    source_code/scctrl.f:99-108."""
+    if irmset > 0:
+        rstart, rturn = findrm(blk, rmin, eredmx, irmset)
+    else:
+        rstart, rturn = rmin, rmin
+    cays = calck(0.0, eredmx, blk.eint)
+    nstep, dr = drset(rstart, rmax, steps, 0.0, cays, drs, 6, 0.0)
+    return Grid(rstart, rmax, nstep, dr), rturn
+
+
+def smatrix_ld(blk: Block, ered: float, grid: Grid, iprop: int = 6):
+    """YINIT + MDPROP/LDPROP/ODPROP + YTOK + KTOS for one energy.
+
+    Returns dict with Y (final), K, SR, SI (NB-ordered open channels), NB,
+    NOPEN, WVEC, NODES."""
+    wvec, nopen = gtwvec(ered, blk.eint)
+    nb = chnsrt(blk.eint)
+    y = yinit(blk, ered, grid.rstart)
+    hm = lambda r: blk.hammat(r, ered)
+    if iprop == 6 and blk.n > 1:
+        y, nodes = mdprop(y, hm, grid.rstart, grid.nstep, grid.dr)
+    elif iprop == 6:
+        yy, nodes = odprop(float(y[0, 0]), blk, ered, grid.rstart, grid.nstep, grid.dr)
+        y = np.array([[yy]])
+    elif iprop == 5:
+        y, nodes = ldprop(y, hm, grid.rstart, grid.nstep, grid.dr)
+    else:
+        raise ValueError(iprop)
+    out = dict(Y=y, NB=nb, NOPEN=nopen, WVEC=wvec, NODES=nodes)
+    if nopen > 0:
+        k = ytok(nb, wvec, blk.n, nopen, y, grid.rstop, blk.cent)
+        sr, si = ktos(k)
+        out.update(K=k, SR=sr, SI=si)
+    return out
