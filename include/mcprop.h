@@ -1,0 +1,121 @@
+/*
+ * mcprop: MI355X (gfx950) engine for the coupled-channel log-derivative
+ * propagation and matching path of MOLSCAT / BOUND / FIELD.
+ *
+ * C ABI.  All matrices are column-major FP64 with leading dimension N unless
+ * stated otherwise; integers are 32-bit; every entry point returns MCP_OK (0)
+ * or a negative MCP_ERR_* code, and mcp_last_error() gives the message.
+ * Per-instance failures (a singular inversion at one energy) are reported in
+ * the per-instance status array and never affect the other instances of the
+ * batch.  A context is used from one host thread at a time.
+ *
+ * Units are the reference's reduced units: energies in 1/RUNIT^2 (ERED,
+ * EINT = CM2RU * E[cm-1]), radii in RUNIT, P_lambda(R) already multiplied by
+ * EP2RU (HAMMAT steps 1-6, source_code/hammat.f:38-73, done by the caller).
+ */
+#ifndef MCPROP_H
+#define MCPROP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCP_OK 0
+#define MCP_ERR_ARG -1
+#define MCP_ERR_HIP -2
+#define MCP_ERR_UNSUPPORTED -3
+#define MCP_ERR_NOMEM -4
+
+typedef struct mcp_ctx mcp_ctx;
+
+/* Context on HIP device `device`.  No reference counterpart (the reference
+ * keeps its state in COMMON blocks, e.g. /PRPSCR/ at source_code/mdprop.f:56). */
+int mcp_ctx_create(int device, mcp_ctx **ctx);
+void mcp_ctx_destroy(mcp_ctx *ctx);
+const char *mcp_last_error(const mcp_ctx *ctx);
+/* Largest N the LDS-resident kernels take for the MDPROP / LDPROP / eigen
+ * paths on this device, the LDS bytes per workgroup and the CU count. */
+int mcp_ctx_info(const mcp_ctx *ctx, int32_t *max_n_md, int32_t *max_n_ld, int32_t *max_n_eig, int32_t *lds_bytes,
+                 int32_t *num_cu);
+
+/* SYMINV for a batch: replaces SYMINV(A,IA,N,IFAIL) (source_code/syminv.f:1,
+ * with SYMINL source_code/syminl-alloc.f:1 for N > 30).  Reads the lower
+ * triangle of each a[m], writes the full symmetric inverse to ainv[m] and
+ * kount[m] = number of negative eigenvalues (IFAIL), N+1 for an exactly zero
+ * pivot when N <= 30, -INFO (< 0) for an exactly singular column when N > 30. */
+int mcp_syminv_batched(mcp_ctx *ctx, int32_t n, int32_t nmat, const double *a, double *ainv, int32_t *kount);
+
+/* DIAGVC for a batch (source_code/diagvc.f:29-60, DSYEVR): eigenvalues
+ * ascending, eigenvectors as columns (evecs may be NULL). */
+int mcp_eigh_batched(mcp_ctx *ctx, int32_t n, int32_t nmat, const double *a, double *evals, double *evecs);
+
+/* Upload one (JTOT, IB) block: the coupling VL(NHAM, N(N+1)/2) with NHAM =
+ * mxlam + nconst + nrsq (lambda fastest, upper triangle column by column, as
+ * COUPLE/CPL9 build it: source_code/couple.f:120-127, sumlam.f:31-36), EINT
+ * and CENT (source_code/base.f:386-409).  conlen <= 0 disables the harmonic
+ * confinement term (hammat.f:101-106). */
+int mcp_block_create(mcp_ctx *ctx, int32_t n, int32_t mxlam, int32_t nconst, int32_t nrsq, const double *vl,
+                     const double *eint, const double *cent, double conlen, int32_t *block_id);
+int mcp_block_destroy(mcp_ctx *ctx, int32_t block_id);
+
+/* Upload a radial grid: npts radii and P(lambda, R) = EP2RU * POTENL(R*RSCALE)
+ * (clamped at 1e60, after SCAPOT/PERTRB), ptab[k*mxlam + l].  For the LD
+ * propagators the grid is R0 = RSTART, R_{2k-1} = midpoint, R_{2k} = end of
+ * step k, accumulated as R = R + H (source_code/mdprop.f:113,203), so npts =
+ * 2*NSTEP + 1, and dr is the step DR (H = DR/2 enters the propagator
+ * coefficients, mdprop.f:64-68); dr = 0 for a grid not used by LDPROP/MDPROP. */
+int mcp_grid_create(mcp_ctx *ctx, int32_t mxlam, int32_t npts, const double *r, const double *ptab, double dr,
+                    int32_t *grid_id);
+int mcp_grid_destroy(mcp_ctx *ctx, int32_t grid_id);
+
+/* HAMMAT (source_code/hammat.f:1-116) at nr grid points idx[] and energy
+ * ered; pconst = the NCONST constants (x EP2RU) or NULL.  w: nr full N x N. */
+int mcp_hammat(mcp_ctx *ctx, int32_t block_id, int32_t grid_id, int32_t nr, const int32_t *idx,
+               const double *pconst, double ered, double *w);
+
+/* YINIT (source_code/yinit.f:1-158, ADIAMN=WKBMN=.TRUE.) at grid point k0 for
+ * ninst instances; y: ninst N x N. */
+int mcp_yinit(mcp_ctx *ctx, int32_t block_id, int32_t grid_id, int32_t k0, int32_t outward, double bcyo,
+              int32_t ninst, const double *ered, const double *pconst, double *y);
+
+/* Log-derivative propagation over the whole grid for ninst instances of one
+ * block: iprop = 6 replaces MDPROP (source_code/mdprop.f:1-4), iprop = 5
+ * replaces LDPROP (source_code/ldprop.f:1-4).  y holds the starting log-
+ * derivative matrices on entry and Y(RSTOP) on exit; nodes = NODES; status =
+ * 0 or the 1-based step at which an inversion failed ("MATRIX INVERSION
+ * ERROR AT STEP K", mdprop.f:282).  pconst: ninst x nconst or NULL. */
+int mcp_prop_ld(mcp_ctx *ctx, int32_t block_id, int32_t grid_id, int32_t iprop, int32_t ninst, const double *ered,
+                const double *pconst, double *y, int32_t *nodes, int32_t *status);
+
+/* YTOK + KTOS (source_code/ytok.f:1, ytokg.f:1, ktos.f:1) for ninst
+ * instances: y (N x N), wvec (N, GTWVEC's signed wave vectors), nopen, nb
+ * (N, 0-based CHNSRT order), cent (N), rval = RSTOP.  Writes kmat, sr, si as
+ * nopen x nopen matrices with leading dimension nopen at stride N*N.
+ * status: 0, a Bessel failure code, or 100+k for a singular DGESV pivot. */
+int mcp_ytok_ktos(mcp_ctx *ctx, int32_t n, int32_t ninst, const double *y, const double *wvec,
+                  const int32_t *nopen, const int32_t *nb, const double *cent, double rval, double *kmat,
+                  double *sr, double *si, int32_t *status);
+
+/* Whole MOLSCAT solve for one block on one LD grid (SCCTRL with one segment,
+ * source_code/scctrl.f:80-289): YINIT at R0, propagation, YTOK, KTOS, with
+ * everything kept on the device; per-instance nopen/nb/wvec as for
+ * mcp_ytok_ktos; rval is the segment's nominal RSTOP, which YTOK matches at
+ * (scctrl.f:287), not the radius accumulated step by step.  sr/si/kmat as
+ * there; nodes/status per instance (status < 0: propagation failed at step
+ * -status). */
+int mcp_smatrix_ld(mcp_ctx *ctx, int32_t block_id, int32_t grid_id, int32_t iprop, int32_t ninst,
+                   const double *ered, const double *pconst, const double *wvec, const int32_t *nopen,
+                   const int32_t *nb, double rval, double *kmat, double *sr, double *si, int32_t *nodes,
+                   int32_t *status);
+
+/* Wall time in milliseconds of the last mcp_prop_ld / mcp_smatrix_ld call's
+ * device work, measured with HIP events on the library's stream, split into
+ * the energy-independent sector matrices and the propagation loop. */
+int mcp_last_timing(const mcp_ctx *ctx, double *ms_sector, double *ms_prop, double *ms_total);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,642 @@
+// C-ABI entry points of libmcprop (see include/mcprop.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/mcprop.h"
+#include "mcp_kernels.hpp"
+
+namespace {
+
+using mcp::kNT;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t ensure(size_t b) {
+    if (b <= bytes && p) return hipSuccess;
+    if (p) {
+      (void)hipFree(p);
+      p = nullptr;
+      bytes = 0;
+    }
+    hipError_t e = hipMalloc(&p, b ? b : 16);
+    if (e == hipSuccess) bytes = b;
+    return e;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+struct Block {
+  int n, mxlam, nconst, nrsq, nham;
+  double conlen4inv;
+  DevBuf vl, eint, cent;
+  std::vector<double> h_cent;
+};
+
+struct Grid {
+  int mxlam, npts;
+  double dr = 0.0;
+  DevBuf r, ptab;
+  std::vector<double> h_r;
+};
+
+// LDS carve sizes, mirroring LdsCarve::take (16-byte rounded pieces).
+struct LdsSize {
+  size_t b = 0;
+  template <class T>
+  LdsSize& add(size_t count) {
+    b += (count * sizeof(T) + 15) & ~(size_t)15;
+    return *this;
+  }
+  LdsSize& sweep(int n) {
+    add<double>(n).add<double>(n).add<double>(n).add<double>(n).add<int>(n);
+    add<double>(kNT / 64).add<int>(kNT / 64);
+    return *this;
+  }
+};
+
+size_t lds_syminv(int n) { return LdsSize().add<double>((size_t)n * n).sweep(n).b; }
+size_t lds_uc(int n, int nham) { return LdsSize().add<double>((size_t)n * n).add<double>(nham).sweep(n).b; }
+size_t lds_md(int n, int nham) {
+  return LdsSize().add<double>((size_t)n * n).add<double>(nham).add<double>(n).add<double>(n).add<double>(n).sweep(n).b;
+}
+size_t lds_ld(int n, int nham) {
+  return LdsSize().add<double>((size_t)n * n).add<double>((size_t)n * n).add<double>(nham).sweep(n).b;
+}
+size_t lds_eig(int n) {
+  const int half = (n + 1) / 2 + 1;
+  return LdsSize()
+      .add<double>((size_t)n * n)
+      .add<double>((size_t)n * n)
+      .add<double>(half)
+      .add<double>(half)
+      .add<int>(half)
+      .add<int>(half)
+      .add<double>(kNT / 64)
+      .add<double>(n)
+      .add<int>(n)
+      .add<double>(n)
+      .b;
+}
+size_t lds_yinit(int n, int nham) {
+  const int half = (n + 1) / 2 + 1;
+  return LdsSize()
+      .add<double>((size_t)n * n)
+      .add<double>((size_t)n * n)
+      .add<double>(half)
+      .add<double>(half)
+      .add<int>(half)
+      .add<int>(half)
+      .add<double>(kNT / 64)
+      .add<double>(nham)
+      .add<double>(n)
+      .b;
+}
+
+}  // namespace
+
+struct mcp_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  size_t lds_max = 65536;
+  int num_cu = 0;
+  std::string err;
+  std::map<int, std::unique_ptr<Block>> blocks;
+  std::map<int, std::unique_ptr<Grid>> grids;
+  int next_id = 1;
+  DevBuf s_a, s_b, s_c, s_d, s_e, s_f, s_g, s_h, s_i, s_j, s_k;
+  double t_sector = 0.0, t_prop = 0.0, t_total = 0.0;
+
+  int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    err = buf;
+    return code;
+  }
+  int hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return MCP_OK;
+    return fail(MCP_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+  }
+};
+
+#define HIPCHK(ctx, call)                               \
+  do {                                                  \
+    int _rc = (ctx)->hip((call), #call);                \
+    if (_rc != MCP_OK) return _rc;                      \
+  } while (0)
+
+namespace {
+
+int set_lds_limits(mcp_ctx* c) {
+  const void* fns[] = {(const void*)mcp::k_syminv,     (const void*)mcp::k_mdprop_uc, (const void*)mcp::k_mdprop_lds,
+                       (const void*)mcp::k_ldprop_lds, (const void*)mcp::k_eigh,      (const void*)mcp::k_yinit};
+  for (const void* f : fns) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_max);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      c->lds_max = 65536;
+      return MCP_OK;
+    }
+  }
+  return MCP_OK;
+}
+
+int max_n_for(size_t lds_max, size_t (*f)(int, int), int nham) {
+  int n = 1;
+  while (n < 4096 && f(n + 1, nham) <= lds_max) ++n;
+  return n;
+}
+
+size_t w_uc(int n, int nham) { return lds_uc(n, nham); }
+size_t w_md(int n, int nham) { return std::max(lds_md(n, nham), lds_uc(n, nham)); }
+size_t w_ld(int n, int nham) { return lds_ld(n, nham); }
+size_t w_eig(int n, int) { return lds_eig(n); }
+
+template <class T>
+int upload(mcp_ctx* c, DevBuf& b, const T* h, size_t count) {
+  HIPCHK(c, b.ensure(count * sizeof(T)));
+  if (count) HIPCHK(c, hipMemcpyAsync(b.p, h, count * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  return MCP_OK;
+}
+
+template <class T>
+int download(mcp_ctx* c, T* h, const DevBuf& b, size_t count) {
+  if (count) HIPCHK(c, hipMemcpyAsync(h, b.p, count * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+  return MCP_OK;
+}
+
+mcp::BlockArgs block_args(const Block& b) {
+  mcp::BlockArgs a;
+  a.n = b.n;
+  a.nham = b.nham;
+  a.mxlam = b.mxlam;
+  a.nconst = b.nconst;
+  a.nrsq = b.nrsq;
+  a.vl = b.vl.as<double>();
+  a.eint = b.eint.as<double>();
+  a.cent = b.cent.as<double>();
+  a.conlen4inv = b.conlen4inv;
+  return a;
+}
+
+mcp::GridArgs grid_args(const Grid& g) {
+  mcp::GridArgs a;
+  a.npts = g.npts;
+  a.r = g.r.as<double>();
+  a.ptab = g.ptab.as<double>();
+  return a;
+}
+
+int lookup(mcp_ctx* c, int32_t bid, int32_t gid, Block** b, Grid** g) {
+  auto ib = c->blocks.find(bid);
+  if (ib == c->blocks.end()) return c->fail(MCP_ERR_ARG, "unknown block id %d", bid);
+  *b = ib->second.get();
+  if (g) {
+    auto ig = c->grids.find(gid);
+    if (ig == c->grids.end()) return c->fail(MCP_ERR_ARG, "unknown grid id %d", gid);
+    *g = ig->second.get();
+    if ((*g)->mxlam != (*b)->mxlam)
+      return c->fail(MCP_ERR_ARG, "grid has %d radial terms, block expects %d", (*g)->mxlam, (*b)->mxlam);
+  }
+  return MCP_OK;
+}
+
+// Device-side propagation of ninst instances whose Y already sits in dy.
+int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const double* d_ered, const double* d_pconst,
+                   double* dy, int* d_nodes, int* d_status) {
+  if (iprop != 5 && iprop != 6) return c->fail(MCP_ERR_ARG, "iprop %d: only 5 (LDPROP) and 6 (MDPROP)", iprop);
+  if ((g.npts - 1) % 2 != 0 || g.npts < 3) return c->fail(MCP_ERR_ARG, "LD grid needs 2*NSTEP+1 points");
+  const int nstep = (g.npts - 1) / 2;
+  const int n = b.n;
+  if (!(g.dr > 0.0)) return c->fail(MCP_ERR_ARG, "grid %d was not created with an LD step DR", 0);
+  const double h = g.dr / 2.0;  // H = DR/2 (mdprop.f:64)
+  const int bk = n > 30 ? 1 : 0;
+  mcp::BlockArgs B = block_args(b);
+  mcp::GridArgs G = grid_args(g);
+  mcp::InstArgs I;
+  I.ered = d_ered;
+  I.pconst = d_pconst;
+  I.y = dy;
+  I.nodes = d_nodes;
+  I.status = d_status;
+  HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+  if (iprop == 6) {
+    const size_t need = lds_md(n, b.nham);
+    if (need > c->lds_max || lds_uc(n, b.nham) > c->lds_max)
+      return c->fail(MCP_ERR_UNSUPPORTED, "MDPROP: N=%d exceeds the LDS-resident limit on this device", n);
+    // energy-independent sector matrices (shared by all instances)
+    HIPCHK(c, c->s_e.ensure((size_t)nstep * n * n * sizeof(double)));
+    HIPCHK(c, c->s_f.ensure((size_t)nstep * sizeof(int)));
+    hipLaunchKernelGGL(mcp::k_mdprop_uc, dim3(nstep), dim3(kNT), lds_uc(n, b.nham), c->stream, B, G, d_pconst, 1,
+                       nstep, h, bk, c->s_e.as<double>(), c->s_f.as<int>());
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    hipLaunchKernelGGL(mcp::k_mdprop_lds, dim3(ninst), dim3(kNT), need, c->stream, B, G, I, nstep, h,
+                       c->s_e.as<double>(), c->s_f.as<int>(), bk);
+    HIPCHK(c, hipGetLastError());
+  } else {
+    const size_t need = lds_ld(n, b.nham);
+    if (need > c->lds_max)
+      return c->fail(MCP_ERR_UNSUPPORTED, "LDPROP: N=%d exceeds the LDS-resident limit on this device", n);
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    hipLaunchKernelGGL(mcp::k_ldprop_lds, dim3(ninst), dim3(kNT), need, c->stream, B, G, I, nstep, h, bk);
+    HIPCHK(c, hipGetLastError());
+  }
+  HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+  return MCP_OK;
+}
+
+int record_timing(mcp_ctx* c) {
+  float a = 0.f, b = 0.f;
+  HIPCHK(c, hipEventSynchronize(c->ev[2]));
+  HIPCHK(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+  HIPCHK(c, hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+  c->t_sector = a;
+  c->t_prop = b;
+  c->t_total = (double)a + (double)b;
+  return MCP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mcp_ctx_create(int device, mcp_ctx** out) {
+  if (!out) return MCP_ERR_ARG;
+  *out = nullptr;
+  auto c = std::make_unique<mcp_ctx>();
+  c->device = device;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) return MCP_ERR_HIP;
+  if (hipSetDevice(device) != hipSuccess) return MCP_ERR_HIP;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return MCP_ERR_HIP;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    fprintf(stderr, "mcprop: built for gfx950, device %d is %s\n", device, prop.gcnArchName);
+    return MCP_ERR_UNSUPPORTED;
+  }
+  c->num_cu = prop.multiProcessorCount;
+  c->lds_max = prop.sharedMemPerBlock;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return MCP_ERR_HIP;
+  for (auto& e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) return MCP_ERR_HIP;
+  set_lds_limits(c.get());
+  *out = c.release();
+  return MCP_OK;
+}
+
+void mcp_ctx_destroy(mcp_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->blocks.clear();
+  c->grids.clear();
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* mcp_last_error(const mcp_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int mcp_ctx_info(const mcp_ctx* c, int32_t* max_n_md, int32_t* max_n_ld, int32_t* max_n_eig, int32_t* lds_bytes,
+                 int32_t* num_cu) {
+  if (!c) return MCP_ERR_ARG;
+  if (max_n_md) *max_n_md = max_n_for(c->lds_max, w_md, 8);
+  if (max_n_ld) *max_n_ld = max_n_for(c->lds_max, w_ld, 8);
+  if (max_n_eig) *max_n_eig = max_n_for(c->lds_max, w_eig, 8);
+  if (lds_bytes) *lds_bytes = (int32_t)c->lds_max;
+  if (num_cu) *num_cu = c->num_cu;
+  return MCP_OK;
+}
+
+int mcp_syminv_batched(mcp_ctx* c, int32_t n, int32_t nmat, const double* a, double* ainv, int32_t* kount) {
+  if (!c || n <= 0 || nmat < 0 || !a || !ainv || !kount) return c ? c->fail(MCP_ERR_ARG, "syminv: bad args") : MCP_ERR_ARG;
+  if (nmat == 0) return MCP_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t lds = lds_syminv(n);
+  if (lds > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "syminv: N=%d exceeds the LDS-resident limit", n);
+  const size_t cnt = (size_t)nmat * n * n;
+  int rc;
+  if ((rc = upload(c, c->s_a, a, cnt))) return rc;
+  HIPCHK(c, c->s_b.ensure(cnt * sizeof(double)));
+  HIPCHK(c, c->s_c.ensure((size_t)nmat * sizeof(int)));
+  hipLaunchKernelGGL(mcp::k_syminv, dim3(nmat), dim3(kNT), lds, c->stream, n, c->s_a.as<double>(),
+                     c->s_b.as<double>(), c->s_c.as<int>(), n > 30 ? 1 : 0);
+  HIPCHK(c, hipGetLastError());
+  if ((rc = download(c, ainv, c->s_b, cnt))) return rc;
+  if ((rc = download(c, kount, c->s_c, (size_t)nmat))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MCP_OK;
+}
+
+int mcp_eigh_batched(mcp_ctx* c, int32_t n, int32_t nmat, const double* a, double* evals, double* evecs) {
+  if (!c || n <= 0 || nmat < 0 || !a || !evals) return c ? c->fail(MCP_ERR_ARG, "eigh: bad args") : MCP_ERR_ARG;
+  if (nmat == 0) return MCP_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t lds = lds_eig(n);
+  if (lds > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "eigh: N=%d exceeds the LDS-resident limit", n);
+  const size_t cnt = (size_t)nmat * n * n;
+  int rc;
+  if ((rc = upload(c, c->s_a, a, cnt))) return rc;
+  HIPCHK(c, c->s_b.ensure(cnt * sizeof(double)));
+  HIPCHK(c, c->s_c.ensure((size_t)nmat * n * sizeof(double)));
+  hipLaunchKernelGGL(mcp::k_eigh, dim3(nmat), dim3(kNT), lds, c->stream, n, c->s_a.as<double>(),
+                     c->s_c.as<double>(), evecs ? c->s_b.as<double>() : nullptr, (int*)nullptr);
+  HIPCHK(c, hipGetLastError());
+  if ((rc = download(c, evals, c->s_c, (size_t)nmat * n))) return rc;
+  if (evecs && (rc = download(c, evecs, c->s_b, cnt))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MCP_OK;
+}
+
+int mcp_block_create(mcp_ctx* c, int32_t n, int32_t mxlam, int32_t nconst, int32_t nrsq, const double* vl,
+                     const double* eint, const double* cent, double conlen, int32_t* block_id) {
+  if (!c || n <= 0 || mxlam < 0 || nconst < 0 || nrsq < 0 || !vl || !eint || !cent || !block_id)
+    return c ? c->fail(MCP_ERR_ARG, "block_create: bad args") : MCP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  auto b = std::make_unique<Block>();
+  b->n = n;
+  b->mxlam = mxlam;
+  b->nconst = nconst;
+  b->nrsq = nrsq;
+  b->nham = mxlam + nconst + nrsq;
+  b->conlen4inv = conlen > 0.0 ? 1.0 / (conlen * conlen * conlen * conlen) : 0.0;
+  const size_t ntri = (size_t)n * (n + 1) / 2;
+  int rc;
+  if ((rc = upload(c, b->vl, vl, ntri * b->nham))) return rc;
+  if ((rc = upload(c, b->eint, eint, (size_t)n))) return rc;
+  if ((rc = upload(c, b->cent, cent, (size_t)n))) return rc;
+  b->h_cent.assign(cent, cent + n);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const int id = c->next_id++;
+  c->blocks[id] = std::move(b);
+  *block_id = id;
+  return MCP_OK;
+}
+
+int mcp_block_destroy(mcp_ctx* c, int32_t id) {
+  if (!c) return MCP_ERR_ARG;
+  (void)hipStreamSynchronize(c->stream);
+  return c->blocks.erase(id) ? MCP_OK : c->fail(MCP_ERR_ARG, "unknown block id %d", id);
+}
+
+int mcp_grid_create(mcp_ctx* c, int32_t mxlam, int32_t npts, const double* r, const double* ptab, double dr,
+                    int32_t* grid_id) {
+  if (!c || mxlam < 0 || npts <= 0 || !r || (mxlam > 0 && !ptab) || !grid_id)
+    return c ? c->fail(MCP_ERR_ARG, "grid_create: bad args") : MCP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  auto g = std::make_unique<Grid>();
+  g->mxlam = mxlam;
+  g->npts = npts;
+  g->dr = dr;
+  int rc;
+  if ((rc = upload(c, g->r, r, (size_t)npts))) return rc;
+  if ((rc = upload(c, g->ptab, ptab, (size_t)npts * mxlam))) return rc;
+  g->h_r.assign(r, r + npts);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const int id = c->next_id++;
+  c->grids[id] = std::move(g);
+  *grid_id = id;
+  return MCP_OK;
+}
+
+int mcp_grid_destroy(mcp_ctx* c, int32_t id) {
+  if (!c) return MCP_ERR_ARG;
+  (void)hipStreamSynchronize(c->stream);
+  return c->grids.erase(id) ? MCP_OK : c->fail(MCP_ERR_ARG, "unknown grid id %d", id);
+}
+
+int mcp_hammat(mcp_ctx* c, int32_t bid, int32_t gid, int32_t nr, const int32_t* idx, const double* pconst,
+               double ered, double* w) {
+  if (!c || nr < 0 || !idx || !w) return c ? c->fail(MCP_ERR_ARG, "hammat: bad args") : MCP_ERR_ARG;
+  Block* b;
+  Grid* g;
+  int rc;
+  if ((rc = lookup(c, bid, gid, &b, &g))) return rc;
+  if (nr == 0) return MCP_OK;
+  if (b->nconst > 0 && !pconst) return c->fail(MCP_ERR_ARG, "hammat: block has NCONST=%d but pconst is NULL", b->nconst);
+  for (int m = 0; m < nr; ++m)
+    if (idx[m] < 0 || idx[m] >= g->npts) return c->fail(MCP_ERR_ARG, "hammat: grid index %d out of range", idx[m]);
+  HIPCHK(c, hipSetDevice(c->device));
+  const int n = b->n;
+  if ((rc = upload(c, c->s_a, idx, (size_t)nr))) return rc;
+  if (b->nconst > 0 && (rc = upload(c, c->s_d, pconst, (size_t)b->nconst))) return rc;
+  HIPCHK(c, c->s_b.ensure((size_t)nr * n * n * sizeof(double)));
+  const size_t ntri = (size_t)n * (n + 1) / 2;
+  const int gx = (int)std::min<size_t>((ntri + 255) / 256, 1024);
+  hipLaunchKernelGGL(mcp::k_hammat, dim3(gx, nr), dim3(256), 0, c->stream, block_args(*b), grid_args(*g),
+                     c->s_a.as<int>(), nr, b->nconst > 0 ? c->s_d.as<double>() : nullptr, ered, c->s_b.as<double>());
+  HIPCHK(c, hipGetLastError());
+  if ((rc = download(c, w, c->s_b, (size_t)nr * n * n))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MCP_OK;
+}
+
+static int upload_inst(mcp_ctx* c, const Block& b, int ninst, const double* ered, const double* pconst) {
+  int rc;
+  if ((rc = upload(c, c->s_g, ered, (size_t)ninst))) return rc;
+  if (b.nconst > 0) {
+    if (!pconst) return c->fail(MCP_ERR_ARG, "block has NCONST=%d but pconst is NULL", b.nconst);
+    if ((rc = upload(c, c->s_h, pconst, (size_t)ninst * b.nconst))) return rc;
+  }
+  return MCP_OK;
+}
+
+int mcp_yinit(mcp_ctx* c, int32_t bid, int32_t gid, int32_t k0, int32_t outward, double bcyo, int32_t ninst,
+              const double* ered, const double* pconst, double* y) {
+  if (!c || ninst < 0 || !ered || !y) return c ? c->fail(MCP_ERR_ARG, "yinit: bad args") : MCP_ERR_ARG;
+  Block* b;
+  Grid* g;
+  int rc;
+  if ((rc = lookup(c, bid, gid, &b, &g))) return rc;
+  if (k0 < 0 || k0 >= g->npts) return c->fail(MCP_ERR_ARG, "yinit: grid index %d out of range", k0);
+  if (ninst == 0) return MCP_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  const int n = b->n;
+  const size_t lds = lds_yinit(n, b->nham);
+  if (lds > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "yinit: N=%d exceeds the LDS-resident limit", n);
+  if ((rc = upload_inst(c, *b, ninst, ered, pconst))) return rc;
+  HIPCHK(c, c->s_i.ensure((size_t)ninst * n * n * sizeof(double)));
+  mcp::InstArgs I;
+  I.ered = c->s_g.as<double>();
+  I.pconst = b->nconst > 0 ? c->s_h.as<double>() : nullptr;
+  I.y = c->s_i.as<double>();
+  I.nodes = nullptr;
+  I.status = nullptr;
+  hipLaunchKernelGGL(mcp::k_yinit, dim3(ninst), dim3(kNT), lds, c->stream, block_args(*b), grid_args(*g), I, k0,
+                     outward, bcyo, 1);
+  HIPCHK(c, hipGetLastError());
+  if ((rc = download(c, y, c->s_i, (size_t)ninst * n * n))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MCP_OK;
+}
+
+int mcp_prop_ld(mcp_ctx* c, int32_t bid, int32_t gid, int32_t iprop, int32_t ninst, const double* ered,
+                const double* pconst, double* y, int32_t* nodes, int32_t* status) {
+  if (!c || ninst < 0 || !ered || !y || !nodes || !status)
+    return c ? c->fail(MCP_ERR_ARG, "prop_ld: bad args") : MCP_ERR_ARG;
+  Block* b;
+  Grid* g;
+  int rc;
+  if ((rc = lookup(c, bid, gid, &b, &g))) return rc;
+  if (ninst == 0) return MCP_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  const int n = b->n;
+  if ((rc = upload_inst(c, *b, ninst, ered, pconst))) return rc;
+  if ((rc = upload(c, c->s_i, y, (size_t)ninst * n * n))) return rc;
+  HIPCHK(c, c->s_j.ensure((size_t)ninst * 2 * sizeof(int)));
+  int* dn = c->s_j.as<int>();
+  if ((rc = prop_ld_device(c, *b, *g, iprop, ninst, c->s_g.as<double>(), b->nconst > 0 ? c->s_h.as<double>() : nullptr,
+                           c->s_i.as<double>(), dn, dn + ninst)))
+    return rc;
+  if ((rc = download(c, y, c->s_i, (size_t)ninst * n * n))) return rc;
+  std::vector<int> tmp(2 * (size_t)ninst);
+  HIPCHK(c, hipMemcpyAsync(tmp.data(), dn, 2 * ninst * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < ninst; ++i) {
+    nodes[i] = tmp[i];
+    status[i] = tmp[ninst + i];
+  }
+  return record_timing(c);
+}
+
+static int match_device(mcp_ctx* c, int n, int ninst, const double* d_y, const double* wvec, const int32_t* nopen,
+                        const int32_t* nb, const double* d_cent, double rval) {
+  int rc;
+  if ((rc = upload(c, c->s_a, wvec, (size_t)ninst * n))) return rc;
+  if ((rc = upload(c, c->s_c, nopen, (size_t)ninst))) return rc;
+  if ((rc = upload(c, c->s_d, nb, (size_t)ninst * n))) return rc;
+  HIPCHK(c, c->s_b.ensure((size_t)ninst * (2 * (size_t)n * n + 4 * (size_t)n) * sizeof(double)));
+  HIPCHK(c, c->s_k.ensure((size_t)ninst * 3 * n * n * sizeof(double) + (size_t)ninst * sizeof(int)));
+  mcp::MatchArgs M;
+  M.n = n;
+  M.y = d_y;
+  M.wvec = c->s_a.as<double>();
+  M.nopen = c->s_c.as<int>();
+  M.nb = c->s_d.as<int>();
+  M.cent = d_cent;
+  M.rval = rval;
+  M.work = c->s_b.as<double>();
+  M.kmat = c->s_k.as<double>();
+  M.sr = M.kmat + (size_t)ninst * n * n;
+  M.si = M.sr + (size_t)ninst * n * n;
+  M.status = (int*)(M.si + (size_t)ninst * n * n);
+  hipLaunchKernelGGL(mcp::k_match, dim3(ninst), dim3(kNT), 0, c->stream, M);
+  HIPCHK(c, hipGetLastError());
+  return MCP_OK;
+}
+
+static int fetch_match(mcp_ctx* c, int n, int ninst, const int32_t* nopen, double* kmat, double* sr, double* si,
+                       int32_t* status) {
+  std::vector<double> all((size_t)ninst * 3 * n * n);
+  std::vector<int> st(ninst);
+  HIPCHK(c, hipMemcpyAsync(all.data(), c->s_k.p, all.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(st.data(), (char*)c->s_k.p + all.size() * sizeof(double), ninst * sizeof(int),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const size_t nn = (size_t)n * n;
+  for (int i = 0; i < ninst; ++i) {
+    const int nop = nopen[i];
+    for (size_t e = 0; e < (size_t)nop * nop; ++e) {
+      if (kmat) kmat[i * nn + e] = all[i * nn + e];
+      sr[i * nn + e] = all[(size_t)ninst * nn + i * nn + e];
+      si[i * nn + e] = all[2 * (size_t)ninst * nn + i * nn + e];
+    }
+    status[i] = st[i];
+  }
+  return MCP_OK;
+}
+
+int mcp_ytok_ktos(mcp_ctx* c, int32_t n, int32_t ninst, const double* y, const double* wvec, const int32_t* nopen,
+                  const int32_t* nb, const double* cent, double rval, double* kmat, double* sr, double* si,
+                  int32_t* status) {
+  if (!c || n <= 0 || ninst < 0 || !y || !wvec || !nopen || !nb || !cent || !sr || !si || !status)
+    return c ? c->fail(MCP_ERR_ARG, "ytok_ktos: bad args") : MCP_ERR_ARG;
+  if (ninst == 0) return MCP_OK;
+  for (int i = 0; i < ninst; ++i)
+    if (nopen[i] < 0 || nopen[i] > n) return c->fail(MCP_ERR_ARG, "ytok_ktos: nopen[%d]=%d out of range", i, nopen[i]);
+  for (size_t e = 0; e < (size_t)ninst * n; ++e)
+    if (nb[e] < 0 || nb[e] >= n) return c->fail(MCP_ERR_ARG, "ytok_ktos: nb out of range");
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc;
+  if ((rc = upload(c, c->s_i, y, (size_t)ninst * n * n))) return rc;
+  if ((rc = upload(c, c->s_e, cent, (size_t)n))) return rc;
+  if ((rc = match_device(c, n, ninst, c->s_i.as<double>(), wvec, nopen, nb, c->s_e.as<double>(), rval))) return rc;
+  return fetch_match(c, n, ninst, nopen, kmat, sr, si, status);
+}
+
+int mcp_smatrix_ld(mcp_ctx* c, int32_t bid, int32_t gid, int32_t iprop, int32_t ninst, const double* ered,
+                   const double* pconst, const double* wvec, const int32_t* nopen, const int32_t* nb, double rval,
+                   double* kmat, double* sr, double* si, int32_t* nodes, int32_t* status) {
+  if (!c || ninst < 0 || !ered || !wvec || !nopen || !nb || !sr || !si || !nodes || !status)
+    return c ? c->fail(MCP_ERR_ARG, "smatrix_ld: bad args") : MCP_ERR_ARG;
+  Block* b;
+  Grid* g;
+  int rc;
+  if ((rc = lookup(c, bid, gid, &b, &g))) return rc;
+  if (ninst == 0) return MCP_OK;
+  const int n = b->n;
+  for (int i = 0; i < ninst; ++i)
+    if (nopen[i] < 0 || nopen[i] > n) return c->fail(MCP_ERR_ARG, "smatrix_ld: nopen[%d]=%d out of range", i, nopen[i]);
+  for (size_t e = 0; e < (size_t)ninst * n; ++e)
+    if (nb[e] < 0 || nb[e] >= n) return c->fail(MCP_ERR_ARG, "smatrix_ld: nb out of range");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t lds = lds_yinit(n, b->nham);
+  if (lds > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "yinit: N=%d exceeds the LDS-resident limit", n);
+  if ((rc = upload_inst(c, *b, ninst, ered, pconst))) return rc;
+  HIPCHK(c, c->s_i.ensure((size_t)ninst * n * n * sizeof(double)));
+  HIPCHK(c, c->s_j.ensure((size_t)ninst * 2 * sizeof(int)));
+  mcp::InstArgs I;
+  I.ered = c->s_g.as<double>();
+  I.pconst = b->nconst > 0 ? c->s_h.as<double>() : nullptr;
+  I.y = c->s_i.as<double>();
+  I.nodes = nullptr;
+  I.status = nullptr;
+  hipLaunchKernelGGL(mcp::k_yinit, dim3(ninst), dim3(kNT), lds, c->stream, block_args(*b), grid_args(*g), I, 0, 1,
+                     0.0, 1);
+  HIPCHK(c, hipGetLastError());
+  int* dn = c->s_j.as<int>();
+  if ((rc = prop_ld_device(c, *b, *g, iprop, ninst, I.ered, I.pconst, I.y, dn, dn + ninst))) return rc;
+  if (!(rval > 0.0)) return c->fail(MCP_ERR_ARG, "smatrix_ld: matching radius %g must be positive", rval);
+  if ((rc = match_device(c, n, ninst, I.y, wvec, nopen, nb, b->cent.as<double>(), rval))) return rc;
+  std::vector<int> tmp(2 * (size_t)ninst);
+  HIPCHK(c, hipMemcpyAsync(tmp.data(), dn, 2 * ninst * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  if ((rc = fetch_match(c, n, ninst, nopen, kmat, sr, si, status))) return rc;
+  for (int i = 0; i < ninst; ++i) {
+    nodes[i] = tmp[i];
+    if (tmp[ninst + i] != 0) status[i] = -tmp[ninst + i];  // propagation failure: -(step)
+  }
+  return record_timing(c);
+}
+
+int mcp_last_timing(const mcp_ctx* c, double* ms_sector, double* ms_prop, double* ms_total) {
+  if (!c) return MCP_ERR_ARG;
+  if (ms_sector) *ms_sector = c->t_sector;
+  if (ms_prop) *ms_prop = c->t_prop;
+  if (ms_total) *ms_total = c->t_total;
+  return MCP_OK;
+}
+
+}  // extern "C"

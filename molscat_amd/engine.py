@@ -1,0 +1,228 @@
+"""Host-side handle on the MI355X engine: one HIP context, device-resident
+blocks (VL, EINT, CENT) and radial grids (R, P table), and the batched entry
+points of the C ABI with numpy arrays.
+
+Matrices cross the boundary column-major; numpy arrays here are
+(batch, N, N) with element [m, i, j] = A_m(i, j), stored so that the
+column-major view of each N x N slab is the matrix itself (the arrays are
+transposed copies of C-order data where needed).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import lib as _lib
+from .lib import McpError, dptr, iptr
+
+
+def _colmajor(batch: np.ndarray) -> np.ndarray:
+    """(m, N, N) logical matrices -> contiguous buffer whose m-th slab is the
+    column-major storage of batch[m]."""
+    return np.ascontiguousarray(np.transpose(batch, (0, 2, 1)), dtype=np.float64)
+
+
+def _from_colmajor(buf: np.ndarray) -> np.ndarray:
+    return np.ascontiguousarray(np.transpose(buf, (0, 2, 1)))
+
+
+@dataclass
+class BlockHandle:
+    id: int
+    n: int
+    mxlam: int
+    nconst: int
+    nrsq: int
+    cent: np.ndarray
+
+
+@dataclass
+class GridHandle:
+    id: int
+    r: np.ndarray
+    mxlam: int
+
+
+class Engine:
+    """One MI355X device context (mcp_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _lib.load()
+        h = ctypes.c_void_p()
+        rc = self.lib.mcp_ctx_create(device, ctypes.byref(h))
+        if rc != 0 or not h.value:
+            raise McpError(f"mcp_ctx_create(device={device}) failed with code {rc} (is a gfx950 GPU visible?)")
+        self.ctx = h
+
+    def close(self):
+        if getattr(self, "ctx", None) is not None and self.ctx.value:
+            self.lib.mcp_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            msg = self.lib.mcp_last_error(self.ctx)
+            raise McpError(f"{what}: code {rc}: {msg.decode() if msg else ''}")
+
+    def info(self) -> dict:
+        v = [ctypes.c_int32() for _ in range(5)]
+        self._check(self.lib.mcp_ctx_info(self.ctx, *[ctypes.byref(x) for x in v]), "mcp_ctx_info")
+        return dict(max_n_md=v[0].value, max_n_ld=v[1].value, max_n_eig=v[2].value, lds_bytes=v[3].value,
+                    num_cu=v[4].value)
+
+    # -- dense kernels ------------------------------------------------------
+    def syminv(self, a: np.ndarray):
+        """Batched SYMINV: a (m, N, N) symmetric (lower triangle read).
+        Returns (inverse (m, N, N), kount (m,))."""
+        a = np.asarray(a, dtype=np.float64)
+        single = a.ndim == 2
+        if single:
+            a = a[None]
+        m, n, _ = a.shape
+        buf = _colmajor(a)
+        out = np.empty_like(buf)
+        k = np.empty(m, dtype=np.int32)
+        self._check(self.lib.mcp_syminv_batched(self.ctx, n, m, dptr(buf), dptr(out), iptr(k)), "mcp_syminv_batched")
+        inv = _from_colmajor(out)
+        return (inv[0], int(k[0])) if single else (inv, k)
+
+    def eigh(self, a: np.ndarray, vectors: bool = True):
+        a = np.asarray(a, dtype=np.float64)
+        single = a.ndim == 2
+        if single:
+            a = a[None]
+        m, n, _ = a.shape
+        buf = _colmajor(a)
+        ev = np.empty((m, n))
+        vec = np.empty_like(buf) if vectors else None
+        self._check(self.lib.mcp_eigh_batched(self.ctx, n, m, dptr(buf), dptr(ev), dptr(vec)), "mcp_eigh_batched")
+        v = _from_colmajor(vec) if vectors else None
+        if single:
+            return ev[0], (v[0] if vectors else None)
+        return ev, v
+
+    # -- device-resident data ----------------------------------------------
+    def block(self, vl: np.ndarray, eint: np.ndarray, cent: np.ndarray, mxlam: int, nconst: int = 0,
+              nrsq: int = 0, conlen: float = 0.0) -> BlockHandle:
+        """Upload VL(NHAM, N(N+1)/2) (numpy shape (NHAM, ntri)), EINT, CENT."""
+        vl = np.asarray(vl, dtype=np.float64)
+        n = len(eint)
+        nham = mxlam + nconst + nrsq
+        if vl.shape != (nham, n * (n + 1) // 2):
+            raise ValueError(f"VL shape {vl.shape}, expected {(nham, n * (n + 1) // 2)}")
+        vlf = np.ascontiguousarray(vl.T)  # [ntri][nham]: lambda fastest
+        bid = ctypes.c_int32()
+        self._check(self.lib.mcp_block_create(self.ctx, n, mxlam, nconst, nrsq, dptr(vlf),
+                                              dptr(np.ascontiguousarray(eint, dtype=np.float64)),
+                                              dptr(np.ascontiguousarray(cent, dtype=np.float64)), float(conlen),
+                                              ctypes.byref(bid)), "mcp_block_create")
+        return BlockHandle(bid.value, n, mxlam, nconst, nrsq, np.array(cent, dtype=np.float64))
+
+    def grid(self, r: np.ndarray, ptab: np.ndarray, dr: float = 0.0) -> GridHandle:
+        """Upload radii r (npts,), the P(lambda, R) table ptab (npts, MXLAM) and,
+        for an LD grid, its step DR."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        ptab = np.ascontiguousarray(ptab, dtype=np.float64)
+        mxlam = ptab.shape[1] if ptab.ndim == 2 else 0
+        gid = ctypes.c_int32()
+        self._check(self.lib.mcp_grid_create(self.ctx, mxlam, len(r), dptr(r), dptr(ptab), float(dr),
+                                             ctypes.byref(gid)),
+                    "mcp_grid_create")
+        return GridHandle(gid.value, r, mxlam)
+
+    def free_block(self, b: BlockHandle):
+        self._check(self.lib.mcp_block_destroy(self.ctx, b.id), "mcp_block_destroy")
+
+    def free_grid(self, g: GridHandle):
+        self._check(self.lib.mcp_grid_destroy(self.ctx, g.id), "mcp_grid_destroy")
+
+    # -- path entry points ----------------------------------------------------
+    def hammat(self, b: BlockHandle, g: GridHandle, idx, ered: float, pconst=None) -> np.ndarray:
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        w = np.empty((len(idx), b.n, b.n))
+        pc = None if pconst is None else np.ascontiguousarray(pconst, dtype=np.float64)
+        self._check(self.lib.mcp_hammat(self.ctx, b.id, g.id, len(idx), iptr(idx), dptr(pc), float(ered), dptr(w)),
+                    "mcp_hammat")
+        return _from_colmajor(w)
+
+    def yinit(self, b: BlockHandle, g: GridHandle, ered, k0: int = 0, outward: bool = True, bcyo: float = 0.0,
+              pconst=None) -> np.ndarray:
+        ered = np.ascontiguousarray(np.atleast_1d(ered), dtype=np.float64)
+        m = len(ered)
+        y = np.empty((m, b.n, b.n))
+        pc = None if pconst is None else np.ascontiguousarray(pconst, dtype=np.float64)
+        self._check(self.lib.mcp_yinit(self.ctx, b.id, g.id, k0, 1 if outward else 0, float(bcyo), m, dptr(ered),
+                                       dptr(pc), dptr(y)), "mcp_yinit")
+        return _from_colmajor(y)
+
+    def prop_ld(self, b: BlockHandle, g: GridHandle, iprop: int, ered, y0: np.ndarray, pconst=None):
+        """Propagate ninst = len(ered) instances from y0 (m, N, N); returns
+        (Y(RSTOP), nodes, status)."""
+        ered = np.ascontiguousarray(np.atleast_1d(ered), dtype=np.float64)
+        m = len(ered)
+        y = _colmajor(np.asarray(y0, dtype=np.float64).reshape(m, b.n, b.n))
+        nodes = np.empty(m, dtype=np.int32)
+        st = np.empty(m, dtype=np.int32)
+        pc = None if pconst is None else np.ascontiguousarray(pconst, dtype=np.float64)
+        self._check(self.lib.mcp_prop_ld(self.ctx, b.id, g.id, iprop, m, dptr(ered), dptr(pc), dptr(y), iptr(nodes),
+                                         iptr(st)), "mcp_prop_ld")
+        return _from_colmajor(y), nodes, st
+
+    def ytok_ktos(self, y: np.ndarray, wvec: np.ndarray, nopen, nb: np.ndarray, cent: np.ndarray, rval: float):
+        y = np.asarray(y, dtype=np.float64)
+        m, n, _ = y.shape
+        nopen = np.ascontiguousarray(np.atleast_1d(nopen), dtype=np.int32)
+        nb = np.ascontiguousarray(np.broadcast_to(nb, (m, n)), dtype=np.int32)
+        wv = np.ascontiguousarray(np.broadcast_to(wvec, (m, n)), dtype=np.float64)
+        k = np.zeros((m, n * n))
+        sr = np.zeros((m, n * n))
+        si = np.zeros((m, n * n))
+        st = np.empty(m, dtype=np.int32)
+        self._check(self.lib.mcp_ytok_ktos(self.ctx, n, m, dptr(_colmajor(y)), dptr(wv), iptr(nopen), iptr(nb),
+                                           dptr(np.ascontiguousarray(cent, dtype=np.float64)), float(rval), dptr(k),
+                                           dptr(sr), dptr(si), iptr(st)), "mcp_ytok_ktos")
+        return _unpack_square(k, nopen), _unpack_square(sr, nopen), _unpack_square(si, nopen), st
+
+    def smatrix_ld(self, b: BlockHandle, g: GridHandle, iprop: int, ered, wvec, nopen, nb, rval: float,
+                   pconst=None):
+        """YINIT + propagation + YTOK/KTOS on the device; rval = nominal RSTOP."""
+        ered = np.ascontiguousarray(np.atleast_1d(ered), dtype=np.float64)
+        m = len(ered)
+        n = b.n
+        nopen = np.ascontiguousarray(np.broadcast_to(nopen, (m,)), dtype=np.int32)
+        nb = np.ascontiguousarray(np.broadcast_to(nb, (m, n)), dtype=np.int32)
+        wv = np.ascontiguousarray(np.broadcast_to(wvec, (m, n)), dtype=np.float64)
+        k = np.zeros((m, n * n))
+        sr = np.zeros((m, n * n))
+        si = np.zeros((m, n * n))
+        nodes = np.empty(m, dtype=np.int32)
+        st = np.empty(m, dtype=np.int32)
+        pc = None if pconst is None else np.ascontiguousarray(pconst, dtype=np.float64)
+        self._check(self.lib.mcp_smatrix_ld(self.ctx, b.id, g.id, iprop, m, dptr(ered), dptr(pc), dptr(wv),
+                                            iptr(nopen), iptr(nb), float(rval), dptr(k), dptr(sr), dptr(si),
+                                            iptr(nodes),
+                                            iptr(st)), "mcp_smatrix_ld")
+        return dict(K=_unpack_square(k, nopen), SR=_unpack_square(sr, nopen), SI=_unpack_square(si, nopen),
+                    NODES=nodes, STATUS=st)
+
+    def last_timing(self) -> dict:
+        v = [ctypes.c_double() for _ in range(3)]
+        self._check(self.lib.mcp_last_timing(self.ctx, *[ctypes.byref(x) for x in v]), "mcp_last_timing")
+        return dict(ms_sector=v[0].value, ms_prop=v[1].value, ms_total=v[2].value)
+
+
+def _unpack_square(flat: np.ndarray, nopen: np.ndarray):
+    out = []
+    for m, nop in enumerate(nopen):
+        a = flat[m, : nop * nop].reshape(nop, nop).T  # column-major nop x nop
+        out.append(np.ascontiguousarray(a))
+    return out

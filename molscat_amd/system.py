@@ -1,0 +1,279 @@
+"""Host-side problem set-up for the propagation path: units, the atom +
+rigid-rotor (ITYPE=1) channel basis and Percival-Seaton coupling, POTENL
+power/exponential radial terms, LD radial grids, and the O(N) channel
+bookkeeping done before and after propagation (GTWVEC, CHNSRT, cross
+sections, scattering length).  None of this is on the device: it runs once
+per (JTOT, IB) block or once per S-matrix, as in the reference driver.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from fractions import Fraction
+from functools import lru_cache
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+# Physical constants, CODATA 2022 as in source_code/physical_constants_module.f:7-88.
+_PI = 3.141592653589793
+_HBAR = 6.62607015e-34 / 2.0 / _PI
+_C_CM = 2.99792458e8 * 1.0e2
+_AMU = 1.66053906892e-27
+BFCT = _HBAR / (4.0 * _PI * _C_CM) / (1.0e-10) ** 2 / _AMU  # cm-1 amu A^2 (hbar^2/2)
+MHZ_IN_INV_CM = 3.335640951e-11 * 1.0e6
+
+
+def cm2ru(ured: float, runit: float = 1.0) -> float:
+    """CM2RU = URED*MUNIT*RUNIT^2/BFCT (source_code/mol.driver.f:734)."""
+    return ured * 1.0 * runit * runit / BFCT
+
+
+# ---------------------------------------------------------------------------
+# angular momentum algebra (exact rational arithmetic)
+# ---------------------------------------------------------------------------
+@lru_cache(maxsize=None)
+def _fact(n: int) -> int:
+    return math.factorial(n)
+
+
+@lru_cache(maxsize=None)
+def threej000(a: int, b: int, c: int) -> float:
+    """(a b c; 0 0 0) by its closed form."""
+    j = a + b + c
+    if j % 2 or c < abs(a - b) or c > a + b:
+        return 0.0
+    g = j // 2
+    sq = Fraction(_fact(j - 2 * a) * _fact(j - 2 * b) * _fact(j - 2 * c), _fact(j + 1))
+    rat = Fraction(_fact(g), _fact(g - a) * _fact(g - b) * _fact(g - c))
+    v = math.sqrt(float(sq * rat * rat))
+    return -v if g % 2 else v
+
+
+def _tri(a, b, c) -> bool:
+    return abs(a - b) <= c <= a + b
+
+
+@lru_cache(maxsize=None)
+def sixj(a: int, b: int, c: int, d: int, e: int, f: int) -> float:
+    """{a b c; d e f} by Racah's formula in exact rational arithmetic."""
+    if not (_tri(a, b, c) and _tri(a, e, f) and _tri(d, b, f) and _tri(d, e, c)):
+        return 0.0
+
+    def delta2(x, y, z):
+        return Fraction(_fact(x + y - z) * _fact(x - y + z) * _fact(-x + y + z), _fact(x + y + z + 1))
+
+    d2 = delta2(a, b, c) * delta2(a, e, f) * delta2(d, b, f) * delta2(d, e, c)
+    tmin = max(a + b + c, a + e + f, d + b + f, d + e + c)
+    tmax = min(a + b + d + e, a + c + d + f, b + c + e + f)
+    s = Fraction(0)
+    for t in range(tmin, tmax + 1):
+        den = (_fact(t - a - b - c) * _fact(t - a - e - f) * _fact(t - d - b - f) * _fact(t - d - e - c)
+               * _fact(a + b + d + e - t) * _fact(a + c + d + f - t) * _fact(b + c + e + f - t))
+        term = Fraction(_fact(t + 1), den)
+        s += -term if t % 2 else term
+    if s == 0:
+        return 0.0
+    v = math.sqrt(float(s * s * d2))
+    return v if s > 0 else -v
+
+
+# ---------------------------------------------------------------------------
+# ITYPE=1 atom + linear rigid rotor
+# ---------------------------------------------------------------------------
+def rotor_channels(jtot: int, ib: int, jlevels: Sequence[int]) -> List[Tuple[int, int]]:
+    """(state index, L) list of block IB, in BASE's order (source_code/base.f:276-352)."""
+    jmax = max(max(jlevels), 0)
+    jmin = min(min(jlevels), 0)
+    lmax = jtot + jmax
+    lmin = jtot - jmax
+    if lmin < 0:
+        lmin = max(jmin - jtot, 0)
+    ipty = ib % 2
+    out = []
+    for li in range(lmin, lmax + 1):
+        for i, ji in enumerate(jlevels):
+            if (li + ji + jtot) % 2 != ipty:
+                continue
+            if ji < abs(jtot - li) or ji > jtot + li:
+                continue
+            out.append((i, li))
+    return out
+
+
+def rotor_coupling(jtot: int, chans, jlevels, lambdas: Sequence[int]) -> np.ndarray:
+    """VL(NHAM, N(N+1)/2), Percival-Seaton coefficients (source_code/couple.f:51-143)."""
+    n = len(chans)
+    vl = np.zeros((len(lambdas), n * (n + 1) // 2))
+    idx = 0
+    for icol in range(n):
+        jc, lc = jlevels[chans[icol][0]], chans[icol][1]
+        for irow in range(icol + 1):
+            jr, lr = jlevels[chans[irow][0]], chans[irow][1]
+            for il, lam in enumerate(lambdas):
+                if abs(jc - jr) > lam or abs(lc - lr) > lam:
+                    continue
+                t = threej000(jc, lam, jr)
+                if t == 0.0:
+                    continue
+                u = threej000(lc, lam, lr)
+                if u == 0.0:
+                    continue
+                s = sixj(jr, lr, jtot, lc, jc, lam)
+                if s == 0.0:
+                    continue
+                v = math.sqrt((2 * jc + 1) * (2 * jr + 1) * (2 * lc + 1) * (2 * lr + 1)) * t * u * s
+                vl[il, idx] = -v if (jc + jr + jtot) % 2 else v
+            idx += 1
+    return vl
+
+
+@dataclass
+class PowerPotential:
+    """&POTL case-1 radial terms (source_code/potenl.f:160-200): per lambda a
+    list of (A, n) for A*R**n or (A, 0, E) for A*exp(E*R); R in RM units,
+    result in EPSIL units."""
+
+    terms: List[List[tuple]]
+    rm: float = 1.0
+    epsil: float = 1.0
+
+    def __call__(self, r: float) -> np.ndarray:
+        out = np.zeros(len(self.terms))
+        for i, tl in enumerate(self.terms):
+            s = 0.0
+            for t in tl:
+                if t[1] != 0:
+                    rr = 1.0e-8 if (t[1] < 0 and abs(r) < 1.0e-8) else r
+                    s += t[0] * rr ** t[1]
+                else:
+                    s += t[0] * math.exp(t[2] * r)
+            out[i] = s
+        return out
+
+
+def lj_rotor_potential(cvals: Sequence[float], rm: float, epsil: float) -> PowerPotential:
+    return PowerPotential([[(c, -12), (-2.0 * c, -6)] for c in cvals], rm=rm, epsil=epsil)
+
+
+@dataclass
+class RotorBlock:
+    """One (JTOT, IB) block of an atom + rigid rotor system in reduced units."""
+
+    jtot: int
+    ib: int
+    chans: list
+    vl: np.ndarray
+    eint: np.ndarray
+    cent: np.ndarray
+    lvals: np.ndarray
+    levidx: np.ndarray
+    cm2ru: float
+    ep2ru: float
+    rscale: float
+    potl: PowerPotential
+    runit: float
+    mxlam: int = field(init=False)
+
+    def __post_init__(self):
+        self.mxlam = self.vl.shape[0]
+
+    @property
+    def n(self) -> int:
+        return len(self.eint)
+
+    def ptab(self, r: np.ndarray) -> np.ndarray:
+        """P(lambda, R) * EP2RU with HAMMAT's clamp (hammat.f:38-73)."""
+        out = np.empty((len(r), self.mxlam))
+        for k, rr in enumerate(r):
+            p = self.potl(rr * self.rscale)
+            p = np.where(np.abs(p) > 1.0e60, np.sign(p) * 1.0e60, p)
+            out[k] = self.ep2ru * p
+        return out
+
+
+def rotor_block(jtot, ib, jlevels, be, lambdas, potl: PowerPotential, ured, runit=None) -> RotorBlock:
+    """BASE + COUPLE for ITYPE=1 (units: source_code/mol.driver.f:734-735,
+    source_code/chckru.f: RUNIT defaults to RM)."""
+    if runit is None:
+        runit = potl.rm
+    c2r = cm2ru(ured, runit)
+    chans = rotor_channels(jtot, ib, jlevels)
+    elev = [be * float(j * (j + 1)) for j in jlevels]
+    eint = np.array([c2r * elev[c[0]] for c in chans])
+    lv = np.array([c[1] for c in chans], dtype=np.int64)
+    cent = np.array([float(l * (l + 1)) for l in lv])
+    vl = rotor_coupling(jtot, chans, jlevels, lambdas)
+    return RotorBlock(jtot=jtot, ib=ib, chans=chans, vl=vl, eint=eint, cent=cent, lvals=lv,
+                      levidx=np.array([c[0] for c in chans], dtype=np.int64), cm2ru=c2r,
+                      ep2ru=potl.epsil * c2r, rscale=runit / potl.rm, potl=potl, runit=runit)
+
+
+# ---------------------------------------------------------------------------
+# grids and channel bookkeeping
+# ---------------------------------------------------------------------------
+def ld_radii(rstart: float, dr: float, nstep: int) -> np.ndarray:
+    """Radii visited by MDPROP/LDPROP: RSTART, then R = R + H twice per step
+    (source_code/mdprop.f:72,113,203), accumulated in the same order."""
+    h = dr / 2.0
+    r = np.empty(2 * nstep + 1)
+    x = rstart
+    r[0] = x
+    for k in range(1, 2 * nstep + 1):
+        x = x + h
+        r[k] = x
+    return r
+
+
+def drset_constant(rstart: float, rstop: float, drfit: float) -> Tuple[int, float]:
+    """Constant-step branch of DRSET (source_code/drset.f:62-64)."""
+    prange = abs(rstop - rstart)
+    nstep = max(int(math.floor(prange / drfit + 0.5)), 1)
+    return nstep, prange / nstep
+
+
+def gtwvec(ered: float, eint: np.ndarray) -> Tuple[np.ndarray, int]:
+    """GTWVEC: source_code/gtwvec.f:18-30."""
+    dif = ered - eint
+    w = np.copysign(np.sqrt(np.abs(dif)), dif)
+    return w, int(np.sum(dif > 0.0))
+
+
+def chnsrt(eint: np.ndarray) -> np.ndarray:
+    """CHNSRT: source_code/chnsrt.f (exchange sort; 0-based)."""
+    nb = list(range(len(eint)))
+    for i in range(len(eint) - 1):
+        for j in range(i + 1, len(eint)):
+            if eint[nb[i]] <= eint[nb[j]] or abs(eint[nb[i]] - eint[nb[j]]) < 1.0e-11:
+                continue
+            nb[i], nb[j] = nb[j], nb[i]
+    return np.array(nb, dtype=np.int32)
+
+
+def partial_sigma(sr, si, nb, nopen, levidx, jlevels, wvec, jtot, runit, nsig, wght=1.0) -> np.ndarray:
+    """SPROC partial cross sections (source_code/sproc.f:380-430), ITYPE=1
+    degeneracy 2j_i+1 (source_code/base.f:1126-1129).  SIGMA[F, I]."""
+    xj = float(2 * jtot + 1) * math.pi * (wght if wght > 0.0 else 1.0)
+    sig = np.zeros((nsig, nsig))
+    for icol in range(nopen):
+        lcol = levidx[nb[icol]]
+        for irow in range(nopen):
+            lrow = levidx[nb[irow]]
+            if lrow >= nsig or lcol >= nsig:
+                continue
+            s_r, s_i = sr[irow, icol], si[irow, icol]
+            smag = (1.0 - s_r) ** 2 + s_i ** 2 if irow == icol else s_r ** 2 + s_i ** 2
+            dd = wvec[nb[irow]] ** 2 / runit ** 2
+            sig[lcol, lrow] += smag * xj / (float(2 * jlevels[lrow] + 1) * dd)
+    return sig
+
+
+def scattering_length(sr_ii: float, si_ii: float, k: float, ll: int, lrpow: int = 6) -> complex:
+    """CALCA: source_code/calca.f:42-59."""
+    smag = math.hypot(sr_ii, si_ii)
+    if smag > 1.0:
+        sr_ii, si_ii = sr_ii / smag, si_ii / smag
+    if sr_ii <= -1.0:
+        return complex(1.0e30, 0.0)
+    m = min(lrpow - 2, 2 * ll + 1)
+    return complex(1.0 - sr_ii, -si_ii) / (complex(1.0 + sr_ii, si_ii) * complex(0.0, k ** m))
