@@ -1,0 +1,141 @@
+"""Benchmark of the propagation path on one MI355X.
+
+Workload: atom + rigid rotor, even j = 0..40, JTOT = 60, parity block 2
+(N = 441 channels), LJ 12-6 Legendre terms lambda = 0..6, E = 300 cm-1;
+diabatic modified log-derivative propagation (MDPROP) from 2 to 20 A at
+DR = 0.005 A (3600 sectors), YINIT at 2 A, Y->K->S matching at 20 A.  One
+"step" is one complete S-matrix solve with the block's VL and the P(R) table
+already resident in HBM.
+
+Prints one JSON line (see the README of the driver contract):
+value = S-matrices/s, plus sector-steps/s, the roofline of the dominant
+kernel (the MDPROP recurrence kernel, FP64) measured with HIP events on the
+library's stream, and a CPU baseline (the oracle's MDPROP on a bounded
+sample of sectors on this host).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "S-matrices/s and sector-steps/s at N=441, mixed vs FP64 at max|ΔS|≤1e-8"
+PEAK_FP64 = 78.6e12  # MI355X dense FP64 (vector and matrix), FLOP/s
+
+
+def _cpu_baseline(blk, seg, ered, nsamp: int) -> dict:
+    """Oracle MDPROP (C SYMINV + numpy) on the first nsamp sectors, one thread."""
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    from oracle import molscat_oracle as O
+
+    pot = O.PowerPotential(blk.potl.terms, rm=blk.potl.rm, epsil=blk.potl.epsil)
+    ob = O.Block(n=blk.n, vl=blk.vl, eint=blk.eint, cent=blk.cent, lvals=blk.lvals, levidx=blk.levidx, potl=pot,
+                 rscale=blk.rscale, ep2ru=blk.ep2ru, cm2ru=blk.cm2ru)
+    y0 = np.diag(np.sqrt(np.abs(np.diag(ob.hammat(seg.rstart, ered)[0]))))
+    t0 = time.perf_counter()
+    O.mdprop(y0, lambda r: ob.hammat(r, ered), seg.rstart, nsamp, seg.dr)
+    dt = time.perf_counter() - t0
+    return {"value": nsamp / dt, "unit": "sector-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle MDPROP, first {nsamp} of {seg.nstep} sectors at N={blk.n}, single thread "
+                      f"({dt:.1f} s)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-sectors", type=int, default=12, help="sectors in the CPU baseline sample (0: skip)")
+    ap.add_argument("--team", type=int, default=0, help="workgroups per instance on the large-N path (0: auto)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              f"(independent (J, parity, energy) instances would shard across GPUs with no exchange "
+              f"but the final cross-section sum)", file=sys.stderr)
+        return 2
+
+    from molscat_amd import decks
+    from molscat_amd import system as S
+    from molscat_amd.engine import Engine
+
+    deck = decks.rotor441()
+    blk = deck.block()
+    seg = deck.ld
+    n, nstep = blk.n, seg.nstep
+    r = seg.radii()
+    ered = deck.energies[0] * blk.cm2ru
+    wv, nopen = S.gtwvec(ered, blk.eint)
+    nb = S.chnsrt(blk.eint)
+
+    eng = Engine(0)
+    eng.set_team_size(args.team)
+    b = eng.block(blk.vl, blk.eint, blk.cent, mxlam=blk.mxlam)
+    g = eng.grid(r, blk.ptab(r), seg.dr)
+
+    def solve():
+        out = eng.smatrix_ld(b, g, 6, [ered], wv, [nopen], nb, seg.rstop)
+        if out["STATUS"][0] != 0:
+            raise RuntimeError(f"solve failed: status {out['STATUS'][0]}")
+        return out
+
+    for _ in range(args.warmup):
+        solve()
+    t_prop, t_sec = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = solve()
+        tm = eng.last_timing()
+        t_prop.append(tm["ms_prop"])
+        t_sec.append(tm["ms_sector"])
+    wall = time.perf_counter() - t0
+    s = out["SR"][0] + 1j * out["SI"][0]
+    unit_err = float(np.abs(s @ s.conj().T - np.eye(len(s))).max())
+
+    ms_step = 1e3 * wall / args.steps
+    smat_per_s = args.steps / wall
+    nham = blk.mxlam
+    flops_prop = nstep * (2.0 * n ** 3 + nham * n * (n + 1) + 12.0 * n * n)
+    ms_prop = float(np.mean(t_prop))
+    achieved = flops_prop / (ms_prop * 1e-3)
+    line = {
+        "metric": METRIC,
+        "value": smat_per_s,
+        "unit": "S-matrices/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"rotor N={n} JTOT=60 E=300 cm-1, LDMD 2-20 A DR={seg.dr:.4f} A ({nstep} sectors), "
+                        f"YINIT + YTOK/KTOS matching at 20 A, one instance per solve",
+            "n_channels": n, "sectors": nstep, "precision": "fp64", "batch_instances": 1,
+        },
+        "sector_steps_per_s": nstep * smat_per_s,
+        "kernel_ms": {"mdprop_sector_matrices": float(np.mean(t_sec)), "mdprop_recurrence": ms_prop},
+        "unitarity_err": unit_err,
+        "roofline": {"bound": "mfma", "kernel": "k_mdprop_large", "achieved": achieved / 1e12,
+                     "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
+                     "traffic": None,
+                     "flops_per_launch": flops_prop},
+    }
+    if args.cpu_sectors > 0:
+        line["cpu_baseline"] = _cpu_baseline(blk, seg, ered, args.cpu_sectors)
+    eng.close()
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

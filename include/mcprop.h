@@ -110,6 +110,11 @@ int mcp_smatrix_ld(mcp_ctx *ctx, int32_t block_id, int32_t grid_id, int32_t ipro
                    const int32_t *nb, double rval, double *kmat, double *sr, double *si, int32_t *nodes,
                    int32_t *status);
 
+/* Workgroups that cooperate on one instance on the large-N path (N beyond the
+ * LDS-resident kernels): g >= 1 fixes it, 0 restores the automatic choice
+ * (up to 32, limited by CUs / instances).  No reference counterpart. */
+int mcp_set_team_size(mcp_ctx *ctx, int32_t g);
+
 /* Wall time in milliseconds of the last mcp_prop_ld / mcp_smatrix_ld call's
  * device work, measured with HIP events on the library's stream, split into
  * the energy-independent sector matrices and the propagation loop. */

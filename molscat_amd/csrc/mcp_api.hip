@@ -12,6 +12,7 @@
 
 #include "../../include/mcprop.h"
 #include "mcp_kernels.hpp"
+#include "mcp_large.hpp"
 
 namespace {
 
@@ -110,6 +111,18 @@ size_t lds_yinit(int n, int nham) {
       .b;
 }
 
+size_t lds_large(int n, int nham, int nvec) {
+  LdsSize z;
+  z.add<double>((size_t)mcp::kPB * n).add<double>((size_t)mcp::kPB * n).add<double>(mcp::kPB * mcp::kPB);
+  z.add<double>(n).add<double>(n).add<double>(2 * mcp::kPB).add<double>(mcp::kPB);
+  z.add<int>(n).add<int>(n).add<int>(mcp::kPB);
+  z.add<int>(mcp::kPB).add<int>(mcp::kPB).add<int>(8);
+  z.add<double>(kNT / 64).add<int>(kNT / 64);
+  if (nham > 0) z.add<double>(nham);
+  for (int v = 0; v < nvec; ++v) z.add<double>(n);
+  return z.b;
+}
+
 }  // namespace
 
 struct mcp_ctx {
@@ -123,6 +136,10 @@ struct mcp_ctx {
   std::map<int, std::unique_ptr<Grid>> grids;
   int next_id = 1;
   DevBuf s_a, s_b, s_c, s_d, s_e, s_f, s_g, s_h, s_i, s_j, s_k;
+  DevBuf s_y;  // YINIT square-root workspace (large N)
+  DevBuf s_bar;  // per-instance team barrier counters + abort flag
+  int team = 0;  // workgroups per instance on the large-N path (0: automatic)
+  DevBuf s_u;  // LDPROP sector-matrix workspace (large N)
   double t_sector = 0.0, t_prop = 0.0, t_total = 0.0;
 
   int fail(int code, const char* fmt, ...) {
@@ -148,17 +165,31 @@ struct mcp_ctx {
 
 namespace {
 
+// Raise every LDS kernel's dynamic-LDS limit to what the device allows after
+// its static LDS; lds_max becomes the smallest such limit.
 int set_lds_limits(mcp_ctx* c) {
-  const void* fns[] = {(const void*)mcp::k_syminv,     (const void*)mcp::k_mdprop_uc, (const void*)mcp::k_mdprop_lds,
-                       (const void*)mcp::k_ldprop_lds, (const void*)mcp::k_eigh,      (const void*)mcp::k_yinit};
+  const void* fns[] = {(const void*)mcp::k_syminv,         (const void*)mcp::k_mdprop_uc,
+                       (const void*)mcp::k_mdprop_lds,     (const void*)mcp::k_ldprop_lds,
+                       (const void*)mcp::k_eigh,           (const void*)mcp::k_yinit,
+                       (const void*)mcp::k_syminv_large,   (const void*)mcp::k_mdprop_uc_large,
+                       (const void*)mcp::k_mdprop_large,   (const void*)mcp::k_ldprop_large,
+                       (const void*)mcp::k_yinit_large};
+  const size_t total = c->lds_max;
+  size_t lim = total;
   for (const void* f : fns) {
-    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_max);
-    if (e != hipSuccess) {
+    hipFuncAttributes at;
+    size_t stat = 0;
+    if (hipFuncGetAttributes(&at, f) == hipSuccess) stat = at.sharedSizeBytes;
+    (void)hipGetLastError();
+    const size_t dyn = total > stat + 64 ? total - stat - 64 : 0;
+    if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) != hipSuccess) {
       (void)hipGetLastError();
-      c->lds_max = 65536;
-      return MCP_OK;
+      lim = std::min<size_t>(lim, 65536 - stat - 64);
+    } else {
+      lim = std::min(lim, dyn);
     }
   }
+  c->lds_max = lim;
   return MCP_OK;
 }
 
@@ -222,6 +253,22 @@ int lookup(mcp_ctx* c, int32_t bid, int32_t gid, Block** b, Grid** g) {
   return MCP_OK;
 }
 
+// Workgroups per instance for the large-N propagators: all instances' teams
+// must be co-resident (one workgroup per CU at this LDS size).
+int team_size(mcp_ctx* c, int ninst) {
+  int cap = std::max(1, c->num_cu / std::max(1, ninst));
+  int g = c->team > 0 ? c->team : 32;
+  return std::max(1, std::min(g, cap));
+}
+
+int prep_team(mcp_ctx* c, int ninst) {
+  HIPCHK(c, c->s_bar.ensure((size_t)(ninst + 1) * sizeof(unsigned)));
+  hipLaunchKernelGGL(mcp::k_zero_u32, dim3((ninst + 1 + 255) / 256), dim3(256), 0, c->stream, c->s_bar.as<unsigned>(),
+                     ninst + 1);
+  HIPCHK(c, hipGetLastError());
+  return MCP_OK;
+}
+
 // Device-side propagation of ninst instances whose Y already sits in dy.
 int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const double* d_ered, const double* d_pconst,
                    double* dy, int* d_nodes, int* d_status) {
@@ -242,26 +289,56 @@ int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const do
   I.status = d_status;
   HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
   if (iprop == 6) {
-    const size_t need = lds_md(n, b.nham);
-    if (need > c->lds_max || lds_uc(n, b.nham) > c->lds_max)
-      return c->fail(MCP_ERR_UNSUPPORTED, "MDPROP: N=%d exceeds the LDS-resident limit on this device", n);
     // energy-independent sector matrices (shared by all instances)
     HIPCHK(c, c->s_e.ensure((size_t)nstep * n * n * sizeof(double)));
     HIPCHK(c, c->s_f.ensure((size_t)nstep * sizeof(int)));
-    hipLaunchKernelGGL(mcp::k_mdprop_uc, dim3(nstep), dim3(kNT), lds_uc(n, b.nham), c->stream, B, G, d_pconst, 1,
-                       nstep, h, bk, c->s_e.as<double>(), c->s_f.as<int>());
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-    hipLaunchKernelGGL(mcp::k_mdprop_lds, dim3(ninst), dim3(kNT), need, c->stream, B, G, I, nstep, h,
-                       c->s_e.as<double>(), c->s_f.as<int>(), bk);
-    HIPCHK(c, hipGetLastError());
+    const size_t need = lds_md(n, b.nham);
+    if (need <= c->lds_max && lds_uc(n, b.nham) <= c->lds_max) {
+      hipLaunchKernelGGL(mcp::k_mdprop_uc, dim3(nstep), dim3(kNT), lds_uc(n, b.nham), c->stream, B, G, d_pconst, 1,
+                         nstep, h, bk, c->s_e.as<double>(), c->s_f.as<int>());
+      HIPCHK(c, hipGetLastError());
+      HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+      hipLaunchKernelGGL(mcp::k_mdprop_lds, dim3(ninst), dim3(kNT), need, c->stream, B, G, I, nstep, h,
+                         c->s_e.as<double>(), c->s_f.as<int>(), bk);
+      HIPCHK(c, hipGetLastError());
+    } else {
+      const size_t lu = lds_large(n, b.nham, 0), lm = lds_large(n, b.nham, 3);
+      if (lm > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "MDPROP: N=%d exceeds the large-N panel limit", n);
+      hipLaunchKernelGGL(mcp::k_mdprop_uc_large, dim3(nstep), dim3(kNT), lu, c->stream, B, G, d_pconst, h, bk,
+                         c->s_e.as<double>(), c->s_f.as<int>());
+      HIPCHK(c, hipGetLastError());
+      HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+      const int g = team_size(c, ninst);
+      int rc;
+      if ((rc = prep_team(c, ninst))) return rc;
+      const double* ucp = c->s_e.as<double>();
+      const int* uck = c->s_f.as<int>();
+      unsigned* ctrs = c->s_bar.as<unsigned>();
+      int* abort_flag = (int*)(ctrs + ninst);
+      void* args[] = {&B, &G, &I, (void*)&nstep, (void*)&h, &ucp, &uck, (void*)&bk, (void*)&g, &ctrs, &abort_flag};
+      HIPCHK(c, hipLaunchCooperativeKernel((const void*)mcp::k_mdprop_large, dim3(ninst * g), dim3(kNT), args, lm,
+                                           c->stream));
+    }
   } else {
     const size_t need = lds_ld(n, b.nham);
-    if (need > c->lds_max)
-      return c->fail(MCP_ERR_UNSUPPORTED, "LDPROP: N=%d exceeds the LDS-resident limit on this device", n);
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-    hipLaunchKernelGGL(mcp::k_ldprop_lds, dim3(ninst), dim3(kNT), need, c->stream, B, G, I, nstep, h, bk);
-    HIPCHK(c, hipGetLastError());
+    if (need <= c->lds_max) {
+      hipLaunchKernelGGL(mcp::k_ldprop_lds, dim3(ninst), dim3(kNT), need, c->stream, B, G, I, nstep, h, bk);
+      HIPCHK(c, hipGetLastError());
+    } else {
+      const size_t ll = lds_large(n, b.nham, 0);
+      if (ll > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "LDPROP: N=%d exceeds the large-N panel limit", n);
+      HIPCHK(c, c->s_u.ensure((size_t)ninst * n * n * sizeof(double)));
+      const int g = team_size(c, ninst);
+      int rc;
+      if ((rc = prep_team(c, ninst))) return rc;
+      double* uw = c->s_u.as<double>();
+      unsigned* ctrs = c->s_bar.as<unsigned>();
+      int* abort_flag = (int*)(ctrs + ninst);
+      void* args[] = {&B, &G, &I, (void*)&nstep, (void*)&h, &uw, (void*)&bk, (void*)&g, &ctrs, &abort_flag};
+      HIPCHK(c, hipLaunchCooperativeKernel((const void*)mcp::k_ldprop_large, dim3(ninst * g), dim3(kNT), args, ll,
+                                           c->stream));
+    }
   }
   HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
   return MCP_OK;
@@ -336,9 +413,21 @@ int mcp_syminv_batched(mcp_ctx* c, int32_t n, int32_t nmat, const double* a, dou
   if (nmat == 0) return MCP_OK;
   HIPCHK(c, hipSetDevice(c->device));
   const size_t lds = lds_syminv(n);
-  if (lds > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "syminv: N=%d exceeds the LDS-resident limit", n);
   const size_t cnt = (size_t)nmat * n * n;
   int rc;
+  if (lds > c->lds_max) {
+    const size_t ll = lds_large(n, 0, 0);
+    if (ll > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "syminv: N=%d exceeds the large-N panel limit", n);
+    if ((rc = upload(c, c->s_b, a, cnt))) return rc;
+    HIPCHK(c, c->s_c.ensure((size_t)nmat * sizeof(int)));
+    hipLaunchKernelGGL(mcp::k_syminv_large, dim3(nmat), dim3(kNT), ll, c->stream, n, c->s_b.as<double>(),
+                       c->s_c.as<int>(), n > 30 ? 1 : 0);
+    HIPCHK(c, hipGetLastError());
+    if ((rc = download(c, ainv, c->s_b, cnt))) return rc;
+    if ((rc = download(c, kount, c->s_c, (size_t)nmat))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MCP_OK;
+  }
   if ((rc = upload(c, c->s_a, a, cnt))) return rc;
   HIPCHK(c, c->s_b.ensure(cnt * sizeof(double)));
   HIPCHK(c, c->s_c.ensure((size_t)nmat * sizeof(int)));
@@ -454,6 +543,30 @@ int mcp_hammat(mcp_ctx* c, int32_t bid, int32_t gid, int32_t nr, const int32_t* 
   return MCP_OK;
 }
 
+// YINIT on the device for ninst instances into I.y (LDS eigen path, or the
+// large-N square root).  d_status (ninst ints) receives 1 for an instance
+// whose start matrix is not positive definite on the large path.
+static int yinit_device(mcp_ctx* c, Block& b, Grid& g, mcp::InstArgs I, int ninst, int k0, int outward, double bcyo,
+                        int* d_status) {
+  const int n = b.n;
+  const size_t lds = lds_yinit(n, b.nham);
+  if (lds <= c->lds_max) {
+    hipLaunchKernelGGL(mcp::k_yinit, dim3(ninst), dim3(kNT), lds, c->stream, block_args(b), grid_args(g), I, k0,
+                       outward, bcyo, 1);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemsetAsync(d_status, 0, ninst * sizeof(int), c->stream));
+    return MCP_OK;
+  }
+  const size_t ll = lds_large(n, b.nham, 0);
+  if (ll > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "yinit: N=%d exceeds the large-N panel limit", n);
+  HIPCHK(c, c->s_y.ensure((size_t)ninst * 3 * n * n * sizeof(double)));
+  I.status = d_status;
+  hipLaunchKernelGGL(mcp::k_yinit_large, dim3(ninst), dim3(kNT), ll, c->stream, block_args(b), grid_args(g), I, k0,
+                     outward, c->s_y.as<double>());
+  HIPCHK(c, hipGetLastError());
+  return MCP_OK;
+}
+
 static int upload_inst(mcp_ctx* c, const Block& b, int ninst, const double* ered, const double* pconst) {
   int rc;
   if ((rc = upload(c, c->s_g, ered, (size_t)ninst))) return rc;
@@ -475,21 +588,22 @@ int mcp_yinit(mcp_ctx* c, int32_t bid, int32_t gid, int32_t k0, int32_t outward,
   if (ninst == 0) return MCP_OK;
   HIPCHK(c, hipSetDevice(c->device));
   const int n = b->n;
-  const size_t lds = lds_yinit(n, b->nham);
-  if (lds > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "yinit: N=%d exceeds the LDS-resident limit", n);
   if ((rc = upload_inst(c, *b, ninst, ered, pconst))) return rc;
   HIPCHK(c, c->s_i.ensure((size_t)ninst * n * n * sizeof(double)));
+  HIPCHK(c, c->s_j.ensure((size_t)ninst * 2 * sizeof(int)));
   mcp::InstArgs I;
   I.ered = c->s_g.as<double>();
   I.pconst = b->nconst > 0 ? c->s_h.as<double>() : nullptr;
   I.y = c->s_i.as<double>();
   I.nodes = nullptr;
   I.status = nullptr;
-  hipLaunchKernelGGL(mcp::k_yinit, dim3(ninst), dim3(kNT), lds, c->stream, block_args(*b), grid_args(*g), I, k0,
-                     outward, bcyo, 1);
-  HIPCHK(c, hipGetLastError());
+  if ((rc = yinit_device(c, *b, *g, I, ninst, k0, outward, bcyo, c->s_j.as<int>()))) return rc;
   if ((rc = download(c, y, c->s_i, (size_t)ninst * n * n))) return rc;
+  std::vector<int> st(ninst);
+  HIPCHK(c, hipMemcpyAsync(st.data(), c->s_j.p, ninst * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < ninst; ++i)
+    if (st[i]) return c->fail(MCP_ERR_UNSUPPORTED, "yinit: locally open channel at the start radius (instance %d)", i);
   return MCP_OK;
 }
 
@@ -603,32 +717,35 @@ int mcp_smatrix_ld(mcp_ctx* c, int32_t bid, int32_t gid, int32_t iprop, int32_t 
   for (size_t e = 0; e < (size_t)ninst * n; ++e)
     if (nb[e] < 0 || nb[e] >= n) return c->fail(MCP_ERR_ARG, "smatrix_ld: nb out of range");
   HIPCHK(c, hipSetDevice(c->device));
-  const size_t lds = lds_yinit(n, b->nham);
-  if (lds > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "yinit: N=%d exceeds the LDS-resident limit", n);
   if ((rc = upload_inst(c, *b, ninst, ered, pconst))) return rc;
   HIPCHK(c, c->s_i.ensure((size_t)ninst * n * n * sizeof(double)));
-  HIPCHK(c, c->s_j.ensure((size_t)ninst * 2 * sizeof(int)));
+  HIPCHK(c, c->s_j.ensure((size_t)ninst * 3 * sizeof(int)));
   mcp::InstArgs I;
   I.ered = c->s_g.as<double>();
   I.pconst = b->nconst > 0 ? c->s_h.as<double>() : nullptr;
   I.y = c->s_i.as<double>();
   I.nodes = nullptr;
   I.status = nullptr;
-  hipLaunchKernelGGL(mcp::k_yinit, dim3(ninst), dim3(kNT), lds, c->stream, block_args(*b), grid_args(*g), I, 0, 1,
-                     0.0, 1);
-  HIPCHK(c, hipGetLastError());
   int* dn = c->s_j.as<int>();
+  if ((rc = yinit_device(c, *b, *g, I, ninst, 0, 1, 0.0, dn + 2 * ninst))) return rc;
   if ((rc = prop_ld_device(c, *b, *g, iprop, ninst, I.ered, I.pconst, I.y, dn, dn + ninst))) return rc;
   if (!(rval > 0.0)) return c->fail(MCP_ERR_ARG, "smatrix_ld: matching radius %g must be positive", rval);
   if ((rc = match_device(c, n, ninst, I.y, wvec, nopen, nb, b->cent.as<double>(), rval))) return rc;
-  std::vector<int> tmp(2 * (size_t)ninst);
-  HIPCHK(c, hipMemcpyAsync(tmp.data(), dn, 2 * ninst * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  std::vector<int> tmp(3 * (size_t)ninst);
+  HIPCHK(c, hipMemcpyAsync(tmp.data(), dn, 3 * ninst * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   if ((rc = fetch_match(c, n, ninst, nopen, kmat, sr, si, status))) return rc;
   for (int i = 0; i < ninst; ++i) {
     nodes[i] = tmp[i];
     if (tmp[ninst + i] != 0) status[i] = -tmp[ninst + i];  // propagation failure: -(step)
+    if (tmp[2 * ninst + i] != 0) status[i] = -1000000;       // open channel at the start radius
   }
   return record_timing(c);
+}
+
+int mcp_set_team_size(mcp_ctx* c, int32_t g) {
+  if (!c || g < 0) return MCP_ERR_ARG;
+  c->team = g;
+  return MCP_OK;
 }
 
 int mcp_last_timing(const mcp_ctx* c, double* ms_sector, double* ms_prop, double* ms_total) {

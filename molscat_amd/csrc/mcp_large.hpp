@@ -1,0 +1,832 @@
+// Large-N path (matrices that do not fit in LDS, e.g. N = 441): blocked
+// symmetric sweep with Bunch-Kaufman pivot choice on a matrix in global
+// memory, stored in its lower triangle like the reference's propagators
+// (mdprop.f:89-93), with one panel of kPB pivots in LDS.
+//
+// One instance (one matrix, or one propagation) runs on a team of g
+// workgroups.  Within a panel every workgroup repeats the pivot selection on
+// identical data (wavefront 0 alone, wave-synchronous: no workgroup barrier
+// per pivot), then each workgroup applies the panel's rank-kPB update to the
+// columns it owns; a per-instance barrier separates panels.  g = 1 needs no
+// barrier.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mcp_device.hpp"
+#include "mcp_kernels.hpp"
+
+namespace mcp {
+
+constexpr int kPB = 16;  // pivots per panel
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void wave_argmax(double& v, int& idx) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ov = __shfl_xor(v, off, 64);
+    const int oi = __shfl_xor(idx, off, 64);
+    if (ov > v || (ov == v && oi < idx)) {
+      v = ov;
+      idx = oi;
+    }
+  }
+}
+
+// element (i, c) of a symmetric matrix held in its lower triangle
+__device__ __forceinline__ double lget(const double* A, int n, int i, int c) {
+  return i >= c ? A[i + (size_t)c * n] : A[c + (size_t)i * n];
+}
+
+// --------------------------------------------------------------------------
+// per-instance workgroup team and barrier
+// --------------------------------------------------------------------------
+struct Team {
+  int g;          // workgroups per instance
+  int rank;       // this workgroup's rank in [0, g)
+  unsigned* ctr;  // barrier counter of the instance (zero at launch)
+  unsigned epoch; // barriers passed so far
+  int* abort;     // set when a barrier wait exceeds its bound
+};
+
+constexpr unsigned kSpinLimit = 1u << 26;  // bounded wait (~seconds), never a hang
+
+__device__ inline bool team_sync(Team& t) {
+  __syncthreads();
+  if (t.g == 1) return true;
+  __shared__ int s_ok;
+  if (threadIdx.x == 0) {
+    t.epoch += 1;
+    const unsigned target = t.epoch * (unsigned)t.g;
+    __hip_atomic_fetch_add(t.ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned spins = 0;
+    int ok = 1;
+    while (__hip_atomic_load(t.ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > kSpinLimit ||
+          __hip_atomic_load(t.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        __hip_atomic_store(t.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 0;
+        break;
+      }
+    }
+    s_ok = ok;
+  } else {
+    t.epoch += 1;
+  }
+  __syncthreads();
+  return s_ok != 0;
+}
+
+// storage column c (4-wide groups, interleaved) belongs to rank (c/4) % g
+__device__ __forceinline__ bool owns_col(const Team& t, int c) { return ((c >> 2) % t.g) == t.rank; }
+
+struct LargeScratch {
+  double* X;     // [kPB][n]  panel columns of the current matrix times D^-1
+  double* G;     // [kPB][n]  original columns of the panel candidates (slots)
+  double* Dinv;  // [kPB*kPB] inverse of the panel pivot block
+  double* mc;    // [n] current column of the candidate
+  double* mci;   // [n] current column of the row candidate
+  double* wv;    // [2*kPB]
+  double* gc;    // [kPB] G row of the candidate at the panel pivots
+  int* perm;     // [n] pivot order (positions -> index)
+  int* inP;      // [n] panel pivot number + 1, or 0
+  int* pidx;     // [kPB] pivot indices of the panel
+  int* pslot;    // [kPB] G slot of each pivot
+  int* slotpos;  // [kPB] G slot of panel position kstart+s
+  int* ctl;      // [8] panel control shared with the other waves
+  double* sv;
+  int* si;
+};
+
+__device__ inline LargeScratch carve_large(LdsCarve& c, int n) {
+  LargeScratch s;
+  s.X = c.take<double>((size_t)kPB * n);
+  s.G = c.take<double>((size_t)kPB * n);
+  s.Dinv = c.take<double>(kPB * kPB);
+  s.mc = c.take<double>(n);
+  s.mci = c.take<double>(n);
+  s.wv = c.take<double>(2 * kPB);
+  s.gc = c.take<double>(kPB);
+  s.perm = c.take<int>(n);
+  s.inP = c.take<int>(n);
+  s.pidx = c.take<int>(kPB);
+  s.pslot = c.take<int>(kPB);
+  s.slotpos = c.take<int>(kPB);
+  s.ctl = c.take<int>(8);
+  s.sv = c.take<double>(kNT / 64);
+  s.si = c.take<int>(kNT / 64);
+  return s;
+}
+
+// wave 0: v = a_c - X * G[c, pslot[.]] for the original column a_c of index c
+// (a_c from a G slot, or gathered from the lower-stored matrix when slot < 0)
+__device__ __forceinline__ void wave_cur_col(const double* A, int n, int c, int slot, int np, const LargeScratch& s,
+                                             double* v) {
+  const int lane = threadIdx.x & 63;
+  for (int t = lane; t < np; t += 64) s.gc[t] = s.G[(size_t)s.pslot[t] * n + c];
+  wave_sync();
+  const double* ac = slot >= 0 ? s.G + (size_t)slot * n : nullptr;
+  for (int i = lane; i < n; i += 64) {
+    double x = ac ? ac[i] : lget(A, n, i, c);
+#pragma unroll
+    for (int t = 0; t < kPB; ++t)
+      if (t < np) x -= s.X[(size_t)t * n + i] * s.gc[t];
+    v[i] = x;
+  }
+  wave_sync();
+}
+
+// Panel pivot selection and factor update by wavefront 0 alone.  Leaves in
+// s.ctl: [0] pivots in the panel, [1] inertia of the panel, [2] error code
+// (0, -(k+1) for a zero candidate column, n+1 for a zero pivot), [3] next
+// position.  The caller makes the results visible with a workgroup barrier.
+__device__ inline void wave_panel(const double* A, int n, int kstart, bool bk, const LargeScratch& s) {
+  const int lane = threadIdx.x & 63;
+  int np = 0, kount = 0, err = 0;
+  int k = kstart;
+  const int npre = min(kPB, n - kstart);
+  while (np <= kPB - 2 && k < n) {
+    const int ck = s.perm[k];
+    wave_cur_col(A, n, ck, s.slotpos[np], np, s, s.mc);
+    int p = ck, q = -1;
+    const double* colp = s.mc;
+    if (bk) {
+      const double absakk = fabs(s.mc[ck]);
+      double v = -1.0;
+      int idx = INT_MAX;
+      for (int i = k + 1 + lane; i < n; i += 64) {
+        const double tv = fabs(s.mc[s.perm[i]]);
+        if (tv > v) {
+          v = tv;
+          idx = i;
+        }
+      }
+      wave_argmax(v, idx);
+      const double colmax = v < 0.0 ? 0.0 : v;
+      if (fmax(absakk, colmax) == 0.0) {
+        err = -(k + 1);
+        break;
+      }
+      if (!(absakk >= kBkAlpha * colmax)) {
+        const int imaxpos = idx;
+        const int ci = s.perm[imaxpos];
+        const bool pre = (imaxpos - kstart) < npre;
+        wave_cur_col(A, n, ci, pre ? s.slotpos[imaxpos - kstart] : -1, np, s, s.mci);
+        double rv = -1.0;
+        int ri = INT_MAX;
+        for (int j = k + lane; j < n; j += 64) {
+          if (j == imaxpos) continue;
+          const double tv = fabs(s.mci[s.perm[j]]);
+          if (tv > rv) {
+            rv = tv;
+            ri = j;
+          }
+        }
+        wave_argmax(rv, ri);
+        const double rowmax = rv;
+        int kp = k, kstep = 1;
+        if (absakk >= kBkAlpha * colmax * (colmax / rowmax)) {
+          kp = k;
+        } else if (fabs(s.mci[ci]) >= kBkAlpha * rowmax) {
+          kp = imaxpos;
+          p = ci;
+          colp = s.mci;
+        } else {
+          kp = imaxpos;
+          kstep = 2;
+          p = ck;
+          q = ci;
+        }
+        const int kk = k + kstep - 1;
+        if (kp != kk) {
+          const int pkk = s.perm[kk], pkp = s.perm[kp];
+          const int skk = s.slotpos[kk - kstart];
+          const int skp = pre ? s.slotpos[kp - kstart] : -1;
+          wave_sync();
+          if (lane == 0) {
+            s.perm[kk] = pkp;
+            s.perm[kp] = pkk;
+            if (pre) {
+              s.slotpos[kk - kstart] = skp;
+              s.slotpos[kp - kstart] = skk;
+            }
+          }
+          if (!pre) {
+            double* dst = s.G + (size_t)skk * n;
+            for (int i = lane; i < n; i += 64) dst[i] = lget(A, n, i, pkp);
+          }
+          wave_sync();
+        }
+      }
+    }
+    if (q < 0) {
+      const double d = colp[p];
+      if (d == 0.0) {
+        err = n + 1;
+        break;
+      }
+      if (d < 0.0) ++kount;
+      const double dinv = 1.0 / d;
+      for (int t = lane; t < np; t += 64) s.wv[t] = s.X[(size_t)t * n + p];
+      wave_sync();
+      for (int i = lane; i < n; i += 64) {
+        const double md = colp[i] * dinv;
+#pragma unroll
+        for (int t = 0; t < kPB; ++t)
+          if (t < np) s.X[(size_t)t * n + i] -= md * s.wv[t];
+        s.X[(size_t)np * n + i] = md;
+      }
+      for (int e = lane; e < (np + 1) * (np + 1); e += 64) {
+        const int a = e % (np + 1), b = e / (np + 1);
+        double v;
+        if (a < np && b < np)
+          v = s.Dinv[a + b * kPB] + (s.wv[a > b ? a : b] * dinv) * s.wv[a > b ? b : a];
+        else if (a < np)
+          v = -s.wv[a] * dinv;
+        else if (b < np)
+          v = -s.wv[b] * dinv;
+        else
+          v = dinv;
+        s.Dinv[a + b * kPB] = v;
+      }
+      if (lane == 0) {
+        s.pidx[np] = p;
+        s.pslot[np] = s.slotpos[np];
+        s.inP[p] = np + 1;
+      }
+      wave_sync();
+      np += 1;
+      k += 1;
+    } else {
+      const double a = s.mc[p], b = s.mc[q], c = s.mci[q];
+      {
+        const double tt = fabs(b);
+        const double dd = (c / tt) * a - tt;
+        if (dd < 0.0) ++kount;
+      }
+      const double det = a * c - b * b;
+      const double s00 = c / det, s01 = -b / det, s11 = a / det;
+      for (int t = lane; t < np; t += 64) {
+        s.wv[t] = s.X[(size_t)t * n + p];
+        s.wv[kPB + t] = s.X[(size_t)t * n + q];
+      }
+      wave_sync();
+      for (int i = lane; i < n; i += 64) {
+        const double m0 = s.mc[i], m1 = s.mci[i];
+        const double z0 = m0 * s00 + m1 * s01;
+        const double z1 = m0 * s01 + m1 * s11;
+#pragma unroll
+        for (int t = 0; t < kPB; ++t)
+          if (t < np) s.X[(size_t)t * n + i] -= z0 * s.wv[t] + z1 * s.wv[kPB + t];
+        s.X[(size_t)np * n + i] = z0;
+        s.X[(size_t)(np + 1) * n + i] = z1;
+      }
+      for (int e = lane; e < (np + 2) * (np + 2); e += 64) {
+        const int aa = e % (np + 2), bb = e / (np + 2);
+        double v;
+        if (aa < np && bb < np) {
+          const int hi = aa > bb ? aa : bb, lo = aa > bb ? bb : aa;
+          const double wph = s.wv[hi], wqh = s.wv[kPB + hi];
+          const double wpl = s.wv[lo], wql = s.wv[kPB + lo];
+          v = s.Dinv[aa + bb * kPB] + (wph * (s00 * wpl + s01 * wql) + wqh * (s01 * wpl + s11 * wql));
+        } else if (aa < np || bb < np) {
+          const int r = aa < np ? aa : bb;
+          const int col = aa < np ? bb - np : aa - np;
+          const double wp = s.wv[r], wq = s.wv[kPB + r];
+          v = (col == 0) ? -(wp * s00 + wq * s01) : -(wp * s01 + wq * s11);
+        } else {
+          const int a2 = aa - np, b2 = bb - np;
+          v = (a2 == 0 && b2 == 0) ? s00 : ((a2 == 1 && b2 == 1) ? s11 : s01);
+        }
+        s.Dinv[aa + bb * kPB] = v;
+      }
+      if (lane == 0) {
+        s.pidx[np] = p;
+        s.pidx[np + 1] = q;
+        s.pslot[np] = s.slotpos[np];
+        s.pslot[np + 1] = s.slotpos[np + 1];
+        s.inP[p] = np + 1;
+        s.inP[q] = np + 2;
+      }
+      wave_sync();
+      np += 2;
+      k += 2;
+    }
+  }
+  if (lane == 0) {
+    s.ctl[0] = np;
+    s.ctl[1] = kount;
+    s.ctl[2] = err;
+    s.ctl[3] = k;
+  }
+}
+
+// Apply the panel to the lower-stored matrix on this workgroup's columns:
+// rank-np update outside the panel, then the panel rows/columns and pivot
+// block.  All threads of the workgroup.
+__device__ inline void apply_panel(double* A, int n, int np, const LargeScratch& s, const Team& t) {
+  if (np == 0) return;
+  const int ngrp = (n + 3) / 4;
+  for (int m = t.rank; m < ngrp; m += t.g) {
+    const int c0 = 4 * m;
+    double gv[4][kPB];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int q = 0; q < kPB; ++q)
+        gv[b][q] = (q < np && c0 + b < n) ? s.G[(size_t)s.pslot[q] * n + c0 + b] : 0.0;
+    for (int i = c0 + threadIdx.x; i < n; i += kNT) {
+      if (s.inP[i]) continue;
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int q = 0; q < kPB; ++q) {
+        if (q < np) {
+          const double x = s.X[(size_t)q * n + i];
+#pragma unroll
+          for (int b = 0; b < 4; ++b) acc[b] += x * gv[b][q];
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int cc = c0 + b;
+        if (cc < n && i >= cc && !s.inP[cc]) A[i + (size_t)cc * n] -= acc[b];
+      }
+    }
+  }
+  // panel rows/columns: element (i, p_q) = X[q][i] for i outside the panel,
+  // stored at column min(i, p_q), written by that column's owner
+  for (int e = threadIdx.x; e < np * n; e += kNT) {
+    const int q = e / n, i = e - q * n;
+    if (s.inP[i]) continue;
+    const int p = s.pidx[q];
+    const int lo = i < p ? i : p, hi = i < p ? p : i;
+    if (!owns_col(t, lo)) continue;
+    A[hi + (size_t)lo * n] = s.X[(size_t)q * n + i];
+  }
+  if (t.rank == 0) {
+    for (int e = threadIdx.x; e < np * np; e += kNT) {
+      const int a = e % np, b = e / np;
+      const int pa = s.pidx[a], pb = s.pidx[b];
+      if (pa >= pb) A[pa + (size_t)pb * n] = -s.Dinv[a + b * kPB];
+    }
+  }
+}
+
+// In-place -inv(A) of the symmetric n x n matrix A held in its lower
+// triangle (global, ld = n), by the blocked sweep on a team of workgroups:
+// on exit the lower triangle holds -inv(A) (the caller folds the sign into
+// its next pass).  Returns the inertia (as sweep_inv), -(k+1) for an exactly
+// zero candidate column, n+1 for an exactly zero pivot, or n+2 when a team
+// barrier gave up.  Must be entered after a team barrier.
+__device__ int sweep_neginv_lower(double* A, int n, bool bk, const LargeScratch& s, Team& tm) {
+  const int tid = threadIdx.x;
+  for (int i = tid; i < n; i += kNT) {
+    s.perm[i] = i;
+    s.inP[i] = 0;
+  }
+  __syncthreads();
+  int kount = 0;
+  int k = 0;
+  while (k < n) {
+    const int kstart = k;
+    const int npre = min(kPB, n - k);
+    for (int e = tid; e < npre * n; e += kNT) {
+      const int sl = e / n, i = e - sl * n;
+      s.G[(size_t)sl * n + i] = lget(A, n, i, s.perm[kstart + sl]);
+    }
+    for (int sl = tid; sl < kPB; sl += kNT) s.slotpos[sl] = sl;
+    __syncthreads();
+    if (tid < 64) wave_panel(A, n, kstart, bk, s);
+    __syncthreads();
+    const int np = s.ctl[0];
+    kount += s.ctl[1];
+    const int err = s.ctl[2];
+    if (err != 0) return err;
+    k = s.ctl[3];
+    apply_panel(A, n, np, s, tm);
+    __syncthreads();
+    for (int t = tid; t < np; t += kNT) s.inP[s.pidx[t]] = 0;
+    if (!team_sync(tm)) return n + 2;
+  }
+  return kount;
+}
+
+__device__ inline Team make_team(int g, unsigned* ctrs, int inst, int* abort_flag) {
+  Team t;
+  t.g = g;
+  t.rank = (int)blockIdx.x % g;
+  t.ctr = ctrs ? ctrs + inst : nullptr;
+  t.epoch = 0;
+  t.abort = abort_flag;
+  return t;
+}
+
+// column loops over this workgroup's storage columns
+#define MCP_OWNED_COLS(tm, n, j)                                   \
+  for (int j##0 = 4 * (tm).rank; j##0 < (n); j##0 += 4 * (tm).g) \
+    for (int j = j##0; j < j##0 + 4 && j < (n); ++j)
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+// zero the per-instance barrier counters (a fill kernel, graph-capture safe)
+__global__ void k_zero_u32(unsigned* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+__global__ void __launch_bounds__(kNT) k_syminv_large(int n, double* __restrict__ a, int* __restrict__ kount, int bk) {
+  extern __shared__ double smem[];
+  LdsCarve c(smem);
+  LargeScratch s = carve_large(c, n);
+  Team tm = make_team(1, nullptr, 0, nullptr);
+  double* A = a + (size_t)blockIdx.x * n * n;
+  const int k = sweep_neginv_lower(A, n, bk != 0, s, tm);
+  // full symmetric inverse out (SYMINL hands back the lower triangle)
+  for (size_t e = threadIdx.x; e < (size_t)n * n; e += kNT) {
+    const int j = (int)(e / n), i = (int)(e - (size_t)j * n);
+    if (i >= j) A[e] = -A[e];
+  }
+  __syncthreads();
+  for (size_t e = threadIdx.x; e < (size_t)n * n; e += kNT) {
+    const int j = (int)(e / n), i = (int)(e - (size_t)j * n);
+    if (i < j) A[e] = A[j + (size_t)i * n];
+  }
+  if (threadIdx.x == 0) kount[blockIdx.x] = k;
+}
+
+// MDPROP sector matrices UC(k) = [I/8 + D4 W_off]^-1 - 8I, lower triangle.
+__global__ void __launch_bounds__(kNT) k_mdprop_uc_large(BlockArgs B, GridArgs G, const double* __restrict__ pconst,
+                                                         double h, int bk, double* __restrict__ uc,
+                                                         int* __restrict__ kount) {
+  extern __shared__ double smem[];
+  const int ks = 1 + blockIdx.x;
+  const int n = B.n;
+  LdsCarve c(smem);
+  LargeScratch s = carve_large(c, n);
+  double* pv = c.take<double>(B.nham);
+  Team tm = make_team(1, nullptr, 0, nullptr);
+  const double d1 = h * h / 3.0;
+  const double d4 = -d1 / 16.0;
+  load_pvec(B, G, 2 * ks - 1, pconst, pv);
+  __syncthreads();
+  double* A = uc + (size_t)blockIdx.x * n * n;
+  for (int j = 0; j < n; ++j)
+    for (int i = j + threadIdx.x; i < n; i += kNT)
+      A[i + (size_t)j * n] = (i == j) ? 0.125 : d4 * sumlam(B, i, j, pv);
+  __syncthreads();
+  const int k = sweep_neginv_lower(A, n, bk != 0, s, tm);
+  for (int j = 0; j < n; ++j)
+    for (int i = j + threadIdx.x; i < n; i += kNT) {
+      const size_t e = i + (size_t)j * n;
+      A[e] = (i == j) ? -A[e] - 8.0 : -A[e];
+    }
+  if (threadIdx.x == 0) kount[blockIdx.x] = k;
+}
+
+// MDPROP on a team of g workgroups per instance (grid = ninst * g); Y held in
+// its lower triangle during the propagation, written in full at the end.
+__global__ void __launch_bounds__(kNT) k_mdprop_large(BlockArgs B, GridArgs G, InstArgs I, int nstep, double h,
+                                                      const double* __restrict__ uc,
+                                                      const int* __restrict__ uc_kount, int bk, int g,
+                                                      unsigned* __restrict__ ctrs, int* __restrict__ abort_flag) {
+  extern __shared__ double smem[];
+  const int inst = blockIdx.x / g;
+  const int n = B.n;
+  LdsCarve c(smem);
+  LargeScratch s = carve_large(c, n);
+  double* pv = c.take<double>(B.nham);
+  double* y14 = c.take<double>(n);
+  double* y23 = c.take<double>(n);
+  double* u2 = c.take<double>(n);
+  Team tm = make_team(g, ctrs, inst, abort_flag);
+  const double ered = I.ered[inst];
+  const double* pconst = I.pconst ? I.pconst + (size_t)inst * B.nconst : nullptr;
+  const double d1 = h * h / 3.0;
+  const double d2 = 2.0 * d1;
+  const double half = 0.5 * h;
+  double* Y = I.y + (size_t)inst * n * n;
+
+  int nsave = 0;
+  {
+    double ps = 0.0, pbad = 0.0;
+    for (int k = threadIdx.x; k < nstep; k += kNT) {
+      const int kc = uc_kount[k];
+      ps += (double)kc;
+      if (kc > n || kc < 0) pbad += 1.0;
+    }
+    ps = block_sum<kNT>(ps, s.sv);
+    pbad = block_sum<kNT>(pbad, s.sv);
+    if (pbad > 0.0) {
+      if (threadIdx.x == 0 && tm.rank == 0) {
+        I.nodes[inst] = 0;
+        I.status[inst] = -1;
+      }
+      return;
+    }
+    nsave = (int)ps;
+  }
+  load_pvec(B, G, 0, pconst, pv);
+  __syncthreads();
+  {
+    const double r0 = G.r[0], rsq0 = rsq_of(r0);
+    MCP_OWNED_COLS(tm, n, j)
+    for (int i = j + threadIdx.x; i < n; i += kNT) {
+      double w = sumlam(B, i, j, pv);
+      if (i == j) w = hammat_diag(B, i, w, r0, rsq0, ered);
+      const size_t e = i + (size_t)j * n;
+      Y[e] = h * Y[e] + d1 * w;
+    }
+  }
+  int nodes = 0, status = 0;
+  bool ok = true;
+  for (int ks = 1; ks <= nstep; ++ks) {
+    // DIAG(C) terms (mdprop.f:143-164): every workgroup, redundantly
+    load_pvec(B, G, 2 * ks - 1, pconst, pv);
+    __syncthreads();
+    {
+      const double rc = G.r[2 * ks - 1], rsqc = rsq_of(rc);
+      for (int i = threadIdx.x; i < n; i += kNT) {
+        const double wref = hammat_diag(B, i, sumlam(B, i, i, pv), rc, rsqc, ered);
+        const double arg = half * sqrt(fabs(wref));
+        double a14, a23;
+        if (wref < 0.0) {
+          const double tn = tan(arg);
+          a14 = arg / tn - arg * tn;
+          a23 = arg / tn + arg * tn;
+        } else {
+          const double th = tanh(arg);
+          a14 = arg / th + arg * th;
+          a23 = arg / th - arg * th;
+        }
+        u2[i] = 2.0 * a14;
+        const double y14c = a14 - d1 * wref;
+        y14[i] = y14c;
+        y23[i] = a23;
+        if (owns_col(tm, i)) Y[i + (size_t)i * n] = Y[i + (size_t)i * n] + y14c;
+      }
+    }
+    if (!(ok = team_sync(tm))) break;
+    int kc = sweep_neginv_lower(Y, n, bk != 0, s, tm);
+    if (kc > n || kc < 0) {
+      status = ks;
+      break;
+    }
+    nodes += kc;
+    // Y <- UC + 2 Y14 - Y23 Y Y23 with Y holding -inv (mdprop.f:181-185)
+    {
+      const double* U = uc + (size_t)(ks - 1) * n * n;
+      MCP_OWNED_COLS(tm, n, j)
+      for (int i = j + threadIdx.x; i < n; i += kNT) {
+        const size_t e = i + (size_t)j * n;
+        double uij = U[e];
+        if (i == j) uij = uij + u2[i];
+        Y[e] = uij + y23[i] * Y[e] * y23[j];
+      }
+    }
+    if (!(ok = team_sync(tm))) break;
+    kc = sweep_neginv_lower(Y, n, bk != 0, s, tm);
+    if (kc > n || kc < 0) {
+      status = ks;
+      break;
+    }
+    nodes += kc;
+    // Y <- D2 W(B) - Y23 Y Y23 + Y14 with Y holding -inv (mdprop.f:197-230)
+    const double dd2 = (ks == nstep) ? d1 : d2;
+    load_pvec(B, G, 2 * ks, pconst, pv);
+    __syncthreads();
+    {
+      const double rb = G.r[2 * ks], rsqb = rsq_of(rb);
+      MCP_OWNED_COLS(tm, n, j)
+      for (int i = j + threadIdx.x; i < n; i += kNT) {
+        const size_t e = i + (size_t)j * n;
+        double w = sumlam(B, i, j, pv);
+        if (i == j) w = hammat_diag(B, i, w, rb, rsqb, ered);
+        double v = dd2 * w + y23[i] * Y[e] * y23[j];
+        if (i == j) v = v + y14[i];
+        Y[e] = v;
+      }
+    }
+    if (!(ok = team_sync(tm))) break;
+  }
+  if (ok && status == 0) {
+    // Y/H, written in full
+    const double hi = 1.0 / h;
+    MCP_OWNED_COLS(tm, n, j)
+    for (int i = j + threadIdx.x; i < n; i += kNT) {
+      const double v = hi * Y[i + (size_t)j * n];
+      Y[i + (size_t)j * n] = v;
+      Y[j + (size_t)i * n] = v;
+    }
+  }
+  if (threadIdx.x == 0 && tm.rank == 0) {
+    I.nodes[inst] = nodes - nsave;
+    I.status[inst] = ok ? status : -2;
+  }
+}
+
+// LDPROP on a team of g workgroups per instance; Z and U (per-instance
+// workspace) held in their lower triangles.
+__global__ void __launch_bounds__(kNT) k_ldprop_large(BlockArgs B, GridArgs G, InstArgs I, int nstep, double h,
+                                                      double* __restrict__ uwork, int bk, int g,
+                                                      unsigned* __restrict__ ctrs, int* __restrict__ abort_flag) {
+  extern __shared__ double smem[];
+  const int inst = blockIdx.x / g;
+  const int n = B.n;
+  LdsCarve c(smem);
+  LargeScratch s = carve_large(c, n);
+  double* pv = c.take<double>(B.nham);
+  Team tm = make_team(g, ctrs, inst, abort_flag);
+  const double ered = I.ered[inst];
+  const double* pconst = I.pconst ? I.pconst + (size_t)inst * B.nconst : nullptr;
+  const double d1 = h * h / 3.0;
+  const double d2 = 2.0 * d1;
+  const double d4 = -d1 / 16.0;
+  double* Z = I.y + (size_t)inst * n * n;
+  double* U = uwork + (size_t)inst * n * n;
+  load_pvec(B, G, 0, pconst, pv);
+  __syncthreads();
+  {
+    const double r0 = G.r[0], rsq0 = rsq_of(r0);
+    MCP_OWNED_COLS(tm, n, j)
+    for (int i = j + threadIdx.x; i < n; i += kNT) {
+      double w = sumlam(B, i, j, pv);
+      if (i == j) w = hammat_diag(B, i, w, r0, rsq0, ered);
+      const size_t e = i + (size_t)j * n;
+      double v = h * Z[e] + d1 * w;
+      if (i == j) v = 1.0 + v;
+      Z[e] = v;
+    }
+  }
+  int nodes = 0, status = 0;
+  bool ok = true;
+  for (int ks = 1; ks <= nstep; ++ks) {
+    load_pvec(B, G, 2 * ks - 1, pconst, pv);
+    __syncthreads();
+    {
+      const double rc = G.r[2 * ks - 1], rsqc = rsq_of(rc);
+      MCP_OWNED_COLS(tm, n, j)
+      for (int i = j + threadIdx.x; i < n; i += kNT) {
+        double w = sumlam(B, i, j, pv);
+        if (i == j) w = hammat_diag(B, i, w, rc, rsqc, ered);
+        double v = d4 * w;
+        if (i == j) v = 0.125 + v;
+        U[i + (size_t)j * n] = v;
+      }
+    }
+    if (!(ok = team_sync(tm))) break;
+    const int ncu = sweep_neginv_lower(U, n, bk != 0, s, tm);
+    if (ncu > n || ncu < 0) {
+      status = ks;
+      break;
+    }
+    int ncz = sweep_neginv_lower(Z, n, bk != 0, s, tm);
+    if (ncz > n || ncz < 0) {
+      status = ks;
+      break;
+    }
+    nodes += ncz;
+    // Z <- U - Z - 6I with both holding -inv
+    MCP_OWNED_COLS(tm, n, j)
+    for (int i = j + threadIdx.x; i < n; i += kNT) {
+      const size_t e = i + (size_t)j * n;
+      double v = -U[e] + Z[e];
+      if (i == j) v = v - 6.0;
+      Z[e] = v;
+    }
+    if (!(ok = team_sync(tm))) break;
+    ncz = sweep_neginv_lower(Z, n, bk != 0, s, tm);
+    if (ncz > n || ncz < 0) {
+      status = ks;
+      break;
+    }
+    nodes += ncz - ncu;
+    const double dd2 = (ks == nstep) ? d1 : d2;
+    load_pvec(B, G, 2 * ks, pconst, pv);
+    __syncthreads();
+    {
+      const double rb = G.r[2 * ks], rsqb = rsq_of(rb);
+      MCP_OWNED_COLS(tm, n, j)
+      for (int i = j + threadIdx.x; i < n; i += kNT) {
+        const size_t e = i + (size_t)j * n;
+        double w = sumlam(B, i, j, pv);
+        if (i == j) w = hammat_diag(B, i, w, rb, rsqb, ered);
+        double u = dd2 * w;
+        if (i == j) u = u + 2.0;
+        Z[e] = u + Z[e];
+      }
+    }
+    if (!(ok = team_sync(tm))) break;
+  }
+  if (ok && status == 0) {
+    const double hi = 1.0 / h;
+    MCP_OWNED_COLS(tm, n, j)
+    for (int i = j + threadIdx.x; i < n; i += kNT) {
+      double v = hi * Z[i + (size_t)j * n];
+      if (i == j) v = v - hi;
+      Z[i + (size_t)j * n] = v;
+      Z[j + (size_t)i * n] = v;
+    }
+  }
+  if (threadIdx.x == 0 && tm.rank == 0) {
+    I.nodes[inst] = nodes;
+    I.status[inst] = ok ? status : -2;
+  }
+}
+
+// YINIT for N beyond the LDS eigen-solver with every channel locally closed:
+// the adiabatic WKB start V diag(sqrt(w)) V^T of yinit.f:76-150 is the
+// principal square root of W, computed by the Denman-Beavers iteration
+// (Y <- (Y + Z^-1)/2, Z <- (Z + Y^-1)/2 from Y = W/c, Z = I) with the
+// blocked sweep (lower triangles).  status = 0, or 1 when W is not positive
+// definite (an open channel at the start radius needs BCYOMN,
+// yinit.f:124-131).
+__global__ void __launch_bounds__(kNT) k_yinit_large(BlockArgs B, GridArgs G, InstArgs I, int k0, int outward,
+                                                     double* __restrict__ work) {
+  extern __shared__ double smem[];
+  const int inst = blockIdx.x;
+  const int n = B.n;
+  const size_t nn = (size_t)n * n;
+  LdsCarve c(smem);
+  LargeScratch s = carve_large(c, n);
+  double* pv = c.take<double>(B.nham);
+  Team tm = make_team(1, nullptr, 0, nullptr);
+  const double ered = I.ered[inst];
+  const double* pconst = I.pconst ? I.pconst + (size_t)inst * B.nconst : nullptr;
+  double* Y = I.y + (size_t)inst * nn;
+  double* Z = work + (size_t)inst * 3 * nn;
+  double* T = Z + nn;
+  double* U = T + nn;
+  load_pvec(B, G, k0, pconst, pv);
+  __syncthreads();
+  const double r0 = G.r[k0], rsq0 = rsq_of(r0);
+  double tr = 0.0;
+  for (int j = 0; j < n; ++j)
+    for (int i = j + threadIdx.x; i < n; i += kNT) {
+      double w = sumlam(B, i, j, pv);
+      if (i == j) {
+        w = hammat_diag(B, i, w, r0, rsq0, ered);
+        tr += w;
+      }
+      Y[i + (size_t)j * n] = w;
+    }
+  tr = block_sum<kNT>(tr, s.sv);
+  const double scale = tr / n;
+  bool bad = !(scale > 0.0);
+  if (!bad) {
+    for (int j = 0; j < n; ++j)
+      for (int i = j + threadIdx.x; i < n; i += kNT) {
+        const size_t e = i + (size_t)j * n;
+        Y[e] /= scale;
+        Z[e] = (i == j) ? 1.0 : 0.0;
+      }
+    __syncthreads();
+    for (int it = 0; it < 40; ++it) {
+      for (int j = 0; j < n; ++j)
+        for (int i = j + threadIdx.x; i < n; i += kNT) {
+          const size_t e = i + (size_t)j * n;
+          T[e] = Y[e];
+          U[e] = Z[e];
+        }
+      __syncthreads();
+      const int ky = sweep_neginv_lower(T, n, true, s, tm);
+      const int kz = sweep_neginv_lower(U, n, true, s, tm);
+      if (ky != 0 || kz != 0) {
+        bad = true;
+        break;
+      }
+      double dmax = 0.0, ymax = 0.0;
+      for (int j = 0; j < n; ++j)
+        for (int i = j + threadIdx.x; i < n; i += kNT) {
+          const size_t e = i + (size_t)j * n;
+          const double yn = 0.5 * (Y[e] - U[e]);
+          const double zn = 0.5 * (Z[e] - T[e]);
+          dmax = fmax(dmax, fabs(yn - Y[e]));
+          ymax = fmax(ymax, fabs(yn));
+          Y[e] = yn;
+          Z[e] = zn;
+        }
+      int dummy = 0;
+      block_argmax<kNT>(dmax, dummy, s.sv, s.si);
+      int dummy2 = 0;
+      block_argmax<kNT>(ymax, dummy2, s.sv, s.si);
+      if (it >= 2 && dmax <= 1.0e-15 * ymax) break;
+    }
+  }
+  const double f = (outward ? 1.0 : -1.0) * sqrt(scale > 0.0 ? scale : 0.0);
+  __syncthreads();
+  for (int j = 0; j < n; ++j)
+    for (int i = j + threadIdx.x; i < n; i += kNT) {
+      const double v = f * Y[i + (size_t)j * n];
+      Y[i + (size_t)j * n] = v;
+      Y[j + (size_t)i * n] = v;
+    }
+  if (threadIdx.x == 0 && I.status) I.status[inst] = bad ? 1 : 0;
+}
+
+}  // namespace mcp

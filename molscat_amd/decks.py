@@ -1,0 +1,87 @@
+"""Synthetic decks for the propagation-path workloads (atom + homonuclear
+rigid rotor, LJ 12-6 Legendre terms with seeded anisotropy, RUNIT = 1 A,
+IRMSET = 0).  Each deck is the &INPUT/&BASIS/&POTL content the reference
+would read, reduced to what the propagation path consumes.
+
+  rotor36   B=60 cm-1, even j=0..10, JTOT=10, block IB=2 (N=36), URED=1.5,
+            lambda=0,2,4 with c ~ U[0.1,0.4] (seed 0), LDMD 2->30 A at
+            DR=0.01 (2800 sectors), E=100 cm-1.
+  rotor441  even j=0..40, JTOT=60, IB=2 (N=441), lambda=0..6 with
+            c ~ U[0.05,0.4] (seed 1), LDMD 2->20 A at DR=0.005 (3600
+            sectors), then AIRY 20->200 A, E=300 cm-1.
+  sweep     the rotor441 system for JTOT=0..120, IB=1,2 (blocks of N from 20
+            to 441) at 64 energies uniform in [10, 500] cm-1.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import List
+
+import numpy as np
+
+from . import system as S
+
+RM = 3.5       # Angstrom
+EPSIL = 50.0   # cm-1
+URED = 1.5     # amu
+BE = 60.0      # cm-1
+
+
+def anisotropy(seed: int, lo: float, hi: float, nlam: int) -> np.ndarray:
+    """c_lambda for lambda > 0, drawn U[lo, hi] from numpy's PCG64(seed)."""
+    return np.random.default_rng(seed).uniform(lo, hi, size=nlam)
+
+
+@dataclass
+class LDSegment:
+    rstart: float
+    rstop: float
+    dr_in: float
+
+    @property
+    def nstep(self) -> int:
+        return S.drset_constant(self.rstart, self.rstop, self.dr_in)[0]
+
+    @property
+    def dr(self) -> float:
+        return S.drset_constant(self.rstart, self.rstop, self.dr_in)[1]
+
+    def radii(self) -> np.ndarray:
+        return S.ld_radii(self.rstart, self.dr, self.nstep)
+
+
+@dataclass
+class Deck:
+    name: str
+    jlevels: List[int]
+    lambdas: List[int]
+    cvals: List[float]
+    jtot: int
+    ib: int
+    energies: List[float]
+    ld: LDSegment
+    airy_stop: float = 0.0      # 0: no AIRY segment
+    airy_dr: float = 0.0        # first AIRY step (TOLHIL=0, POWRL=4/3)
+    extra: dict = field(default_factory=dict)
+
+    def potential(self) -> S.PowerPotential:
+        return S.lj_rotor_potential([1.0, *self.cvals], RM, EPSIL)
+
+    def block(self, jtot=None, ib=None) -> S.RotorBlock:
+        return S.rotor_block(self.jtot if jtot is None else jtot, self.ib if ib is None else ib, self.jlevels, BE,
+                             self.lambdas, self.potential(), URED, runit=1.0)
+
+
+def rotor36() -> Deck:
+    c = anisotropy(0, 0.1, 0.4, 2)
+    return Deck("rotor36", list(range(0, 11, 2)), [0, 2, 4], list(c), 10, 2, [100.0], LDSegment(2.0, 30.0, 0.01))
+
+
+def rotor441() -> Deck:
+    c = anisotropy(1, 0.05, 0.4, 3)
+    return Deck("rotor441", list(range(0, 41, 2)), [0, 2, 4, 6], list(c), 60, 2, [300.0],
+                LDSegment(2.0, 20.0, 0.005), airy_stop=200.0, airy_dr=0.2)
+
+
+def sweep_energies() -> np.ndarray:
+    return np.linspace(10.0, 500.0, 64)

@@ -214,6 +214,10 @@ class Engine:
         return dict(K=_unpack_square(k, nopen), SR=_unpack_square(sr, nopen), SI=_unpack_square(si, nopen),
                     NODES=nodes, STATUS=st)
 
+    def set_team_size(self, g: int):
+        """Workgroups per instance on the large-N path (0 = automatic)."""
+        self._check(self.lib.mcp_set_team_size(self.ctx, int(g)), "mcp_set_team_size")
+
     def last_timing(self) -> dict:
         v = [ctypes.c_double() for _ in range(3)]
         self._check(self.lib.mcp_last_timing(self.ctx, *[ctypes.byref(x) for x in v]), "mcp_last_timing")
