@@ -113,9 +113,12 @@ size_t lds_yinit(int n, int nham) {
 
 size_t lds_large(int n, int nham, int nvec) {
   LdsSize z;
-  z.add<double>((size_t)mcp::kPB * n).add<double>((size_t)mcp::kPB * n).add<double>(mcp::kPB * mcp::kPB);
+  // same pieces, in the same order, as mcp::carve_large
+  const size_t ld = mcp::panel_ld(n);
+  z.add<double>(mcp::kPB * ld).add<double>(mcp::kPB * ld).add<double>(mcp::kPB * mcp::kPB);
+  z.add<double>(mcp::kPB * mcp::kPB).add<double>(mcp::kPB * mcp::kPB).add<double>((kNT / 64) * mcp::kPB);
   z.add<double>(n).add<double>(n).add<double>(2 * mcp::kPB).add<double>(mcp::kPB);
-  z.add<int>(n).add<int>(n).add<int>(mcp::kPB);
+  z.add<int>(n).add<int>(n).add<int>(n).add<int>(mcp::kPB);
   z.add<int>(mcp::kPB).add<int>(mcp::kPB).add<int>(8);
   z.add<double>(kNT / 64).add<int>(kNT / 64);
   if (nham > 0) z.add<double>(nham);

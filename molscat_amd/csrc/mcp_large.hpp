@@ -1,14 +1,23 @@
 // Large-N path (matrices that do not fit in LDS, e.g. N = 441): blocked
-// symmetric sweep with Bunch-Kaufman pivot choice on a matrix in global
-// memory, stored in its lower triangle like the reference's propagators
-// (mdprop.f:89-93), with one panel of kPB pivots in LDS.
+// symmetric sweep with the Bunch-Kaufman pivot choice of dsytf2.f on a matrix
+// in global memory, stored in its lower triangle like the reference's
+// propagators (mdprop.f:89-93), with one panel of up to kPB pivots in LDS.
+//
+// Panel step.  The kPB candidate columns are loaded into LDS.  Wavefront 0
+// factors the candidate block S = L D L^T in natural order; every thread then
+// forms W = A(:,P) L^-T, whose column k is the Schur column of pivot k at the
+// moment it would be taken, so the Bunch-Kaufman acceptance test
+// |d_k| >= alpha * colmax_k is evaluated for all candidates at once.  The
+// leading run of candidates that passes is swept as one block pivot
+// (X = A(:,P) S^-1, A <- A - X A(P,:), A(:,P) <- X, A(P,P) <- -S^-1).  When the
+// first candidate fails, the panel falls back to pivot-by-pivot selection
+// with row interchanges (wave_panel).
 //
 // One instance (one matrix, or one propagation) runs on a team of g
-// workgroups.  Within a panel every workgroup repeats the pivot selection on
-// identical data (wavefront 0 alone, wave-synchronous: no workgroup barrier
-// per pivot), then each workgroup applies the panel's rank-kPB update to the
-// columns it owns; a per-instance barrier separates panels.  g = 1 needs no
-// barrier.
+// workgroups; the lower triangle is cut into kTile x kTile tiles and tile t
+// belongs to rank t % g.  Every workgroup repeats the panel factorisation on
+// identical data, applies the update to its own tiles, and a per-instance
+// barrier separates panels (g = 1 needs none).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -17,7 +26,8 @@
 
 namespace mcp {
 
-constexpr int kPB = 16;  // pivots per panel
+constexpr int kPB = 16;    // pivots per panel
+constexpr int kTile = 64;  // ownership / update tile edge
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -35,6 +45,12 @@ __device__ __forceinline__ void wave_argmax(double& v, int& idx) {
       idx = oi;
     }
   }
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
 }
 
 // element (i, c) of a symmetric matrix held in its lower triangle
@@ -82,19 +98,55 @@ __device__ inline bool team_sync(Team& t) {
   return s_ok != 0;
 }
 
-// storage column c (4-wide groups, interleaved) belongs to rank (c/4) % g
-__device__ __forceinline__ bool owns_col(const Team& t, int c) { return ((c >> 2) % t.g) == t.rank; }
+// tile row of lower-triangle tile number tau (tiles numbered row by row)
+__device__ __forceinline__ int tile_row(int tau) {
+  int bi = 0;
+  while ((bi + 1) * (bi + 2) / 2 <= tau) ++bi;
+  return bi;
+}
+
+// owner rank of lower element (r, c), r >= c
+__device__ __forceinline__ bool owns(const Team& t, int r, int c) {
+  if (t.g == 1) return true;
+  const int bi = r / kTile, bj = c / kTile;
+  return (bi * (bi + 1) / 2 + bj) % t.g == t.rank;
+}
+
+// f(i, j) for every lower element (i >= j) of this workgroup's tiles; within
+// a tile consecutive threads take consecutive rows of one column
+template <class F>
+__device__ __forceinline__ void for_owned_lower(const Team& t, int n, F&& f) {
+  const int nt = (n + kTile - 1) / kTile;
+  const int ntt = nt * (nt + 1) / 2;
+  for (int tau = t.rank; tau < ntt; tau += t.g) {
+    const int bi = tile_row(tau);
+    const int bj = tau - bi * (bi + 1) / 2;
+    for (int e = threadIdx.x; e < kTile * kTile; e += kNT) {
+      const int i = bi * kTile + (e & (kTile - 1));
+      const int j = bj * kTile + e / kTile;
+      if (i < n && j < n && i >= j) f(i, j);
+    }
+  }
+}
+
+// leading dimension of the X and G panels in LDS: n rounded up to whole tiles
+__host__ __device__ constexpr int panel_ld(int n) { return (n + kTile - 1) / kTile * kTile; }
 
 struct LargeScratch {
-  double* X;     // [kPB][n]  panel columns of the current matrix times D^-1
-  double* G;     // [kPB][n]  original columns of the panel candidates (slots)
+  int ld;        // panel_ld(n)
+  double* X;     // [kPB][ld] A(:,P) S^-1 (block path) / pivot-by-pivot factors
+  double* G;     // [kPB][ld] candidate columns of the panel (slots)
   double* Dinv;  // [kPB*kPB] inverse of the panel pivot block
+  double* Lb;    // [kPB*kPB] LDL^T of the candidate block (D on the diagonal)
+  double* Li;    // [kPB*kPB] L^-1
+  double* cmax;  // [kNT/64][kPB] per-wave partial column maxima
   double* mc;    // [n] current column of the candidate
   double* mci;   // [n] current column of the row candidate
   double* wv;    // [2*kPB]
   double* gc;    // [kPB] G row of the candidate at the panel pivots
   int* perm;     // [n] pivot order (positions -> index)
   int* inP;      // [n] panel pivot number + 1, or 0
+  int* swept;    // [n] 1 once the index has been pivoted
   int* pidx;     // [kPB] pivot indices of the panel
   int* pslot;    // [kPB] G slot of each pivot
   int* slotpos;  // [kPB] G slot of panel position kstart+s
@@ -105,15 +157,20 @@ struct LargeScratch {
 
 __device__ inline LargeScratch carve_large(LdsCarve& c, int n) {
   LargeScratch s;
-  s.X = c.take<double>((size_t)kPB * n);
-  s.G = c.take<double>((size_t)kPB * n);
+  s.ld = panel_ld(n);
+  s.X = c.take<double>((size_t)kPB * s.ld);
+  s.G = c.take<double>((size_t)kPB * s.ld);
   s.Dinv = c.take<double>(kPB * kPB);
+  s.Lb = c.take<double>(kPB * kPB);
+  s.Li = c.take<double>(kPB * kPB);
+  s.cmax = c.take<double>((kNT / 64) * kPB);
   s.mc = c.take<double>(n);
   s.mci = c.take<double>(n);
   s.wv = c.take<double>(2 * kPB);
   s.gc = c.take<double>(kPB);
   s.perm = c.take<int>(n);
   s.inP = c.take<int>(n);
+  s.swept = c.take<int>(n);
   s.pidx = c.take<int>(kPB);
   s.pslot = c.take<int>(kPB);
   s.slotpos = c.take<int>(kPB);
@@ -123,29 +180,185 @@ __device__ inline LargeScratch carve_large(LdsCarve& c, int n) {
   return s;
 }
 
-// wave 0: v = a_c - X * G[c, pslot[.]] for the original column a_c of index c
-// (a_c from a G slot, or gathered from the lower-stored matrix when slot < 0)
+// --------------------------------------------------------------------------
+// block step on the candidate window
+// --------------------------------------------------------------------------
+// broadcast of a double from lane src of the wavefront
+__device__ __forceinline__ double bcast(double x, int src) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, src);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), src);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Natural-order block pivot over the candidate window (positions kstart ..
+// kstart+npre-1, loaded in G slots 0..npre-1).  All threads; returns the
+// number np of accepted pivots (0: use the pivot-by-pivot path).  On np > 0:
+// X holds A(:,P) S^-1 (rows outside P), Dinv holds S^-1, pidx/pslot/inP the
+// pivots, ctl[1] the inertia of the accepted pivots.
+__device__ __attribute__((always_inline)) int block_panel(int n, int kstart, int npre, bool bk,
+                                                          const LargeScratch& s) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wid = tid >> 6;
+  const int ld = s.ld;
+  if (tid < 64) {
+    // S = L D L^T in natural order: lane j holds row j of the Schur
+    // complement; the pivot row is broadcast from its lane (dsytf2.f 1x1 step)
+    if (lane < npre) s.inP[s.perm[kstart + lane]] = lane + 1;
+    const int pj = lane < npre ? s.perm[kstart + lane] : 0;
+    double r[kPB];
+#pragma unroll
+    for (int c = 0; c < kPB; ++c) r[c] = (lane < npre && c < npre) ? s.G[c * ld + pj] : 0.0;
+    int f = npre;
+#pragma unroll
+    for (int k = 0; k < kPB; ++k) {
+      if (k < f) {
+        const double d = bcast(r[k], k);
+        if (d == 0.0) {
+          f = k;
+        } else {
+          const double dinv = 1.0 / d;
+          const double l = r[k] * dinv;
+#pragma unroll
+          for (int c = k + 1; c < kPB; ++c) {
+            const double rkc = bcast(r[c], k);
+            if (lane > k) r[c] -= l * rkc;
+          }
+          if (lane > k) r[k] = l;
+        }
+      }
+    }
+    if (lane < kPB) {
+#pragma unroll
+      for (int t = 0; t < kPB; ++t) s.Lb[lane + t * kPB] = (t <= lane && t < f && lane < f) ? r[t] : 0.0;
+    }
+    wave_sync();
+    // L^-1 by columns: lane c forms L^-1 e_c (rows of the factor beyond f are 0)
+    if (lane < kPB) {
+      double x[kPB];
+#pragma unroll
+      for (int j = 0; j < kPB; ++j) {
+        double v = (j == lane) ? 1.0 : 0.0;
+#pragma unroll
+        for (int m = 0; m < j; ++m) v -= s.Lb[j + m * kPB] * x[m];
+        x[j] = (j < f && lane < f) ? v : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < kPB; ++j) s.Li[j + lane * kPB] = x[j];
+    }
+    if (lane == 0) s.ctl[4] = f;
+  }
+  __syncthreads();
+  const int f = s.ctl[4];
+  // W = A(:,P) L^-T: column k is the Schur column of candidate k at its pivot
+  // time; maxima over the rows still unpivoted then (remaining rows, and
+  // window rows after k)
+  double cm[kPB];
+#pragma unroll
+  for (int k = 0; k < kPB; ++k) cm[k] = 0.0;
+  for (int i = tid; i < n; i += kNT) {
+    const int rk = s.swept[i] ? 0 : (s.inP[i] ? s.inP[i] - 1 : kPB);
+    double g[kPB];
+#pragma unroll
+    for (int t = 0; t < kPB; ++t) g[t] = s.G[t * ld + i];
+#pragma unroll
+    for (int k = 0; k < kPB; ++k) {
+      double w = 0.0;
+#pragma unroll
+      for (int t = 0; t <= k; ++t) w += g[t] * s.Li[k + t * kPB];
+      if (k < rk) cm[k] = fmax(cm[k], fabs(w));
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kPB; ++k) {
+    const double v = wave_max(cm[k]);
+    if (lane == 0) s.cmax[wid * kPB + k] = v;
+  }
+  __syncthreads();
+  // accepted run: the candidates before the first one that fails the test;
+  // then S^-1 = L^-T D^-1 L^-1 of the accepted block
+  if (tid < 64) {
+    bool pass = false, neg = false;
+    double d = 1.0;
+    if (lane < f) {
+      double colmax = 0.0;
+#pragma unroll
+      for (int wq = 0; wq < kNT / 64; ++wq) colmax = fmax(colmax, s.cmax[wq * kPB + lane]);
+      d = s.Lb[lane + lane * kPB];
+      pass = !bk || (fabs(d) >= kBkAlpha * colmax);
+      neg = d < 0.0;
+    }
+    const int np = __ffsll((unsigned long long)__ballot(!pass)) - 1;  // lanes >= f fail
+    const int kount = __popcll((unsigned long long)__ballot(neg) & ((1ull << np) - 1ull));
+    if (lane >= np && lane < npre) s.inP[s.perm[kstart + lane]] = 0;
+    if (lane < kPB) {
+      s.pidx[lane] = lane < np ? s.perm[kstart + lane] : 0;
+      s.pslot[lane] = lane;
+      s.wv[lane] = lane < np ? 1.0 / d : 0.0;  // D^-1 of the accepted pivots
+    }
+    if (lane == 0) {
+      s.ctl[0] = np;
+      s.ctl[1] = kount;
+      s.ctl[2] = 0;
+      s.ctl[3] = kstart + np;
+    }
+    wave_sync();
+    for (int e = lane; e < kPB * kPB; e += 64) {
+      const int a = e & (kPB - 1), b = e / kPB;
+      double v = 0.0;
+#pragma unroll
+      for (int k = 0; k < kPB; ++k)
+        if (k >= a && k >= b && k < np) v += s.Li[k + a * kPB] * s.wv[k] * s.Li[k + b * kPB];
+      s.Dinv[e] = v;
+    }
+  }
+  __syncthreads();
+  const int np = s.ctl[0];
+  if (np == 0) return 0;
+  // X = A(:,P) S^-1 for the rows outside P (S^-1 is zero beyond np)
+  for (int i = tid; i < n; i += kNT) {
+    if (s.inP[i]) continue;
+    double g[kPB];
+#pragma unroll
+    for (int t = 0; t < kPB; ++t) g[t] = s.G[t * ld + i];
+#pragma unroll
+    for (int q = 0; q < kPB; ++q) {
+      double x = 0.0;
+#pragma unroll
+      for (int t = 0; t < kPB; ++t) x += g[t] * s.Dinv[t + q * kPB];
+      s.X[q * ld + i] = x;
+    }
+  }
+  __syncthreads();
+  return np;
+}
+
+// --------------------------------------------------------------------------
+// pivot-by-pivot panel with Bunch-Kaufman interchanges (wavefront 0)
+// --------------------------------------------------------------------------
+// v = a_c - X * G[c, pslot[.]] for the original column a_c of index c (a_c
+// from a G slot, or gathered from the lower-stored matrix when slot < 0).
+// X rows at and beyond np are zero.
 __device__ __forceinline__ void wave_cur_col(const double* A, int n, int c, int slot, int np, const LargeScratch& s,
                                              double* v) {
   const int lane = threadIdx.x & 63;
-  for (int t = lane; t < np; t += 64) s.gc[t] = s.G[(size_t)s.pslot[t] * n + c];
+  if (lane < kPB) s.gc[lane] = (lane < np) ? s.G[(size_t)s.pslot[lane] * s.ld + c] : 0.0;
   wave_sync();
-  const double* ac = slot >= 0 ? s.G + (size_t)slot * n : nullptr;
+  const double* ac = slot >= 0 ? s.G + (size_t)slot * s.ld : nullptr;
   for (int i = lane; i < n; i += 64) {
     double x = ac ? ac[i] : lget(A, n, i, c);
 #pragma unroll
-    for (int t = 0; t < kPB; ++t)
-      if (t < np) x -= s.X[(size_t)t * n + i] * s.gc[t];
+    for (int t = 0; t < kPB; ++t) x -= s.X[(size_t)t * s.ld + i] * s.gc[t];
     v[i] = x;
   }
   wave_sync();
 }
 
-// Panel pivot selection and factor update by wavefront 0 alone.  Leaves in
-// s.ctl: [0] pivots in the panel, [1] inertia of the panel, [2] error code
-// (0, -(k+1) for a zero candidate column, n+1 for a zero pivot), [3] next
-// position.  The caller makes the results visible with a workgroup barrier.
-__device__ inline void wave_panel(const double* A, int n, int kstart, bool bk, const LargeScratch& s) {
+// Leaves in s.ctl: [0] pivots in the panel, [1] inertia of the panel, [2]
+// error code (0, -(k+1) for a zero candidate column, n+1 for a zero pivot),
+// [3] next position.  X must be zero on entry.  The caller makes the results
+// visible with a workgroup barrier.
+__device__ __attribute__((always_inline)) void wave_panel(const double* A, int n, int kstart, bool bk, const LargeScratch& s) {
   const int lane = threadIdx.x & 63;
   int np = 0, kount = 0, err = 0;
   int k = kstart;
@@ -217,7 +430,7 @@ __device__ inline void wave_panel(const double* A, int n, int kstart, bool bk, c
             }
           }
           if (!pre) {
-            double* dst = s.G + (size_t)skk * n;
+            double* dst = s.G + (size_t)skk * s.ld;
             for (int i = lane; i < n; i += 64) dst[i] = lget(A, n, i, pkp);
           }
           wave_sync();
@@ -232,14 +445,15 @@ __device__ inline void wave_panel(const double* A, int n, int kstart, bool bk, c
       }
       if (d < 0.0) ++kount;
       const double dinv = 1.0 / d;
-      for (int t = lane; t < np; t += 64) s.wv[t] = s.X[(size_t)t * n + p];
+      if (lane < kPB) s.wv[lane] = (lane < np) ? s.X[(size_t)lane * s.ld + p] : 0.0;
       wave_sync();
       for (int i = lane; i < n; i += 64) {
         const double md = colp[i] * dinv;
 #pragma unroll
-        for (int t = 0; t < kPB; ++t)
-          if (t < np) s.X[(size_t)t * n + i] -= md * s.wv[t];
-        s.X[(size_t)np * n + i] = md;
+        for (int t = 0; t < kPB; ++t) {
+          const double x = s.X[(size_t)t * s.ld + i];
+          s.X[(size_t)t * s.ld + i] = (t < np) ? x - md * s.wv[t] : ((t == np) ? md : x);
+        }
       }
       for (int e = lane; e < (np + 1) * (np + 1); e += 64) {
         const int a = e % (np + 1), b = e / (np + 1);
@@ -271,9 +485,9 @@ __device__ inline void wave_panel(const double* A, int n, int kstart, bool bk, c
       }
       const double det = a * c - b * b;
       const double s00 = c / det, s01 = -b / det, s11 = a / det;
-      for (int t = lane; t < np; t += 64) {
-        s.wv[t] = s.X[(size_t)t * n + p];
-        s.wv[kPB + t] = s.X[(size_t)t * n + q];
+      if (lane < kPB) {
+        s.wv[lane] = (lane < np) ? s.X[(size_t)lane * s.ld + p] : 0.0;
+        s.wv[kPB + lane] = (lane < np) ? s.X[(size_t)lane * s.ld + q] : 0.0;
       }
       wave_sync();
       for (int i = lane; i < n; i += 64) {
@@ -281,10 +495,11 @@ __device__ inline void wave_panel(const double* A, int n, int kstart, bool bk, c
         const double z0 = m0 * s00 + m1 * s01;
         const double z1 = m0 * s01 + m1 * s11;
 #pragma unroll
-        for (int t = 0; t < kPB; ++t)
-          if (t < np) s.X[(size_t)t * n + i] -= z0 * s.wv[t] + z1 * s.wv[kPB + t];
-        s.X[(size_t)np * n + i] = z0;
-        s.X[(size_t)(np + 1) * n + i] = z1;
+        for (int t = 0; t < kPB; ++t) {
+          const double x = s.X[(size_t)t * s.ld + i];
+          s.X[(size_t)t * s.ld + i] =
+              (t < np) ? x - (z0 * s.wv[t] + z1 * s.wv[kPB + t]) : ((t == np) ? z0 : ((t == np + 1) ? z1 : x));
+        }
       }
       for (int e = lane; e < (np + 2) * (np + 2); e += 64) {
         const int aa = e % (np + 2), bb = e / (np + 2);
@@ -326,54 +541,73 @@ __device__ inline void wave_panel(const double* A, int n, int kstart, bool bk, c
   }
 }
 
-// Apply the panel to the lower-stored matrix on this workgroup's columns:
-// rank-np update outside the panel, then the panel rows/columns and pivot
-// block.  All threads of the workgroup.
-__device__ inline void apply_panel(double* A, int n, int np, const LargeScratch& s, const Team& t) {
+// --------------------------------------------------------------------------
+// panel update of this workgroup's tiles
+// --------------------------------------------------------------------------
+// A(i,j) -= sum_q X(i,q) A(p_q,j) for i, j outside the panel (4 x 4 register
+// tiles: rows 4tx + a, columns 4ty + b), then the panel rows/columns
+// A(i,p_q) = X(i,q) and the pivot block -Dinv.  X rows beyond np are zero.
+__device__ __attribute__((always_inline)) void apply_panel(double* A, int n, int np, const LargeScratch& s,
+                                                           const Team& t) {
   if (np == 0) return;
-  const int ngrp = (n + 3) / 4;
-  for (int m = t.rank; m < ngrp; m += t.g) {
-    const int c0 = 4 * m;
-    double gv[4][kPB];
+  const int ld = s.ld;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int nt = (n + kTile - 1) / kTile;
+  const int ntt = nt * (nt + 1) / 2;
+  const double* gslot[kPB];
 #pragma unroll
-    for (int b = 0; b < 4; ++b)
+  for (int q = 0; q < kPB; ++q) gslot[q] = s.G + s.pslot[q] * ld;
+  for (int tau = t.rank; tau < ntt; tau += t.g) {
+    const int bi = tile_row(tau);
+    const int bj = tau - bi * (bi + 1) / 2;
+    const int r0 = bi * kTile + 4 * tx, c0 = bj * kTile + 4 * ty;
+    if (c0 >= n || r0 >= n || r0 + 3 < c0) continue;
+    // current tile values first, so their latency overlaps the products
+    double av[4][4], acc[4][4];
 #pragma unroll
-      for (int q = 0; q < kPB; ++q)
-        gv[b][q] = (q < np && c0 + b < n) ? s.G[(size_t)s.pslot[q] * n + c0 + b] : 0.0;
-    for (int i = c0 + threadIdx.x; i < n; i += kNT) {
-      if (s.inP[i]) continue;
-      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < 4; ++b) {
+      const int cc = min(c0 + b, n - 1);
 #pragma unroll
-      for (int q = 0; q < kPB; ++q) {
-        if (q < np) {
-          const double x = s.X[(size_t)q * n + i];
-#pragma unroll
-          for (int b = 0; b < 4; ++b) acc[b] += x * gv[b][q];
-        }
+      for (int a = 0; a < 4; ++a) {
+        av[a][b] = A[min(r0 + a, n - 1) + (size_t)cc * n];
+        acc[a][b] = 0.0;
       }
+    }
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const int cc = c0 + b;
-        if (cc < n && i >= cc && !s.inP[cc]) A[i + (size_t)cc * n] -= acc[b];
+    for (int q = 0; q < kPB; ++q) {
+      const double2* xq = reinterpret_cast<const double2*>(s.X + q * ld + r0);
+      const double2* gq = reinterpret_cast<const double2*>(gslot[q] + c0);
+      const double2 x01 = xq[0], x23 = xq[1], g01 = gq[0], g23 = gq[1];
+      const double xv[4] = {x01.x, x01.y, x23.x, x23.y};
+      const double gv[4] = {g01.x, g01.y, g23.x, g23.y};
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] += xv[a] * gv[b];
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int cc = c0 + b;
+      if (cc >= n || s.inP[cc]) continue;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int i = r0 + a;
+        if (i < n && i >= cc && !s.inP[i]) A[i + (size_t)cc * n] = av[a][b] - acc[a][b];
       }
     }
   }
-  // panel rows/columns: element (i, p_q) = X[q][i] for i outside the panel,
-  // stored at column min(i, p_q), written by that column's owner
-  for (int e = threadIdx.x; e < np * n; e += kNT) {
-    const int q = e / n, i = e - q * n;
-    if (s.inP[i]) continue;
+  for (int q = 0; q < np; ++q) {
     const int p = s.pidx[q];
-    const int lo = i < p ? i : p, hi = i < p ? p : i;
-    if (!owns_col(t, lo)) continue;
-    A[hi + (size_t)lo * n] = s.X[(size_t)q * n + i];
-  }
-  if (t.rank == 0) {
-    for (int e = threadIdx.x; e < np * np; e += kNT) {
-      const int a = e % np, b = e / np;
-      const int pa = s.pidx[a], pb = s.pidx[b];
-      if (pa >= pb) A[pa + (size_t)pb * n] = -s.Dinv[a + b * kPB];
+    for (int i = threadIdx.x; i < n; i += kNT) {
+      if (s.inP[i]) continue;
+      const int lo = i < p ? i : p, hi = i < p ? p : i;
+      if (owns(t, hi, lo)) A[hi + (size_t)lo * n] = s.X[q * ld + i];
     }
+  }
+  for (int e = threadIdx.x; e < np * np; e += kNT) {
+    const int a = e % np, b = e / np;
+    const int pa = s.pidx[a], pb = s.pidx[b];
+    if (pa >= pb && owns(t, pa, pb)) A[pa + (size_t)pb * n] = -s.Dinv[a + b * kPB];
   }
 }
 
@@ -383,11 +617,12 @@ __device__ inline void apply_panel(double* A, int n, int np, const LargeScratch&
 // its next pass).  Returns the inertia (as sweep_inv), -(k+1) for an exactly
 // zero candidate column, n+1 for an exactly zero pivot, or n+2 when a team
 // barrier gave up.  Must be entered after a team barrier.
-__device__ int sweep_neginv_lower(double* A, int n, bool bk, const LargeScratch& s, Team& tm) {
+__device__ __attribute__((always_inline)) int sweep_neginv_lower(double* A, int n, bool bk, const LargeScratch& s, Team& tm) {
   const int tid = threadIdx.x;
   for (int i = tid; i < n; i += kNT) {
     s.perm[i] = i;
     s.inP[i] = 0;
+    s.swept[i] = 0;
   }
   __syncthreads();
   int kount = 0;
@@ -395,22 +630,60 @@ __device__ int sweep_neginv_lower(double* A, int n, bool bk, const LargeScratch&
   while (k < n) {
     const int kstart = k;
     const int npre = min(kPB, n - k);
-    for (int e = tid; e < npre * n; e += kNT) {
-      const int sl = e / n, i = e - sl * n;
-      s.G[(size_t)sl * n + i] = lget(A, n, i, s.perm[kstart + sl]);
+    // candidate columns: the lower part of column p (rows i >= p, coalesced
+    // in i) and the row segment A(p, i < p) (coalesced in p for a natural run)
+    constexpr int kU = 8;  // loads in flight per thread
+    const int ld = s.ld;
+    for (int base = tid; base < kPB * ld; base += kU * kNT) {
+      double v[kU];
+      bool st[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int e = base + u * kNT;
+        const int sl = e / ld, i = e - sl * ld;
+        const int p = (e < kPB * ld && sl < npre) ? s.perm[kstart + sl] : 0;
+        st[u] = e < kPB * ld && (sl >= npre || i >= n || i >= p);
+        v[u] = (st[u] && sl < npre && i < n) ? A[i + (size_t)p * n] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        if (st[u]) s.G[base + u * kNT] = v[u];
     }
-    for (int sl = tid; sl < kPB; sl += kNT) s.slotpos[sl] = sl;
+    for (int base = tid; base < npre * n; base += kU * kNT) {
+      double v[kU];
+      int dst[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int e = base + u * kNT;
+        const int i = e / npre, sl = e - i * npre;
+        const int p = e < npre * n ? s.perm[kstart + sl] : 0;
+        dst[u] = (e < npre * n && i < p) ? sl * ld + i : -1;
+        v[u] = dst[u] >= 0 ? A[p + (size_t)i * n] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        if (dst[u] >= 0) s.G[dst[u]] = v[u];
+    }
+    if (tid < kPB) s.slotpos[tid] = tid;
     __syncthreads();
-    if (tid < 64) wave_panel(A, n, kstart, bk, s);
-    __syncthreads();
-    const int np = s.ctl[0];
+    int np = block_panel(n, kstart, npre, bk, s);
+    if (np == 0) {
+      for (int e = tid; e < kPB * ld; e += kNT) s.X[e] = 0.0;
+      __syncthreads();
+      if (tid < 64) wave_panel(A, n, kstart, bk, s);
+      __syncthreads();
+      np = s.ctl[0];
+    }
     kount += s.ctl[1];
     const int err = s.ctl[2];
     if (err != 0) return err;
     k = s.ctl[3];
     apply_panel(A, n, np, s, tm);
     __syncthreads();
-    for (int t = tid; t < np; t += kNT) s.inP[s.pidx[t]] = 0;
+    if (tid < np) {
+      s.inP[s.pidx[tid]] = 0;
+      s.swept[s.pidx[tid]] = 1;
+    }
     if (!team_sync(tm)) return n + 2;
   }
   return kount;
@@ -425,11 +698,6 @@ __device__ inline Team make_team(int g, unsigned* ctrs, int inst, int* abort_fla
   t.abort = abort_flag;
   return t;
 }
-
-// column loops over this workgroup's storage columns
-#define MCP_OWNED_COLS(tm, n, j)                                   \
-  for (int j##0 = 4 * (tm).rank; j##0 < (n); j##0 += 4 * (tm).g) \
-    for (int j = j##0; j < j##0 + 4 && j < (n); ++j)
 
 // ---------------------------------------------------------------------------
 // kernels
@@ -448,15 +716,11 @@ __global__ void __launch_bounds__(kNT) k_syminv_large(int n, double* __restrict_
   double* A = a + (size_t)blockIdx.x * n * n;
   const int k = sweep_neginv_lower(A, n, bk != 0, s, tm);
   // full symmetric inverse out (SYMINL hands back the lower triangle)
-  for (size_t e = threadIdx.x; e < (size_t)n * n; e += kNT) {
-    const int j = (int)(e / n), i = (int)(e - (size_t)j * n);
-    if (i >= j) A[e] = -A[e];
-  }
+  for_owned_lower(tm, n, [&](int i, int j) { A[i + (size_t)j * n] = -A[i + (size_t)j * n]; });
   __syncthreads();
-  for (size_t e = threadIdx.x; e < (size_t)n * n; e += kNT) {
-    const int j = (int)(e / n), i = (int)(e - (size_t)j * n);
-    if (i < j) A[e] = A[j + (size_t)i * n];
-  }
+  for_owned_lower(tm, n, [&](int i, int j) {
+    if (i != j) A[j + (size_t)i * n] = A[i + (size_t)j * n];
+  });
   if (threadIdx.x == 0) kount[blockIdx.x] = k;
 }
 
@@ -476,16 +740,15 @@ __global__ void __launch_bounds__(kNT) k_mdprop_uc_large(BlockArgs B, GridArgs G
   load_pvec(B, G, 2 * ks - 1, pconst, pv);
   __syncthreads();
   double* A = uc + (size_t)blockIdx.x * n * n;
-  for (int j = 0; j < n; ++j)
-    for (int i = j + threadIdx.x; i < n; i += kNT)
-      A[i + (size_t)j * n] = (i == j) ? 0.125 : d4 * sumlam(B, i, j, pv);
+  for_owned_lower(tm, n, [&](int i, int j) {
+    A[i + (size_t)j * n] = (i == j) ? 0.125 : d4 * sumlam(B, i, j, pv);
+  });
   __syncthreads();
   const int k = sweep_neginv_lower(A, n, bk != 0, s, tm);
-  for (int j = 0; j < n; ++j)
-    for (int i = j + threadIdx.x; i < n; i += kNT) {
-      const size_t e = i + (size_t)j * n;
-      A[e] = (i == j) ? -A[e] - 8.0 : -A[e];
-    }
+  for_owned_lower(tm, n, [&](int i, int j) {
+    const size_t e = i + (size_t)j * n;
+    A[e] = (i == j) ? -A[e] - 8.0 : -A[e];
+  });
   if (threadIdx.x == 0) kount[blockIdx.x] = k;
 }
 
@@ -535,13 +798,12 @@ __global__ void __launch_bounds__(kNT) k_mdprop_large(BlockArgs B, GridArgs G, I
   __syncthreads();
   {
     const double r0 = G.r[0], rsq0 = rsq_of(r0);
-    MCP_OWNED_COLS(tm, n, j)
-    for (int i = j + threadIdx.x; i < n; i += kNT) {
+    for_owned_lower(tm, n, [&](int i, int j) {
       double w = sumlam(B, i, j, pv);
       if (i == j) w = hammat_diag(B, i, w, r0, rsq0, ered);
       const size_t e = i + (size_t)j * n;
       Y[e] = h * Y[e] + d1 * w;
-    }
+    });
   }
   int nodes = 0, status = 0;
   bool ok = true;
@@ -568,7 +830,7 @@ __global__ void __launch_bounds__(kNT) k_mdprop_large(BlockArgs B, GridArgs G, I
         const double y14c = a14 - d1 * wref;
         y14[i] = y14c;
         y23[i] = a23;
-        if (owns_col(tm, i)) Y[i + (size_t)i * n] = Y[i + (size_t)i * n] + y14c;
+        if (owns(tm, i, i)) Y[i + (size_t)i * n] = Y[i + (size_t)i * n] + y14c;
       }
     }
     if (!(ok = team_sync(tm))) break;
@@ -581,13 +843,12 @@ __global__ void __launch_bounds__(kNT) k_mdprop_large(BlockArgs B, GridArgs G, I
     // Y <- UC + 2 Y14 - Y23 Y Y23 with Y holding -inv (mdprop.f:181-185)
     {
       const double* U = uc + (size_t)(ks - 1) * n * n;
-      MCP_OWNED_COLS(tm, n, j)
-      for (int i = j + threadIdx.x; i < n; i += kNT) {
+      for_owned_lower(tm, n, [&](int i, int j) {
         const size_t e = i + (size_t)j * n;
         double uij = U[e];
         if (i == j) uij = uij + u2[i];
         Y[e] = uij + y23[i] * Y[e] * y23[j];
-      }
+      });
     }
     if (!(ok = team_sync(tm))) break;
     kc = sweep_neginv_lower(Y, n, bk != 0, s, tm);
@@ -602,27 +863,25 @@ __global__ void __launch_bounds__(kNT) k_mdprop_large(BlockArgs B, GridArgs G, I
     __syncthreads();
     {
       const double rb = G.r[2 * ks], rsqb = rsq_of(rb);
-      MCP_OWNED_COLS(tm, n, j)
-      for (int i = j + threadIdx.x; i < n; i += kNT) {
+      for_owned_lower(tm, n, [&](int i, int j) {
         const size_t e = i + (size_t)j * n;
         double w = sumlam(B, i, j, pv);
         if (i == j) w = hammat_diag(B, i, w, rb, rsqb, ered);
         double v = dd2 * w + y23[i] * Y[e] * y23[j];
         if (i == j) v = v + y14[i];
         Y[e] = v;
-      }
+      });
     }
     if (!(ok = team_sync(tm))) break;
   }
   if (ok && status == 0) {
     // Y/H, written in full
     const double hi = 1.0 / h;
-    MCP_OWNED_COLS(tm, n, j)
-    for (int i = j + threadIdx.x; i < n; i += kNT) {
+    for_owned_lower(tm, n, [&](int i, int j) {
       const double v = hi * Y[i + (size_t)j * n];
       Y[i + (size_t)j * n] = v;
       Y[j + (size_t)i * n] = v;
-    }
+    });
   }
   if (threadIdx.x == 0 && tm.rank == 0) {
     I.nodes[inst] = nodes - nsave;
@@ -653,15 +912,14 @@ __global__ void __launch_bounds__(kNT) k_ldprop_large(BlockArgs B, GridArgs G, I
   __syncthreads();
   {
     const double r0 = G.r[0], rsq0 = rsq_of(r0);
-    MCP_OWNED_COLS(tm, n, j)
-    for (int i = j + threadIdx.x; i < n; i += kNT) {
+    for_owned_lower(tm, n, [&](int i, int j) {
       double w = sumlam(B, i, j, pv);
       if (i == j) w = hammat_diag(B, i, w, r0, rsq0, ered);
       const size_t e = i + (size_t)j * n;
       double v = h * Z[e] + d1 * w;
       if (i == j) v = 1.0 + v;
       Z[e] = v;
-    }
+    });
   }
   int nodes = 0, status = 0;
   bool ok = true;
@@ -670,14 +928,13 @@ __global__ void __launch_bounds__(kNT) k_ldprop_large(BlockArgs B, GridArgs G, I
     __syncthreads();
     {
       const double rc = G.r[2 * ks - 1], rsqc = rsq_of(rc);
-      MCP_OWNED_COLS(tm, n, j)
-      for (int i = j + threadIdx.x; i < n; i += kNT) {
+      for_owned_lower(tm, n, [&](int i, int j) {
         double w = sumlam(B, i, j, pv);
         if (i == j) w = hammat_diag(B, i, w, rc, rsqc, ered);
         double v = d4 * w;
         if (i == j) v = 0.125 + v;
         U[i + (size_t)j * n] = v;
-      }
+      });
     }
     if (!(ok = team_sync(tm))) break;
     const int ncu = sweep_neginv_lower(U, n, bk != 0, s, tm);
@@ -692,13 +949,12 @@ __global__ void __launch_bounds__(kNT) k_ldprop_large(BlockArgs B, GridArgs G, I
     }
     nodes += ncz;
     // Z <- U - Z - 6I with both holding -inv
-    MCP_OWNED_COLS(tm, n, j)
-    for (int i = j + threadIdx.x; i < n; i += kNT) {
+    for_owned_lower(tm, n, [&](int i, int j) {
       const size_t e = i + (size_t)j * n;
       double v = -U[e] + Z[e];
       if (i == j) v = v - 6.0;
       Z[e] = v;
-    }
+    });
     if (!(ok = team_sync(tm))) break;
     ncz = sweep_neginv_lower(Z, n, bk != 0, s, tm);
     if (ncz > n || ncz < 0) {
@@ -711,27 +967,25 @@ __global__ void __launch_bounds__(kNT) k_ldprop_large(BlockArgs B, GridArgs G, I
     __syncthreads();
     {
       const double rb = G.r[2 * ks], rsqb = rsq_of(rb);
-      MCP_OWNED_COLS(tm, n, j)
-      for (int i = j + threadIdx.x; i < n; i += kNT) {
+      for_owned_lower(tm, n, [&](int i, int j) {
         const size_t e = i + (size_t)j * n;
         double w = sumlam(B, i, j, pv);
         if (i == j) w = hammat_diag(B, i, w, rb, rsqb, ered);
         double u = dd2 * w;
         if (i == j) u = u + 2.0;
         Z[e] = u + Z[e];
-      }
+      });
     }
     if (!(ok = team_sync(tm))) break;
   }
   if (ok && status == 0) {
     const double hi = 1.0 / h;
-    MCP_OWNED_COLS(tm, n, j)
-    for (int i = j + threadIdx.x; i < n; i += kNT) {
+    for_owned_lower(tm, n, [&](int i, int j) {
       double v = hi * Z[i + (size_t)j * n];
       if (i == j) v = v - hi;
       Z[i + (size_t)j * n] = v;
       Z[j + (size_t)i * n] = v;
-    }
+    });
   }
   if (threadIdx.x == 0 && tm.rank == 0) {
     I.nodes[inst] = nodes;
@@ -766,33 +1020,30 @@ __global__ void __launch_bounds__(kNT) k_yinit_large(BlockArgs B, GridArgs G, In
   __syncthreads();
   const double r0 = G.r[k0], rsq0 = rsq_of(r0);
   double tr = 0.0;
-  for (int j = 0; j < n; ++j)
-    for (int i = j + threadIdx.x; i < n; i += kNT) {
-      double w = sumlam(B, i, j, pv);
-      if (i == j) {
-        w = hammat_diag(B, i, w, r0, rsq0, ered);
-        tr += w;
-      }
-      Y[i + (size_t)j * n] = w;
+  for_owned_lower(tm, n, [&](int i, int j) {
+    double w = sumlam(B, i, j, pv);
+    if (i == j) {
+      w = hammat_diag(B, i, w, r0, rsq0, ered);
+      tr += w;
     }
+    Y[i + (size_t)j * n] = w;
+  });
   tr = block_sum<kNT>(tr, s.sv);
   const double scale = tr / n;
   bool bad = !(scale > 0.0);
   if (!bad) {
-    for (int j = 0; j < n; ++j)
-      for (int i = j + threadIdx.x; i < n; i += kNT) {
-        const size_t e = i + (size_t)j * n;
-        Y[e] /= scale;
-        Z[e] = (i == j) ? 1.0 : 0.0;
-      }
+    for_owned_lower(tm, n, [&](int i, int j) {
+      const size_t e = i + (size_t)j * n;
+      Y[e] /= scale;
+      Z[e] = (i == j) ? 1.0 : 0.0;
+    });
     __syncthreads();
     for (int it = 0; it < 40; ++it) {
-      for (int j = 0; j < n; ++j)
-        for (int i = j + threadIdx.x; i < n; i += kNT) {
-          const size_t e = i + (size_t)j * n;
-          T[e] = Y[e];
-          U[e] = Z[e];
-        }
+      for_owned_lower(tm, n, [&](int i, int j) {
+        const size_t e = i + (size_t)j * n;
+        T[e] = Y[e];
+        U[e] = Z[e];
+      });
       __syncthreads();
       const int ky = sweep_neginv_lower(T, n, true, s, tm);
       const int kz = sweep_neginv_lower(U, n, true, s, tm);
@@ -801,16 +1052,15 @@ __global__ void __launch_bounds__(kNT) k_yinit_large(BlockArgs B, GridArgs G, In
         break;
       }
       double dmax = 0.0, ymax = 0.0;
-      for (int j = 0; j < n; ++j)
-        for (int i = j + threadIdx.x; i < n; i += kNT) {
-          const size_t e = i + (size_t)j * n;
-          const double yn = 0.5 * (Y[e] - U[e]);
-          const double zn = 0.5 * (Z[e] - T[e]);
-          dmax = fmax(dmax, fabs(yn - Y[e]));
-          ymax = fmax(ymax, fabs(yn));
-          Y[e] = yn;
-          Z[e] = zn;
-        }
+      for_owned_lower(tm, n, [&](int i, int j) {
+        const size_t e = i + (size_t)j * n;
+        const double yn = 0.5 * (Y[e] - U[e]);
+        const double zn = 0.5 * (Z[e] - T[e]);
+        dmax = fmax(dmax, fabs(yn - Y[e]));
+        ymax = fmax(ymax, fabs(yn));
+        Y[e] = yn;
+        Z[e] = zn;
+      });
       int dummy = 0;
       block_argmax<kNT>(dmax, dummy, s.sv, s.si);
       int dummy2 = 0;
@@ -820,12 +1070,11 @@ __global__ void __launch_bounds__(kNT) k_yinit_large(BlockArgs B, GridArgs G, In
   }
   const double f = (outward ? 1.0 : -1.0) * sqrt(scale > 0.0 ? scale : 0.0);
   __syncthreads();
-  for (int j = 0; j < n; ++j)
-    for (int i = j + threadIdx.x; i < n; i += kNT) {
-      const double v = f * Y[i + (size_t)j * n];
-      Y[i + (size_t)j * n] = v;
-      Y[j + (size_t)i * n] = v;
-    }
+  for_owned_lower(tm, n, [&](int i, int j) {
+    const double v = f * Y[i + (size_t)j * n];
+    Y[i + (size_t)j * n] = v;
+    Y[j + (size_t)i * n] = v;
+  });
   if (threadIdx.x == 0 && I.status) I.status[inst] = bad ? 1 : 0;
 }
 
