@@ -110,6 +110,23 @@ int mcp_smatrix_ld(mcp_ctx *ctx, int32_t block_id, int32_t grid_id, int32_t ipro
                    const int32_t *nb, double rval, double *kmat, double *sr, double *si, int32_t *nodes,
                    int32_t *status);
 
+/* BDCTRL matching (source_code/bdctrl.f:76-257) for ninst energies (or
+ * fields, through pconst) of one block, two log-derivative segments as
+ * BDPSET lays them out for RMID = RMATCH (source_code/bdpset.f:26-42):
+ * YINIT at the first point of grid_out and propagation outward (DR > 0,
+ * iprop_out 6 = MDPROP / 5 = LDPROP) to RMATCH; YINIT with RSTART > RSTOP
+ * at the first point of grid_in and propagation inward (DR < 0) to RMATCH;
+ * matching matrix M = Y_out - Y_in and DIAGVC.  Per instance: nodec = NODEC
+ * (the nodes of both propagations plus NEGCNT = #{eig(M) <= 0}), eigmin =
+ * EIGMIN (the eigenvalue of smallest modulus), imin = its 0-based index in
+ * ascending order, psimch (N, or NULL) = PSIMCH with its largest component
+ * positive, evals (N, or NULL) = eig(M) ascending.  status = 0, -k for an
+ * inversion failure at outward step k, -(100000+k) inward, -1000000 for an
+ * open channel at a start radius. */
+int mcp_bound_match(mcp_ctx *ctx, int32_t block_id, int32_t grid_out, int32_t grid_in, int32_t iprop_out,
+                    int32_t iprop_in, int32_t ninst, const double *ered, const double *pconst, int32_t *nodec,
+                    double *eigmin, int32_t *imin, double *psimch, double *evals, int32_t *status);
+
 /* Workgroups that cooperate on one instance on the large-N path (N beyond the
  * LDS-resident kernels): g >= 1 fixes it, 0 restores the automatic choice
  * (up to 32, limited by CUs / instances).  No reference counterpart. */

@@ -30,6 +30,7 @@ struct DevBuf {
   hipError_t ensure(size_t b) {
     if (b <= bytes && p) return hipSuccess;
     if (p) {
+      (void)hipDeviceSynchronize();  // kernels queued earlier may still read it
       (void)hipFree(p);
       p = nullptr;
       bytes = 0;
@@ -173,7 +174,7 @@ namespace {
 int set_lds_limits(mcp_ctx* c) {
   const void* fns[] = {(const void*)mcp::k_syminv,         (const void*)mcp::k_mdprop_uc,
                        (const void*)mcp::k_mdprop_lds,     (const void*)mcp::k_ldprop_lds,
-                       (const void*)mcp::k_eigh,           (const void*)mcp::k_yinit,
+                       (const void*)mcp::k_eigh,           (const void*)mcp::k_bound_match,           (const void*)mcp::k_yinit,
                        (const void*)mcp::k_syminv_large,   (const void*)mcp::k_mdprop_uc_large,
                        (const void*)mcp::k_mdprop_large,   (const void*)mcp::k_ldprop_large,
                        (const void*)mcp::k_yinit_large};
@@ -279,7 +280,7 @@ int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const do
   if ((g.npts - 1) % 2 != 0 || g.npts < 3) return c->fail(MCP_ERR_ARG, "LD grid needs 2*NSTEP+1 points");
   const int nstep = (g.npts - 1) / 2;
   const int n = b.n;
-  if (!(g.dr > 0.0)) return c->fail(MCP_ERR_ARG, "grid %d was not created with an LD step DR", 0);
+  if (g.dr == 0.0) return c->fail(MCP_ERR_ARG, "grid was not created with an LD step DR");
   const double h = g.dr / 2.0;  // H = DR/2 (mdprop.f:64)
   const int bk = n > 30 ? 1 : 0;
   mcp::BlockArgs B = block_args(b);
@@ -743,6 +744,74 @@ int mcp_smatrix_ld(mcp_ctx* c, int32_t bid, int32_t gid, int32_t iprop, int32_t 
     if (tmp[2 * ninst + i] != 0) status[i] = -1000000;       // open channel at the start radius
   }
   return record_timing(c);
+}
+
+int mcp_bound_match(mcp_ctx* c, int32_t bid, int32_t gid_out, int32_t gid_in, int32_t iprop_out, int32_t iprop_in,
+                    int32_t ninst, const double* ered, const double* pconst, int32_t* nodec, double* eigmin,
+                    int32_t* imin, double* psimch, double* evals, int32_t* status) {
+  if (!c || ninst < 0 || !ered || !nodec || !eigmin || !imin || !status)
+    return c ? c->fail(MCP_ERR_ARG, "bound_match: bad args") : MCP_ERR_ARG;
+  Block* b;
+  Grid* go;
+  Grid* gi;
+  int rc;
+  if ((rc = lookup(c, bid, gid_out, &b, &go))) return rc;
+  if ((rc = lookup(c, bid, gid_in, &b, &gi))) return rc;
+  if (ninst == 0) return MCP_OK;
+  const int n = b->n;
+  if (lds_eig(n) > c->lds_max)
+    return c->fail(MCP_ERR_UNSUPPORTED, "bound_match: N=%d exceeds the LDS eigen-solver limit", n);
+  if (!(go->dr > 0.0) || !(gi->dr < 0.0))
+    return c->fail(MCP_ERR_ARG, "bound_match: the outward grid needs DR > 0 and the inward grid DR < 0");
+  HIPCHK(c, hipSetDevice(c->device));
+  if ((rc = upload_inst(c, *b, ninst, ered, pconst))) return rc;
+  const size_t nn = (size_t)ninst * n * n;
+  HIPCHK(c, c->s_i.ensure(nn * sizeof(double)));
+  HIPCHK(c, c->s_k.ensure(nn * sizeof(double)));
+  HIPCHK(c, c->s_j.ensure((size_t)ninst * 9 * sizeof(int)));
+  HIPCHK(c, c->s_d.ensure((size_t)ninst * (2 + 2 * (size_t)n) * sizeof(double)));
+  // sector-matrix workspace for the longer grid first: no reallocation between the two propagations
+  const int nstep_max = std::max(go->npts, gi->npts) / 2;
+  HIPCHK(c, c->s_e.ensure((size_t)nstep_max * n * n * sizeof(double)));
+  HIPCHK(c, c->s_f.ensure((size_t)nstep_max * sizeof(int)));
+  mcp::InstArgs I;
+  I.ered = c->s_g.as<double>();
+  I.pconst = b->nconst > 0 ? c->s_h.as<double>() : nullptr;
+  I.nodes = nullptr;
+  I.status = nullptr;
+  int* dn = c->s_j.as<int>();  // [nodes_out | status_out | yinit_out | nodes_in | status_in | yinit_in | nodec | imin]
+  double* yo = c->s_i.as<double>();
+  double* yi = c->s_k.as<double>();
+  I.y = yo;
+  if ((rc = yinit_device(c, *b, *go, I, ninst, 0, 1, 0.0, dn + 2 * ninst))) return rc;
+  if ((rc = prop_ld_device(c, *b, *go, iprop_out, ninst, I.ered, I.pconst, yo, dn, dn + ninst))) return rc;
+  I.y = yi;
+  if ((rc = yinit_device(c, *b, *gi, I, ninst, 0, 0, 0.0, dn + 5 * ninst))) return rc;
+  if ((rc = prop_ld_device(c, *b, *gi, iprop_in, ninst, I.ered, I.pconst, yi, dn + 3 * ninst, dn + 4 * ninst)))
+    return rc;
+  double* de = c->s_d.as<double>();  // [eigmin | psimch | evals]
+  hipLaunchKernelGGL(mcp::k_bound_match, dim3(ninst), dim3(kNT), lds_eig(n), c->stream, n, yo, yi, dn, dn + 3 * ninst,
+                     dn + 6 * ninst, de, dn + 7 * ninst, de + ninst, de + ninst + (size_t)ninst * n);
+  HIPCHK(c, hipGetLastError());
+  std::vector<int> tmp(8 * (size_t)ninst);
+  HIPCHK(c, hipMemcpyAsync(tmp.data(), dn, 8 * ninst * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(eigmin, de, ninst * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (psimch)
+    HIPCHK(c, hipMemcpyAsync(psimch, de + ninst, (size_t)ninst * n * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+  if (evals)
+    HIPCHK(c, hipMemcpyAsync(evals, de + ninst + (size_t)ninst * n, (size_t)ninst * n * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < ninst; ++i) {
+    nodec[i] = tmp[6 * ninst + i];
+    imin[i] = tmp[7 * ninst + i];
+    status[i] = 0;
+    if (tmp[ninst + i] != 0) status[i] = -tmp[ninst + i];                // outward inversion failure at step
+    if (tmp[4 * ninst + i] != 0) status[i] = -(100000 + tmp[4 * ninst + i]);  // inward
+    if (tmp[2 * ninst + i] != 0 || tmp[5 * ninst + i] != 0) status[i] = -1000000;
+  }
+  return MCP_OK;
 }
 
 int mcp_set_team_size(mcp_ctx* c, int32_t g) {

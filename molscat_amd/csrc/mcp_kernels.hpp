@@ -535,6 +535,80 @@ __global__ void __launch_bounds__(kNT) k_eigh(int n, const double* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// BDCTRL matching (source_code/bdctrl.f:221-257): M = Y_out - Y_in (the
+// outward and inward log-derivative matrices at RMATCH), DIAGVC of M
+// (ascending), NEGCNT = #{eig <= 0}, EIGMIN = the eigenvalue of smallest
+// modulus (the last such in ascending order, as the reference's scan keeps
+// ties), PSIMCH = its eigenvector with the largest component (the first such)
+// made positive, NODEC = NODES_out + NODES_in + NEGCNT.  One workgroup per
+// instance; nodes_out/nodes_in may be the same array offset by ninst.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kNT) k_bound_match(int n, const double* __restrict__ yout,
+                                                     const double* __restrict__ yin, const int* __restrict__ nodes_out,
+                                                     const int* __restrict__ nodes_in, int* __restrict__ nodec,
+                                                     double* __restrict__ eigmin, int* __restrict__ imin,
+                                                     double* __restrict__ psimch, double* __restrict__ evals) {
+  extern __shared__ double smem[];
+  const int lda = n;
+  const int m = blockIdx.x;
+  LdsCarve c(smem);
+  double* A = c.take<double>((size_t)n * lda);
+  double* V = c.take<double>((size_t)n * lda);
+  const int half = (n + 1) / 2 + 1;
+  double* rc = c.take<double>(half);
+  double* rs = c.take<double>(half);
+  int* rp = c.take<int>(half);
+  int* rq = c.take<int>(half);
+  double* sv = c.take<double>(kNT / 64);
+  double* ev = c.take<double>(n);
+  int* order = c.take<int>(n);
+  double* tmp = c.take<double>(n);
+  __shared__ int s_imin, s_neg, s_big;
+  const double* yo = yout + (size_t)m * n * n;
+  const double* yi = yin + (size_t)m * n * n;
+  for (int e = threadIdx.x; e < n * n; e += kNT) {
+    const int j = e / n, i = e - j * n;
+    const int aa = i > j ? i : j, bb = i > j ? j : i;
+    const size_t src = aa + (size_t)bb * n;
+    A[i + j * lda] = yo[src] - yi[src];
+  }
+  __syncthreads();
+  jacobi_eigh<kNT>(A, V, n, lda, rc, rs, rp, rq, sv);
+  sort_eig<kNT>(A, V, n, lda, ev, order, tmp);
+  if (threadIdx.x == 0) {
+    int neg = 0, im = 0;
+    double em = 1.0e30;
+    for (int i = 0; i < n; ++i) {
+      if (ev[i] <= 0.0) ++neg;
+      if (fabs(ev[i]) > fabs(em)) continue;
+      em = ev[i];
+      im = i;
+    }
+    int big = 0;
+    double bv = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const double v = V[i + im * lda];
+      if (fabs(v) > fabs(bv)) {
+        bv = v;
+        big = i;
+      }
+    }
+    s_imin = im;
+    s_neg = neg;
+    s_big = big;
+    eigmin[m] = em;
+    imin[m] = im;
+    nodec[m] = nodes_out[m] + nodes_in[m] + neg;
+  }
+  __syncthreads();
+  const double sgn = V[s_big + s_imin * lda] < 0.0 ? -1.0 : 1.0;
+  for (int i = threadIdx.x; i < n; i += kNT) {
+    if (psimch) psimch[(size_t)m * n + i] = sgn * V[i + s_imin * lda];
+    if (evals) evals[(size_t)m * n + i] = ev[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // YINIT (source_code/yinit.f:70-150) with ADIAMN/WKBMN: W(R) diagonalised,
 // Y = V diag(y) V^T with y = +sqrt(w) (closed, outward) / -sqrt(w) (closed,
 // inward) / BCYO (open).  One workgroup per instance; grid index k0.

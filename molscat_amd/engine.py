@@ -214,6 +214,24 @@ class Engine:
         return dict(K=_unpack_square(k, nopen), SR=_unpack_square(sr, nopen), SI=_unpack_square(si, nopen),
                     NODES=nodes, STATUS=st)
 
+    def bound_match(self, b: BlockHandle, g_out: GridHandle, g_in: GridHandle, ered, iprop_out: int = 6,
+                    iprop_in: int = 6, pconst=None):
+        """BOUND matching for ninst = len(ered) energies: dict with NODEC,
+        EIGMIN, IMIN (0-based), PSIMCH (m, N), EVALS (m, N), STATUS."""
+        ered = np.ascontiguousarray(np.atleast_1d(ered), dtype=np.float64)
+        m = len(ered)
+        nodec = np.empty(m, dtype=np.int32)
+        eigmin = np.empty(m)
+        imin = np.empty(m, dtype=np.int32)
+        psi = np.empty((m, b.n))
+        ev = np.empty((m, b.n))
+        st = np.empty(m, dtype=np.int32)
+        pc = None if pconst is None else np.ascontiguousarray(pconst, dtype=np.float64)
+        self._check(self.lib.mcp_bound_match(self.ctx, b.id, g_out.id, g_in.id, iprop_out, iprop_in, m, dptr(ered),
+                                             dptr(pc), iptr(nodec), dptr(eigmin), iptr(imin), dptr(psi), dptr(ev),
+                                             iptr(st)), "mcp_bound_match")
+        return dict(NODEC=nodec, EIGMIN=eigmin, IMIN=imin, PSIMCH=psi, EVALS=ev, STATUS=st)
+
     def set_team_size(self, g: int):
         """Workgroups per instance on the large-N path (0 = automatic)."""
         self._check(self.lib.mcp_set_team_size(self.ctx, int(g)), "mcp_set_team_size")

@@ -232,6 +232,17 @@ def drset_constant(rstart: float, rstop: float, drfit: float) -> Tuple[int, floa
     return nstep, prange / nstep
 
 
+def bound_segments(rmin: float, rmatch: float, rmax: float, drs: float, drl: Optional[float] = None):
+    """The two log-derivative segments of BDCTRL when RMID = RMATCH
+    (source_code/bd.prpini.f:24-62, bdpset.f:26-42): outward RMIN -> RMATCH
+    with DRS and inward RMAX -> RMATCH with DRL (DRS when unset), each
+    through DRSET with STEP = -10 (source_code/bdctrl.f:110-111).  Returns
+    ((RSTART, DR, NSTEP) outward, (RSTART, DR < 0, NSTEP) inward)."""
+    nout, dro = drset_constant(rmin, rmatch, drs)
+    nin, dri = drset_constant(rmax, rmatch, drl if drl is not None else drs)
+    return (rmin, dro, nout), (rmax, -dri, nin)
+
+
 def gtwvec(ered: float, eint: np.ndarray) -> Tuple[np.ndarray, int]:
     """GTWVEC: source_code/gtwvec.f:18-30."""
     dif = ered - eint

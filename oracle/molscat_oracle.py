@@ -1225,3 +1225,61 @@ def smatrix_ld(blk: Block, ered: float, grid: Grid, iprop: int = 6):
         sr, si = ktos(k)
         out.update(K=k, SR=sr, SI=si)
     return out
+
+
+# ---------------------------------------------------------------------------
+# BOUND matching: source_code/bdctrl.f:76-257 with the two-segment layout of
+# source_code/bd.prpini.f (RMID = RMATCH) and source_code/bdpset.f:26-42
+# ---------------------------------------------------------------------------
+@dataclass
+class BoundGrid:
+    rmin: float
+    rmatch: float
+    rmax: float
+    nout: int
+    drout: float
+    nin: int
+    drin: float
+
+
+def bound_grid(rmin: float, rmatch: float, rmax: float, drs: float, drl: float = UNSET) -> BoundGrid:
+    """Segments of BDCTRL: outward RMIN -> RMATCH with DRS, inward RMAX ->
+    RMATCH with DRL (DRS when unset), each through DRSET with STEP = -10
+    (source_code/bdctrl.f:110-111, bd.prpini.f:130-150)."""
+    if drl == UNSET:
+        drl = drs
+    nout, drout = drset(rmin, rmatch, -10.0, 0.0, 0.0, drs, 6, 0.0)
+    nin, drin = drset(rmax, rmatch, -10.0, 0.0, 0.0, drl, 6, 0.0)
+    return BoundGrid(rmin, rmatch, rmax, nout, drout, nin, drin)
+
+
+def bound_match(blk: Block, ered: float, g: BoundGrid, iprop_out: int = 6, iprop_in: int = 6):
+    """BDCTRL for one energy: YINIT + propagation outward to RMATCH and
+    inward to RMATCH, matching matrix M = Y_out - Y_in, its eigenvalues
+    (DIAGVC, ascending), NEGCNT = #{eig <= 0}, EIGMIN = the eigenvalue of
+    smallest modulus (the last such in ascending order), PSIMCH = its
+    eigenvector with the largest component made positive, NODEC = nodes of
+    both propagations + NEGCNT (source_code/bdctrl.f:221-257).  Returns a
+    dict with NODEC, EIGMIN, IMIN (0-based), PSIMCH, EVALS, M."""
+    hm = lambda r: blk.hammat(r, ered)
+    props = {6: mdprop, 5: ldprop}
+    y_out = yinit(blk, ered, g.rmin, zout=True)
+    y_out, n_out = props[iprop_out](y_out, hm, g.rmin, g.nout, g.drout)
+    y_in = yinit(blk, ered, g.rmax, zout=False)
+    y_in, n_in = props[iprop_in](y_in, hm, g.rmax, g.nin, g.drin)
+    m = y_out - y_in
+    ev, vec = np.linalg.eigh(m)
+    negcnt = int(np.count_nonzero(ev <= 0.0))
+    eigmin, imin = 1.0e30, 0
+    for i in range(len(ev)):
+        if abs(ev[i]) > abs(eigmin):
+            continue
+        eigmin, imin = float(ev[i]), i
+    psi = vec[:, imin].copy()
+    big = 0.0
+    for v in psi:
+        if abs(v) > abs(big):
+            big = v
+    psi *= math.copysign(1.0, big)
+    return dict(NODEC=n_out + n_in + negcnt, NODES_OUT=n_out, NODES_IN=n_in, NEGCNT=negcnt, EIGMIN=eigmin,
+                IMIN=imin, PSIMCH=psi, EVALS=ev, M=m)

@@ -1,0 +1,110 @@
+"""GPU parity of the BOUND matching (mcp_bound_match) and of the batched
+level search built on it, through the C ABI.
+
+System: the reference's bound-all_propagators deck (ITYPE=1 rotor, BE=30,
+j=0,2,4,6, URED=20, IB=2, RMIN=0.5, RMATCH=1.0, RMAX=20 in units of RM,
+DRS=0.001, DRL=0.005 for MDPROP, 0.001 for LDPROP).  Checks: node counts
+bit-exact against the oracle and the reference; EIGMIN against the oracle
+to 1e-9 of the largest eigenvalue modulus of M; the reference's converged
+level energies as zeros of the GPU's EIGMIN to 1e-10 cm-1; the level
+search reproducing them to 1e-9 cm-1; batch independence (bitwise)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from molscat_amd import bound as BD
+from molscat_amd import system as S
+from oracle import molscat_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+POT_TERMS = [[(1.0, -12), (-2.0, -6)], [(-0.2, -6)]]
+
+
+@pytest.fixture(scope="module")
+def eng():
+    from molscat_amd.engine import Engine
+
+    e = Engine(0)
+    yield e
+    e.close()
+
+
+@pytest.fixture(scope="module")
+def ref(golden_dir):
+    with open(os.path.join(golden_dir, "ref_bound_all_propagators.json")) as f:
+        return json.load(f)["decks"]
+
+
+def _setup(eng, iprop, jtot):
+    pot = S.PowerPotential(POT_TERMS, rm=3.5, epsil=50.0)
+    blk = S.rotor_block(jtot, 2, [0, 2, 4, 6], 30.0, [0, 2], pot, 20.0)
+    (r0, dro, no), (r1, dri, ni) = S.bound_segments(0.5, 1.0, 20.0, 0.001, 0.005 if iprop == 6 else None)
+    ro = S.ld_radii(r0, dro, no)
+    ri = S.ld_radii(r1, dri, ni)
+    b = eng.block(blk.vl, blk.eint, blk.cent, mxlam=blk.mxlam)
+    go = eng.grid(ro, blk.ptab(ro), dro)
+    gi = eng.grid(ri, blk.ptab(ri), dri)
+    opot = O.PowerPotential(pot.terms, rm=pot.rm, epsil=pot.epsil)
+    ob = O.Block(n=blk.n, vl=blk.vl, eint=blk.eint, cent=blk.cent, lvals=blk.lvals, levidx=blk.levidx, potl=opot,
+                 rscale=blk.rscale, ep2ru=blk.ep2ru, cm2ru=blk.cm2ru)
+    og = O.BoundGrid(0.5, 1.0, 20.0, no, dro, ni, dri)
+    return blk, b, go, gi, ob, og
+
+
+def _block(ref, iprop, jtot):
+    return next(x for x in ref[str(iprop)]["blocks"] if x["jtot"] == jtot)
+
+
+@pytest.mark.parametrize("iprop,jtot", [(6, 0), (6, 1), (6, 2), (5, 0), (5, 2)])
+def test_bound_match_vs_oracle_and_reference(eng, ref, iprop, jtot):
+    blk, b, go, gi, ob, og = _setup(eng, iprop, jtot)
+    rb = _block(ref, iprop, jtot)
+    es = [rb["emin"], rb["emax"]] + [st["energy"] for st in rb["states"]]
+    if iprop == 6:  # the LDPROP oracle takes 19 500 steps per energy: keep its list short
+        es += list(np.linspace(-9.5, -1.5, 5))
+    out = eng.bound_match(b, go, gi, np.array(es) * blk.cm2ru, iprop, iprop)
+    assert np.all(out["STATUS"] == 0), out["STATUS"]
+    assert out["NODEC"][0] == rb["nodes_at_emin"] and out["NODEC"][1] == rb["nodes_at_emax"]
+    for k, e in enumerate(es):
+        o = O.bound_match(ob, e * blk.cm2ru, og, iprop, iprop)
+        assert int(out["NODEC"][k]) == o["NODEC"], (e, out["NODEC"][k], o["NODEC"])
+        scale = max(1.0, float(np.abs(o["EVALS"]).max()))
+        assert abs(out["EIGMIN"][k] - o["EIGMIN"]) < 1e-9 * scale, (e, out["EIGMIN"][k], o["EIGMIN"])
+        assert int(out["IMIN"][k]) == o["IMIN"]
+        assert np.abs(out["PSIMCH"][k] - o["PSIMCH"]).max() < 1e-8
+        assert np.abs(out["EVALS"][k] - o["EVALS"]).max() < 1e-9 * scale
+
+
+@pytest.mark.parametrize("iprop", [6, 5])
+def test_reference_levels_are_zeros_of_gpu_eigmin(eng, ref, iprop):
+    blk, b, go, gi, _, _ = _setup(eng, iprop, 0)
+    for st in _block(ref, iprop, 0)["states"]:
+        e = st["energy"]
+        out = eng.bound_match(b, go, gi, np.array([e, e + 1e-5]) * blk.cm2ru, iprop, iprop)
+        slope = (out["EIGMIN"][1] - out["EIGMIN"][0]) / 1e-5
+        assert abs(out["EIGMIN"][0] / slope) < 1e-10, (st["state"], out["EIGMIN"][0], slope)
+
+
+def test_level_search_reproduces_reference(eng, ref):
+    for jtot in (0, 1, 2):
+        blk, b, go, gi, _, _ = _setup(eng, 6, jtot)
+        rb = _block(ref, 6, jtot)
+        levels = BD.find_levels(eng, b, go, gi, blk.cm2ru, rb["emin"], rb["emax"])
+        assert [lv["STATE"] for lv in levels] == [st["state"] for st in rb["states"]]
+        for lv, st in zip(levels, rb["states"]):
+            assert abs(lv["ENERGY"] - st["energy"]) < 1e-9, (jtot, lv, st["energy"])
+            assert lv["NODES_EMIN"] == rb["nodes_at_emin"] and lv["NODES_EMAX"] == rb["nodes_at_emax"]
+
+
+def test_batch_members_are_independent(eng):
+    blk, b, go, gi, _, _ = _setup(eng, 6, 1)
+    es = np.linspace(-9.0, -1.2, 64) * blk.cm2ru
+    all_ = eng.bound_match(b, go, gi, es)
+    for k in (0, 17, 63):
+        one = eng.bound_match(b, go, gi, es[k:k + 1])
+        assert one["NODEC"][0] == all_["NODEC"][k]
+        assert one["EIGMIN"][0] == all_["EIGMIN"][k]
+        assert np.array_equal(one["PSIMCH"][0], all_["PSIMCH"][k])
