@@ -28,6 +28,12 @@ extern "C" {
 #define MCP_ERR_UNSUPPORTED -3
 #define MCP_ERR_NOMEM -4
 
+/* Propagator code for ODPROP (source_code/odprop.f), the single-channel
+ * LDMD propagator, which has no IPROP value of its own in the reference:
+ * SCCTRL selects it for N = 1 under IPROP = 6 (source_code/scctrl.f:183-196),
+ * as mcp_smatrix_ld does. */
+#define MCP_PROP_ODPROP 106
+
 typedef struct mcp_ctx mcp_ctx;
 
 /* Context on HIP device `device`.  No reference counterpart (the reference
@@ -82,7 +88,9 @@ int mcp_yinit(mcp_ctx *ctx, int32_t block_id, int32_t grid_id, int32_t k0, int32
 
 /* Log-derivative propagation over the whole grid for ninst instances of one
  * block: iprop = 6 replaces MDPROP (source_code/mdprop.f:1-4), iprop = 5
- * replaces LDPROP (source_code/ldprop.f:1-4).  y holds the starting log-
+ * replaces LDPROP (source_code/ldprop.f:1-4), iprop = MCP_PROP_ODPROP
+ * replaces ODPROP (source_code/odprop.f:1-4; N = 1, NRSQ = 0; it takes the
+ * sector ends and midpoints from the LD grid).  y holds the starting log-
  * derivative matrices on entry and Y(RSTOP) on exit; nodes = NODES; status =
  * 0 or the 1-based step at which an inversion failed ("MATRIX INVERSION
  * ERROR AT STEP K", mdprop.f:282).  pconst: ninst x nconst or NULL. */

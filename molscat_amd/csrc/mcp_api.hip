@@ -276,7 +276,8 @@ int prep_team(mcp_ctx* c, int ninst) {
 // Device-side propagation of ninst instances whose Y already sits in dy.
 int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const double* d_ered, const double* d_pconst,
                    double* dy, int* d_nodes, int* d_status) {
-  if (iprop != 5 && iprop != 6) return c->fail(MCP_ERR_ARG, "iprop %d: only 5 (LDPROP) and 6 (MDPROP)", iprop);
+  if (iprop != 5 && iprop != 6 && iprop != MCP_PROP_ODPROP)
+    return c->fail(MCP_ERR_ARG, "iprop %d: only 5 (LDPROP), 6 (MDPROP) and %d (ODPROP)", iprop, MCP_PROP_ODPROP);
   if ((g.npts - 1) % 2 != 0 || g.npts < 3) return c->fail(MCP_ERR_ARG, "LD grid needs 2*NSTEP+1 points");
   const int nstep = (g.npts - 1) / 2;
   const int n = b.n;
@@ -292,7 +293,13 @@ int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const do
   I.nodes = d_nodes;
   I.status = d_status;
   HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-  if (iprop == 6) {
+  if (iprop == MCP_PROP_ODPROP) {
+    if (n != 1) return c->fail(MCP_ERR_ARG, "ODPROP needs N = 1 (N = %d)", n);
+    if (b.nrsq != 0) return c->fail(MCP_ERR_UNSUPPORTED, "ODPROP with NRSQ > 0 (the reference leaves those P unset)");
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    hipLaunchKernelGGL(mcp::k_odprop, dim3((ninst + 63) / 64), dim3(64), 0, c->stream, B, G, I, ninst, nstep, g.dr);
+    HIPCHK(c, hipGetLastError());
+  } else if (iprop == 6) {
     // energy-independent sector matrices (shared by all instances)
     HIPCHK(c, c->s_e.ensure((size_t)nstep * n * n * sizeof(double)));
     HIPCHK(c, c->s_f.ensure((size_t)nstep * sizeof(int)));
@@ -732,7 +739,9 @@ int mcp_smatrix_ld(mcp_ctx* c, int32_t bid, int32_t gid, int32_t iprop, int32_t 
   I.status = nullptr;
   int* dn = c->s_j.as<int>();
   if ((rc = yinit_device(c, *b, *g, I, ninst, 0, 1, 0.0, dn + 2 * ninst))) return rc;
-  if ((rc = prop_ld_device(c, *b, *g, iprop, ninst, I.ered, I.pconst, I.y, dn, dn + ninst))) return rc;
+  // SCCTRL runs ODPROP for a single channel (source_code/scctrl.f:183-196)
+  const int ip = (iprop == 6 && n == 1) ? MCP_PROP_ODPROP : iprop;
+  if ((rc = prop_ld_device(c, *b, *g, ip, ninst, I.ered, I.pconst, I.y, dn, dn + ninst))) return rc;
   if (!(rval > 0.0)) return c->fail(MCP_ERR_ARG, "smatrix_ld: matching radius %g must be positive", rval);
   if ((rc = match_device(c, n, ninst, I.y, wvec, nopen, nb, b->cent.as<double>(), rval))) return rc;
   std::vector<int> tmp(3 * (size_t)ninst);

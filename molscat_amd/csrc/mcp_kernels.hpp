@@ -535,6 +535,61 @@ __global__ void __launch_bounds__(kNT) k_eigh(int n, const double* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// ODPROP (source_code/odprop.f): the single-channel LDMD propagator SCCTRL
+// runs for N = 1 (source_code/scctrl.f:183-196).  One thread per instance.
+// U(R) = W(R) + ERED at the sector ends r[2i], W at the midpoints r[2i+1] of
+// the LD grid; Q_i = U_i - W_{i-1}; per sector the reference-potential
+// terms of the midpoint and Y <- Y1 - Y2^2 / (Y + Y1 + (U_i - W_i + Q_i) DR/6);
+// Y(RSTOP) = Y + Q_NPT DR/6.  NODES = 0, as in the reference.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double od_u(const BlockArgs& B, const GridArgs& G, int k, const double* pconst) {
+  const double r = G.r[k];
+  const double rsq = rsq_of(r);
+  const double* v = B.vl;  // N = 1: one packed element
+  double s = 0.0;
+  for (int l = 0; l < B.nham; ++l) {
+    const double pl = l < B.mxlam ? G.ptab[(size_t)k * B.mxlam + l] : (l < B.mxlam + B.nconst ? pconst[l - B.mxlam] : rsq);
+    s += v[l] * pl;
+  }
+  return hammat_diag(B, 0, s, r, rsq, 0.0);
+}
+
+__global__ void k_odprop(BlockArgs B, GridArgs G, InstArgs I, int ninst, int nstep, double dr) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= ninst) return;
+  const double* pconst = I.pconst ? I.pconst + (size_t)m * B.nconst : nullptr;
+  const double ered = I.ered[m];
+  const double dr6 = dr / 6.0;
+  double y = I.y[m];
+  double wprev = 0.0;
+  for (int i = 0; i < nstep; ++i) {
+    const double ui = od_u(B, G, 2 * i, pconst);
+    const double wi = od_u(B, G, 2 * i + 1, pconst);
+    const double qi = i == 0 ? 0.0 : ui - wprev;
+    const double uu = (ui - wi + qi) * dr6;
+    const double wref = wi - ered;
+    const double flam = 0.5 * sqrt(fabs(wref));
+    double y1, y2;
+    if (wref < 0.0) {
+      const double tn = tan(flam * dr);
+      y1 = flam / tn - flam * tn;
+      y2 = flam / tn + flam * tn;
+    } else {
+      const double tn = tanh(flam * dr);
+      y1 = flam / tn + flam * tn;
+      y2 = flam / tn - flam * tn;
+    }
+    y2 = y2 * y2;
+    y = y1 - y2 / (y + (y1 + uu));
+    wprev = wi;
+  }
+  const double qlast = (od_u(B, G, 2 * nstep, pconst) - wprev) * dr6;
+  I.y[m] = y + qlast;
+  I.nodes[m] = 0;
+  I.status[m] = 0;
+}
+
+// ---------------------------------------------------------------------------
 // BDCTRL matching (source_code/bdctrl.f:221-257): M = Y_out - Y_in (the
 // outward and inward log-derivative matrices at RMATCH), DIAGVC of M
 // (ascending), NEGCNT = #{eig <= 0}, EIGMIN = the eigenvalue of smallest

@@ -194,3 +194,42 @@ def test_yinit_and_match_vs_oracle(eng):
     assert st[0] == 0
     assert np.abs(k[0] - ok).max() < 1e-12 * max(1.0, np.abs(ok).max())
     assert np.abs(sr[0] - osr).max() < 1e-12 and np.abs(si[0] - osi).max() < 1e-12
+
+
+def _single_channel_block():
+    # JTOT = 0, j = 0 only: the single channel (j=0, L=0) of parity block 2
+    pot = S.PowerPotential([[(1.0, -12), (-2.0, -6)], [(-0.2, -6)]], rm=3.5, epsil=50.0)
+    return S.rotor_block(0, 2, [0], 30.0, [0, 2], pot, 20.0)
+
+
+def test_odprop_single_channel_vs_oracle(eng):
+    """N = 1: mcp_smatrix_ld follows SCCTRL and runs ODPROP; mcp_prop_ld with
+    iprop 6 runs MDPROP.  Both against the oracle; tolerance 1e-10 on S."""
+    blk = _single_channel_block()
+    assert blk.n == 1
+    rstart, rstop, nstep = 0.64, 20.0, 4000
+    dr = (rstop - rstart) / nstep
+    r = S.ld_radii(rstart, dr, nstep)
+    b = eng.block(blk.vl, blk.eint, blk.cent, mxlam=blk.mxlam)
+    g = eng.grid(r, blk.ptab(r), dr)
+    energies = [50.0, 400.0, 900.0]  # k*DR/2 <= 0.57
+    ered = np.array(energies) * blk.cm2ru
+    nb = S.chnsrt(blk.eint)
+    wv, nops = zip(*[S.gtwvec(e, blk.eint) for e in ered])
+    print(f"ODPROP N=1 nstep={nstep} energies={energies}", flush=True)
+    out = eng.smatrix_ld(b, g, 6, ered, np.array(wv), np.array(nops), nb, rstop)
+    ob = _oracle_block(blk)
+    grid = O.Grid(rstart, rstop, nstep, dr)
+    for m, e in enumerate(ered):
+        ref = O.smatrix_ld(ob, e, grid, iprop=6)  # the oracle also dispatches N=1 to ODPROP
+        assert out["STATUS"][m] == 0 and int(out["NODES"][m]) == 0
+        assert abs(out["SR"][m][0, 0] - ref["SR"][0, 0]) < 1e-10
+        assert abs(out["SI"][m][0, 0] - ref["SI"][0, 0]) < 1e-10
+    # MDPROP at N = 1 through mcp_prop_ld
+    y0 = np.stack([O.yinit(ob, e, rstart) for e in ered])
+    y, nodes, st = eng.prop_ld(b, g, 6, ered, y0)
+    for m, e in enumerate(ered):
+        yr, nr = O.mdprop(y0[m], lambda rr: ob.hammat(rr, e), rstart, nstep, dr)
+        print(f"  MDPROP N=1 E={energies[m]}: nodes GPU {int(nodes[m])} oracle {nr}", flush=True)
+        assert st[m] == 0 and int(nodes[m]) == nr
+        assert abs(y[m][0, 0] - yr[0, 0]) < 1e-10 * max(1.0, abs(yr[0, 0]))

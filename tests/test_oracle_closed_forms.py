@@ -113,3 +113,31 @@ def test_hammat_dense_definition():
     assert np.array_equal(d, np.diag(w))
     # isotropic term is diagonal with coefficient 1 (Percival-Seaton, lambda=0)
     assert np.allclose(blk.vl_full[0], np.eye(blk.n), atol=1e-14)
+
+
+def _single_channel(c0, cent):
+    pot = O.PowerPotential([[(1.0, 0, 0.0)]])  # P(R) = 1
+    return O.Block(n=1, vl=np.array([[c0]]), eint=np.zeros(1), cent=np.array([cent]), lvals=np.zeros(1, dtype=int),
+                   levidx=np.zeros(1, dtype=int), potl=pot, rscale=1.0, ep2ru=1.0, cm2ru=1.0)
+
+
+def test_odprop_constant_potential_is_exact():
+    # ODPROP's reference potential is the midpoint W: exact for a constant W
+    k, rs, re, nstep = 2.3, 0.4, 3.1, 400
+    blk = _single_channel(-k * k, 0.0)
+    y, nodes = O.odprop(k / math.tan(k * rs), blk, 0.0, rs, nstep, (re - rs) / nstep)
+    exact = k / math.tan(k * re)
+    assert abs(y - exact) < 1e-10 * max(1.0, abs(exact))
+    assert nodes == 0  # ODPROP does not count nodes (source_code/odprop.f)
+
+
+def test_odprop_free_particle_converges_to_riccati_bessel():
+    L, k, rs, re = 3, 1.7, 0.05, 6.0
+    blk = _single_channel(-k * k, float(L * (L + 1)))
+    uj, ujp, _, _ = O.rbesjy(float(L), k * re)
+    errs = []
+    for nstep in (200, 400):
+        y, _ = O.odprop(math.sqrt(L * (L + 1) / rs ** 2 - k * k), blk, 0.0, rs, nstep, (re - rs) / nstep)
+        errs.append(abs(y - k * ujp / uj))
+    assert errs[1] < errs[0] / 10.0
+    assert errs[1] < 1e-4
