@@ -76,6 +76,16 @@ int mcp_grid_create(mcp_ctx *ctx, int32_t mxlam, int32_t npts, const double *r, 
                     int32_t *grid_id);
 int mcp_grid_destroy(mcp_ctx *ctx, int32_t grid_id);
 
+/* Upload an AIRY grid (AIPROP with TOLHI = 0, source_code/aiprop.f:86-327):
+ * nstep steps of widths drnow[k] (one sign: positive outward, negative
+ * inward), radii r = {RSTART, RA_0, RB_0, RA_1, RB_1, ...} (2*nstep+1) where
+ * RA_k, RB_k = RNOW_k -+ DRNOW_k/(2*1.732050807) are POTENT's Gauss points
+ * (source_code/potent.f:66-67) and RNOW_k the midpoints accumulated as
+ * AIPROP does; ptab holds P(lambda, R) at those radii as for
+ * mcp_grid_create. */
+int mcp_grid_create_airy(mcp_ctx *ctx, int32_t mxlam, int32_t nstep, const double *r, const double *drnow,
+                         const double *ptab, int32_t *grid_id);
+
 /* HAMMAT (source_code/hammat.f:1-116) at nr grid points idx[] and energy
  * ered; pconst = the NCONST constants (x EP2RU) or NULL.  w: nr full N x N. */
 int mcp_hammat(mcp_ctx *ctx, int32_t block_id, int32_t grid_id, int32_t nr, const int32_t *idx,
@@ -97,6 +107,14 @@ int mcp_yinit(mcp_ctx *ctx, int32_t block_id, int32_t grid_id, int32_t k0, int32
 int mcp_prop_ld(mcp_ctx *ctx, int32_t block_id, int32_t grid_id, int32_t iprop, int32_t ninst, const double *ered,
                 const double *pconst, double *y, int32_t *nodes, int32_t *status);
 
+/* AIPROP (source_code/aiprop.f:1-4, TOLHI = 0, NCONST = 0) over an AIRY grid
+ * for ninst energies of one block: y holds Z (the log-derivative matrix) at
+ * the grid's RSTART on entry and at its end on exit; nodes = NODES (KOUNT
+ * summed outward, N - KOUNT inward); status = 0 or the 1-based step of an
+ * inversion failure ("INSTABILITY IN SYMINV IN AIPROP", aiprop.f:212). */
+int mcp_prop_airy(mcp_ctx *ctx, int32_t block_id, int32_t grid_id, int32_t ninst, const double *ered, double *y,
+                  int32_t *nodes, int32_t *status);
+
 /* YTOK + KTOS (source_code/ytok.f:1, ytokg.f:1, ktos.f:1) for ninst
  * instances: y (N x N), wvec (N, GTWVEC's signed wave vectors), nopen, nb
  * (N, 0-based CHNSRT order), cent (N), rval = RSTOP.  Writes kmat, sr, si as
@@ -117,6 +135,18 @@ int mcp_smatrix_ld(mcp_ctx *ctx, int32_t block_id, int32_t grid_id, int32_t ipro
                    const double *ered, const double *pconst, const double *wvec, const int32_t *nopen,
                    const int32_t *nb, double rval, double *kmat, double *sr, double *si, int32_t *nodes,
                    int32_t *status);
+
+/* SCCTRL with up to 8 segments (source_code/scctrl.f:80-289): YINIT at the
+ * first grid's start, then each segment's propagator (iprops[k] = 5, 6 or
+ * 9; 6 with N = 1 runs ODPROP, scctrl.f:183-196) on grid_ids[k], and YTOK +
+ * KTOS at rval (the last segment's nominal RSTOP).  Outputs as for
+ * mcp_smatrix_ld; nodes = NODES summed over the segments; status < 0:
+ * -(100000*k + step) for a failure in segment k (0-based), -1000000 for an
+ * open channel at the start radius. */
+int mcp_smatrix_segments(mcp_ctx *ctx, int32_t block_id, int32_t nseg, const int32_t *grid_ids,
+                         const int32_t *iprops, int32_t ninst, const double *ered, const double *pconst,
+                         const double *wvec, const int32_t *nopen, const int32_t *nb, double rval, double *kmat,
+                         double *sr, double *si, int32_t *nodes, int32_t *status);
 
 /* BDCTRL matching (source_code/bdctrl.f:76-257) for ninst energies (or
  * fields, through pconst) of one block, two log-derivative segments as

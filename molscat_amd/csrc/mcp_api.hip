@@ -13,6 +13,7 @@
 #include "../../include/mcprop.h"
 #include "mcp_kernels.hpp"
 #include "mcp_large.hpp"
+#include "mcp_airyprop.hpp"
 
 namespace {
 
@@ -57,6 +58,10 @@ struct Grid {
   double dr = 0.0;
   DevBuf r, ptab;
   std::vector<double> h_r;
+  bool airy = false;  // AIRY grid: RSTART, then RA_k, RB_k per step; widths in w
+  int nstep = 0;
+  DevBuf w;
+  std::vector<double> h_w;
 };
 
 // LDS carve sizes, mirroring LdsCarve::take (16-byte rounded pieces).
@@ -112,6 +117,12 @@ size_t lds_yinit(int n, int nham) {
       .b;
 }
 
+size_t lds_airy_sector(int n, int nham) { return lds_eig(n) + LdsSize().add<double>(nham).add<double>(nham).b; }
+size_t lds_airy_prop(int n) {
+  return LdsSize().add<double>((size_t)n * n).add<double>((size_t)n * n).add<double>((size_t)n * n)
+      .add<double>(n).add<double>(n).add<double>(n).sweep(n).b;
+}
+
 size_t lds_large(int n, int nham, int nvec) {
   LdsSize z;
   // same pieces, in the same order, as mcp::carve_large
@@ -144,6 +155,7 @@ struct mcp_ctx {
   DevBuf s_bar;  // per-instance team barrier counters + abort flag
   int team = 0;  // workgroups per instance on the large-N path (0: automatic)
   DevBuf s_u;  // LDPROP sector-matrix workspace (large N)
+  DevBuf s_av, s_at, s_ae, s_ah;  // AIRY step bases, basis changes, eigenvalues, slopes
   double t_sector = 0.0, t_prop = 0.0, t_total = 0.0;
 
   int fail(int code, const char* fmt, ...) {
@@ -177,7 +189,8 @@ int set_lds_limits(mcp_ctx* c) {
                        (const void*)mcp::k_eigh,           (const void*)mcp::k_bound_match,           (const void*)mcp::k_yinit,
                        (const void*)mcp::k_syminv_large,   (const void*)mcp::k_mdprop_uc_large,
                        (const void*)mcp::k_mdprop_large,   (const void*)mcp::k_ldprop_large,
-                       (const void*)mcp::k_yinit_large};
+                       (const void*)mcp::k_yinit_large,    (const void*)mcp::k_airy_sector,
+                       (const void*)mcp::k_airy_prop};
   const size_t total = c->lds_max;
   size_t lim = total;
   for (const void* f : fns) {
@@ -355,6 +368,47 @@ int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const do
   return MCP_OK;
 }
 
+// AIRY propagation of ninst instances whose Z already sits in dy (TOLHI = 0).
+int airy_prop_device(mcp_ctx* c, Block& b, Grid& g, int ninst, const double* d_ered, double* dy, int* d_nodes,
+                     int* d_status) {
+  if (!g.airy) return c->fail(MCP_ERR_ARG, "AIRY propagation needs a grid from mcp_grid_create_airy");
+  if (b.nconst != 0)
+    return c->fail(MCP_ERR_UNSUPPORTED, "AIRY with NCONST > 0 (the local basis would depend on the instance)");
+  const int n = b.n, nstep = g.nstep;
+  const size_t ls = lds_airy_sector(n, b.nham), lp = lds_airy_prop(n);
+  if (ls > c->lds_max || lp > c->lds_max)
+    return c->fail(MCP_ERR_UNSUPPORTED, "AIRY: N=%d exceeds the LDS-resident AIRY limit", n);
+  const size_t nn = (size_t)n * n;
+  HIPCHK(c, c->s_av.ensure((size_t)nstep * nn * sizeof(double)));
+  HIPCHK(c, c->s_at.ensure((size_t)nstep * nn * sizeof(double)));
+  HIPCHK(c, c->s_ae.ensure((size_t)nstep * n * sizeof(double)));
+  HIPCHK(c, c->s_ah.ensure((size_t)nstep * n * sizeof(double)));
+  mcp::BlockArgs B = block_args(b);
+  mcp::GridArgs G = grid_args(g);
+  const double* w = g.w.as<double>();
+  HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+  hipLaunchKernelGGL(mcp::k_airy_sector, dim3(nstep), dim3(kNT), ls, c->stream, B, G, w, c->s_av.as<double>(),
+                     c->s_ae.as<double>(), c->s_ah.as<double>());
+  HIPCHK(c, hipGetLastError());
+  hipLaunchKernelGGL(mcp::k_airy_tmat, dim3(nstep), dim3(kNT), 0, c->stream, n, c->s_av.as<double>(),
+                     c->s_at.as<double>());
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  mcp::InstArgs I;
+  I.ered = d_ered;
+  I.pconst = nullptr;
+  I.y = dy;
+  I.nodes = d_nodes;
+  I.status = d_status;
+  const int outward = g.h_w[0] > 0.0 ? 1 : 0;
+  const double* vlast = c->s_av.as<double>() + (size_t)(nstep - 1) * nn;
+  hipLaunchKernelGGL(mcp::k_airy_prop, dim3(ninst), dim3(kNT), lp, c->stream, n, nstep, c->s_at.as<double>(),
+                     c->s_ae.as<double>(), c->s_ah.as<double>(), w, vlast, I, outward);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+  return MCP_OK;
+}
+
 int record_timing(mcp_ctx* c) {
   float a = 0.f, b = 0.f;
   HIPCHK(c, hipEventSynchronize(c->ev[2]));
@@ -520,6 +574,49 @@ int mcp_grid_create(mcp_ctx* c, int32_t mxlam, int32_t npts, const double* r, co
   c->grids[id] = std::move(g);
   *grid_id = id;
   return MCP_OK;
+}
+
+int mcp_grid_create_airy(mcp_ctx* c, int32_t mxlam, int32_t nstep, const double* r, const double* drnow,
+                         const double* ptab, int32_t* grid_id) {
+  if (!c || mxlam < 0 || nstep <= 0 || !r || !drnow || (mxlam > 0 && !ptab) || !grid_id)
+    return c ? c->fail(MCP_ERR_ARG, "grid_create_airy: bad args") : MCP_ERR_ARG;
+  for (int k = 0; k < nstep; ++k)
+    if (!(drnow[k] != 0.0) || (drnow[k] > 0.0) != (drnow[0] > 0.0))
+      return c->fail(MCP_ERR_ARG, "grid_create_airy: step widths must be nonzero and of one sign");
+  int32_t id = 0;
+  int rc;
+  if ((rc = mcp_grid_create(c, mxlam, 2 * nstep + 1, r, ptab, 0.0, &id))) return rc;
+  Grid& g = *c->grids[id];
+  g.airy = true;
+  g.nstep = nstep;
+  if ((rc = upload(c, g.w, drnow, (size_t)nstep))) return rc;
+  g.h_w.assign(drnow, drnow + nstep);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *grid_id = id;
+  return MCP_OK;
+}
+
+int mcp_prop_airy(mcp_ctx* c, int32_t bid, int32_t gid, int32_t ninst, const double* ered, double* y, int32_t* nodes,
+                  int32_t* status) {
+  if (!c || ninst < 0 || !ered || !y || !nodes || !status)
+    return c ? c->fail(MCP_ERR_ARG, "prop_airy: bad args") : MCP_ERR_ARG;
+  Block* b;
+  Grid* g;
+  int rc;
+  if ((rc = lookup(c, bid, gid, &b, &g))) return rc;
+  if (ninst == 0) return MCP_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t cnt = (size_t)ninst * b->n * b->n;
+  if ((rc = upload(c, c->s_g, ered, (size_t)ninst))) return rc;
+  if ((rc = upload(c, c->s_i, y, cnt))) return rc;
+  HIPCHK(c, c->s_j.ensure((size_t)ninst * 2 * sizeof(int)));
+  int* dn = c->s_j.as<int>();
+  if ((rc = airy_prop_device(c, *b, *g, ninst, c->s_g.as<double>(), c->s_i.as<double>(), dn, dn + ninst))) return rc;
+  if ((rc = download(c, y, c->s_i, cnt))) return rc;
+  HIPCHK(c, hipMemcpyAsync(nodes, dn, ninst * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(status, dn + ninst, ninst * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return record_timing(c);
 }
 
 int mcp_grid_destroy(mcp_ctx* c, int32_t id) {
@@ -753,6 +850,64 @@ int mcp_smatrix_ld(mcp_ctx* c, int32_t bid, int32_t gid, int32_t iprop, int32_t 
     if (tmp[2 * ninst + i] != 0) status[i] = -1000000;       // open channel at the start radius
   }
   return record_timing(c);
+}
+
+int mcp_smatrix_segments(mcp_ctx* c, int32_t bid, int32_t nseg, const int32_t* gids, const int32_t* iprops,
+                         int32_t ninst, const double* ered, const double* pconst, const double* wvec,
+                         const int32_t* nopen, const int32_t* nb, double rval, double* kmat, double* sr, double* si,
+                         int32_t* nodes, int32_t* status) {
+  if (!c || nseg <= 0 || nseg > 8 || !gids || !iprops || ninst < 0 || !ered || !wvec || !nopen || !nb || !sr || !si ||
+      !nodes || !status)
+    return c ? c->fail(MCP_ERR_ARG, "smatrix_segments: bad args") : MCP_ERR_ARG;
+  Block* b;
+  Grid* g[8];
+  int rc;
+  for (int k = 0; k < nseg; ++k)
+    if ((rc = lookup(c, bid, gids[k], &b, &g[k]))) return rc;
+  if (ninst == 0) return MCP_OK;
+  const int n = b->n;
+  for (int i = 0; i < ninst; ++i)
+    if (nopen[i] < 0 || nopen[i] > n) return c->fail(MCP_ERR_ARG, "smatrix_segments: nopen[%d] out of range", i);
+  for (size_t e = 0; e < (size_t)ninst * n; ++e)
+    if (nb[e] < 0 || nb[e] >= n) return c->fail(MCP_ERR_ARG, "smatrix_segments: nb out of range");
+  if (!(rval > 0.0)) return c->fail(MCP_ERR_ARG, "smatrix_segments: matching radius %g must be positive", rval);
+  HIPCHK(c, hipSetDevice(c->device));
+  if ((rc = upload_inst(c, *b, ninst, ered, pconst))) return rc;
+  HIPCHK(c, c->s_i.ensure((size_t)ninst * n * n * sizeof(double)));
+  HIPCHK(c, c->s_j.ensure((size_t)ninst * (2 * nseg + 1) * sizeof(int)));
+  mcp::InstArgs I;
+  I.ered = c->s_g.as<double>();
+  I.pconst = b->nconst > 0 ? c->s_h.as<double>() : nullptr;
+  I.y = c->s_i.as<double>();
+  I.nodes = nullptr;
+  I.status = nullptr;
+  int* dn = c->s_j.as<int>();  // [yinit | nodes, status per segment]
+  // YINIT on the first segment only (source_code/scctrl.f:116-121)
+  if ((rc = yinit_device(c, *b, *g[0], I, ninst, 0, 1, 0.0, dn))) return rc;
+  for (int k = 0; k < nseg; ++k) {
+    int* sn = dn + ninst * (1 + 2 * k);
+    if (iprops[k] == 9) {
+      if ((rc = airy_prop_device(c, *b, *g[k], ninst, I.ered, I.y, sn, sn + ninst))) return rc;
+    } else {
+      const int ip = (iprops[k] == 6 && n == 1) ? MCP_PROP_ODPROP : iprops[k];
+      if ((rc = prop_ld_device(c, *b, *g[k], ip, ninst, I.ered, I.pconst, I.y, sn, sn + ninst))) return rc;
+    }
+  }
+  if ((rc = match_device(c, n, ninst, I.y, wvec, nopen, nb, b->cent.as<double>(), rval))) return rc;
+  std::vector<int> tmp((size_t)ninst * (2 * nseg + 1));
+  HIPCHK(c, hipMemcpyAsync(tmp.data(), dn, tmp.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  if ((rc = fetch_match(c, n, ninst, nopen, kmat, sr, si, status))) return rc;
+  for (int i = 0; i < ninst; ++i) {
+    int nd = 0;
+    for (int k = nseg - 1; k >= 0; --k) {
+      nd += tmp[(size_t)ninst * (1 + 2 * k) + i];
+      const int st = tmp[(size_t)ninst * (2 + 2 * k) + i];
+      if (st != 0) status[i] = -(100000 * k + st);  // segment k (0-based) failed at step st
+    }
+    nodes[i] = nd;
+    if (tmp[i] != 0) status[i] = -1000000;
+  }
+  return MCP_OK;
 }
 
 int mcp_bound_match(mcp_ctx* c, int32_t bid, int32_t gid_out, int32_t gid_in, int32_t iprop_out, int32_t iprop_in,

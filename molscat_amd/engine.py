@@ -139,6 +139,18 @@ class Engine:
                     "mcp_grid_create")
         return GridHandle(gid.value, r, mxlam)
 
+    def airy_grid(self, r: np.ndarray, drnow: np.ndarray, ptab: np.ndarray) -> GridHandle:
+        """Upload an AIRY grid: r = (RSTART, RA_0, RB_0, ...), step widths
+        drnow, P table at r (see system.airy_grid)."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        drnow = np.ascontiguousarray(drnow, dtype=np.float64)
+        ptab = np.ascontiguousarray(ptab, dtype=np.float64)
+        mxlam = ptab.shape[1] if ptab.ndim == 2 else 0
+        gid = ctypes.c_int32()
+        self._check(self.lib.mcp_grid_create_airy(self.ctx, mxlam, len(drnow), dptr(r), dptr(drnow), dptr(ptab),
+                                                  ctypes.byref(gid)), "mcp_grid_create_airy")
+        return GridHandle(gid.value, r, mxlam)
+
     def free_block(self, b: BlockHandle):
         self._check(self.lib.mcp_block_destroy(self.ctx, b.id), "mcp_block_destroy")
 
@@ -176,6 +188,41 @@ class Engine:
         self._check(self.lib.mcp_prop_ld(self.ctx, b.id, g.id, iprop, m, dptr(ered), dptr(pc), dptr(y), iptr(nodes),
                                          iptr(st)), "mcp_prop_ld")
         return _from_colmajor(y), nodes, st
+
+    def prop_airy(self, b: BlockHandle, g: GridHandle, ered, y0: np.ndarray):
+        """AIPROP over an AIRY grid for ninst = len(ered) instances; returns
+        (Z at the grid's end, nodes, status)."""
+        ered = np.ascontiguousarray(np.atleast_1d(ered), dtype=np.float64)
+        m = len(ered)
+        y = _colmajor(np.asarray(y0, dtype=np.float64).reshape(m, b.n, b.n))
+        nodes = np.empty(m, dtype=np.int32)
+        st = np.empty(m, dtype=np.int32)
+        self._check(self.lib.mcp_prop_airy(self.ctx, b.id, g.id, m, dptr(ered), dptr(y), iptr(nodes), iptr(st)),
+                    "mcp_prop_airy")
+        return _from_colmajor(y), nodes, st
+
+    def smatrix_segments(self, b: BlockHandle, grids, iprops, ered, wvec, nopen, nb, rval: float, pconst=None):
+        """SCCTRL over several segments (grids with their propagator codes)."""
+        ered = np.ascontiguousarray(np.atleast_1d(ered), dtype=np.float64)
+        m = len(ered)
+        n = b.n
+        gids = np.ascontiguousarray([g.id for g in grids], dtype=np.int32)
+        ips = np.ascontiguousarray(iprops, dtype=np.int32)
+        nopen = np.ascontiguousarray(np.broadcast_to(nopen, (m,)), dtype=np.int32)
+        nb = np.ascontiguousarray(np.broadcast_to(nb, (m, n)), dtype=np.int32)
+        wv = np.ascontiguousarray(np.broadcast_to(wvec, (m, n)), dtype=np.float64)
+        k = np.zeros((m, n * n))
+        sr = np.zeros((m, n * n))
+        si = np.zeros((m, n * n))
+        nodes = np.empty(m, dtype=np.int32)
+        st = np.empty(m, dtype=np.int32)
+        pc = None if pconst is None else np.ascontiguousarray(pconst, dtype=np.float64)
+        self._check(self.lib.mcp_smatrix_segments(self.ctx, b.id, len(gids), iptr(gids), iptr(ips), m, dptr(ered),
+                                                  dptr(pc), dptr(wv), iptr(nopen), iptr(nb), float(rval), dptr(k),
+                                                  dptr(sr), dptr(si), iptr(nodes), iptr(st)),
+                    "mcp_smatrix_segments")
+        return dict(K=_unpack_square(k, nopen), SR=_unpack_square(sr, nopen), SI=_unpack_square(si, nopen),
+                    NODES=nodes, STATUS=st)
 
     def ytok_ktos(self, y: np.ndarray, wvec: np.ndarray, nopen, nb: np.ndarray, cent: np.ndarray, rval: float):
         y = np.asarray(y, dtype=np.float64)

@@ -44,6 +44,10 @@ SIGNATURES = {
     "mcp_ytok_ktos": (ctypes.c_int, [_vp, _i32, _i32, _pd, _pd, _pi, _pi, _pd, _dbl, _pd, _pd, _pd, _pi]),
     "mcp_smatrix_ld": (ctypes.c_int, [_vp, _i32, _i32, _i32, _i32, _pd, _pd, _pd, _pi, _pi, _dbl, _pd, _pd, _pd,
                                        _pi, _pi]),
+    "mcp_grid_create_airy": (ctypes.c_int, [_vp, _i32, _i32, _pd, _pd, _pd, _pi]),
+    "mcp_prop_airy": (ctypes.c_int, [_vp, _i32, _i32, _i32, _pd, _pd, _pi, _pi]),
+    "mcp_smatrix_segments": (ctypes.c_int, [_vp, _i32, _i32, _pi, _pi, _i32, _pd, _pd, _pd, _pi, _pi, _dbl, _pd, _pd,
+                                             _pd, _pi, _pi]),
     "mcp_bound_match": (ctypes.c_int, [_vp, _i32, _i32, _i32, _i32, _i32, _i32, _pd, _pd, _pi, _pd, _pi, _pd, _pd,
                                         _pi]),
     "mcp_set_team_size": (ctypes.c_int, [_vp, _i32]),

@@ -225,6 +225,78 @@ def ld_radii(rstart: float, dr: float, nstep: int) -> np.ndarray:
     return r
 
 
+def drcalc(rstart: float, rstop: float, kstep: int, nstep: int, powr: float) -> float:
+    """DRCALC: width of step kstep+1 of a grid whose steps grow as R^POWR
+    (source_code/drcalc.f:23-38)."""
+    powint = 1.0 - powr
+    if powint != 0.0:
+        powinv = 1.0 / powint
+        zstart = rstart ** powint
+        dz = (rstop ** powint - zstart) / float(nstep)
+        return (zstart + float(kstep + 1) * dz) ** powinv - (zstart + float(kstep) * dz) ** powinv
+    dz = rstart * ((rstop / rstart) ** (1.0 / float(nstep)) - 1.0)
+    return dz * (1.0 + dz / rstart) ** kstep
+
+
+def drset_airy(rstart: float, rstop: float, drfit: float, powr: float) -> Tuple[int, float]:
+    """DRSET for TOLHI = 0 (source_code/drset.f:62-80): number of steps and
+    the first step's width, constant steps for POWR = 0, steps growing as
+    R^POWR otherwise."""
+    rsmall, rlarge = min(rstart, rstop), max(rstart, rstop)
+    prange = rlarge - rsmall
+    if powr == 0.0:
+        nstep = max(int(math.floor(prange / drfit + 0.5)), 1)
+        dr = prange / float(nstep)
+    else:
+        rratio = drfit / rsmall
+        if powr == 1.0:
+            nstep = int(math.floor(math.log(rlarge / rsmall) / math.log(rratio + 1.0) + 0.5))
+        else:
+            powint = 1.0 - powr
+            zrm1 = (rlarge / rsmall) ** powint - 1.0
+            nstep = int(math.floor(zrm1 / ((rratio + 1.0) ** powint - 1.0) + 0.5))
+        dr = drcalc(rstart, rstop, 0, nstep, powr)
+    if rstop < rstart:
+        dr = -abs(dr)
+    return nstep, dr
+
+
+AIRY_SQ3 = 1.732050807  # the sqrt(3) of POTENT's Gauss points, as written in source_code/potent.f:49
+
+
+def airy_grid(rstart: float, rstop: float, nstep: int, dr0: float, powr: float):
+    """AIPROP's steps for TOLHI = 0 (source_code/aiprop.f:86-91,288-327):
+    (r, drnow) with r = (RSTART, RA_0, RB_0, ...) the Gauss points of each
+    step (RNOW -+ 0.5*DRNOW/1.732050807, source_code/potent.f:66-67) and
+    drnow the widths, accumulated in the reference's order."""
+    rnows, widths = [], []
+    drnow = dr0
+    rlast = rstart
+    rnow = rlast + drnow * 0.5
+    rnext = rlast + drnow
+    last = False
+    kstep = 0
+    while True:
+        kstep += 1
+        rnows.append(rnow)
+        widths.append(drnow)
+        if last:
+            break
+        drnow = drcalc(rstart, rstop, kstep, nstep, powr)
+        rlast = rnext
+        rnext = rnext + drnow
+        last = (nstep - kstep) <= 1
+        if last:
+            rnext = rstop
+            drnow = rnext - rlast
+        rnow = rlast + 0.5 * drnow
+    r = [rstart]
+    for rn, w in zip(rnows, widths):
+        r.append(rn - 0.5 * w / AIRY_SQ3)
+        r.append(rn + 0.5 * w / AIRY_SQ3)
+    return np.array(r), np.array(widths)
+
+
 def drset_constant(rstart: float, rstop: float, drfit: float) -> Tuple[int, float]:
     """Constant-step branch of DRSET (source_code/drset.f:62-64)."""
     prange = abs(rstop - rstart)

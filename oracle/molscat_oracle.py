@@ -1283,3 +1283,117 @@ def bound_match(blk: Block, ered: float, g: BoundGrid, iprop_out: int = 6, iprop
     psi *= math.copysign(1.0, big)
     return dict(NODEC=n_out + n_in + negcnt, NODES_OUT=n_out, NODES_IN=n_in, NEGCNT=negcnt, EIGMIN=eigmin,
                 IMIN=imin, PSIMCH=psi, EVALS=ev, M=m)
+
+
+# ---------------------------------------------------------------------------
+# AIRY propagator (AIPROP, IPROP=9): source_code/aiprop.f:1-392 with POTENT
+# (source_code/potent.f:56-119), SPROPN (source_code/spropn.f:82-258) and the
+# step sequence of DRSET/DRCALC for TOLHI = 0.  The single-channel Airy
+# imbedding propagator is evaluated here from its definition with mpmath at
+# 40 digits (independent of the reference's SCAIRY tables and asymptotic
+# series): for the linear reference potential x = alpha*(beta + r) of a
+# sector with ends x1, x2 and D = Ai(x1)Bi(x2) - Ai(x2)Bi(x1),
+#   y1 = alpha (Bi'(x1)Ai(x2) - Ai'(x1)Bi(x2)) / D,
+#   y2 = alpha / (pi D),
+#   y4 = alpha (Ai(x1)Bi'(x2) - Bi(x1)Ai'(x2)) / D,
+# which is what SPROPN's modulus-phase and asymptotic-product forms compute.
+# ---------------------------------------------------------------------------
+AIRY_SQ3 = 1.732050807  # the sqrt(3) of source_code/potent.f:49 (as written there)
+
+
+def airy_steps(rstart: float, rstop: float, nstep: int, dr0: float, powr: float) -> List[Tuple[float, float]]:
+    """(RNOW, DRNOW) of each AIPROP step for TOLHI = 0, accumulated exactly as
+    source_code/aiprop.f:86-91,288-327 does (RNEXT += DRNOW, last step ends
+    at RSTOP)."""
+    out = []
+    drnow = dr0
+    rlast = rstart
+    rnow = rlast + drnow * 0.5
+    rnext = rlast + drnow
+    last = False
+    kstep = 0
+    while True:
+        kstep += 1
+        out.append((rnow, drnow))
+        if last:
+            break
+        drnow = drcalc(rstart, rstop, kstep, nstep, powr)
+        rlast = rnext
+        rnext = rnext + drnow
+        last = (nstep - kstep) <= 1
+        if last:
+            rnext = rstop
+            drnow = rnext - rlast
+        rnew = rlast + 0.5 * drnow
+        rnow = rnew
+    return out
+
+
+def airy_sector(width: float, eig: np.ndarray, hp: np.ndarray):
+    """y1, y2, y4 per channel for a sector of the given width whose local
+    wavevector-squared eigenvalues are eig with slopes hp (SPROPN's inputs:
+    the reference W is -eig + hp*r about the sector midpoint, so x =
+    alpha*(r + beta) with alpha^3 = hp and beta = -eig/hp)."""
+    import mpmath as mp
+
+    mp.mp.dps = 40
+    n = len(eig)
+    y1 = np.empty(n)
+    y2 = np.empty(n)
+    y4 = np.empty(n)
+    for i in range(n):
+        slope = hp[i] if hp[i] != 0.0 else 1.0e-30
+        alpha = math.copysign(abs(slope) ** (1.0 / 3.0), slope)
+        beta = -eig[i] / slope
+        x1 = mp.mpf(alpha) * (mp.mpf(beta) - mp.mpf(width) / 2)
+        x2 = mp.mpf(alpha) * (mp.mpf(beta) + mp.mpf(width) / 2)
+        a1, b1 = mp.airyai(x1), mp.airybi(x1)
+        a2, b2 = mp.airyai(x2), mp.airybi(x2)
+        ap1, bp1 = mp.airyai(x1, 1), mp.airybi(x1, 1)
+        ap2, bp2 = mp.airyai(x2, 1), mp.airybi(x2, 1)
+        d = a1 * b2 - a2 * b1
+        y1[i] = float(alpha * (bp1 * a2 - ap1 * b2) / d)
+        y2[i] = float(alpha / (mp.pi * d))
+        y4[i] = float(alpha * (a1 * bp2 - b1 * ap2) / d)
+    return y1, y2, y4
+
+
+def airy_potent(blk: Block, ered: float, rnow: float, drnow: float):
+    """POTENT (source_code/potent.f:56-119): W at the two Gauss points, the
+    local wavevector matrix -(W(RA)+W(RB))/2 diagonalised (DIAGVC,
+    ascending), and the diagonal of sqrt(3)(W(RB)-W(RA))/DRNOW in the local
+    basis.  Returns (eig, V, hp) with V's columns the local basis."""
+    ra = rnow - 0.5 * drnow / AIRY_SQ3
+    rb = rnow + 0.5 * drnow / AIRY_SQ3
+    wa, _ = blk.hammat(ra, ered)
+    wb, _ = blk.hammat(rb, ered)
+    wavg = -0.5 * (wa + wb)
+    wd = (AIRY_SQ3 / drnow) * (wb - wa)
+    eig, v = np.linalg.eigh(wavg)
+    hp = np.einsum("ji,jk,ki->i", v, wd, v)
+    return eig, v, hp
+
+
+def aiprop(z: np.ndarray, blk: Block, ered: float, rstart: float, rstop: float, nstep: int, dr0: float,
+           powr: float) -> Tuple[np.ndarray, int]:
+    """AIPROP for TOLHI = 0 (source_code/aiprop.f:86-327): per step the local
+    basis of POTENT, Z <- T^T Z T with T = V_prev^T V_now, the Airy sector
+    propagator, Z <- Y4 - Y2 (Z + Y1)^-1 Y2 with NODES += KOUNT (outward) or
+    NCH - KOUNT (inward); finally back to the primitive basis."""
+    n = blk.n
+    nodes = 0
+    vprev = None
+    for rnow, drnow in airy_steps(rstart, rstop, nstep, dr0, powr):
+        eig, v, hp = airy_potent(blk, ered, rnow, drnow)
+        t = v if vprev is None else vprev.T @ v
+        z = t.T @ z @ t
+        y1, y2, y4 = airy_sector(drnow, eig, hp)
+        zz = z + np.diag(y1)
+        zi, kount = syminv(zz)
+        if kount > n:
+            raise RuntimeError("AIPROP: instability in SYMINV")
+        nodes += kount if rstop > rstart else n - kount
+        z = -(y2[:, None] * zi * y2[None, :]) + np.diag(y4)
+        vprev = v
+    z = vprev @ z @ vprev.T
+    return 0.5 * (z + z.T), nodes
