@@ -166,8 +166,10 @@ int mcp_bound_match(mcp_ctx *ctx, int32_t block_id, int32_t grid_out, int32_t gr
                     double *eigmin, int32_t *imin, double *psimch, double *evals, int32_t *status);
 
 /* Workgroups that cooperate on one instance on the large-N path (N beyond the
- * LDS-resident kernels): g >= 1 fixes it, 0 restores the automatic choice
- * (up to 32, limited by CUs / instances).  No reference counterpart. */
+ * LDS-resident kernels): g >= 1 asks for g (the register-resident MDPROP
+ * needs at least ceil(N/32) and takes that many by default), 0 restores the
+ * automatic choice, g < 0 keeps MDPROP on the global-memory kernel with -g
+ * workgroups.  No reference counterpart. */
 int mcp_set_team_size(mcp_ctx *ctx, int32_t g);
 
 /* Wall time in milliseconds of the last mcp_prop_ld / mcp_smatrix_ld call's

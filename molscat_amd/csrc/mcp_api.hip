@@ -14,6 +14,7 @@
 #include "mcp_kernels.hpp"
 #include "mcp_large.hpp"
 #include "mcp_airyprop.hpp"
+#include "mcp_mfmaprop.hpp"
 
 namespace {
 
@@ -123,6 +124,15 @@ size_t lds_airy_prop(int n) {
       .add<double>(n).add<double>(n).add<double>(n).sweep(n).b;
 }
 
+// same pieces, in the same order, as mcp::carve_reg
+size_t lds_mfma(int n, int nham) {
+  LdsSize z;
+  z.add<double>(mcp::kMW * mcp::kPB * mcp::kPB).add<double>(mcp::kMW * mcp::kPB).add<double>(mcp::kMW * mcp::kPB);
+  z.add<double>(mcp::kPB * mcp::kPB).add<double>(nham).add<double>(n).add<double>(n).add<double>(n);
+  z.add<int>(n).add<int>(n).add<int>(mcp::kPB).add<int>(8).add<double>(mcp::kMW).add<int>(mcp::kMW);
+  return z.b;
+}
+
 size_t lds_large(int n, int nham, int nvec) {
   LdsSize z;
   // same pieces, in the same order, as mcp::carve_large
@@ -154,8 +164,11 @@ struct mcp_ctx {
   DevBuf s_y;  // YINIT square-root workspace (large N)
   DevBuf s_bar;  // per-instance team barrier counters + abort flag
   int team = 0;  // workgroups per instance on the large-N path (0: automatic)
+  bool no_reg = false;  // keep the large-N MDPROP on the global-memory kernel
   DevBuf s_u;  // LDPROP sector-matrix workspace (large N)
   DevBuf s_av, s_at, s_ae, s_ah;  // AIRY step bases, basis changes, eigenvalues, slopes
+  DevBuf s_x;                     // column exchange of the register-resident MDPROP
+  DevBuf s_dg;                    // its per-sector reference-potential terms
   double t_sector = 0.0, t_prop = 0.0, t_total = 0.0;
 
   int fail(int code, const char* fmt, ...) {
@@ -190,7 +203,7 @@ int set_lds_limits(mcp_ctx* c) {
                        (const void*)mcp::k_syminv_large,   (const void*)mcp::k_mdprop_uc_large,
                        (const void*)mcp::k_mdprop_large,   (const void*)mcp::k_ldprop_large,
                        (const void*)mcp::k_yinit_large,    (const void*)mcp::k_airy_sector,
-                       (const void*)mcp::k_airy_prop};
+                       (const void*)mcp::k_airy_prop,      (const void*)mcp::k_mdprop_mfma<1>, (const void*)mcp::k_mdprop_mfma<2>};
   const size_t total = c->lds_max;
   size_t lim = total;
   for (const void* f : fns) {
@@ -332,16 +345,39 @@ int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const do
                          c->s_e.as<double>(), c->s_f.as<int>());
       HIPCHK(c, hipGetLastError());
       HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-      const int g = team_size(c, ninst);
       int rc;
       if ((rc = prep_team(c, ninst))) return rc;
       const double* ucp = c->s_e.as<double>();
       const int* uck = c->s_f.as<int>();
       unsigned* ctrs = c->s_bar.as<unsigned>();
       int* abort_flag = (int*)(ctrs + ninst);
-      void* args[] = {&B, &G, &I, (void*)&nstep, (void*)&h, &ucp, &uck, (void*)&bk, (void*)&g, &ctrs, &abort_flag};
-      HIPCHK(c, hipLaunchCooperativeKernel((const void*)mcp::k_mdprop_large, dim3(ninst * g), dim3(kNT), args, lm,
-                                           c->stream));
+      // register-resident matrix-core kernel when the instances' teams fit
+      // the device: one column block per workgroup where there are CUs for
+      // it, else two
+      const int nblk = (n + mcp::kPB - 1) / mcp::kPB;
+      const int rb = (size_t)nblk * ninst <= (size_t)c->num_cu ? 1 : 2;
+      const int gmin = (nblk + rb - 1) / rb;
+      const size_t lr = lds_mfma(n, b.nham);
+      if (n <= mcp::kMMaxN && lr <= c->lds_max && !c->no_reg && (size_t)gmin * ninst <= (size_t)c->num_cu) {
+        const int g = std::min(std::max(gmin, c->team), std::max(gmin, c->num_cu / ninst));
+        HIPCHK(c, c->s_x.ensure((size_t)ninst * 2 * mcp::kMSlots * mcp::panel_ld(n) * sizeof(double)));
+        HIPCHK(c, c->s_dg.ensure((size_t)ninst * 3 * nstep * n * sizeof(double)));
+        double* xg = c->s_x.as<double>();
+        const double* dgp = c->s_dg.as<double>();
+        const size_t ndiag = (size_t)ninst * nstep * n;
+        hipLaunchKernelGGL(mcp::k_mdprop_diag, dim3((unsigned)((ndiag + 255) / 256)), dim3(256), 0, c->stream, B, G, I,
+                           ninst, nstep, h, c->s_dg.as<double>());
+        HIPCHK(c, hipGetLastError());
+        void* args[] = {&B,  &G, &I, (void*)&nstep, (void*)&h, &ucp, &uck, (void*)&bk, (void*)&g, &ctrs, &abort_flag,
+                        &xg, &dgp};
+        const void* kern = rb == 1 ? (const void*)mcp::k_mdprop_mfma<1> : (const void*)mcp::k_mdprop_mfma<2>;
+        HIPCHK(c, hipLaunchCooperativeKernel(kern, dim3(ninst * g), dim3(mcp::kMT), args, lr, c->stream));
+      } else {
+        const int g = team_size(c, ninst);
+        void* args[] = {&B, &G, &I, (void*)&nstep, (void*)&h, &ucp, &uck, (void*)&bk, (void*)&g, &ctrs, &abort_flag};
+        HIPCHK(c, hipLaunchCooperativeKernel((const void*)mcp::k_mdprop_large, dim3(ninst * g), dim3(kNT), args, lm,
+                                             c->stream));
+      }
     }
   } else {
     const size_t need = lds_ld(n, b.nham);
@@ -979,8 +1015,9 @@ int mcp_bound_match(mcp_ctx* c, int32_t bid, int32_t gid_out, int32_t gid_in, in
 }
 
 int mcp_set_team_size(mcp_ctx* c, int32_t g) {
-  if (!c || g < 0) return MCP_ERR_ARG;
-  c->team = g;
+  if (!c) return MCP_ERR_ARG;
+  c->no_reg = g < 0;
+  c->team = g < 0 ? -g : g;
   return MCP_OK;
 }
 

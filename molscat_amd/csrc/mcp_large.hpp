@@ -724,7 +724,7 @@ __global__ void __launch_bounds__(kNT) k_syminv_large(int n, double* __restrict_
   if (threadIdx.x == 0) kount[blockIdx.x] = k;
 }
 
-// MDPROP sector matrices UC(k) = [I/8 + D4 W_off]^-1 - 8I, lower triangle.
+// MDPROP sector matrices UC(k) = [I/8 + D4 W_off]^-1 - 8I, both triangles.
 __global__ void __launch_bounds__(kNT) k_mdprop_uc_large(BlockArgs B, GridArgs G, const double* __restrict__ pconst,
                                                          double h, int bk, double* __restrict__ uc,
                                                          int* __restrict__ kount) {
@@ -748,6 +748,11 @@ __global__ void __launch_bounds__(kNT) k_mdprop_uc_large(BlockArgs B, GridArgs G
   for_owned_lower(tm, n, [&](int i, int j) {
     const size_t e = i + (size_t)j * n;
     A[e] = (i == j) ? -A[e] - 8.0 : -A[e];
+  });
+  __syncthreads();
+  // full storage: the register-resident propagator reads whole columns
+  for_owned_lower(tm, n, [&](int i, int j) {
+    if (i != j) A[j + (size_t)i * n] = A[i + (size_t)j * n];
   });
   if (threadIdx.x == 0) kount[blockIdx.x] = k;
 }

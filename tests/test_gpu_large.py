@@ -107,7 +107,14 @@ def test_team_size_bitwise_identical(eng):
         y, nodes, st = eng.prop_ld(b, g, 6, [ered], y0)
         assert st[0] == 0
         outs.append((y[0], int(nodes[0])))
+    # the global-memory kernel (negative team size) sums in another order
+    eng.set_team_size(-8)
+    print("team=8 global-memory kernel", flush=True)
+    yg, nodes_g, st = eng.prop_ld(b, g, 6, [ered], y0)
     eng.set_team_size(0)
+    assert st[0] == 0
     for y, nodes in outs[1:]:
         assert nodes == outs[0][1]
         assert np.array_equal(y, outs[0][0])
+    assert int(nodes_g[0]) == outs[0][1]
+    assert np.abs(yg[0] - outs[0][0]).max() < 1e-12 * np.abs(yg[0]).max()
