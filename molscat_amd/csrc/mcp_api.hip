@@ -127,7 +127,7 @@ size_t lds_airy_prop(int n) {
 // same pieces, in the same order, as mcp::carve_reg
 size_t lds_mfma(int n, int nham) {
   LdsSize z;
-  z.add<double>(mcp::kMW * mcp::kPB * mcp::kPB).add<double>(mcp::kMW * mcp::kPB).add<double>(mcp::kMW * mcp::kPB);
+  z.add<double>(mcp::kPB * mcp::kPB).add<double>(2 * mcp::kPB).add<double>(mcp::kMW * mcp::kPB);
   z.add<double>(mcp::kPB * mcp::kPB).add<double>(nham).add<double>(n).add<double>(n).add<double>(n);
   z.add<int>(n).add<int>(n).add<int>(mcp::kPB).add<int>(8).add<double>(mcp::kMW).add<int>(mcp::kMW);
   return z.b;

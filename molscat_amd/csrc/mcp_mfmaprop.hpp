@@ -22,7 +22,7 @@
 // single Bunch-Kaufman step with interchange when the first candidate
 // fails).  For the candidate columns G = A(:,P) (P = positions k..k+15) and
 // S = A(P,P):
-//   S = L D L^T in the registers of lanes 0..15 (every wave, identically);
+//   S = L D L^T in the registers of lanes 0..15 of wave 0;
 //   W = G L^-T, whose column kk over the rows not pivoted before kk is the
 //   column dsytf2 tests at step kk; np = the leading candidates that pass;
 //   S^-1 = L^-T D^-1 L^-1 over the first np; X = G S^-1;
@@ -52,8 +52,8 @@ __device__ __forceinline__ dvec4 mma(double a, double b, dvec4 c) {
 
 struct MfmaScratch {
   int ld;        // exchange leading dimension (whole 64-row tiles)
-  double* Rw;    // [kMW][16*16] each wave's L^-1, row-major
-  double* dw;    // [kMW][16] each wave's D
+  double* Rw;    // [16*16] L^-1, row-major
+  double* dw;    // [32] D and its reciprocal
   double* cmax;  // [kMW][16] column maxima of W per wave
   double* Dinv;  // [16*16] S^-1 of a Bunch-Kaufman step
   double* pv;    // [nham]
@@ -71,8 +71,8 @@ struct MfmaScratch {
 __device__ __forceinline__ MfmaScratch carve_mfma(LdsCarve& c, int n, int nham) {
   MfmaScratch s;
   s.ld = panel_ld(n);
-  s.Rw = c.take<double>(kMW * kPB * kPB);
-  s.dw = c.take<double>(kMW * kPB);
+  s.Rw = c.take<double>(kPB * kPB);
+  s.dw = c.take<double>(2 * kPB);
   s.cmax = c.take<double>(kMW * kPB);
   s.Dinv = c.take<double>(kPB * kPB);
   s.pv = c.take<double>(nham);
@@ -156,8 +156,8 @@ __device__ __attribute__((always_inline)) int mf_sweep(dvec4 (&T)[kMTW][RB], int
   __syncthreads();
   mf_publish<RB>(T, n, ld, tm, E, [&](int j) { return j < kPB ? j : -1; });
   if (!team_sync(tm)) return n + 2;
-  double* Rw = s.Rw + w * kPB * kPB;
-  double* dw = s.dw + w * kPB;
+  const double* Rw = s.Rw;
+  const double* dw = s.dw;
   int kount = 0, k = 0, parity = 0;
   while (k < n) {
     double* Ec = E + (size_t)parity * kMSlots * ld;
@@ -173,49 +173,64 @@ __device__ __attribute__((always_inline)) int mf_sweep(dvec4 (&T)[kMTW][RB], int
         Gb[tt][c] = (t < ntile && u < npre) ? Ec[(size_t)u * ld + 16 * t + r16] : 0.0;
       }
     }
-    // S = L D L^T, row a in lane a (dsytf2.f 1x1 steps in natural order);
-    // R accumulates L^-1 by the same eliminations
-    int f = npre;
-    double dk = 0.0;
-    {
+    // candidate rows A(P,j) at the owned columns: A operands, chunk c =
+    // candidates 4c..4c+3 (beyond np they meet zero B operands)
+    double Ga[RB][4];
+#pragma unroll
+    for (int b = 0; b < RB; ++b) {
+      const int blk = tm.rank + b * tm.g;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int u = 4 * c + q;
+        Ga[b][c] = (blk < ntile && u < npre) ? Ec[(size_t)u * ld + 16 * blk + r16] : 0.0;
+      }
+    }
+    // S = L D L^T in wave 0, row a in lane a (dsytf2.f 1x1 steps in natural
+    // order); R accumulates L^-1 by the same eliminations
+    if (w == 0) {
       double Sr[kPB], Rr[kPB];
       const int prow = r16 < npre ? s.perm[k + r16] : 0;
 #pragma unroll
       for (int c = 0; c < kPB; ++c) Sr[c] = (lane < npre && c < npre) ? Ec[(size_t)c * ld + prow] : 0.0;
-      // at step kk only S(a, kk+1..) and R(a, ..kk) are live: R's column kk
-      // starts as e_kk there
+      // straight-line steps: once a pivot is exactly zero (or the window
+      // ends) every later multiplier is zero.  At step kk only S(a, kk+1..)
+      // and R(a, ..kk) are live: R's column kk starts as e_kk there.
+      bool live = true;
+      int f = 0;
+      double dk = 0.0, dki = 0.0;
 #pragma unroll
       for (int kk = 0; kk < kPB; ++kk) {
         Rr[kk] = lane == kk ? 1.0 : 0.0;
-        if (kk < f) {
-          const double d = bcast(Sr[kk], kk);
-          if (d == 0.0) {
-            f = kk;
-          } else {
-            if (lane == kk) dk = d;
-            const double dinv = 1.0 / d;
-            const double la = Sr[kk] * dinv;
-            const bool below = lane > kk;
-#pragma unroll
-            for (int c = kk + 1; c < kPB; ++c) {
-              const double rk = bcast(Sr[c], kk);
-              if (below) Sr[c] -= la * rk;
-            }
-#pragma unroll
-            for (int c = 0; c <= kk; ++c) {
-              const double rk = bcast(Rr[c], kk);
-              if (below) Rr[c] -= la * rk;
-            }
-          }
+        const double d = bcast(Sr[kk], kk);
+        live = live && kk < npre && d != 0.0;
+        // 1/d: hardware estimate refined by two Newton steps
+        double r = __builtin_amdgcn_rcp(d);
+        r = fma(r, fma(-d, r, 1.0), r);
+        r = fma(r, fma(-d, r, 1.0), r);
+        if (live) f = kk + 1;
+        if (live && lane == kk) {
+          dk = d;
+          dki = r;
         }
+        // rows at and above the pivot take a zero multiplier: x - 0*y
+        // leaves a finite x unchanged, with no predication
+        const double la = (live && lane > kk) ? Sr[kk] * r : 0.0;
+#pragma unroll
+        for (int c = kk + 1; c < kPB; ++c) Sr[c] -= la * bcast(Sr[c], kk);
+#pragma unroll
+        for (int c = 0; c <= kk; ++c) Rr[c] -= la * bcast(Rr[c], kk);
       }
       if (lane < kPB) {
 #pragma unroll
-        for (int c = 0; c < kPB; ++c) Rw[lane * kPB + c] = Rr[c];
-        dw[lane] = dk;
+        for (int c = 0; c < kPB; ++c) s.Rw[lane * kPB + c] = Rr[c];
+        s.dw[lane] = dk;
+        s.dw[kPB + lane] = dki;
       }
+      if (lane == 0) s.ctl[4] = f;
     }
-    wave_sync();
+    __syncthreads();
+    const int f = s.ctl[4];
+    const double dk = lane < kPB ? s.dw[lane] : 0.0;
     // W = G L^-T and its masked column maxima
     {
       double la4[4];
@@ -263,7 +278,6 @@ __device__ __attribute__((always_inline)) int mf_sweep(dvec4 (&T)[kMTW][RB], int
       kpan = __popcll((unsigned long long)__ballot(neg) & ((1ull << np) - 1ull));
     }
     dvec4 Sv = {0.0, 0.0, 0.0, 0.0};  // lane l, register v: S^-1(l/16 + 4v, l%16)
-    int slot4[4];                      // exchange slot of pivot 4c + l/16
     if (np > 0) {
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
@@ -271,10 +285,9 @@ __device__ __attribute__((always_inline)) int mf_sweep(dvec4 (&T)[kMTW][RB], int
         double rv = 0.0, rd = 0.0;
         if (kk < np) {
           rv = Rw[kk * kPB + r16];
-          rd = rv * (1.0 / dw[kk]);
+          rd = rv * dw[kPB + kk];
         }
         Sv = mma(rv, rd, Sv);
-        slot4[c] = kk < np ? kk : -1;
       }
     } else {
       // one Bunch-Kaufman step at position k (dsytf2.f): candidate ck, the
@@ -359,6 +372,8 @@ __device__ __attribute__((always_inline)) int mf_sweep(dvec4 (&T)[kMTW][RB], int
       if (s.ctl[2] != 0) return s.ctl[2];
       np = s.ctl[0];
       kpan = s.ctl[1];
+      // operands of the step's pivots, pivot u from exchange slot sl[u]
+      int slot4[4];
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         Sv[c] = s.Dinv[(q + 4 * c) * kPB + r16];
@@ -371,45 +386,29 @@ __device__ __attribute__((always_inline)) int mf_sweep(dvec4 (&T)[kMTW][RB], int
         for (int c = 0; c < 4; ++c)
           Gb[tt][c] = (t < ntile && slot4[c] >= 0) ? Ec[(size_t)slot4[c] * ld + 16 * t + r16] : 0.0;
       }
+#pragma unroll
+      for (int b = 0; b < RB; ++b) {
+        const int blk = tm.rank + b * tm.g;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          Ga[b][c] = (blk < ntile && slot4[c] >= 0) ? Ec[(size_t)slot4[c] * ld + 16 * blk + r16] : 0.0;
+      }
     }
-    // pivot rows A(P,j) of the owned columns: A operands, chunk c = pivots 4c..4c+3
-    double Ga[RB][4];
+    // X = G S^-1 at the owned columns: register c of Xa[b] in lane l is
+    // X(16B + l%16, 4c + l/16), the A operand of chunk c of the update
+    dvec4 Xa[RB];
 #pragma unroll
     for (int b = 0; b < RB; ++b) {
-      const int blk = tm.rank + b * tm.g;
-#pragma unroll
-      for (int c = 0; c < 4; ++c)
-        Ga[b][c] = (blk < ntile && slot4[c] >= 0) ? Ec[(size_t)slot4[c] * ld + 16 * blk + r16] : 0.0;
-    }
-    // X' = -G S^-1 off P, S^-1(a,:) on pivot row a (lane l, register v:
-    // pivot l/16 + 4v at row 16t + l%16)
-    dvec4 Xt[kMTW];
-    unsigned prow_any = 0u;
-#pragma unroll
-    for (int tt = 0; tt < kMTW; ++tt) {
-      const int t = w + kMW * tt;
       dvec4 acc = {0.0, 0.0, 0.0, 0.0};
-      if (t < ntile) {
+      if (tm.rank + b * tm.g < ntile) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc = mma(Sv[c], Gb[tt][c], acc);
-        const int i = 16 * t + r16;
-        const int ai = i < n ? s.ipos[i] - k : -1;
-        const bool prow = ai >= 0 && ai < np;
-        if (__ballot(prow) != 0ull) {
-          prow_any |= 1u << tt;
-          const int src = (prow ? ai : 0) + 16 * q;
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const double sh = shfl_d(Sv[v], src);
-            acc[v] = prow ? sh : -acc[v];
-          }
-        } else {
-          acc = -acc;
-        }
+        for (int c = 0; c < 4; ++c) acc = mma(Sv[c], Ga[b][c], acc);
       }
-      Xt[tt] = acc;
+      Xa[b] = acc;
     }
-    // the update, then the pivot columns -X'(i, b) (X off P, -S^-1 on P)
+    // A(i,j) <- A(i,j) - G(i,:) X(j,:)^T off P; on pivot row a the B operand
+    // is e_a and the entry starts from 0, giving X(j,a) = (S^-1 A(P,j))_a;
+    // then the pivot columns: X(i,b) off P, -S^-1(a,b) on P
 #pragma unroll
     for (int b = 0; b < RB; ++b) {
       const int blk = tm.rank + b * tm.g;
@@ -431,32 +430,41 @@ __device__ __attribute__((always_inline)) int mf_sweep(dvec4 (&T)[kMTW][RB], int
         for (int tt = 0; tt < kMTW; ++tt) {
           const int t = w + kMW * tt;
           if (t < ntile) {
-            if (prow_any & (1u << tt)) {
-              const int i = 16 * t + r16;
-              const int ai = i < n ? s.ipos[i] - k : -1;
-              if (ai >= 0 && ai < np) {
-#pragma unroll
-                for (int v = 0; v < 4; ++v)
-                  if (!pc[v]) T[tt][b][v] = 0.0;
-              }
-            }
+            const int i = 16 * t + r16;
+            const int ai = i < n ? s.ipos[i] - k : -1;
+            const bool prow = ai >= 0 && ai < np;
             dvec4 acc = T[tt][b];
+            double gn[4];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) acc = mma(Ga[b][c], Xt[tt][c], acc);
+            for (int c = 0; c < 4; ++c) gn[c] = prow ? (4 * c + q == ai ? 1.0 : 0.0) : -Gb[tt][c];
+            if (prow) {
+#pragma unroll
+              for (int v = 0; v < 4; ++v)
+                if (!pc[v]) acc[v] = 0.0;
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc = mma(Xa[b][c], gn[c], acc);
             if (has_pc) {
+              // X at this wave's rows (lane l, register v: X(16t + l%16, l/16 + 4v))
+              dvec4 xt = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+              for (int c = 0; c < 4; ++c) xt = mma(Sv[c], Gb[tt][c], xt);
               if (aligned) {
 #pragma unroll
-                for (int v = 0; v < 4; ++v) acc[v] = -Xt[tt][v];
+                for (int v = 0; v < 4; ++v) acc[v] = prow ? -Sv[v] : xt[v];
               } else {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                   const int bv = pc[v] ? bj[v] : 0;
-                  const int src = r16 + 16 * (bv & 3);
-                  const double x0 = shfl_d(Xt[tt][0], src), x1 = shfl_d(Xt[tt][1], src);
-                  const double x2 = shfl_d(Xt[tt][2], src), x3 = shfl_d(Xt[tt][3], src);
                   const int rr = bv >> 2;
+                  const int sx = r16 + 16 * (bv & 3), ss = (prow ? ai : 0) + 16 * (bv & 3);
+                  const double x0 = shfl_d(xt[0], sx), x1 = shfl_d(xt[1], sx);
+                  const double x2 = shfl_d(xt[2], sx), x3 = shfl_d(xt[3], sx);
+                  const double s0 = shfl_d(Sv[0], ss), s1 = shfl_d(Sv[1], ss);
+                  const double s2 = shfl_d(Sv[2], ss), s3 = shfl_d(Sv[3], ss);
                   const double xv = rr == 0 ? x0 : (rr == 1 ? x1 : (rr == 2 ? x2 : x3));
-                  if (pc[v]) acc[v] = -xv;
+                  const double sw = rr == 0 ? s0 : (rr == 1 ? s1 : (rr == 2 ? s2 : s3));
+                  if (pc[v]) acc[v] = prow ? -sw : xv;
                 }
               }
             }
