@@ -72,6 +72,9 @@ struct Team {
 constexpr unsigned kSpinLimit = 1u << 26;  // bounded wait (~seconds), never a hang
 
 __device__ inline bool team_sync(Team& t) {
+  // every wave's stores have reached the L2 before thread 0's release writes
+  // it back (a workgroup barrier alone does not wait for them)
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (t.g == 1) return true;
   __shared__ int s_ok;
@@ -81,7 +84,9 @@ __device__ inline bool team_sync(Team& t) {
     __hip_atomic_fetch_add(t.ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     unsigned spins = 0;
     int ok = 1;
-    while (__hip_atomic_load(t.ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    // relaxed polls and one acquire after: an acquire per poll would
+    // invalidate the caches on every iteration
+    while (__hip_atomic_load(t.ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       __builtin_amdgcn_s_sleep(1);
       if (++spins > kSpinLimit ||
           __hip_atomic_load(t.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
@@ -90,6 +95,7 @@ __device__ inline bool team_sync(Team& t) {
         break;
       }
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     s_ok = ok;
   } else {
     t.epoch += 1;
