@@ -165,12 +165,21 @@ int mcp_bound_match(mcp_ctx *ctx, int32_t block_id, int32_t grid_out, int32_t gr
                     int32_t iprop_in, int32_t ninst, const double *ered, const double *pconst, int32_t *nodec,
                     double *eigmin, int32_t *imin, double *psimch, double *evals, int32_t *status);
 
-/* Workgroups that cooperate on one instance on the large-N path (N beyond the
- * LDS-resident kernels): g >= 1 asks for g (the register-resident MDPROP
- * needs at least ceil(N/32) and takes that many by default), 0 restores the
+/* Workgroups that cooperate on one instance on the sequential large-N
+ * kernels (N beyond the LDS-resident ones): g >= 1 asks for g (the
+ * register-resident MDPROP takes ceil(N/16) by default, and between
+ * ceil(N/32) and ceil(N/16) when a smaller g is asked for), 0 restores the
  * automatic choice, g < 0 keeps MDPROP on the global-memory kernel with -g
  * workgroups.  No reference counterpart. */
 int mcp_set_team_size(mcp_ctx *ctx, int32_t g);
+
+/* How MDPROP runs for large N: 0 (default) solves the recurrence in parallel
+ * over the radial grid, as a tree of merged sector maps, where the device
+ * memory holds it, else sequentially; 1 always sequentially on the
+ * register-resident matrix-core kernel; 2 sequentially on the global-memory
+ * kernel.  Same S-matrix and node count to rounding.  No reference
+ * counterpart. */
+int mcp_set_large_mode(mcp_ctx *ctx, int32_t mode);
 
 /* Wall time in milliseconds of the last mcp_prop_ld / mcp_smatrix_ld call's
  * device work, measured with HIP events on the library's stream, split into

@@ -15,6 +15,7 @@
 #include "mcp_large.hpp"
 #include "mcp_airyprop.hpp"
 #include "mcp_mfmaprop.hpp"
+#include "mcp_tree.hpp"
 
 namespace {
 
@@ -165,10 +166,14 @@ struct mcp_ctx {
   DevBuf s_bar;  // per-instance team barrier counters + abort flag
   int team = 0;  // workgroups per instance on the large-N path (0: automatic)
   bool no_reg = false;  // keep the large-N MDPROP on the global-memory kernel
+  int large_mode = 0;   // large-N MDPROP: 0 tree where memory allows, 1 sequential matrix-core, 2 global memory
   DevBuf s_u;  // LDPROP sector-matrix workspace (large N)
   DevBuf s_av, s_at, s_ae, s_ah;  // AIRY step bases, basis changes, eigenvalues, slopes
   DevBuf s_x;                     // column exchange of the register-resident MDPROP
   DevBuf s_dg;                    // its per-sector reference-potential terms
+  DevBuf s_seg0, s_seg1;          // segments of the parallel-in-R MDPROP, two tree levels
+  DevBuf s_zb, s_pq;              // its merge pivots / inverses and products
+  DevBuf s_kc, s_acc;             // its negative-pivot counts
   double t_sector = 0.0, t_prop = 0.0, t_total = 0.0;
 
   int fail(int code, const char* fmt, ...) {
@@ -299,6 +304,108 @@ int prep_team(mcp_ctx* c, int ninst) {
   return MCP_OK;
 }
 
+unsigned nblk256(size_t cnt) { return (unsigned)((cnt + 255) / 256); }
+
+template <bool TA, bool TB>
+int tree_gemm(mcp_ctx* c, int n, int nbatch, int per, mcp::SegRef A, mcp::SegRef Bm, mcp::SegRef Ci, mcp::SegOut Co,
+              double alpha, double beta) {
+  const int nt = (n + mcp::kGT - 1) / mcp::kGT;
+  hipLaunchKernelGGL((mcp::k_gemm_seg<TA, TB>), dim3(nbatch, nt * nt), dim3(256), 0, c->stream, n, per, A, Bm, Ci, Co,
+                     alpha, beta);
+  HIPCHK(c, hipGetLastError());
+  return MCP_OK;
+}
+
+// Device bytes the parallel-in-R MDPROP needs for ninst instances.
+size_t tree_bytes(int n, int ninst, int nstep) {
+  const size_t nn = (size_t)n * n, k1 = (size_t)(nstep + 1) / 2;
+  return sizeof(double) * nn * ninst * (3 * (size_t)nstep + 3 * k1 + (size_t)nstep + 2 * k1) +
+         sizeof(int) * ((size_t)ninst * nstep + 2 * (size_t)ninst);
+}
+
+// MDPROP for large N as a tree of imbedding-map merges (mcp_tree.hpp): uc,
+// uck the energy-independent sector matrices and their inertias, diag the
+// output of k_mdprop_diag.  Y in I.y is replaced by Y(RSTOP).
+int tree_mdprop(mcp_ctx* c, const mcp::BlockArgs& B, const mcp::GridArgs& G, const mcp::InstArgs& I, int ninst,
+                int nstep, double h, const double* uc, const int* uck, const double* diag, int bk) {
+  const int n = B.n;
+  const size_t nn = (size_t)n * n, sz = 3 * nn;
+  const int k1 = (nstep + 1) / 2;
+  HIPCHK(c, c->s_seg0.ensure((size_t)ninst * nstep * sz * sizeof(double)));
+  HIPCHK(c, c->s_seg1.ensure((size_t)ninst * std::max(k1, 1) * sz * sizeof(double)));
+  HIPCHK(c, c->s_zb.ensure((size_t)ninst * nstep * nn * sizeof(double)));
+  HIPCHK(c, c->s_pq.ensure((size_t)ninst * std::max(k1, 1) * 2 * nn * sizeof(double)));
+  HIPCHK(c, c->s_kc.ensure((size_t)ninst * nstep * sizeof(int)));
+  HIPCHK(c, c->s_acc.ensure((size_t)2 * ninst * sizeof(int)));
+  double* zb = c->s_zb.as<double>();
+  double* pq = c->s_pq.as<double>();
+  int* kc = c->s_kc.as<int>();
+  int* acc = c->s_acc.as<int>();
+  int* bad = acc + ninst;
+  const size_t ll = lds_large(n, 0, 0);
+  hipLaunchKernelGGL(mcp::k_tree_zero, dim3(nblk256(2 * ninst)), dim3(256), 0, c->stream, acc, 2 * ninst);
+  // leaves: each sector's two half-steps merged, Z = (UC + 2 A14)^-1
+  const size_t nleaf = (size_t)ninst * nstep;
+  hipLaunchKernelGGL(mcp::k_tree_c1, dim3(nblk256(nleaf * nn)), dim3(256), 0, c->stream, n, ninst, nstep, uc, diag, zb);
+  hipLaunchKernelGGL(mcp::k_syminv_large, dim3((unsigned)nleaf), dim3(kNT), ll, c->stream, n, zb, kc, bk);
+  hipLaunchKernelGGL(mcp::k_tree_count, dim3(nblk256(nleaf)), dim3(256), 0, c->stream, n, (int)nleaf, nstep, kc, acc,
+                     bad);
+  double* in = c->s_seg0.as<double>();
+  double* out = c->s_seg1.as<double>();
+  hipLaunchKernelGGL(mcp::k_tree_leaf, dim3(nblk256(nleaf * nn)), dim3(256), 0, c->stream, B, G, I, ninst, nstep, h,
+                     diag, zb, in);
+  HIPCHK(c, hipGetLastError());
+  int rc;
+  // levels: segments (2m, 2m+1) -> m
+  for (int kl = nstep; kl > 1;) {
+    const int per = kl / 2, kn = (kl + 1) / 2;
+    const int nb = ninst * per;
+    hipLaunchKernelGGL(mcp::k_tree_w, dim3(nblk256((size_t)nb * nn)), dim3(256), 0, c->stream, n, ninst, kl, per, in,
+                       zb);
+    hipLaunchKernelGGL(mcp::k_syminv_large, dim3(nb), dim3(kNT), ll, c->stream, n, zb, kc, bk);
+    hipLaunchKernelGGL(mcp::k_tree_count, dim3(nblk256(nb)), dim3(256), 0, c->stream, n, nb, per, kc, acc, bad);
+    HIPCHK(c, hipGetLastError());
+    const size_t si = (size_t)kl * sz, sm = 2 * sz;
+    const mcp::SegRef Z{zb, (size_t)per * nn, nn};
+    const mcp::SegRef L1{in, si, sm}, L2{in + nn, si, sm};
+    const mcp::SegRef R2{in + sz + nn, si, sm}, R4{in + sz + 2 * nn, si, sm};
+    const mcp::SegRef P{pq, (size_t)per * 2 * nn, 2 * nn}, Q{pq + nn, (size_t)per * 2 * nn, 2 * nn};
+    const mcp::SegOut Po{pq, (size_t)per * 2 * nn, 2 * nn}, Qo{pq + nn, (size_t)per * 2 * nn, 2 * nn};
+    const mcp::SegOut O1{out, (size_t)kn * sz, sz}, O2{out + nn, (size_t)kn * sz, sz}, O4{out + 2 * nn, (size_t)kn * sz, sz};
+    // P = Z l2^T; Y1 = l1 - l2 P; Y2 = P^T r2; Q = Z r2; Y4 = r4 - r2^T Q
+    if ((rc = tree_gemm<false, true>(c, n, nb, per, Z, L2, Z, Po, 1.0, 0.0))) return rc;
+    if ((rc = tree_gemm<false, false>(c, n, nb, per, L2, P, L1, O1, -1.0, 1.0))) return rc;
+    if ((rc = tree_gemm<true, false>(c, n, nb, per, P, R2, P, O2, 1.0, 0.0))) return rc;
+    if ((rc = tree_gemm<false, false>(c, n, nb, per, Z, R2, Z, Qo, 1.0, 0.0))) return rc;
+    if ((rc = tree_gemm<true, false>(c, n, nb, per, R2, Q, R4, O4, -1.0, 1.0))) return rc;
+    if (kl & 1) {
+      hipLaunchKernelGGL(mcp::k_tree_pass, dim3(nblk256((size_t)ninst * sz)), dim3(256), 0, c->stream, n, ninst, kl, kn,
+                         in, out);
+      HIPCHK(c, hipGetLastError());
+    }
+    std::swap(in, out);
+    kl = kn;
+  }
+  // the starting matrix: Y = Y4 - Y2^T (A0 + Y1)^-1 Y2
+  hipLaunchKernelGGL(mcp::k_tree_final_w, dim3(nblk256((size_t)ninst * nn)), dim3(256), 0, c->stream, B, G, I, ninst, h,
+                     in, zb);
+  hipLaunchKernelGGL(mcp::k_syminv_large, dim3(ninst), dim3(kNT), ll, c->stream, n, zb, kc, bk);
+  hipLaunchKernelGGL(mcp::k_tree_count, dim3(nblk256(ninst)), dim3(256), 0, c->stream, n, ninst, 1, kc, acc, bad);
+  HIPCHK(c, hipGetLastError());
+  {
+    const mcp::SegRef Z{zb, nn, 0}, Y2{in + nn, sz, 0}, Y4{in + 2 * nn, sz, 0}, Q{pq, nn, 0};
+    const mcp::SegOut Qo{pq, nn, 0}, Ao{out, nn, 0};
+    if ((rc = tree_gemm<false, false>(c, n, ninst, 1, Z, Y2, Z, Qo, 1.0, 0.0))) return rc;
+    if ((rc = tree_gemm<true, false>(c, n, ninst, 1, Y2, Q, Y4, Ao, -1.0, 1.0))) return rc;
+  }
+  hipLaunchKernelGGL(mcp::k_tree_out, dim3(nblk256((size_t)ninst * nn)), dim3(256), 0, c->stream, n, ninst, h, out,
+                     I.y);
+  hipLaunchKernelGGL(mcp::k_tree_nodes, dim3(nblk256(ninst)), dim3(256), 0, c->stream, ninst, nstep, uck, acc, bad,
+                     I.nodes, I.status);
+  HIPCHK(c, hipGetLastError());
+  return MCP_OK;
+}
+
 // Device-side propagation of ninst instances whose Y already sits in dy.
 int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const double* d_ered, const double* d_pconst,
                    double* dy, int* d_nodes, int* d_status) {
@@ -351,15 +458,35 @@ int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const do
       const int* uck = c->s_f.as<int>();
       unsigned* ctrs = c->s_bar.as<unsigned>();
       int* abort_flag = (int*)(ctrs + ninst);
+      // parallel-in-R tree where the device memory holds two of its levels
+      if (c->large_mode == 0 && !c->no_reg) {
+        size_t freeb = 0, totalb = 0;
+        (void)hipMemGetInfo(&freeb, &totalb);
+        const size_t held = c->s_seg0.bytes + c->s_seg1.bytes + c->s_zb.bytes + c->s_pq.bytes;
+        const size_t dgb = (size_t)ninst * 3 * nstep * n * sizeof(double);
+        if (tree_bytes(n, ninst, nstep) + dgb + (size_t(2) << 30) <= freeb + held) {
+          HIPCHK(c, c->s_dg.ensure(dgb));
+          const size_t ndiag = (size_t)ninst * nstep * n;
+          hipLaunchKernelGGL(mcp::k_mdprop_diag, dim3((unsigned)((ndiag + 255) / 256)), dim3(256), 0, c->stream, B, G,
+                             I, ninst, nstep, h, c->s_dg.as<double>());
+          HIPCHK(c, hipGetLastError());
+          if ((rc = tree_mdprop(c, B, G, I, ninst, nstep, h, ucp, uck, c->s_dg.as<double>(), bk))) return rc;
+          HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+          return MCP_OK;
+        }
+      }
       // register-resident matrix-core kernel when the instances' teams fit
-      // the device: one column block per workgroup where there are CUs for
-      // it, else two
+      // the device: one column block per workgroup, or two where there are
+      // not CUs enough (or a smaller team was asked for)
       const int nblk = (n + mcp::kPB - 1) / mcp::kPB;
-      const int rb = (size_t)nblk * ninst <= (size_t)c->num_cu ? 1 : 2;
-      const int gmin = (nblk + rb - 1) / rb;
+      int rb = 1, g = nblk;
+      if ((c->team > 0 && c->team < nblk) || (size_t)nblk * ninst > (size_t)c->num_cu) {
+        rb = 2;
+        g = std::min(nblk, std::max(c->team, (nblk + 1) / 2));
+      }
       const size_t lr = lds_mfma(n, b.nham);
-      if (n <= mcp::kMMaxN && lr <= c->lds_max && !c->no_reg && (size_t)gmin * ninst <= (size_t)c->num_cu) {
-        const int g = std::min(std::max(gmin, c->team), std::max(gmin, c->num_cu / ninst));
+      if (n <= mcp::kMMaxN && lr <= c->lds_max && !c->no_reg && c->large_mode <= 1 &&
+          (size_t)g * ninst <= (size_t)c->num_cu) {
         HIPCHK(c, c->s_x.ensure((size_t)ninst * 2 * mcp::kMSlots * mcp::panel_ld(n) * sizeof(double)));
         HIPCHK(c, c->s_dg.ensure((size_t)ninst * 3 * nstep * n * sizeof(double)));
         double* xg = c->s_x.as<double>();
@@ -1011,6 +1138,12 @@ int mcp_bound_match(mcp_ctx* c, int32_t bid, int32_t gid_out, int32_t gid_in, in
     if (tmp[4 * ninst + i] != 0) status[i] = -(100000 + tmp[4 * ninst + i]);  // inward
     if (tmp[2 * ninst + i] != 0 || tmp[5 * ninst + i] != 0) status[i] = -1000000;
   }
+  return MCP_OK;
+}
+
+int mcp_set_large_mode(mcp_ctx* c, int32_t mode) {
+  if (!c || mode < 0 || mode > 2) return c ? c->fail(MCP_ERR_ARG, "large mode %d: 0, 1 or 2", mode) : MCP_ERR_ARG;
+  c->large_mode = mode;
   return MCP_OK;
 }
 

@@ -1,0 +1,265 @@
+// MDPROP for large N solved in parallel over the radial grid.
+//
+// The recurrence of mdprop.f:118-233 is a chain of maps of the form
+//   X -> y4 - y3 (X + y1)^-1 y2,
+// two per sector: H1 with y1 = diag(Y14), y2 = y3 = diag(Y23), y4 = UC +
+// diag(2 A14) (the first inversion and mdprop.f:181-185), and H2 with y1 = 0,
+// y2 = y3 = diag(Y23), y4 = D2 W(B) + diag(Y14) (the second inversion and
+// mdprop.f:197-230).  Two such maps compose into one of the same form
+// (R after L):
+//   Z = (l4 + r1)^-1,
+//   Y1 = l1 - l2 Z l3,  Y2 = l2 Z r2,  Y3 = r3 Z l3,  Y4 = r4 - r3 Z r2,
+// with Y1, Y4 symmetric and Y3 = Y2^T when the parts are.  The whole
+// propagation is therefore reduced pairwise in a tree of log2(2 nstep)
+// levels, every level one batch of independent merges (an inversion and five
+// matrix products each), and applied to the starting matrix at the end.
+//
+// Node count.  The sequential recurrence inverts the pivots of the block
+// L D L^T of one block-tridiagonal matrix; the tree inverts the pivots of
+// another elimination order of the same matrix (each merge pivot l4 + r1 is
+// the Schur complement of an interface block).  By the inertia additivity of
+// Schur complements both sums of negative-pivot counts equal the matrix's
+// number of negative eigenvalues, so sum(KOUNT) - NSAVE is the same integer.
+//
+// Storage: a segment is (Y1, Y2, Y4), three n x n column-major matrices.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mcp_mfmaprop.hpp"
+
+namespace mcp {
+
+// matrix of batch item (inst, m): base + inst*si + m*sm
+struct SegRef {
+  const double* base;
+  size_t si, sm;
+  __device__ __forceinline__ const double* at(int inst, int m) const { return base + inst * si + m * sm; }
+};
+
+struct SegOut {
+  double* base;
+  size_t si, sm;
+  __device__ __forceinline__ double* at(int inst, int m) const { return base + inst * si + m * sm; }
+};
+
+// W(i, j) at grid point k (HAMMAT: sumlam plus the diagonal terms)
+__device__ __forceinline__ double w_elem(const BlockArgs& B, const GridArgs& G, int k, const double* pconst, int i,
+                                         int j, double ered) {
+  const double r = G.r[k], rsq = rsq_of(r);
+  const double* v = B.vl + tri_index(i, j) * (size_t)B.nham;
+  double s = 0.0;
+  for (int l = 0; l < B.nham; ++l) {
+    const double pl = l < B.mxlam ? G.ptab[(size_t)k * B.mxlam + l]
+                                  : (l < B.mxlam + B.nconst ? pconst[l - B.mxlam] : rsq);
+    s += v[l] * pl;
+  }
+  if (i == j) s = hammat_diag(B, i, s, r, rsq, ered);
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// Batched n x n products on the FP64 matrix cores:
+//   Co = alpha op(A) op(B) + beta Ci,
+// batch item b = (b / per, b % per) on grid x, one 64 x 64 tile of Co per
+// workgroup on grid y
+// (four waves of 32 x 32, each 2 x 2 MFMA tiles), k in steps of 16 through LDS.
+// The MFMA computes the tile transposed (op(B)^T op(A)^T), so that register v
+// of lane l is Co(m0 + l%16, n0 + l/16 + 4v) and stores run down columns.
+// ---------------------------------------------------------------------------
+constexpr int kGT = 64;   // tile edge
+constexpr int kGK = 16;   // k step
+constexpr int kGPad = 4;  // LDS row padding (doubles)
+
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(256) k_gemm_seg(int n, int per, SegRef A, SegRef Bm, SegRef Ci, SegOut Co,
+                                                  double alpha, double beta) {
+  __shared__ double As[kGK][kGT + kGPad];
+  __shared__ double Bs[kGK][kGT + kGPad];
+  const int bt = blockIdx.x;
+  const int inst = bt / per, m = bt - inst * per;
+  const int nt = (n + kGT - 1) / kGT;
+  const int m0 = (int)(blockIdx.y % nt) * kGT, n0 = (int)(blockIdx.y / nt) * kGT;
+  const double* a = A.at(inst, m);
+  const double* bb = Bm.at(inst, m);
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int q = lane >> 4, r16 = lane & 15;
+  const int mb = (w & 1) * 32, nb = (w >> 1) * 32;
+  dvec4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = dvec4{0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < n; k0 += kGK) {
+#pragma unroll
+    for (int e0 = 0; e0 < kGT * kGK; e0 += 256) {
+      const int e = e0 + tid;
+      {
+        const int mm = TA ? e >> 4 : e & 63, kk = TA ? e & 15 : e >> 6;
+        const int gm = m0 + mm, gk = k0 + kk;
+        double v = 0.0;
+        if (gm < n && gk < n) v = TA ? a[gk + (size_t)gm * n] : a[gm + (size_t)gk * n];
+        As[kk][mm] = v;
+      }
+      {
+        const int nn = TB ? e & 63 : e >> 4, kk = TB ? e >> 6 : e & 15;
+        const int gn = n0 + nn, gk = k0 + kk;
+        double v = 0.0;
+        if (gn < n && gk < n) v = TB ? bb[gn + (size_t)gk * n] : bb[gk + (size_t)gn * n];
+        Bs[kk][nn] = v;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int kk = 4 * c + q;
+      double av[2], bv[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) av[i] = As[kk][mb + 16 * i + r16];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) bv[j] = Bs[kk][nb + 16 * j + r16];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = mma(bv[j], av[i], acc[i][j]);
+    }
+    __syncthreads();
+  }
+  const double* ci = beta != 0.0 ? Ci.at(inst, m) : nullptr;
+  double* co = Co.at(inst, m);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int gm = m0 + mb + 16 * i + r16, gn = n0 + nb + 16 * j + q + 4 * v;
+        if (gm < n && gn < n) {
+          const size_t idx = gm + (size_t)gn * n;
+          double r = alpha * acc[i][j][v];
+          if (ci) r += beta * ci[idx];
+          co[idx] = r;
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// elementwise stages; one thread per element of the batch
+// ---------------------------------------------------------------------------
+// c1 = UC(k) + diag(2 A14) of every (instance, sector), into Zb
+__global__ void k_tree_c1(int n, int ninst, int nstep, const double* __restrict__ uc, const double* __restrict__ diag,
+                          double* __restrict__ zb) {
+  const size_t nn = (size_t)n * n;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nn * nstep * ninst) return;
+  const size_t mat = e / nn, r = e - mat * nn;
+  const int inst = (int)(mat / nstep), k = (int)(mat - (size_t)inst * nstep);
+  const int j = (int)(r / n), i = (int)(r - (size_t)j * n);
+  double v = uc[(size_t)k * nn + r];
+  if (i == j) v += diag[((size_t)inst * 3 + 2) * nstep * n + (size_t)k * n + i];
+  zb[e] = v;
+}
+
+// leaf segments from Z = c1^-1: M = D Z D, Y1 = diag(Y14) - M, Y2 = M,
+// Y4 = D2 W(B) + diag(Y14) - M
+__global__ void k_tree_leaf(BlockArgs B, GridArgs G, InstArgs I, int ninst, int nstep, double h,
+                            const double* __restrict__ diag, const double* __restrict__ zb, double* __restrict__ seg) {
+  const int n = B.n;
+  const size_t nn = (size_t)n * n;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nn * nstep * ninst) return;
+  const size_t mat = e / nn, r = e - mat * nn;
+  const int inst = (int)(mat / nstep), k = (int)(mat - (size_t)inst * nstep);
+  const int j = (int)(r / n), i = (int)(r - (size_t)j * n);
+  const size_t per = (size_t)nstep * n;
+  const double* dg = diag + (size_t)inst * 3 * per + (size_t)k * n;
+  const double* y14 = dg;
+  const double* y23 = dg + per;
+  const double mij = y23[i] * zb[e] * y23[j];
+  const double* pconst = I.pconst ? I.pconst + (size_t)inst * B.nconst : nullptr;
+  const double d1 = h * h / 3.0;
+  const double dd2 = (k == nstep - 1) ? d1 : 2.0 * d1;
+  double wb = dd2 * w_elem(B, G, 2 * (k + 1), pconst, i, j, I.ered[inst]);
+  double* s = seg + mat * 3 * nn;
+  s[r] = (i == j ? y14[i] : 0.0) - mij;
+  s[nn + r] = mij;
+  s[2 * nn + r] = wb + (i == j ? y14[i] : 0.0) - mij;
+}
+
+// merge pivots l4 + r1 of segments (2m, 2m+1) of level kl into Zb[inst*per + m]
+__global__ void k_tree_w(int n, int ninst, int kl, int per, const double* __restrict__ seg,
+                         double* __restrict__ zb) {
+  const size_t nn = (size_t)n * n;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nn * per * ninst) return;
+  const size_t mat = e / nn, r = e - mat * nn;
+  const int inst = (int)(mat / per), m = (int)(mat - (size_t)inst * per);
+  const double* L = seg + ((size_t)inst * kl + 2 * m) * 3 * nn;
+  const double* R = L + 3 * nn;
+  zb[e] = L[2 * nn + r] + R[r];
+}
+
+// the odd last segment of a level passes to the next unchanged
+__global__ void k_tree_pass(int n, int ninst, int kl, int kn, const double* __restrict__ in, double* __restrict__ out) {
+  const size_t sz = 3 * (size_t)n * n;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= sz * ninst) return;
+  const int inst = (int)(e / sz);
+  const size_t r = e - (size_t)inst * sz;
+  out[((size_t)inst * kn + kn - 1) * sz + r] = in[((size_t)inst * kl + kl - 1) * sz + r];
+}
+
+// final pivot A0 + Y1, A0 = h Y(RSTART) + D1 W(RSTART) (mdprop.f:118-126)
+__global__ void k_tree_final_w(BlockArgs B, GridArgs G, InstArgs I, int ninst, double h, const double* __restrict__ seg,
+                               double* __restrict__ zb) {
+  const int n = B.n;
+  const size_t nn = (size_t)n * n;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nn * ninst) return;
+  const int inst = (int)(e / nn);
+  const size_t r = e - (size_t)inst * nn;
+  const int j = (int)(r / n), i = (int)(r - (size_t)j * n);
+  const double* pconst = I.pconst ? I.pconst + (size_t)inst * B.nconst : nullptr;
+  const double d1 = h * h / 3.0;
+  const double a0 = h * I.y[e] + d1 * w_elem(B, G, 0, pconst, i, j, I.ered[inst]);
+  zb[e] = a0 + seg[(size_t)inst * 3 * nn + r];
+}
+
+// Y(RSTOP) = A / h
+__global__ void k_tree_out(int n, int ninst, double h, const double* __restrict__ a, double* __restrict__ y) {
+  const size_t nn = (size_t)n * n;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nn * ninst) return;
+  y[e] = a[e] / h;
+}
+
+// negative-pivot counts of a batch of inversions (item b of instance b / per)
+// into the instances' totals; a failed inversion marks the instance
+__global__ void k_tree_count(int n, int nb, int per, const int* __restrict__ kount, int* __restrict__ acc,
+                             int* __restrict__ bad) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int inst = b / per;
+  const int kc = kount[b];
+  if (kc < 0 || kc > n)
+    atomicMax(bad + inst, 1);
+  else
+    atomicAdd(acc + inst, kc);
+}
+
+__global__ void k_tree_zero(int* __restrict__ p, int cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < cnt) p[i] = 0;
+}
+
+// NODES = sum(KOUNT) - NSAVE and STATUS
+__global__ void k_tree_nodes(int ninst, int nstep, const int* __restrict__ uc_kount, const int* __restrict__ acc,
+                             const int* __restrict__ bad, int* __restrict__ nodes, int* __restrict__ status) {
+  const int inst = blockIdx.x * blockDim.x + threadIdx.x;
+  if (inst >= ninst) return;
+  int nsave = 0;
+  for (int k = 0; k < nstep; ++k) nsave += uc_kount[k];
+  nodes[inst] = acc[inst] - nsave;
+  status[inst] = bad[inst] ? 1 : 0;
+}
+
+}  // namespace mcp

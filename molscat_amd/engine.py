@@ -283,6 +283,11 @@ class Engine:
         """Workgroups per instance on the large-N path (0 = automatic)."""
         self._check(self.lib.mcp_set_team_size(self.ctx, int(g)), "mcp_set_team_size")
 
+    def set_large_mode(self, mode: int):
+        """Large-N MDPROP: 0 tree over the radial grid where memory allows
+        (default), 1 sequential matrix-core kernel, 2 global-memory kernel."""
+        self._check(self.lib.mcp_set_large_mode(self.ctx, int(mode)), "mcp_set_large_mode")
+
     def last_timing(self) -> dict:
         v = [ctypes.c_double() for _ in range(3)]
         self._check(self.lib.mcp_last_timing(self.ctx, *[ctypes.byref(x) for x in v]), "mcp_last_timing")

@@ -51,6 +51,7 @@ SIGNATURES = {
     "mcp_bound_match": (ctypes.c_int, [_vp, _i32, _i32, _i32, _i32, _i32, _i32, _pd, _pd, _pi, _pd, _pi, _pd, _pd,
                                         _pi]),
     "mcp_set_team_size": (ctypes.c_int, [_vp, _i32]),
+    "mcp_set_large_mode": (ctypes.c_int, [_vp, _i32]),
     "mcp_last_timing": (ctypes.c_int, [_vp, _pd, _pd, _pd]),
 }
 

@@ -101,6 +101,7 @@ def test_team_size_bitwise_identical(eng):
     ered = 2000.0 * blk.cm2ru
     y0 = eng.yinit(b, g, [ered])
     outs = []
+    eng.set_large_mode(1)  # the sequential matrix-core kernel
     for team in (1, 4, 32):
         eng.set_team_size(team)
         print(f"team={team}", flush=True)
@@ -112,9 +113,38 @@ def test_team_size_bitwise_identical(eng):
     print("team=8 global-memory kernel", flush=True)
     yg, nodes_g, st = eng.prop_ld(b, g, 6, [ered], y0)
     eng.set_team_size(0)
+    eng.set_large_mode(0)
     assert st[0] == 0
     for y, nodes in outs[1:]:
         assert nodes == outs[0][1]
         assert np.array_equal(y, outs[0][0])
     assert int(nodes_g[0]) == outs[0][1]
     assert np.abs(yg[0] - outs[0][0]).max() < 1e-12 * np.abs(yg[0]).max()
+
+
+@pytest.mark.parametrize("nstep", [300, 301])
+def test_tree_matches_sequential(eng, nstep):
+    """The parallel-in-R MDPROP (tree of merged sector maps) against the
+    sequential recurrence on the same grid: node counts identical (the
+    inertia of one block-tridiagonal matrix in two elimination orders), Y to
+    1e-10 of its largest element.  An odd sector count passes a segment
+    through a level unmerged."""
+    blk = _big_block()
+    rstart, rstop = 2.0, 8.0
+    dr = (rstop - rstart) / nstep
+    r = S.ld_radii(rstart, dr, nstep)
+    b = eng.block(blk.vl, blk.eint, blk.cent, mxlam=blk.mxlam)
+    g = eng.grid(r, blk.ptab(r), dr)
+    ered = np.array([300.0, 2000.0]) * blk.cm2ru
+    y0 = eng.yinit(b, g, ered)
+    print(f"tree vs sequential N={blk.n} nstep={nstep}", flush=True)
+    eng.set_large_mode(1)
+    ys, ns, ss = eng.prop_ld(b, g, 6, ered, y0)
+    eng.set_large_mode(0)
+    yt, nt, st = eng.prop_ld(b, g, 6, ered, y0)
+    for m in range(len(ered)):
+        assert ss[m] == 0 and st[m] == 0
+        assert int(nt[m]) == int(ns[m])
+        err = np.abs(yt[m] - ys[m]).max() / np.abs(ys[m]).max()
+        print(f"  E={ered[m] / blk.cm2ru:.0f} nodes={int(nt[m])} rel|dY|={err:.2e}", flush=True)
+        assert err < 1e-10
