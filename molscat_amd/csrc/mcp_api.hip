@@ -174,6 +174,7 @@ struct mcp_ctx {
   DevBuf s_seg0, s_seg1;          // segments of the parallel-in-R MDPROP, two tree levels
   DevBuf s_zb, s_pq;              // its merge pivots / inverses and products
   DevBuf s_kc, s_acc;             // its negative-pivot counts
+  DevBuf s_yv;                    // large-N YINIT: scales, flags, inertias
   double t_sector = 0.0, t_prop = 0.0, t_total = 0.0;
 
   int fail(int code, const char* fmt, ...) {
@@ -208,7 +209,8 @@ int set_lds_limits(mcp_ctx* c) {
                        (const void*)mcp::k_syminv_large,   (const void*)mcp::k_mdprop_uc_large,
                        (const void*)mcp::k_mdprop_large,   (const void*)mcp::k_ldprop_large,
                        (const void*)mcp::k_yinit_large,    (const void*)mcp::k_airy_sector,
-                       (const void*)mcp::k_airy_prop,      (const void*)mcp::k_mdprop_mfma<1>, (const void*)mcp::k_mdprop_mfma<2>};
+                       (const void*)mcp::k_airy_prop,      (const void*)mcp::k_mdprop_mfma<1>, (const void*)mcp::k_mdprop_mfma<2>,
+                       (const void*)mcp::k_syminv_mfma<1>};
   const size_t total = c->lds_max;
   size_t lim = total;
   for (const void* f : fns) {
@@ -305,6 +307,38 @@ int prep_team(mcp_ctx* c, int ninst) {
 }
 
 unsigned nblk256(size_t cnt) { return (unsigned)((cnt + 255) / 256); }
+
+// Full symmetric inverses and inertias of nb n x n matrices at a (stride
+// n^2) in place, each on a team of ceil(n/16) workgroups on the matrix cores
+// (for a few matrices: latency-bound, the whole device on them).  False when
+// the device cannot hold one team.
+bool team_inverse_fits(mcp_ctx* c, int n) {
+  const int g = (n + mcp::kPB - 1) / mcp::kPB;
+  return n <= mcp::kMMaxN && g <= c->num_cu && lds_mfma(n, 0) <= c->lds_max;
+}
+
+int team_inverse(mcp_ctx* c, int n, int nb, double* a, int* kount, int bk) {
+  const int g = (n + mcp::kPB - 1) / mcp::kPB;
+  const size_t lr = lds_mfma(n, 0);
+  const int chunk = std::max(1, std::min(nb, c->num_cu / g));
+  HIPCHK(c, c->s_x.ensure((size_t)chunk * 2 * mcp::kMSlots * mcp::panel_ld(n) * sizeof(double)));
+  HIPCHK(c, c->s_bar.ensure((size_t)(chunk + 1) * sizeof(unsigned)));
+  double* xg = c->s_x.as<double>();
+  for (int b0 = 0; b0 < nb; b0 += chunk) {
+    const int m = std::min(chunk, nb - b0);
+    unsigned* ctrs = c->s_bar.as<unsigned>();
+    int* abort_flag = (int*)(ctrs + m);
+    hipLaunchKernelGGL(mcp::k_zero_u32, dim3((m + 1 + 255) / 256), dim3(256), 0, c->stream, ctrs, m + 1);
+    HIPCHK(c, hipGetLastError());
+    double* ab = a + (size_t)b0 * n * n;
+    int* kb = kount + b0;
+    int n_ = n, bk_ = bk, g_ = g;
+    void* args[] = {&n_, &ab, &kb, &bk_, &g_, &ctrs, &abort_flag, &xg};
+    HIPCHK(c, hipLaunchCooperativeKernel((const void*)mcp::k_syminv_mfma<1>, dim3(m * g), dim3(mcp::kMT), args, lr,
+                                         c->stream));
+  }
+  return MCP_OK;
+}
 
 template <bool TA, bool TB>
 int tree_gemm(mcp_ctx* c, int n, int nbatch, int per, mcp::SegRef A, mcp::SegRef Bm, mcp::SegRef Ci, mcp::SegOut Co,
@@ -832,6 +866,41 @@ static int yinit_device(mcp_ctx* c, Block& b, Grid& g, mcp::InstArgs I, int nins
   if (ll > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "yinit: N=%d exceeds the large-N panel limit", n);
   HIPCHK(c, c->s_y.ensure((size_t)ninst * 3 * n * n * sizeof(double)));
   I.status = d_status;
+  const int nblk = (n + mcp::kPB - 1) / mcp::kPB;
+  if (team_inverse_fits(c, n) && (size_t)2 * ninst * nblk <= (size_t)c->num_cu) {
+    // the square-root iteration of k_yinit_large in stages, both inverses
+    // of an iteration on teams of workgroups
+    const size_t nn = (size_t)n * n;
+    double* work = c->s_y.as<double>();
+    double* zs = work + 2 * nn * ninst;
+    HIPCHK(c, c->s_yv.ensure((size_t)ninst * (sizeof(double) + 4 * sizeof(int))));
+    double* scale = c->s_yv.as<double>();
+    int* done = (int*)(scale + ninst);
+    int* kc = done + ninst;
+    hipLaunchKernelGGL(mcp::k_yl_start, dim3(ninst), dim3(kNT), 0, c->stream, block_args(b), grid_args(g), I, k0, zs,
+                       scale, d_status);
+    HIPCHK(c, hipMemsetAsync(done, 0, ninst * sizeof(int), c->stream));
+    std::vector<int> hd(2 * (size_t)ninst);
+    int rc;
+    for (int it = 0; it < 40; ++it) {
+      hipLaunchKernelGGL(mcp::k_yl_copy, dim3(nblk256(ninst * nn)), dim3(256), 0, c->stream, n, ninst, I.y, zs, work);
+      HIPCHK(c, hipGetLastError());
+      if ((rc = team_inverse(c, n, 2 * ninst, work, kc, 1))) return rc;
+      hipLaunchKernelGGL(mcp::k_yl_update, dim3(ninst), dim3(kNT), 0, c->stream, n, it, I.y, zs, work, kc, done,
+                         d_status);
+      HIPCHK(c, hipGetLastError());
+      HIPCHK(c, hipMemcpyAsync(hd.data(), done, ninst * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipMemcpyAsync(hd.data() + ninst, d_status, ninst * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      bool all = true;
+      for (int i = 0; i < ninst; ++i) all = all && (hd[i] || hd[ninst + i]);
+      if (all) break;
+    }
+    hipLaunchKernelGGL(mcp::k_yl_finish, dim3(nblk256(ninst * nn)), dim3(256), 0, c->stream, n, ninst, outward, scale,
+                       I.y);
+    HIPCHK(c, hipGetLastError());
+    return MCP_OK;
+  }
   hipLaunchKernelGGL(mcp::k_yinit_large, dim3(ninst), dim3(kNT), ll, c->stream, block_args(b), grid_args(g), I, k0,
                      outward, c->s_y.as<double>());
   HIPCHK(c, hipGetLastError());

@@ -563,44 +563,74 @@ __device__ __attribute__((always_inline)) void apply_panel(double* A, int n, int
   const double* gslot[kPB];
 #pragma unroll
   for (int q = 0; q < kPB; ++q) gslot[q] = s.G + s.pslot[q] * ld;
-  for (int tau = t.rank; tau < ntt; tau += t.g) {
+  // the owned tiles in turn, each tile's values fetched while the previous
+  // one is updated (one workgroup per CU leaves no other wave to hide the
+  // load latency)
+  auto tile_at = [&](int tau, int& r0, int& c0) {
     const int bi = tile_row(tau);
     const int bj = tau - bi * (bi + 1) / 2;
-    const int r0 = bi * kTile + 4 * tx, c0 = bj * kTile + 4 * ty;
-    if (c0 >= n || r0 >= n || r0 + 3 < c0) continue;
-    // current tile values first, so their latency overlaps the products
-    double av[4][4], acc[4][4];
+    r0 = bi * kTile + 4 * tx;
+    c0 = bj * kTile + 4 * ty;
+  };
+  auto fetch = [&](int r0, int c0, double (&av)[4][4]) {
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       const int cc = min(c0 + b, n - 1);
 #pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        av[a][b] = A[min(r0 + a, n - 1) + (size_t)cc * n];
-        acc[a][b] = 0.0;
-      }
+      for (int a = 0; a < 4; ++a) av[a][b] = A[min(r0 + a, n - 1) + (size_t)cc * n];
     }
-#pragma unroll
-    for (int q = 0; q < kPB; ++q) {
-      const double2* xq = reinterpret_cast<const double2*>(s.X + q * ld + r0);
-      const double2* gq = reinterpret_cast<const double2*>(gslot[q] + c0);
-      const double2 x01 = xq[0], x23 = xq[1], g01 = gq[0], g23 = gq[1];
-      const double xv[4] = {x01.x, x01.y, x23.x, x23.y};
-      const double gv[4] = {g01.x, g01.y, g23.x, g23.y};
+  };
+  int tau = t.rank;
+  double av[4][4];
+  int r0 = 0, c0 = 0;
+  if (tau < ntt) {
+    tile_at(tau, r0, c0);
+    fetch(r0, c0, av);
+  }
+  while (tau < ntt) {
+    const int tn = tau + t.g;
+    int rn = 0, cn = 0;
+    double an[4][4];
+    if (tn < ntt) {
+      tile_at(tn, rn, cn);
+      fetch(rn, cn, an);
+    }
+    if (!(c0 >= n || r0 >= n || r0 + 3 < c0)) {
+      double acc[4][4];
 #pragma unroll
       for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] += xv[a] * gv[b];
-    }
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int cc = c0 + b;
-      if (cc >= n || s.inP[cc]) continue;
+      for (int q = 0; q < kPB; ++q) {
+        const double2* xq = reinterpret_cast<const double2*>(s.X + q * ld + r0);
+        const double2* gq = reinterpret_cast<const double2*>(gslot[q] + c0);
+        const double2 x01 = xq[0], x23 = xq[1], g01 = gq[0], g23 = gq[1];
+        const double xv[4] = {x01.x, x01.y, x23.x, x23.y};
+        const double gv[4] = {g01.x, g01.y, g23.x, g23.y};
 #pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        const int i = r0 + a;
-        if (i < n && i >= cc && !s.inP[i]) A[i + (size_t)cc * n] = av[a][b] - acc[a][b];
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) acc[a][b] += xv[a] * gv[b];
+      }
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int cc = c0 + b;
+        if (cc >= n || s.inP[cc]) continue;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          const int i = r0 + a;
+          if (i < n && i >= cc && !s.inP[i]) A[i + (size_t)cc * n] = av[a][b] - acc[a][b];
+        }
       }
     }
+    tau = tn;
+    r0 = rn;
+    c0 = cn;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) av[a][b] = an[a][b];
   }
   for (int q = 0; q < np; ++q) {
     const int p = s.pidx[q];
@@ -1087,6 +1117,132 @@ __global__ void __launch_bounds__(kNT) k_yinit_large(BlockArgs B, GridArgs G, In
     Y[j + (size_t)i * n] = v;
   });
   if (threadIdx.x == 0 && I.status) I.status[inst] = bad ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// large-N YINIT as separate stages around team inversions (the iteration of
+// k_yinit_large, for one or a few instances): one workgroup per instance.
+// work: [ninst][2][n*n] the two matrices of an iteration (Y, Z copies),
+// then their inverses; zs: [ninst][n*n] Z.
+// ---------------------------------------------------------------------------
+// Y = W(R0) / scale, scale = trace / n, Z = I; st[inst] = 1 when scale <= 0
+__global__ void __launch_bounds__(kNT) k_yl_start(BlockArgs B, GridArgs G, InstArgs I, int k0,
+                                                  double* __restrict__ zs, double* __restrict__ scale,
+                                                  int* __restrict__ st) {
+  __shared__ double sv[kNT / 64];
+  __shared__ double pv[64];
+  const int inst = blockIdx.x;
+  const int n = B.n;
+  const size_t nn = (size_t)n * n;
+  const double ered = I.ered[inst];
+  const double* pconst = I.pconst ? I.pconst + (size_t)inst * B.nconst : nullptr;
+  double* Y = I.y + (size_t)inst * nn;
+  double* Z = zs + (size_t)inst * nn;
+  const bool lds_pv = B.nham <= 64;
+  if (lds_pv) load_pvec(B, G, k0, pconst, pv);
+  __syncthreads();
+  const double r0 = G.r[k0], rsq0 = rsq_of(r0);
+  double tr = 0.0;
+  for (size_t e = threadIdx.x; e < nn; e += kNT) {
+    const int j = (int)(e / n), i = (int)(e - (size_t)j * n);
+    double w;
+    if (lds_pv) {
+      w = sumlam(B, i, j, pv);
+    } else {
+      w = 0.0;
+      const double* v = B.vl + tri_index(i, j) * (size_t)B.nham;
+      for (int l = 0; l < B.nham; ++l) {
+        const double pl = l < B.mxlam ? G.ptab[(size_t)k0 * B.mxlam + l]
+                                      : (l < B.mxlam + B.nconst ? pconst[l - B.mxlam] : rsq0);
+        w += v[l] * pl;
+      }
+    }
+    if (i == j) {
+      w = hammat_diag(B, i, w, r0, rsq0, ered);
+      tr += w;
+    }
+    Y[e] = w;
+    Z[e] = i == j ? 1.0 : 0.0;
+  }
+  tr = block_sum<kNT>(tr, sv);
+  const double sc = tr / n;
+  __syncthreads();
+  if (sc > 0.0)
+    for (size_t e = threadIdx.x; e < nn; e += kNT) Y[e] /= sc;
+  if (threadIdx.x == 0) {
+    scale[inst] = sc;
+    st[inst] = sc > 0.0 ? 0 : 1;
+  }
+}
+
+// the iteration's matrices: work[inst] = (Y, Z)
+__global__ void k_yl_copy(int n, int ninst, const double* __restrict__ y, const double* __restrict__ zs,
+                          double* __restrict__ work) {
+  const size_t nn = (size_t)n * n;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nn * ninst) return;
+  const int inst = (int)(e / nn);
+  const size_t r = e - (size_t)inst * nn;
+  work[(size_t)inst * 2 * nn + r] = y[e];
+  work[(size_t)inst * 2 * nn + nn + r] = zs[e];
+}
+
+// Y <- (Y + Z^-1)/2, Z <- (Z + Y^-1)/2 from the inverses in work; done[inst]
+// = 1 once converged (the test of k_yinit_large), st = 1 on a non-positive
+// inverse
+__global__ void __launch_bounds__(kNT) k_yl_update(int n, int it, double* __restrict__ y, double* __restrict__ zs,
+                                                   const double* __restrict__ work, const int* __restrict__ kc,
+                                                   int* __restrict__ done, int* __restrict__ st) {
+  __shared__ double sv[kNT / 64];
+  __shared__ int si[kNT / 64];
+  const int inst = blockIdx.x;
+  if (done[inst] || st[inst]) return;
+  if (kc[2 * inst] != 0 || kc[2 * inst + 1] != 0) {
+    if (threadIdx.x == 0) st[inst] = 1;
+    return;
+  }
+  const size_t nn = (size_t)n * n;
+  double* Y = y + (size_t)inst * nn;
+  double* Z = zs + (size_t)inst * nn;
+  const double* Yi = work + (size_t)inst * 2 * nn;
+  const double* Zi = Yi + nn;
+  // from the lower triangles, mirrored (the iterates stay exactly symmetric)
+  double dmax = 0.0, ymax = 0.0;
+  for (size_t e = threadIdx.x; e < nn; e += kNT) {
+    const int j = (int)(e / n), i = (int)(e - (size_t)j * n);
+    if (i < j) continue;
+    const double yn = 0.5 * (Y[e] + Zi[e]);
+    const double zn = 0.5 * (Z[e] + Yi[e]);
+    dmax = fmax(dmax, fabs(yn - Y[e]));
+    ymax = fmax(ymax, fabs(yn));
+    const size_t et = j + (size_t)i * n;
+    Y[e] = yn;
+    Y[et] = yn;
+    Z[e] = zn;
+    Z[et] = zn;
+  }
+  int d0 = 0;
+  block_argmax<kNT>(dmax, d0, sv, si);
+  int d1 = 0;
+  block_argmax<kNT>(ymax, d1, sv, si);
+  if (threadIdx.x == 0 && it >= 2 && dmax <= 1.0e-15 * ymax) done[inst] = 1;
+}
+
+// Y <- +-sqrt(scale) Y, symmetrised from the lower triangle
+__global__ void k_yl_finish(int n, int ninst, int outward, const double* __restrict__ scale, double* __restrict__ y) {
+  const size_t nn = (size_t)n * n;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nn * ninst) return;
+  const int inst = (int)(e / nn);
+  const size_t r = e - (size_t)inst * nn;
+  const int j = (int)(r / n), i = (int)(r - (size_t)j * n);
+  if (i < j) return;
+  const double sc = scale[inst];
+  const double f = (outward ? 1.0 : -1.0) * sqrt(sc > 0.0 ? sc : 0.0);
+  double* Y = y + (size_t)inst * nn;
+  const double v = f * Y[r];
+  Y[r] = v;
+  Y[j + (size_t)i * n] = v;
 }
 
 }  // namespace mcp

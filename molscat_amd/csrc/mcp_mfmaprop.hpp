@@ -488,6 +488,40 @@ __device__ __attribute__((always_inline)) int mf_sweep(dvec4 (&T)[kMTW][RB], int
   return kount;
 }
 
+// Symmetric inversion with inertia on the matrix cores for a few matrices:
+// matrix b (n x n, full, column-major at a + b n^2) on a team of g
+// workgroups of kMT threads (latency-bound: for one or a handful of
+// matrices, where a single workgroup per matrix would leave the device
+// idle).  a is overwritten by the full inverse, kount[b] by the number of
+// negative pivots (or the sweep's error code, the matrix then left as it
+// was).  E: exchange areas, 2 x kMSlots x ld doubles per matrix.
+template <int RB>
+__global__ void __launch_bounds__(kMT) k_syminv_mfma(int n, double* __restrict__ a, int* __restrict__ kount, int bk,
+                                                     int g, unsigned* __restrict__ ctrs, int* __restrict__ abort_flag,
+                                                     double* __restrict__ xchg) {
+  extern __shared__ double smem[];
+  const int inst = blockIdx.x / g;
+  LdsCarve c(smem);
+  MfmaScratch s = carve_mfma(c, n, 0);
+  Team tm = make_team(g, ctrs, inst, abort_flag);
+  double* E = xchg + (size_t)inst * 2 * kMSlots * s.ld;
+  double* A = a + (size_t)inst * n * n;
+  dvec4 T[kMTW][RB];
+#pragma unroll
+  for (int tt = 0; tt < kMTW; ++tt)
+#pragma unroll
+    for (int b = 0; b < RB; ++b) T[tt][b] = dvec4{0.0, 0.0, 0.0, 0.0};
+  mf_for_owned<RB>(T, n, tm, [&](int i, int j, double) { return A[i + (size_t)j * n]; });
+  const int k = mf_sweep<RB>(T, n, bk != 0, s, tm, E);
+  // the sweep leaves -inv
+  if (k >= 0 && k <= n)
+    mf_for_owned<RB>(T, n, tm, [&](int i, int j, double v) {
+      A[i + (size_t)j * n] = -v;
+      return v;
+    });
+  if (threadIdx.x == 0 && tm.rank == 0) kount[inst] = k;
+}
+
 // DIAG(C) reference-potential terms of every sector for every instance
 // (mdprop.f:143-164): y14 = A14 - D1 W_ii(C), y23 = A23, u2 = 2 A14, laid
 // out [inst][3][nstep][n].  One thread per (instance, sector, channel).

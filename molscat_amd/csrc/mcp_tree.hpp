@@ -62,13 +62,41 @@ __device__ __forceinline__ double w_elem(const BlockArgs& B, const GridArgs& G, 
 //   Co = alpha op(A) op(B) + beta Ci,
 // batch item b = (b / per, b % per) on grid x, one 64 x 64 tile of Co per
 // workgroup on grid y
-// (four waves of 32 x 32, each 2 x 2 MFMA tiles), k in steps of 16 through LDS.
+// (four waves of 32 x 32, each 2 x 2 MFMA tiles), k in steps of 32 through
+// LDS, the next step's operands fetched into registers during the products.
 // The MFMA computes the tile transposed (op(B)^T op(A)^T), so that register v
 // of lane l is Co(m0 + l%16, n0 + l/16 + 4v) and stores run down columns.
 // ---------------------------------------------------------------------------
 constexpr int kGT = 64;   // tile edge
-constexpr int kGK = 16;   // k step
+constexpr int kGK = 32;   // k step
 constexpr int kGPad = 4;  // LDS row padding (doubles)
+constexpr int kGPer = kGT * kGK / 256;  // elements of each operand tile per thread
+
+// one k step of op(X) (rows = the tile's m or n side) into registers:
+// element e = tid + 256 s; along the contiguous direction of X
+template <bool T>
+__device__ __forceinline__ void gemm_fetch(const double* x, int n, int r0, int k0, double (&v)[kGPer]) {
+#pragma unroll
+  for (int s = 0; s < kGPer; ++s) {
+    const int e = (int)threadIdx.x + 256 * s;
+    // not transposed: rows contiguous (e % 64 = row); transposed: k contiguous (e % 32 = k)
+    const int rr = T ? e / kGK : e % kGT, kk = T ? e % kGK : e / kGT;
+    const int gr = r0 + rr, gk = k0 + kk;
+    double u = 0.0;
+    if (gr < n && gk < n) u = T ? x[gk + (size_t)gr * n] : x[gr + (size_t)gk * n];
+    v[s] = u;
+  }
+}
+
+template <bool T>
+__device__ __forceinline__ void gemm_stash(double (*S)[kGT + kGPad], const double (&v)[kGPer]) {
+#pragma unroll
+  for (int s = 0; s < kGPer; ++s) {
+    const int e = (int)threadIdx.x + 256 * s;
+    const int rr = T ? e / kGK : e % kGT, kk = T ? e % kGK : e / kGT;
+    S[kk][rr] = v[s];
+  }
+}
 
 template <bool TA, bool TB>
 __global__ void __launch_bounds__(256) k_gemm_seg(int n, int per, SegRef A, SegRef Bm, SegRef Ci, SegOut Co,
@@ -89,28 +117,23 @@ __global__ void __launch_bounds__(256) k_gemm_seg(int n, int per, SegRef A, SegR
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = dvec4{0.0, 0.0, 0.0, 0.0};
+  // op(A) rows are the tile's m, op(B)^T rows its n; B transposed is stored
+  // like A not transposed and vice versa
+  double va[kGPer], vb[kGPer];
+  gemm_fetch<TA>(a, n, m0, 0, va);
+  gemm_fetch<!TB>(bb, n, n0, 0, vb);
   for (int k0 = 0; k0 < n; k0 += kGK) {
-#pragma unroll
-    for (int e0 = 0; e0 < kGT * kGK; e0 += 256) {
-      const int e = e0 + tid;
-      {
-        const int mm = TA ? e >> 4 : e & 63, kk = TA ? e & 15 : e >> 6;
-        const int gm = m0 + mm, gk = k0 + kk;
-        double v = 0.0;
-        if (gm < n && gk < n) v = TA ? a[gk + (size_t)gm * n] : a[gm + (size_t)gk * n];
-        As[kk][mm] = v;
-      }
-      {
-        const int nn = TB ? e & 63 : e >> 4, kk = TB ? e >> 6 : e & 15;
-        const int gn = n0 + nn, gk = k0 + kk;
-        double v = 0.0;
-        if (gn < n && gk < n) v = TB ? bb[gn + (size_t)gk * n] : bb[gk + (size_t)gn * n];
-        Bs[kk][nn] = v;
-      }
-    }
     __syncthreads();
+    gemm_stash<TA>(As, va);
+    gemm_stash<!TB>(Bs, vb);
+    __syncthreads();
+    // the next k step's operands are in flight during this step's products
+    if (k0 + kGK < n) {
+      gemm_fetch<TA>(a, n, m0, k0 + kGK, va);
+      gemm_fetch<!TB>(bb, n, n0, k0 + kGK, vb);
+    }
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < kGK / 4; ++c) {
       const int kk = 4 * c + q;
       double av[2], bv[2];
 #pragma unroll
@@ -122,7 +145,6 @@ __global__ void __launch_bounds__(256) k_gemm_seg(int n, int per, SegRef A, SegR
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = mma(bv[j], av[i], acc[i][j]);
     }
-    __syncthreads();
   }
   const double* ci = beta != 0.0 ? Ci.at(inst, m) : nullptr;
   double* co = Co.at(inst, m);
