@@ -493,18 +493,30 @@ int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const do
       unsigned* ctrs = c->s_bar.as<unsigned>();
       int* abort_flag = (int*)(ctrs + ninst);
       // parallel-in-R tree where the device memory holds two of its levels
-      if (c->large_mode == 0 && !c->no_reg) {
+      if (c->large_mode == 0) {
         size_t freeb = 0, totalb = 0;
         (void)hipMemGetInfo(&freeb, &totalb);
-        const size_t held = c->s_seg0.bytes + c->s_seg1.bytes + c->s_zb.bytes + c->s_pq.bytes;
-        const size_t dgb = (size_t)ninst * 3 * nstep * n * sizeof(double);
-        if (tree_bytes(n, ninst, nstep) + dgb + (size_t(2) << 30) <= freeb + held) {
-          HIPCHK(c, c->s_dg.ensure(dgb));
-          const size_t ndiag = (size_t)ninst * nstep * n;
-          hipLaunchKernelGGL(mcp::k_mdprop_diag, dim3((unsigned)((ndiag + 255) / 256)), dim3(256), 0, c->stream, B, G,
-                             I, ninst, nstep, h, c->s_dg.as<double>());
-          HIPCHK(c, hipGetLastError());
-          if ((rc = tree_mdprop(c, B, G, I, ninst, nstep, h, ucp, uck, c->s_dg.as<double>(), bk))) return rc;
+        const size_t held = c->s_seg0.bytes + c->s_seg1.bytes + c->s_zb.bytes + c->s_pq.bytes + c->s_dg.bytes;
+        const size_t per = tree_bytes(n, 1, nstep) + (size_t)3 * nstep * n * sizeof(double);
+        const size_t room = freeb + held > (size_t(2) << 30) ? freeb + held - (size_t(2) << 30) : 0;
+        // as many instances per tree as the device memory holds, in turn
+        const int chunk = (int)std::min<size_t>((size_t)ninst, room / per);
+        if (chunk >= 1) {
+          for (int i0 = 0; i0 < ninst; i0 += chunk) {
+            const int m = std::min(chunk, ninst - i0);
+            mcp::InstArgs Ic = I;
+            Ic.ered = I.ered + i0;
+            Ic.pconst = I.pconst ? I.pconst + (size_t)i0 * b.nconst : nullptr;
+            Ic.y = I.y + (size_t)i0 * n * n;
+            Ic.nodes = I.nodes + i0;
+            Ic.status = I.status + i0;
+            HIPCHK(c, c->s_dg.ensure((size_t)m * 3 * nstep * n * sizeof(double)));
+            const size_t ndiag = (size_t)m * nstep * n;
+            hipLaunchKernelGGL(mcp::k_mdprop_diag, dim3((unsigned)((ndiag + 255) / 256)), dim3(256), 0, c->stream, B,
+                               G, Ic, m, nstep, h, c->s_dg.as<double>());
+            HIPCHK(c, hipGetLastError());
+            if ((rc = tree_mdprop(c, B, G, Ic, m, nstep, h, ucp, uck, c->s_dg.as<double>(), bk))) return rc;
+          }
           HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
           return MCP_OK;
         }
@@ -867,7 +879,7 @@ static int yinit_device(mcp_ctx* c, Block& b, Grid& g, mcp::InstArgs I, int nins
   HIPCHK(c, c->s_y.ensure((size_t)ninst * 3 * n * n * sizeof(double)));
   I.status = d_status;
   const int nblk = (n + mcp::kPB - 1) / mcp::kPB;
-  if (team_inverse_fits(c, n) && (size_t)2 * ninst * nblk <= (size_t)c->num_cu) {
+  if (!c->no_reg && team_inverse_fits(c, n) && (size_t)2 * ninst * nblk <= (size_t)c->num_cu) {
     // the square-root iteration of k_yinit_large in stages, both inverses
     // of an iteration on teams of workgroups
     const size_t nn = (size_t)n * n;
