@@ -26,6 +26,16 @@
 
 namespace mcp {
 
+typedef double dvec4 __attribute__((ext_vector_type(4)));
+
+// D = A B + C on the FP64 matrix cores (v_mfma_f64_16x16x4f64): A operand of
+// lane l = element (l%16, l/16) of the 16x4 factor, B operand = element
+// (l/16, l%16) of the 4x16 factor, register v of the accumulator = element
+// (l/16 + 4v, l%16) of the 16x16 product.
+__device__ __forceinline__ dvec4 mma(double a, double b, dvec4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
 constexpr int kPB = 16;    // pivots per panel
 constexpr int kTile = 64;  // ownership / update tile edge
 
@@ -550,87 +560,110 @@ __device__ __attribute__((always_inline)) void wave_panel(const double* A, int n
 // --------------------------------------------------------------------------
 // panel update of this workgroup's tiles
 // --------------------------------------------------------------------------
-// A(i,j) -= sum_q X(i,q) A(p_q,j) for i, j outside the panel (4 x 4 register
-// tiles: rows 4tx + a, columns 4ty + b), then the panel rows/columns
+// A(i,j) -= sum_q X(i,q) A(p_q,j) for i, j outside the panel (64 x 64 tiles on
+// the matrix cores), then the panel rows/columns
 // A(i,p_q) = X(i,q) and the pivot block -Dinv.  X rows beyond np are zero.
 __device__ __attribute__((always_inline)) void apply_panel(double* A, int n, int np, const LargeScratch& s,
                                                            const Team& t) {
   if (np == 0) return;
   const int ld = s.ld;
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  // 64 x 64 tiles of the lower triangle on the matrix cores: wave w takes the
+  // 32 x 32 quadrant (w & 1, w >> 1), four 16 x 16 products over the 16
+  // pivots each, computed transposed (G X^T) so that register v of lane l
+  // is A(row0 + l%16, col0 + l/16 + 4v) and loads and stores run down
+  // columns.  Tile values are fetched two tiles ahead of the products (one
+  // workgroup per CU: the load latency has no other wave to hide behind).
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int q4 = lane >> 4, r16 = lane & 15;
+  const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
   const int nt = (n + kTile - 1) / kTile;
   const int ntt = nt * (nt + 1) / 2;
-  const double* gslot[kPB];
+  // operand rows of the pivots 4c + l/16 (X rows beyond np are zero)
+  const double* gq[4];
+  const double* xq[4];
 #pragma unroll
-  for (int q = 0; q < kPB; ++q) gslot[q] = s.G + s.pslot[q] * ld;
-  // the owned tiles in turn, each tile's values fetched while the previous
-  // one is updated (one workgroup per CU leaves no other wave to hide the
-  // load latency)
+  for (int c = 0; c < 4; ++c) {
+    gq[c] = s.G + s.pslot[4 * c + q4] * ld;
+    xq[c] = s.X + (4 * c + q4) * ld;
+  }
   auto tile_at = [&](int tau, int& r0, int& c0) {
     const int bi = tile_row(tau);
     const int bj = tau - bi * (bi + 1) / 2;
-    r0 = bi * kTile + 4 * tx;
-    c0 = bj * kTile + 4 * ty;
+    r0 = bi * kTile;
+    c0 = bj * kTile;
   };
-  auto fetch = [&](int r0, int c0, double (&av)[4][4]) {
+  auto fetch = [&](int r0, int c0, dvec4 (&av)[2][2]) {
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int cc = min(c0 + b, n - 1);
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-      for (int a = 0; a < 4; ++a) av[a][b] = A[min(r0 + a, n - 1) + (size_t)cc * n];
-    }
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = min(r0 + wm + 16 * a + r16, n - 1);
+          const int j = min(c0 + wn + 16 * b + q4 + 4 * v, n - 1);
+          av[a][b][v] = A[i + (size_t)j * n];
+        }
   };
+  // two tiles ahead: av the current tile, an the next, the one after that
+  // fetched at the top of each iteration
   int tau = t.rank;
-  double av[4][4];
-  int r0 = 0, c0 = 0;
+  dvec4 av[2][2], an[2][2];
+  int r0 = 0, c0 = 0, rn = 0, cn = 0;
   if (tau < ntt) {
     tile_at(tau, r0, c0);
     fetch(r0, c0, av);
   }
+  if (tau + t.g < ntt) {
+    tile_at(tau + t.g, rn, cn);
+    fetch(rn, cn, an);
+  }
   while (tau < ntt) {
-    const int tn = tau + t.g;
-    int rn = 0, cn = 0;
-    double an[4][4];
-    if (tn < ntt) {
-      tile_at(tn, rn, cn);
-      fetch(rn, cn, an);
+    const int tn2 = tau + 2 * t.g;
+    int r2 = 0, c2 = 0;
+    dvec4 a2[2][2];
+    if (tn2 < ntt) {
+      tile_at(tn2, r2, c2);
+      fetch(r2, c2, a2);
     }
-    if (!(c0 >= n || r0 >= n || r0 + 3 < c0)) {
-      double acc[4][4];
+    dvec4 acc[2][2];
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+      for (int b = 0; b < 2; ++b) acc[a][b] = dvec4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int q = 0; q < kPB; ++q) {
-        const double2* xq = reinterpret_cast<const double2*>(s.X + q * ld + r0);
-        const double2* gq = reinterpret_cast<const double2*>(gslot[q] + c0);
-        const double2 x01 = xq[0], x23 = xq[1], g01 = gq[0], g23 = gq[1];
-        const double xv[4] = {x01.x, x01.y, x23.x, x23.y};
-        const double gv[4] = {g01.x, g01.y, g23.x, g23.y};
+    for (int c = 0; c < 4; ++c) {
+      double xo[2], go[2];
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < 2; ++a) xo[a] = xq[c][r0 + wm + 16 * a + r16];
 #pragma unroll
-          for (int b = 0; b < 4; ++b) acc[a][b] += xv[a] * gv[b];
-      }
+      for (int b = 0; b < 2; ++b) go[b] = gq[c][c0 + wn + 16 * b + r16];
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const int cc = c0 + b;
-        if (cc >= n || s.inP[cc]) continue;
+      for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          const int i = r0 + a;
-          if (i < n && i >= cc && !s.inP[i]) A[i + (size_t)cc * n] = av[a][b] - acc[a][b];
+        for (int b = 0; b < 2; ++b) acc[a][b] = mma(go[b], xo[a], acc[a][b]);
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = r0 + wm + 16 * a + r16;
+          const int j = c0 + wn + 16 * b + q4 + 4 * v;
+          if (i < n && j < n && i >= j && !s.inP[i] && !s.inP[j]) A[i + (size_t)j * n] = av[a][b][v] - acc[a][b][v];
         }
-      }
-    }
-    tau = tn;
+    tau += t.g;
     r0 = rn;
     c0 = cn;
+    rn = r2;
+    cn = c2;
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-      for (int b = 0; b < 4; ++b) av[a][b] = an[a][b];
+      for (int b = 0; b < 2; ++b) {
+        av[a][b] = an[a][b];
+        an[a][b] = a2[a][b];
+      }
   }
   for (int q = 0; q < np; ++q) {
     const int p = s.pidx[q];

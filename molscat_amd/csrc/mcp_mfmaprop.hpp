@@ -38,17 +38,11 @@
 
 namespace mcp {
 
-typedef double dvec4 __attribute__((ext_vector_type(4)));
-
 constexpr int kMW = 8;                   // waves per workgroup
 constexpr int kMT = kMW * 64;            // threads per workgroup
 constexpr int kMTW = 4;                  // row tiles per wave
 constexpr int kMMaxN = kMW * kMTW * 16;  // largest N
 constexpr int kMSlots = kPB + 1;         // exchange slots: the window and one interchanged column
-
-__device__ __forceinline__ dvec4 mma(double a, double b, dvec4 c) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
 
 struct MfmaScratch {
   int ld;        // exchange leading dimension (whole 64-row tiles)
