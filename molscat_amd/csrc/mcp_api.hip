@@ -210,7 +210,7 @@ int set_lds_limits(mcp_ctx* c) {
                        (const void*)mcp::k_mdprop_large,   (const void*)mcp::k_ldprop_large,
                        (const void*)mcp::k_yinit_large,    (const void*)mcp::k_airy_sector,
                        (const void*)mcp::k_airy_prop,      (const void*)mcp::k_mdprop_mfma<1>, (const void*)mcp::k_mdprop_mfma<2>,
-                       (const void*)mcp::k_syminv_mfma<1>};
+                       (const void*)mcp::k_syminv_mfma<1>, (const void*)mcp::k_match_lu};
   const size_t total = c->lds_max;
   size_t lim = total;
   for (const void* f : fns) {
@@ -1009,7 +1009,29 @@ static int match_device(mcp_ctx* c, int n, int ninst, const double* d_y, const d
   M.sr = M.kmat + (size_t)ninst * n * n;
   M.si = M.sr + (size_t)ninst * n * n;
   M.status = (int*)(M.si + (size_t)ninst * n * n);
-  hipLaunchKernelGGL(mcp::k_match, dim3(ninst), dim3(kNT), 0, c->stream, M);
+  hipLaunchKernelGGL(mcp::k_match, dim3(ninst), dim3(kNT), 0, c->stream, M, mcp::kMatchSetup);
+  HIPCHK(c, hipGetLastError());
+  {
+    // the forward elimination on teams of workgroups: g ranks per instance
+    // where the CUs hold them (a single workgroup for many instances)
+    const int g = std::max(1, std::min({32, (n + 7) / 8, c->num_cu / std::max(1, ninst)}));
+    const size_t lu_lds = (size_t)5 * n * sizeof(double);
+    if (lu_lds > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "matching: N=%d exceeds the LU's LDS", n);
+    int rc;
+    if ((rc = prep_team(c, ninst))) return rc;
+    unsigned* ctrs = c->s_bar.as<unsigned>();
+    int* abort_flag = (int*)(ctrs + ninst);
+    if (g == 1) {
+      hipLaunchKernelGGL(mcp::k_match_lu, dim3(ninst), dim3(kNT), lu_lds, c->stream, M, 1, ctrs, abort_flag);
+      HIPCHK(c, hipGetLastError());
+    } else {
+      int g_ = g;
+      void* args[] = {&M, &g_, &ctrs, &abort_flag};
+      HIPCHK(c, hipLaunchCooperativeKernel((const void*)mcp::k_match_lu, dim3(ninst * g), dim3(kNT), args, lu_lds,
+                                           c->stream));
+    }
+  }
+  hipLaunchKernelGGL(mcp::k_match, dim3(ninst), dim3(kNT), 0, c->stream, M, mcp::kMatchFinish);
   HIPCHK(c, hipGetLastError());
   return MCP_OK;
 }

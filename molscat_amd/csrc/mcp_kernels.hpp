@@ -746,9 +746,14 @@ struct MatchArgs {
   int* status;         // [ninst]
 };
 
-__global__ void __launch_bounds__(kNT) k_match(MatchArgs M) {
-  __shared__ double sv[kNT / 64];
-  __shared__ int si[kNT / 64];
+// k_match runs in two phases around k_match_lu (the forward elimination of
+// YTOKG's DGESV): kMatchSetup builds the Riccati-Bessel functions and the two
+// matching matrices, kMatchFinish does the back substitution, KSYM and KTOS.
+// The diagonal of U is kept in the (no longer needed) SJ slot of the work area.
+constexpr int kMatchSetup = 0;
+constexpr int kMatchFinish = 1;
+
+__global__ void __launch_bounds__(kNT) k_match(MatchArgs M, int phase) {
   __shared__ int bad;
   const int inst = blockIdx.x;
   const int n = M.n;
@@ -768,6 +773,9 @@ __global__ void __launch_bounds__(kNT) k_match(MatchArgs M) {
     if (threadIdx.x == 0) M.status[inst] = 0;
     return;
   }
+  if (phase == kMatchFinish) {
+    if (M.status[inst] != 0) return;
+  } else {
   // Riccati-Bessel functions per channel (ytok.f:24-45)
   for (int ii = threadIdx.x; ii < n; ii += kNT) {
     const int nx = nb[ii];
@@ -810,68 +818,24 @@ __global__ void __launch_bounds__(kNT) k_match(MatchArgs M) {
     }
   }
   __syncthreads();
-  // DGESV: LU with partial pivoting, applied to the nop right-hand sides
-  for (int k = 0; k < n; ++k) {
-    double v = -1.0;
-    int idx = INT_MAX;
-    for (int i = k + threadIdx.x; i < n; i += kNT) {
-      const double a = fabs(km[i + (size_t)k * n]);
-      if (a > v) {
-        v = a;
-        idx = i;
-      }
-    }
-    block_argmax<kNT>(v, idx, sv, si);
-    if (v == 0.0) {
-      if (threadIdx.x == 0) bad = 100 + k;
-      break;
-    }
-    const int piv = idx;
-    if (piv != k) {
-      for (int j = threadIdx.x; j < n; j += kNT) {
-        const double x = km[k + (size_t)j * n];
-        km[k + (size_t)j * n] = km[piv + (size_t)j * n];
-        km[piv + (size_t)j * n] = x;
-      }
-      for (int j = threadIdx.x; j < nop; j += kNT) {
-        const double x = t[k + (size_t)j * n];
-        t[k + (size_t)j * n] = t[piv + (size_t)j * n];
-        t[piv + (size_t)j * n] = x;
-      }
-    }
-    __syncthreads();
-    const double dinv = 1.0 / km[k + (size_t)k * n];
-    for (int i = k + 1 + threadIdx.x; i < n; i += kNT) km[i + (size_t)k * n] *= dinv;
-    __syncthreads();
-    const int rem = n - k - 1;
-    for (int e = threadIdx.x; e < rem * (rem + nop); e += kNT) {
-      const int i = k + 1 + e % rem;
-      const int jj = e / rem;
-      const double l = km[i + (size_t)k * n];
-      if (jj < rem) {
-        const int j = k + 1 + jj;
-        km[i + (size_t)j * n] -= l * km[k + (size_t)j * n];
-      } else {
-        const int j = jj - rem;
-        t[i + (size_t)j * n] -= l * t[k + (size_t)j * n];
-      }
-    }
-    __syncthreads();
-  }
   __syncthreads();
-  if (bad) {
-    if (threadIdx.x == 0) M.status[inst] = bad;
-    return;
+  if (threadIdx.x == 0) M.status[inst] = bad;
+  return;
   }
-  // back substitution U x = t (columns independent: one thread per column)
-  for (int j = threadIdx.x; j < nop; j += kNT) {
-    for (int i = n - 1; i >= 0; --i) {
-      double s = t[i + (size_t)j * n];
-      for (int k = i + 1; k < n; ++k) s -= km[i + (size_t)k * n] * t[k + (size_t)j * n];
-      t[i + (size_t)j * n] = s / km[i + (size_t)i * n];
+  const double* ud = sj;  // diagonal of U (k_match_lu)
+  // back substitution U x = t by columns of U: x_i = t_i / U_ii, then
+  // t_r -= U_ri x_i for r < i, every thread on its rows (column i of U is
+  // contiguous); the right-hand sides share the pass
+  for (int i = n - 1; i >= 0; --i) {
+    const double uii = ud[i];
+    for (int e = threadIdx.x; e < i * nop; e += kNT) {
+      const int j = e / i, r = e - j * i;
+      t[r + (size_t)j * n] -= km[r + (size_t)i * n] * (t[i + (size_t)j * n] / uii);
     }
+    __syncthreads();
+    for (int j = threadIdx.x; j < nop; j += kNT) t[i + (size_t)j * n] /= uii;
+    __syncthreads();
   }
-  __syncthreads();
   // K = sym(X[0:nop, 0:nop])  (ksym.f)
   double* K = M.kmat + (size_t)inst * n * n;
   for (int e = threadIdx.x; e < nop * nop; e += kNT) {

@@ -1278,4 +1278,79 @@ __global__ void k_yl_finish(int n, int ninst, int outward, const double* __restr
   Y[j + (size_t)i * n] = v;
 }
 
+// DGESV's forward elimination (LU with partial pivoting, the right-hand
+// sides carried along; ytokg.f:56) of KM (n x n) and T (n x nop) in the
+// work area, on a team of g workgroups per instance: column j (and right-hand
+// side c, as column n + c) belongs to rank j % g.  Every rank searches the
+// pivot column itself (same data, same choice: the first of the largest
+// |values|, as IDAMAX), takes the multipliers with the interchange applied
+// virtually, and swaps and updates only its own columns; the lower part of
+// the pivot column is left as it was and U's diagonal goes to the SJ slot.
+// One team barrier per column.  LDS: 5n doubles (dynamic).
+__global__ void __launch_bounds__(kNT) k_match_lu(MatchArgs M, int g, unsigned* __restrict__ ctrs,
+                                                  int* __restrict__ abort_flag) {
+  __shared__ double sv[kNT / 64];
+  __shared__ int si[kNT / 64];
+  extern __shared__ double smem[];
+  const int inst = blockIdx.x / g;
+  Team tm = make_team(g, ctrs, inst, abort_flag);
+  const int n = M.n;
+  const int nop = M.nopen[inst];
+  if (nop <= 0 || M.status[inst] != 0) return;
+  double* km = M.work + (size_t)inst * (2 * (size_t)n * n + 4 * (size_t)n);
+  double* t = km + (size_t)n * n;
+  double* ud = t + (size_t)n * n;
+  double* lcol = smem;      // [n] multipliers
+  double* urow = smem + n;  // [2 (n + nop)] rows piv and k at the owned columns
+  const int ncol = n + nop;
+  for (int k = 0; k < n; ++k) {
+    double v = -1.0;
+    int idx = INT_MAX;
+    for (int i = k + threadIdx.x; i < n; i += kNT) {
+      const double a = fabs(km[i + (size_t)k * n]);
+      if (a > v) {
+        v = a;
+        idx = i;
+      }
+    }
+    block_argmax<kNT>(v, idx, sv, si);
+    if (v == 0.0) {
+      if (threadIdx.x == 0 && tm.rank == 0) M.status[inst] = 100 + k;
+      break;
+    }
+    const int piv = idx;
+    const double pv = km[piv + (size_t)k * n], akk = km[k + (size_t)k * n];
+    if (threadIdx.x == 0 && tm.rank == k % g) ud[k] = pv;
+    const double dinv = 1.0 / pv;
+    for (int i = k + 1 + threadIdx.x; i < n; i += kNT) lcol[i] = (i == piv ? akk : km[i + (size_t)k * n]) * dinv;
+    // the owned columns j > k: their rows k and piv first
+    const int j0 = k + 1 + ((tm.rank - (k + 1) % g) + g) % g;  // first owned column > k
+    for (int j = j0 + threadIdx.x * g; j < ncol; j += kNT * g) {
+      double* col = j < n ? km + (size_t)j * n : t + (size_t)(j - n) * n;
+      urow[j] = col[piv];
+      urow[n + nop + j] = col[k];
+    }
+    __syncthreads();
+    const int own = j0 < ncol ? (ncol - j0 + g - 1) / g : 0;
+    const int rows = n - k - 1;
+    for (int e = threadIdx.x; e < own * (rows + 1); e += kNT) {
+      const int jc = e / (rows + 1), r = e - jc * (rows + 1);
+      const int j = j0 + jc * g;
+      double* col = j < n ? km + (size_t)j * n : t + (size_t)(j - n) * n;
+      const double ukj = urow[j];
+      if (r == rows) {
+        col[k] = ukj;
+      } else {
+        const int i = k + 1 + r;
+        const double a = (i == piv) ? urow[n + nop + j] : col[i];
+        col[i] = a - lcol[i] * ukj;
+      }
+    }
+    if (!team_sync(tm)) {
+      if (threadIdx.x == 0) M.status[inst] = 200;
+      break;
+    }
+  }
+}
+
 }  // namespace mcp
