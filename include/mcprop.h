@@ -168,11 +168,11 @@ int mcp_bound_match(mcp_ctx *ctx, int32_t block_id, int32_t grid_out, int32_t gr
 /* Workgroups that cooperate on one instance on the sequential large-N
  * kernels (N beyond the LDS-resident ones): g >= 1 asks for g (the
  * register-resident MDPROP takes ceil(N/16) by default, and between
- * ceil(N/32) and ceil(N/16) when a smaller g is asked for), 0 restores the
- * automatic choice, g < 0 keeps the sequential MDPROP on the global-memory
- * kernel with -g workgroups and the large-N YINIT on one workgroup per
- * instance (no cooperative launches besides that kernel's).  No reference
- * counterpart. */
+ * ceil(N/32) and ceil(N/16) when a smaller g is asked for; the matching's
+ * LU takes at most |g|), 0 restores the automatic choice, g < 0 keeps the
+ * sequential MDPROP on the global-memory kernel with -g workgroups and the
+ * large-N YINIT on one workgroup per instance (with g = -1 no cooperative
+ * launch outside that sequential kernel).  No reference counterpart. */
 int mcp_set_team_size(mcp_ctx *ctx, int32_t g);
 
 /* How MDPROP runs for large N: 0 (default) solves the recurrence in parallel

@@ -175,6 +175,7 @@ struct mcp_ctx {
   DevBuf s_zb, s_pq;              // its merge pivots / inverses and products
   DevBuf s_kc, s_acc;             // its negative-pivot counts
   DevBuf s_yv;                    // large-N YINIT: scales, flags, inertias
+  DevBuf s_flag;                  // a device-side decision flag
   double t_sector = 0.0, t_prop = 0.0, t_total = 0.0;
 
   int fail(int code, const char* fmt, ...) {
@@ -357,11 +358,12 @@ size_t tree_bytes(int n, int ninst, int nstep) {
          sizeof(int) * ((size_t)ninst * nstep + 2 * (size_t)ninst);
 }
 
-// MDPROP for large N as a tree of imbedding-map merges (mcp_tree.hpp): uc,
-// uck the energy-independent sector matrices and their inertias, diag the
-// output of k_mdprop_diag.  Y in I.y is replaced by Y(RSTOP).
+// MDPROP for large N as a tree of imbedding-map merges (mcp_tree.hpp): diag
+// the output of k_mdprop_diag; fused: the leaves from M = V + E^-1 (no U(C)),
+// else uc, uck the energy-independent sector matrices and their inertias.
+// Y in I.y is replaced by Y(RSTOP).
 int tree_mdprop(mcp_ctx* c, const mcp::BlockArgs& B, const mcp::GridArgs& G, const mcp::InstArgs& I, int ninst,
-                int nstep, double h, const double* uc, const int* uck, const double* diag, int bk) {
+                int nstep, double h, bool fused, const double* uc, const int* uck, const double* diag, int bk) {
   const int n = B.n;
   const size_t nn = (size_t)n * n, sz = 3 * nn;
   const int k1 = (nstep + 1) / 2;
@@ -369,7 +371,7 @@ int tree_mdprop(mcp_ctx* c, const mcp::BlockArgs& B, const mcp::GridArgs& G, con
   HIPCHK(c, c->s_seg1.ensure((size_t)ninst * std::max(k1, 1) * sz * sizeof(double)));
   HIPCHK(c, c->s_zb.ensure((size_t)ninst * nstep * nn * sizeof(double)));
   HIPCHK(c, c->s_pq.ensure((size_t)ninst * std::max(k1, 1) * 2 * nn * sizeof(double)));
-  HIPCHK(c, c->s_kc.ensure((size_t)ninst * nstep * sizeof(int)));
+  HIPCHK(c, c->s_kc.ensure((size_t)2 * ninst * nstep * sizeof(int)));
   HIPCHK(c, c->s_acc.ensure((size_t)2 * ninst * sizeof(int)));
   double* zb = c->s_zb.as<double>();
   double* pq = c->s_pq.as<double>();
@@ -380,10 +382,22 @@ int tree_mdprop(mcp_ctx* c, const mcp::BlockArgs& B, const mcp::GridArgs& G, con
   hipLaunchKernelGGL(mcp::k_tree_zero, dim3(nblk256(2 * ninst)), dim3(256), 0, c->stream, acc, 2 * ninst);
   // leaves: each sector's two half-steps merged, Z = (UC + 2 A14)^-1
   const size_t nleaf = (size_t)ninst * nstep;
-  hipLaunchKernelGGL(mcp::k_tree_c1, dim3(nblk256(nleaf * nn)), dim3(256), 0, c->stream, n, ninst, nstep, uc, diag, zb);
-  hipLaunchKernelGGL(mcp::k_syminv_large, dim3((unsigned)nleaf), dim3(kNT), ll, c->stream, n, zb, kc, bk);
-  hipLaunchKernelGGL(mcp::k_tree_count, dim3(nblk256(nleaf)), dim3(256), 0, c->stream, n, (int)nleaf, nstep, kc, acc,
-                     bad);
+  if (fused) {
+    int* epos = kc + nleaf;
+    hipLaunchKernelGGL(mcp::k_tree_zero, dim3(nblk256(nleaf)), dim3(256), 0, c->stream, epos, (int)nleaf);
+    hipLaunchKernelGGL(mcp::k_tree_m, dim3(nblk256(nleaf * nn)), dim3(256), 0, c->stream, B, G, I, ninst, nstep, h,
+                       diag, zb, epos);
+    hipLaunchKernelGGL(mcp::k_syminv_large, dim3((unsigned)nleaf), dim3(kNT), ll, c->stream, n, zb, kc, bk);
+    hipLaunchKernelGGL(mcp::k_tree_count_leaf, dim3(nblk256(nleaf)), dim3(256), 0, c->stream, n, (int)nleaf, nstep, kc,
+                       epos, acc, bad);
+    hipLaunchKernelGGL(mcp::k_tree_z, dim3(nblk256(nleaf * nn)), dim3(256), 0, c->stream, n, ninst, nstep, diag, zb);
+  } else {
+    hipLaunchKernelGGL(mcp::k_tree_c1, dim3(nblk256(nleaf * nn)), dim3(256), 0, c->stream, n, ninst, nstep, uc, diag,
+                       zb);
+    hipLaunchKernelGGL(mcp::k_syminv_large, dim3((unsigned)nleaf), dim3(kNT), ll, c->stream, n, zb, kc, bk);
+    hipLaunchKernelGGL(mcp::k_tree_count, dim3(nblk256(nleaf)), dim3(256), 0, c->stream, n, (int)nleaf, nstep, kc,
+                       acc, bad);
+  }
   double* in = c->s_seg0.as<double>();
   double* out = c->s_seg1.as<double>();
   hipLaunchKernelGGL(mcp::k_tree_leaf, dim3(nblk256(nleaf * nn)), dim3(256), 0, c->stream, B, G, I, ninst, nstep, h,
@@ -434,8 +448,8 @@ int tree_mdprop(mcp_ctx* c, const mcp::BlockArgs& B, const mcp::GridArgs& G, con
   }
   hipLaunchKernelGGL(mcp::k_tree_out, dim3(nblk256((size_t)ninst * nn)), dim3(256), 0, c->stream, n, ninst, h, out,
                      I.y);
-  hipLaunchKernelGGL(mcp::k_tree_nodes, dim3(nblk256(ninst)), dim3(256), 0, c->stream, ninst, nstep, uck, acc, bad,
-                     I.nodes, I.status);
+  hipLaunchKernelGGL(mcp::k_tree_nodes, dim3(nblk256(ninst)), dim3(256), 0, c->stream, ninst, nstep,
+                     fused ? nullptr : uck, acc, bad, I.nodes, I.status);
   HIPCHK(c, hipGetLastError());
   return MCP_OK;
 }
@@ -482,16 +496,20 @@ int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const do
     } else {
       const size_t lu = lds_large(n, b.nham, 0), lm = lds_large(n, b.nham, 3);
       if (lm > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "MDPROP: N=%d exceeds the large-N panel limit", n);
-      hipLaunchKernelGGL(mcp::k_mdprop_uc_large, dim3(nstep), dim3(kNT), lu, c->stream, B, G, d_pconst, h, bk,
-                         c->s_e.as<double>(), c->s_f.as<int>());
-      HIPCHK(c, hipGetLastError());
-      HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
       int rc;
-      if ((rc = prep_team(c, ninst))) return rc;
       const double* ucp = c->s_e.as<double>();
       const int* uck = c->s_f.as<int>();
-      unsigned* ctrs = c->s_bar.as<unsigned>();
-      int* abort_flag = (int*)(ctrs + ninst);
+      // the energy-independent U(C) sector matrices, when a path needs them
+      bool uc_done = false;
+      auto need_uc = [&]() -> int {
+        if (!uc_done) {
+          hipLaunchKernelGGL(mcp::k_mdprop_uc_large, dim3(nstep), dim3(kNT), lu, c->stream, B, G, d_pconst, h, bk,
+                             c->s_e.as<double>(), c->s_f.as<int>());
+          HIPCHK(c, hipGetLastError());
+          uc_done = true;
+        }
+        return MCP_OK;
+      };
       // parallel-in-R tree where the device memory holds two of its levels
       if (c->large_mode == 0) {
         size_t freeb = 0, totalb = 0;
@@ -502,6 +520,8 @@ int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const do
         // as many instances per tree as the device memory holds, in turn
         const int chunk = (int)std::min<size_t>((size_t)ninst, room / per);
         if (chunk >= 1) {
+          HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+          HIPCHK(c, c->s_flag.ensure(sizeof(int)));
           for (int i0 = 0; i0 < ninst; i0 += chunk) {
             const int m = std::min(chunk, ninst - i0);
             mcp::InstArgs Ic = I;
@@ -515,12 +535,27 @@ int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const do
             hipLaunchKernelGGL(mcp::k_mdprop_diag, dim3((unsigned)((ndiag + 255) / 256)), dim3(256), 0, c->stream, B,
                                G, Ic, m, nstep, h, c->s_dg.as<double>());
             HIPCHK(c, hipGetLastError());
-            if ((rc = tree_mdprop(c, B, G, Ic, m, nstep, h, ucp, uck, c->s_dg.as<double>(), bk))) return rc;
+            // leaves without U(C) unless some |2 A14 - 8| is small
+            int flag = 0;
+            HIPCHK(c, hipMemsetAsync(c->s_flag.p, 0, sizeof(int), c->stream));
+            hipLaunchKernelGGL(mcp::k_tree_guard, dim3((unsigned)((ndiag + 255) / 256)), dim3(256), 0, c->stream, n, m,
+                               nstep, c->s_dg.as<double>(), c->s_flag.as<int>());
+            HIPCHK(c, hipGetLastError());
+            HIPCHK(c, hipMemcpyAsync(&flag, c->s_flag.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            const bool fused = flag == 0;
+            if (!fused && (rc = need_uc())) return rc;
+            if ((rc = tree_mdprop(c, B, G, Ic, m, nstep, h, fused, ucp, uck, c->s_dg.as<double>(), bk))) return rc;
           }
           HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
           return MCP_OK;
         }
       }
+      if ((rc = need_uc())) return rc;
+      HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+      if ((rc = prep_team(c, ninst))) return rc;
+      unsigned* ctrs = c->s_bar.as<unsigned>();
+      int* abort_flag = (int*)(ctrs + ninst);
       // register-resident matrix-core kernel when the instances' teams fit
       // the device: one column block per workgroup, or two where there are
       // not CUs enough (or a smaller team was asked for)
@@ -1013,8 +1048,10 @@ static int match_device(mcp_ctx* c, int n, int ninst, const double* d_y, const d
   HIPCHK(c, hipGetLastError());
   {
     // the forward elimination on teams of workgroups: g ranks per instance
-    // where the CUs hold them (a single workgroup for many instances)
-    const int g = std::max(1, std::min({32, (n + 7) / 8, c->num_cu / std::max(1, ninst)}));
+    // where the CUs hold them (a single workgroup for many instances); a
+    // team size set on the context caps it
+    const int gmax = c->team > 0 ? c->team : 32;
+    const int g = std::max(1, std::min({gmax, (n + 7) / 8, c->num_cu / std::max(1, ninst)}));
     const size_t lu_lds = (size_t)5 * n * sizeof(double);
     if (lu_lds > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "matching: N=%d exceeds the LU's LDS", n);
     int rc;

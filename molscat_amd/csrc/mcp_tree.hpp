@@ -181,6 +181,85 @@ __global__ void k_tree_c1(int n, int ninst, int nstep, const double* __restrict_
   zb[e] = v;
 }
 
+// The leaves without U(C).  c1 = UC + diag(2 A14) = V^-1 + E with V = I/8 +
+// D4 W_off(C) (mdprop.f:127-135) and E = diag(2 A14 - 8), so by Woodbury
+// c1^-1 = E^-1 - E^-1 (V + E^-1)^-1 E^-1: one inversion of the sector's
+// M = V + E^-1 replaces the energy-independent inversion of V and the leaf's
+// inversion of c1.  The node count needs neg(c1) - neg(V) (the leaf's KOUNT
+// less its share of NSAVE); the inertias of [[V, I], [I, -E]] eliminated in
+// either order give neg(c1) - neg(V) = n - pos(E) - neg(M).  Used when every
+// |E_ii| is well away from zero (k_tree_guard).
+
+// flag = 1 when some |2 A14 - 8| < 0.5 in the batch
+__global__ void k_tree_guard(int n, int ninst, int nstep, const double* __restrict__ diag, int* __restrict__ flag) {
+  const size_t per = (size_t)nstep * n;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= per * ninst) return;
+  const int inst = (int)(e / per);
+  const size_t r = e - (size_t)inst * per;
+  const double u2 = diag[(size_t)inst * 3 * per + 2 * per + r];
+  if (!(fabs(u2 - 8.0) >= 0.5)) atomicMax(flag, 1);
+}
+
+// M = V + E^-1 of every (instance, sector) into zb; epos[inst*nstep + k] += pos(E)
+__global__ void k_tree_m(BlockArgs B, GridArgs G, InstArgs I, int ninst, int nstep, double h,
+                         const double* __restrict__ diag, double* __restrict__ zb, int* __restrict__ epos) {
+  const int n = B.n;
+  const size_t nn = (size_t)n * n;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nn * nstep * ninst) return;
+  const size_t mat = e / nn, r = e - mat * nn;
+  const int inst = (int)(mat / nstep), k = (int)(mat - (size_t)inst * nstep);
+  const int j = (int)(r / n), i = (int)(r - (size_t)j * n);
+  double v;
+  if (i == j) {
+    const double u2 = diag[((size_t)inst * 3 + 2) * nstep * n + (size_t)k * n + i];
+    v = 0.125 + 1.0 / (u2 - 8.0);
+    if (u2 - 8.0 > 0.0) atomicAdd(epos + mat, 1);
+  } else {
+    const double* pconst = I.pconst ? I.pconst + (size_t)inst * B.nconst : nullptr;
+    const double d4 = -(h * h / 3.0) / 16.0;
+    const int kc = 2 * (k + 1) - 1;
+    const double rsq = rsq_of(G.r[kc]);
+    const double* vl = B.vl + tri_index(i, j) * (size_t)B.nham;
+    double sw = 0.0;
+    for (int l = 0; l < B.nham; ++l) {
+      const double pl = l < B.mxlam ? G.ptab[(size_t)kc * B.mxlam + l]
+                                    : (l < B.mxlam + B.nconst ? pconst[l - B.mxlam] : rsq);
+      sw += vl[l] * pl;
+    }
+    v = d4 * sw;
+  }
+  zb[e] = v;
+}
+
+// Z = E^-1 - E^-1 X E^-1 in place, X = M^-1
+__global__ void k_tree_z(int n, int ninst, int nstep, const double* __restrict__ diag, double* __restrict__ zb) {
+  const size_t nn = (size_t)n * n;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nn * nstep * ninst) return;
+  const size_t mat = e / nn, r = e - mat * nn;
+  const int inst = (int)(mat / nstep), k = (int)(mat - (size_t)inst * nstep);
+  const int j = (int)(r / n), i = (int)(r - (size_t)j * n);
+  const double* u2 = diag + ((size_t)inst * 3 + 2) * nstep * n + (size_t)k * n;
+  const double ei = 1.0 / (u2[i] - 8.0), ej = 1.0 / (u2[j] - 8.0);
+  const double z = -ei * zb[e] * ej;
+  zb[e] = i == j ? z + ei : z;
+}
+
+// leaf counts n - pos(E) - neg(M) into the instances' totals
+__global__ void k_tree_count_leaf(int n, int nb, int per, const int* __restrict__ kount, const int* __restrict__ epos,
+                                  int* __restrict__ acc, int* __restrict__ bad) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int inst = b / per;
+  const int kc = kount[b];
+  if (kc < 0 || kc > n)
+    atomicMax(bad + inst, 1);
+  else
+    atomicAdd(acc + inst, n - epos[b] - kc);
+}
+
 // leaf segments from Z = c1^-1: M = D Z D, Y1 = diag(Y14) - M, Y2 = M,
 // Y4 = D2 W(B) + diag(Y14) - M
 __global__ void k_tree_leaf(BlockArgs B, GridArgs G, InstArgs I, int ninst, int nstep, double h,
@@ -273,13 +352,15 @@ __global__ void k_tree_zero(int* __restrict__ p, int cnt) {
   if (i < cnt) p[i] = 0;
 }
 
-// NODES = sum(KOUNT) - NSAVE and STATUS
+// NODES = sum(KOUNT) - NSAVE and STATUS (uc_kount null: NSAVE already in the
+// leaf counts)
 __global__ void k_tree_nodes(int ninst, int nstep, const int* __restrict__ uc_kount, const int* __restrict__ acc,
                              const int* __restrict__ bad, int* __restrict__ nodes, int* __restrict__ status) {
   const int inst = blockIdx.x * blockDim.x + threadIdx.x;
   if (inst >= ninst) return;
   int nsave = 0;
-  for (int k = 0; k < nstep; ++k) nsave += uc_kount[k];
+  if (uc_kount)
+    for (int k = 0; k < nstep; ++k) nsave += uc_kount[k];
   nodes[inst] = acc[inst] - nsave;
   status[inst] = bad[inst] ? 1 : 0;
 }
