@@ -2,19 +2,19 @@
 
 Workload: atom + rigid rotor, even j = 0..40, JTOT = 60, parity block 2
 (N = 441 channels), LJ 12-6 Legendre terms lambda = 0..6, E = 300 cm-1;
-diabatic modified log-derivative propagation (MDPROP) from 2 to 20 A at
-DR = 0.005 A (3600 sectors), YINIT at 2 A, Y->K->S matching at 20 A.  The
-AIRY segment 20-200 A of the N=441 deck is not part of the timed solve (the
-large-N AIRY basis needs an eigensolver beyond the LDS-resident one).  One
-"step" is one complete S-matrix solve with the block's VL and the P(R) table
-already resident in HBM.
+YINIT at 2 A, diabatic modified log-derivative propagation (MDPROP) from 2
+to 20 A at DR = 0.005 A (3600 sectors), AIRY propagation from 20 to 200 A
+(TOLHIL = 0, 162 steps growing as R^(4/3) from 0.2 A), Y->K->S matching at
+200 A.  One "step" is one complete S-matrix solve with the block's VL and
+the P(R) tables already resident in HBM.
 
 Prints one JSON line: value = S-matrices/s, plus sector-steps/s; the
 roofline of the LDMD propagation stage (the energy-independent sector
 matrices and the recurrence, solved as a tree of merged sector maps),
 priced at the reference algorithm's work per sector-step and timed with HIP
-events on the library's stream; and a CPU baseline (the oracle's MDPROP on
-a bounded sample of sectors on this host, one thread).
+events on the library's stream; the AIRY segment's times beside it; and a
+CPU baseline (the oracle's MDPROP on a bounded sample of sectors on this
+host, one thread).
 """
 from __future__ import annotations
 
@@ -50,7 +50,8 @@ def _cpu_baseline(blk, seg, ered, nsamp: int) -> dict:
     return {"value": rate / seg.nstep, "unit": "S-matrices/s", "cores": 1, "kind": "port",
             "sector_steps_per_s": rate,
             "sample": f"oracle MDPROP, first {nsamp} of {seg.nstep} sectors at N={blk.n}, single thread "
-                      f"({dt:.1f} s, {rate:.2f} sector-steps/s; YINIT and matching not counted)"}
+                      f"({dt:.1f} s, {rate:.2f} sector-steps/s; YINIT, the AIRY segment and matching not "
+                      f"counted)"}
 
 
 def main() -> int:
@@ -80,26 +81,32 @@ def main() -> int:
     wv, nopen = S.gtwvec(ered, blk.eint)
     nb = S.chnsrt(blk.eint)
 
+    ra, wa = deck.airy_grid()
+    nairy = len(wa)
+
     eng = Engine(0)
     eng.set_team_size(args.team)
     b = eng.block(blk.vl, blk.eint, blk.cent, mxlam=blk.mxlam)
     g = eng.grid(r, blk.ptab(r), seg.dr)
+    ga = eng.airy_grid(ra, wa, blk.ptab(ra))
 
     def solve():
-        out = eng.smatrix_ld(b, g, 6, [ered], wv, [nopen], nb, seg.rstop)
+        out = eng.smatrix_segments(b, [g, ga], [6, 9], [ered], wv, [nopen], nb, deck.airy_stop)
         if out["STATUS"][0] != 0:
             raise RuntimeError(f"solve failed: status {out['STATUS'][0]}")
         return out
 
     for _ in range(args.warmup):
         solve()
-    t_prop, t_sec = [], []
+    t_prop, t_sec, t_abas, t_aprop = [], [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         out = solve()
-        tm = eng.last_timing()
-        t_prop.append(tm["ms_prop"])
-        t_sec.append(tm["ms_sector"])
+        tl, ta = eng.last_segment_timing(0), eng.last_segment_timing(1)
+        t_prop.append(tl["ms_prop"])
+        t_sec.append(tl["ms_sector"])
+        t_abas.append(ta["ms_sector"])
+        t_aprop.append(ta["ms_prop"])
     wall = time.perf_counter() - t0
     s = out["SR"][0] + 1j * out["SI"][0]
     unit_err = float(np.abs(s @ s.conj().T - np.eye(len(s))).max())
@@ -116,9 +123,10 @@ def main() -> int:
     ms_prop = float(np.mean(t_prop))
     ms_stage = ms_sec + ms_prop
     achieved = flops_alg / (ms_stage * 1e-3)
-    # what the tree executes: U(C) and leaf inversions, (nstep - 1) merges of
-    # one inversion and five n x n products, the final application
-    flops_exec = (2.0 * nstep * n ** 3 + (nstep - 1) * 11.0 * n ** 3 + 5.0 * n ** 3)
+    # what the tree executes: one leaf inversion per sector (V + E^-1, no
+    # U(C) at this step size), (nstep - 1) merges of one inversion and five
+    # n x n products, the final application
+    flops_exec = (1.0 * nstep * n ** 3 + (nstep - 1) * 11.0 * n ** 3 + 5.0 * n ** 3)
     line = {
         "metric": METRIC,
         "value": smat_per_s,
@@ -133,17 +141,18 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"rotor N={n} JTOT=60 E=300 cm-1, LDMD 2-20 A DR={seg.dr:.4f} A ({nstep} sectors), "
-                        f"YINIT + YTOK/KTOS matching at 20 A, one instance per solve "
-                        f"(the deck's AIRY segment 20-200 A not included)",
-            "n_channels": n, "sectors": nstep, "precision": "fp64", "batch_instances": 1,
+            "workload": f"rotor N={n} JTOT=60 E=300 cm-1, YINIT at 2 A, LDMD 2-20 A DR={seg.dr:.4f} A "
+                        f"({nstep} sectors), AIRY 20-{deck.airy_stop:.0f} A ({nairy} steps, TOLHIL=0, R^(4/3)), "
+                        f"YTOK/KTOS matching at {deck.airy_stop:.0f} A, one instance per solve",
+            "n_channels": n, "sectors": nstep, "airy_steps": nairy, "precision": "fp64", "batch_instances": 1,
         },
         "sector_steps_per_s": nstep * smat_per_s,
-        "kernel_ms": {"mdprop_sector_matrices": ms_sec, "mdprop_recurrence": ms_prop},
+        "kernel_ms": {"mdprop_sector_matrices": ms_sec, "mdprop_recurrence": ms_prop,
+                      "airy_step_bases": float(np.mean(t_abas)), "airy_propagation": float(np.mean(t_aprop))},
         "unitarity_err": unit_err,
         "roofline": {"bound": "mfma",
-                     "kernel": "LDMD propagation stage: k_mdprop_uc_large + tree (k_syminv_large, k_gemm_seg, "
-                               "k_tree_*)",
+                     "kernel": "LDMD propagation stage: k_mdprop_diag + tree (k_tree_m, k_syminv_large, "
+                               "k_gemm_seg, k_tree_*)",
                      "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64, "traffic": None,
                      "flops_per_launch": flops_alg, "ms_per_launch": ms_stage,

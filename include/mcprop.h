@@ -54,7 +54,10 @@ int mcp_ctx_info(const mcp_ctx *ctx, int32_t *max_n_md, int32_t *max_n_ld, int32
 int mcp_syminv_batched(mcp_ctx *ctx, int32_t n, int32_t nmat, const double *a, double *ainv, int32_t *kount);
 
 /* DIAGVC for a batch (source_code/diagvc.f:29-60, DSYEVR): eigenvalues
- * ascending, eigenvectors as columns (evecs may be NULL). */
+ * ascending, eigenvectors as columns (evecs may be NULL).  LDS-resident
+ * Jacobi up to mcp_ctx_info's max_n_eig; beyond it, up to N = 512, a
+ * global-memory solver (Householder tridiagonalisation, bisection, inverse
+ * iteration) that reads the lower triangle. */
 int mcp_eigh_batched(mcp_ctx *ctx, int32_t n, int32_t nmat, const double *a, double *evals, double *evecs);
 
 /* Upload one (JTOT, IB) block: the coupling VL(NHAM, N(N+1)/2) with NHAM =
@@ -179,14 +182,23 @@ int mcp_set_team_size(mcp_ctx *ctx, int32_t g);
  * over the radial grid, as a tree of merged sector maps, where the device
  * memory holds it, else sequentially; 1 always sequentially on the
  * register-resident matrix-core kernel; 2 sequentially on the global-memory
- * kernel.  Same S-matrix and node count to rounding.  No reference
- * counterpart. */
+ * kernel.  Adding 4 puts DIAGVC (mcp_eigh_batched) and AIRY on the
+ * global-memory eigen-solver for every N (by default only N beyond the
+ * LDS-resident one, up to 512).  Same S-matrix and node count to rounding.
+ * No reference counterpart. */
 int mcp_set_large_mode(mcp_ctx *ctx, int32_t mode);
 
 /* Wall time in milliseconds of the last mcp_prop_ld / mcp_smatrix_ld call's
  * device work, measured with HIP events on the library's stream, split into
  * the energy-independent sector matrices and the propagation loop. */
 int mcp_last_timing(const mcp_ctx *ctx, double *ms_sector, double *ms_prop, double *ms_total);
+
+/* The same two times for segment k (0-based) of the last
+ * mcp_smatrix_segments call: for LD, the sector matrices and the
+ * propagation; for AIRY, the grid's energy-independent step bases, slopes
+ * and basis changes, and the propagation.  MCP_ERR_ARG for k beyond the
+ * segments of that call.  No reference counterpart. */
+int mcp_last_segment_timing(const mcp_ctx *ctx, int32_t k, double *ms_sector, double *ms_prop);
 
 #ifdef __cplusplus
 }

@@ -16,6 +16,7 @@
 #include "mcp_airyprop.hpp"
 #include "mcp_mfmaprop.hpp"
 #include "mcp_tree.hpp"
+#include "mcp_eig.hpp"
 
 namespace {
 
@@ -167,6 +168,7 @@ struct mcp_ctx {
   int team = 0;  // workgroups per instance on the large-N path (0: automatic)
   bool no_reg = false;  // keep the large-N MDPROP on the global-memory kernel
   int large_mode = 0;   // large-N MDPROP: 0 tree where memory allows, 1 sequential matrix-core, 2 global memory
+  bool eig_global = false;  // DIAGVC and AIRY on the global-memory eigen-solver for every N
   DevBuf s_u;  // LDPROP sector-matrix workspace (large N)
   DevBuf s_av, s_at, s_ae, s_ah;  // AIRY step bases, basis changes, eigenvalues, slopes
   DevBuf s_x;                     // column exchange of the register-resident MDPROP
@@ -176,7 +178,12 @@ struct mcp_ctx {
   DevBuf s_kc, s_acc;             // its negative-pivot counts
   DevBuf s_yv;                    // large-N YINIT: scales, flags, inertias
   DevBuf s_flag;                  // a device-side decision flag
+  DevBuf s_aa, s_ad, s_az;        // large-N AIRY: step matrices / reflectors, slopes' matrices, Z of T / D V
+  DevBuf s_ed, s_ew;              // global-memory eigen-solver: d, e, tau; inverse-iteration work
+  DevBuf s_ay, s_ax, s_am, s_akc; // large-N AIRY per instance: Y1 Y2 Y4, products, pivots, inertias
   double t_sector = 0.0, t_prop = 0.0, t_total = 0.0;
+  int seg_timed = 0;                       // segments of the last mcp_smatrix_segments call
+  double seg_ms[8][2] = {};                // their (energy-independent, propagation) times
 
   int fail(int code, const char* fmt, ...) {
     char buf[512];
@@ -612,6 +619,123 @@ int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const do
   return MCP_OK;
 }
 
+// DIAGVC on the global-memory solver (mcp_eig.hpp): nmat matrices at a
+// (device; lower triangle read, overwritten by the reflectors) to evals
+// (nmat x n, ascending) and evecs (nmat x n x n, columns); zt scratch of
+// nmat n^2 doubles.
+int eig_global(mcp_ctx* c, int n, int nmat, double* a, double* evals, double* evecs, double* zt) {
+  if (n > mcp::kEMaxN) return c->fail(MCP_ERR_UNSUPPORTED, "eigen-solver: N=%d above %d", n, mcp::kEMaxN);
+  HIPCHK(c, c->s_ed.ensure((size_t)3 * nmat * n * sizeof(double)));
+  HIPCHK(c, c->s_ew.ensure((size_t)nmat * mcp::kSArrays * n * 64 * sizeof(double)));
+  double* d = c->s_ed.as<double>();
+  double* e = d + (size_t)nmat * n;
+  double* tau = e + (size_t)nmat * n;
+  for (int m0 = 0; m0 < nmat; m0 += 65535) {
+    const int m = std::min(65535, nmat - m0);
+    const size_t o = (size_t)m0 * n, onn = o * n;
+    hipLaunchKernelGGL(mcp::k_sytrd, dim3(m), dim3(mcp::kET), 0, c->stream, n, a + onn, d + o, e + o, tau + o);
+    hipLaunchKernelGGL(mcp::k_stebz, dim3(m), dim3(mcp::kET), 0, c->stream, n, d + o, e + o, evals + o);
+    hipLaunchKernelGGL(mcp::k_stein, dim3(m), dim3(64), 0, c->stream, n, d + o, e + o, evals + o, zt + onn,
+                       c->s_ew.as<double>() + o * mcp::kSArrays * 64);
+    hipLaunchKernelGGL(mcp::k_ormtr, dim3((n + 15) / 16, m), dim3(256), 0, c->stream, n, a + onn, tau + o, zt + onn,
+                       evecs + onn);
+    HIPCHK(c, hipGetLastError());
+  }
+  return MCP_OK;
+}
+
+// AIRY for N beyond the LDS-resident kernels: the energy-independent step
+// data of the grid on the global-memory eigen-solver and the batched
+// matrix-core products, then per step the two products, the team inversion
+// and the elementwise updates for all instances together.
+int airy_prop_global(mcp_ctx* c, Block& b, Grid& g, int ninst, const double* d_ered, double* dy, int* d_nodes,
+                     int* d_status) {
+  const int n = b.n, nstep = g.nstep;
+  if (n > mcp::kEMaxN) return c->fail(MCP_ERR_UNSUPPORTED, "AIRY: N=%d above %d", n, mcp::kEMaxN);
+  if (b.nham > mcp::kAiryMaxHam) return c->fail(MCP_ERR_UNSUPPORTED, "AIRY: more than %d VL terms", mcp::kAiryMaxHam);
+  const size_t nn = (size_t)n * n, sn = (size_t)nstep * nn;
+  HIPCHK(c, c->s_aa.ensure(sn * sizeof(double)));
+  HIPCHK(c, c->s_ad.ensure(sn * sizeof(double)));
+  HIPCHK(c, c->s_az.ensure(sn * sizeof(double)));
+  HIPCHK(c, c->s_av.ensure(sn * sizeof(double)));
+  HIPCHK(c, c->s_at.ensure(sn * sizeof(double)));
+  HIPCHK(c, c->s_ae.ensure((size_t)nstep * n * sizeof(double)));
+  HIPCHK(c, c->s_ah.ensure((size_t)nstep * n * sizeof(double)));
+  HIPCHK(c, c->s_ay.ensure((size_t)ninst * nstep * 3 * n * sizeof(double)));
+  HIPCHK(c, c->s_ax.ensure((size_t)ninst * nn * sizeof(double)));
+  HIPCHK(c, c->s_am.ensure((size_t)ninst * nn * sizeof(double)));
+  HIPCHK(c, c->s_akc.ensure((size_t)ninst * sizeof(int)));
+  double* aa = c->s_aa.as<double>();
+  double* ad = c->s_ad.as<double>();
+  double* az = c->s_az.as<double>();
+  double* av = c->s_av.as<double>();
+  double* at = c->s_at.as<double>();
+  double* ae = c->s_ae.as<double>();
+  double* ah = c->s_ah.as<double>();
+  double* ay = c->s_ay.as<double>();
+  double* ax = c->s_ax.as<double>();
+  double* am = c->s_am.as<double>();
+  int* akc = c->s_akc.as<int>();
+  const mcp::BlockArgs B = block_args(b);
+  const mcp::GridArgs G = grid_args(g);
+  const double* w = g.w.as<double>();
+  int rc;
+  HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+  // every step's basis V_k, eigenvalues, slopes HP = diag(V^T D V), and the
+  // basis changes T_0 = V_0, T_k = V_{k-1}^T V_k
+  hipLaunchKernelGGL(mcp::k_airy_wd, dim3(nblk256(nn)), dim3(256), 0, c->stream, B, G, nstep, w, mcp::kPotentSq3, aa,
+                     ad);
+  HIPCHK(c, hipGetLastError());
+  if ((rc = eig_global(c, n, nstep, aa, ae, av, az))) return rc;
+  const mcp::SegRef Dk{ad, nn, 0}, Vk{av, nn, 0}, Vn{av + nn, nn, 0};
+  if ((rc = tree_gemm<false, false>(c, n, nstep, 1, Dk, Vk, Dk, mcp::SegOut{az, nn, 0}, 1.0, 0.0))) return rc;
+  hipLaunchKernelGGL(mcp::k_airy_hp, dim3(nblk256((size_t)nstep * n)), dim3(256), 0, c->stream, n, nstep, av, az, ah);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(at, av, nn * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  if (nstep > 1 &&
+      (rc = tree_gemm<true, false>(c, n, nstep - 1, 1, Vk, Vn, Vk, mcp::SegOut{at + nn, nn, 0}, 1.0, 0.0)))
+    return rc;
+  HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  // the steps, all instances together
+  hipLaunchKernelGGL(mcp::k_airy_y, dim3(nblk256((size_t)ninst * nstep * n)), dim3(256), 0, c->stream, n, nstep, ninst,
+                     ae, ah, w, d_ered, ay);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemsetAsync(d_nodes, 0, ninst * sizeof(int), c->stream));
+  HIPCHK(c, hipMemsetAsync(d_status, 0, ninst * sizeof(int), c->stream));
+  const int outward = g.h_w[0] > 0.0 ? 1 : 0;
+  const int bk = n > 30 ? 1 : 0;
+  const bool team = team_inverse_fits(c, n);
+  const size_t ll = lds_large(n, 0, 0);
+  if (!team && ll > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "AIRY: N=%d exceeds the inversion limit", n);
+  const unsigned ne = nblk256((size_t)ninst * nn);
+  const mcp::SegRef Z{dy, nn, 0}, X{ax, nn, 0};
+  for (int k = 0; k < nstep; ++k) {
+    const mcp::SegRef Tk{at + (size_t)k * nn, 0, 0};
+    // Z <- T^T (Z T), then M = sym(Z) + diag(Y1)
+    if ((rc = tree_gemm<false, false>(c, n, ninst, 1, Z, Tk, Z, mcp::SegOut{ax, nn, 0}, 1.0, 0.0))) return rc;
+    if ((rc = tree_gemm<true, false>(c, n, ninst, 1, Tk, X, X, mcp::SegOut{dy, nn, 0}, 1.0, 0.0))) return rc;
+    hipLaunchKernelGGL(mcp::k_airy_m, dim3(ne), dim3(256), 0, c->stream, n, ninst, k, nstep, dy, ay, am);
+    HIPCHK(c, hipGetLastError());
+    if (team) {
+      if ((rc = team_inverse(c, n, ninst, am, akc, bk))) return rc;
+    } else {
+      hipLaunchKernelGGL(mcp::k_syminv_large, dim3(ninst), dim3(kNT), ll, c->stream, n, am, akc, bk);
+      HIPCHK(c, hipGetLastError());
+    }
+    hipLaunchKernelGGL(mcp::k_airy_z, dim3(ne), dim3(256), 0, c->stream, n, ninst, k, nstep, am, akc, ay, dy, d_nodes,
+                       d_status, outward);
+    HIPCHK(c, hipGetLastError());
+  }
+  // back to the primitive basis: Z <- sym(V Z V^T), V of the last step
+  const mcp::SegRef Vl{av + (size_t)(nstep - 1) * nn, 0, 0};
+  if ((rc = tree_gemm<false, false>(c, n, ninst, 1, Vl, Z, Vl, mcp::SegOut{ax, nn, 0}, 1.0, 0.0))) return rc;
+  if ((rc = tree_gemm<false, true>(c, n, ninst, 1, X, Vl, X, mcp::SegOut{am, nn, 0}, 1.0, 0.0))) return rc;
+  hipLaunchKernelGGL(mcp::k_airy_out, dim3(ne), dim3(256), 0, c->stream, n, ninst, am, dy);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+  return MCP_OK;
+}
+
 // AIRY propagation of ninst instances whose Z already sits in dy (TOLHI = 0).
 int airy_prop_device(mcp_ctx* c, Block& b, Grid& g, int ninst, const double* d_ered, double* dy, int* d_nodes,
                      int* d_status) {
@@ -620,8 +744,8 @@ int airy_prop_device(mcp_ctx* c, Block& b, Grid& g, int ninst, const double* d_e
     return c->fail(MCP_ERR_UNSUPPORTED, "AIRY with NCONST > 0 (the local basis would depend on the instance)");
   const int n = b.n, nstep = g.nstep;
   const size_t ls = lds_airy_sector(n, b.nham), lp = lds_airy_prop(n);
-  if (ls > c->lds_max || lp > c->lds_max)
-    return c->fail(MCP_ERR_UNSUPPORTED, "AIRY: N=%d exceeds the LDS-resident AIRY limit", n);
+  if (ls > c->lds_max || lp > c->lds_max || c->eig_global)
+    return airy_prop_global(c, b, g, ninst, d_ered, dy, d_nodes, d_status);
   const size_t nn = (size_t)n * n;
   HIPCHK(c, c->s_av.ensure((size_t)nstep * nn * sizeof(double)));
   HIPCHK(c, c->s_at.ensure((size_t)nstep * nn * sizeof(double)));
@@ -754,9 +878,21 @@ int mcp_eigh_batched(mcp_ctx* c, int32_t n, int32_t nmat, const double* a, doubl
   if (nmat == 0) return MCP_OK;
   HIPCHK(c, hipSetDevice(c->device));
   const size_t lds = lds_eig(n);
-  if (lds > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "eigh: N=%d exceeds the LDS-resident limit", n);
   const size_t cnt = (size_t)nmat * n * n;
   int rc;
+  if (lds > c->lds_max || c->eig_global) {
+    if ((rc = upload(c, c->s_a, a, cnt))) return rc;
+    HIPCHK(c, c->s_b.ensure(cnt * sizeof(double)));
+    HIPCHK(c, c->s_c.ensure((size_t)nmat * n * sizeof(double)));
+    HIPCHK(c, c->s_d.ensure(cnt * sizeof(double)));
+    if ((rc = eig_global(c, n, nmat, c->s_a.as<double>(), c->s_c.as<double>(), c->s_b.as<double>(),
+                         c->s_d.as<double>())))
+      return rc;
+    if ((rc = download(c, evals, c->s_c, (size_t)nmat * n))) return rc;
+    if (evecs && (rc = download(c, evecs, c->s_b, cnt))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MCP_OK;
+  }
   if ((rc = upload(c, c->s_a, a, cnt))) return rc;
   HIPCHK(c, c->s_b.ensure(cnt * sizeof(double)));
   HIPCHK(c, c->s_c.ensure((size_t)nmat * n * sizeof(double)));
@@ -1187,6 +1323,7 @@ int mcp_smatrix_segments(mcp_ctx* c, int32_t bid, int32_t nseg, const int32_t* g
   int* dn = c->s_j.as<int>();  // [yinit | nodes, status per segment]
   // YINIT on the first segment only (source_code/scctrl.f:116-121)
   if ((rc = yinit_device(c, *b, *g[0], I, ninst, 0, 1, 0.0, dn))) return rc;
+  c->seg_timed = 0;
   for (int k = 0; k < nseg; ++k) {
     int* sn = dn + ninst * (1 + 2 * k);
     if (iprops[k] == 9) {
@@ -1195,6 +1332,10 @@ int mcp_smatrix_segments(mcp_ctx* c, int32_t bid, int32_t nseg, const int32_t* g
       const int ip = (iprops[k] == 6 && n == 1) ? MCP_PROP_ODPROP : iprops[k];
       if ((rc = prop_ld_device(c, *b, *g[k], ip, ninst, I.ered, I.pconst, I.y, sn, sn + ninst))) return rc;
     }
+    if ((rc = record_timing(c))) return rc;
+    c->seg_ms[k][0] = c->t_sector;
+    c->seg_ms[k][1] = c->t_prop;
+    c->seg_timed = k + 1;
   }
   if ((rc = match_device(c, n, ninst, I.y, wvec, nopen, nb, b->cent.as<double>(), rval))) return rc;
   std::vector<int> tmp((size_t)ninst * (2 * nseg + 1));
@@ -1282,8 +1423,10 @@ int mcp_bound_match(mcp_ctx* c, int32_t bid, int32_t gid_out, int32_t gid_in, in
 }
 
 int mcp_set_large_mode(mcp_ctx* c, int32_t mode) {
-  if (!c || mode < 0 || mode > 2) return c ? c->fail(MCP_ERR_ARG, "large mode %d: 0, 1 or 2", mode) : MCP_ERR_ARG;
-  c->large_mode = mode;
+  if (!c || mode < 0 || mode > 7 || (mode & 3) > 2)
+    return c ? c->fail(MCP_ERR_ARG, "large mode %d: 0, 1 or 2, plus 4", mode) : MCP_ERR_ARG;
+  c->large_mode = mode & 3;
+  c->eig_global = (mode & 4) != 0;
   return MCP_OK;
 }
 
@@ -1291,6 +1434,13 @@ int mcp_set_team_size(mcp_ctx* c, int32_t g) {
   if (!c) return MCP_ERR_ARG;
   c->no_reg = g < 0;
   c->team = g < 0 ? -g : g;
+  return MCP_OK;
+}
+
+int mcp_last_segment_timing(const mcp_ctx* c, int32_t k, double* ms_sector, double* ms_prop) {
+  if (!c || k < 0 || k >= c->seg_timed) return MCP_ERR_ARG;
+  if (ms_sector) *ms_sector = c->seg_ms[k][0];
+  if (ms_prop) *ms_prop = c->seg_ms[k][1];
   return MCP_OK;
 }
 

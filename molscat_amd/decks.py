@@ -67,6 +67,15 @@ class Deck:
     def potential(self) -> S.PowerPotential:
         return S.lj_rotor_potential([1.0, *self.cvals], RM, EPSIL)
 
+    def airy_grid(self):
+        """(r, drnow) of the AIRY segment from the LD segment's end to
+        airy_stop: TOLHIL = 0, steps growing as R^POWRL with POWRL = 4/3
+        (the AIRY default, source_code/mol.prpini.f:111-116), the first
+        airy_dr wide (DRSET, source_code/drset.f:62-80)."""
+        powr = 4.0 / 3.0
+        nstep, dr0 = S.drset_airy(self.ld.rstop, self.airy_stop, self.airy_dr, powr)
+        return S.airy_grid(self.ld.rstop, self.airy_stop, nstep, dr0, powr)
+
     def block(self, jtot=None, ib=None) -> S.RotorBlock:
         return S.rotor_block(self.jtot if jtot is None else jtot, self.ib if ib is None else ib, self.jlevels, BE,
                              self.lambdas, self.potential(), URED, runit=1.0)

@@ -285,13 +285,22 @@ class Engine:
 
     def set_large_mode(self, mode: int):
         """Large-N MDPROP: 0 tree over the radial grid where memory allows
-        (default), 1 sequential matrix-core kernel, 2 global-memory kernel."""
+        (default), 1 sequential matrix-core kernel, 2 global-memory kernel;
+        plus 4: DIAGVC and AIRY on the global-memory eigen-solver for every
+        N."""
         self._check(self.lib.mcp_set_large_mode(self.ctx, int(mode)), "mcp_set_large_mode")
 
     def last_timing(self) -> dict:
         v = [ctypes.c_double() for _ in range(3)]
         self._check(self.lib.mcp_last_timing(self.ctx, *[ctypes.byref(x) for x in v]), "mcp_last_timing")
         return dict(ms_sector=v[0].value, ms_prop=v[1].value, ms_total=v[2].value)
+
+    def last_segment_timing(self, k: int) -> dict:
+        """Device times of segment k of the last smatrix_segments call."""
+        v = [ctypes.c_double() for _ in range(2)]
+        self._check(self.lib.mcp_last_segment_timing(self.ctx, int(k), *[ctypes.byref(x) for x in v]),
+                    "mcp_last_segment_timing")
+        return dict(ms_sector=v[0].value, ms_prop=v[1].value)
 
 
 def _unpack_square(flat: np.ndarray, nopen: np.ndarray):
