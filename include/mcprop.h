@@ -174,8 +174,9 @@ int mcp_bound_match(mcp_ctx *ctx, int32_t block_id, int32_t grid_out, int32_t gr
  * ceil(N/32) and ceil(N/16) when a smaller g is asked for; the matching's
  * LU takes at most |g|), 0 restores the automatic choice, g < 0 keeps the
  * sequential MDPROP on the global-memory kernel with -g workgroups and the
- * large-N YINIT on one workgroup per instance (with g = -1 no cooperative
- * launch outside that sequential kernel).  No reference counterpart. */
+ * large-N YINIT and AIRY inversions on one workgroup per instance (with
+ * g = -1 no cooperative launch outside that sequential kernel).  No
+ * reference counterpart. */
 int mcp_set_team_size(mcp_ctx *ctx, int32_t g);
 
 /* How MDPROP runs for large N: 0 (default) solves the recurrence in parallel

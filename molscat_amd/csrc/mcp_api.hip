@@ -179,7 +179,7 @@ struct mcp_ctx {
   DevBuf s_yv;                    // large-N YINIT: scales, flags, inertias
   DevBuf s_flag;                  // a device-side decision flag
   DevBuf s_aa, s_ad, s_az;        // large-N AIRY: step matrices / reflectors, slopes' matrices, Z of T / D V
-  DevBuf s_ed, s_ew;              // global-memory eigen-solver: d, e, tau; inverse-iteration work
+  DevBuf s_ed, s_ew;              // global-memory eigen-solver: d, e, tau; inverse-iteration factors
   DevBuf s_ay, s_ax, s_am, s_akc; // large-N AIRY per instance: Y1 Y2 Y4, products, pivots, inertias
   double t_sector = 0.0, t_prop = 0.0, t_total = 0.0;
   int seg_timed = 0;                       // segments of the last mcp_smatrix_segments call
@@ -625,18 +625,21 @@ int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const do
 // nmat n^2 doubles.
 int eig_global(mcp_ctx* c, int n, int nmat, double* a, double* evals, double* evecs, double* zt) {
   if (n > mcp::kEMaxN) return c->fail(MCP_ERR_UNSUPPORTED, "eigen-solver: N=%d above %d", n, mcp::kEMaxN);
+  // the inverse iteration's factors take 5 n^2 doubles a matrix: batches of
+  // at most 256 matrices
+  const int chunk = std::min(nmat, 256);
   HIPCHK(c, c->s_ed.ensure((size_t)3 * nmat * n * sizeof(double)));
-  HIPCHK(c, c->s_ew.ensure((size_t)nmat * mcp::kSArrays * n * 64 * sizeof(double)));
+  HIPCHK(c, c->s_ew.ensure((size_t)5 * chunk * n * n * sizeof(double)));
   double* d = c->s_ed.as<double>();
   double* e = d + (size_t)nmat * n;
   double* tau = e + (size_t)nmat * n;
-  for (int m0 = 0; m0 < nmat; m0 += 65535) {
-    const int m = std::min(65535, nmat - m0);
+  for (int m0 = 0; m0 < nmat; m0 += chunk) {
+    const int m = std::min(chunk, nmat - m0);
     const size_t o = (size_t)m0 * n, onn = o * n;
     hipLaunchKernelGGL(mcp::k_sytrd, dim3(m), dim3(mcp::kET), 0, c->stream, n, a + onn, d + o, e + o, tau + o);
     hipLaunchKernelGGL(mcp::k_stebz, dim3(m), dim3(mcp::kET), 0, c->stream, n, d + o, e + o, evals + o);
-    hipLaunchKernelGGL(mcp::k_stein, dim3(m), dim3(64), 0, c->stream, n, d + o, e + o, evals + o, zt + onn,
-                       c->s_ew.as<double>() + o * mcp::kSArrays * 64);
+    hipLaunchKernelGGL(mcp::k_stein, dim3(m), dim3(mcp::kET), 0, c->stream, n, d + o, e + o, evals + o, zt + onn,
+                       c->s_ew.as<double>());
     hipLaunchKernelGGL(mcp::k_ormtr, dim3((n + 15) / 16, m), dim3(256), 0, c->stream, n, a + onn, tau + o, zt + onn,
                        evecs + onn);
     HIPCHK(c, hipGetLastError());
@@ -704,7 +707,8 @@ int airy_prop_global(mcp_ctx* c, Block& b, Grid& g, int ninst, const double* d_e
   HIPCHK(c, hipMemsetAsync(d_status, 0, ninst * sizeof(int), c->stream));
   const int outward = g.h_w[0] > 0.0 ? 1 : 0;
   const int bk = n > 30 ? 1 : 0;
-  const bool team = team_inverse_fits(c, n);
+  // a negative team size keeps the inversions on one workgroup per instance
+  const bool team = !c->no_reg && team_inverse_fits(c, n);
   const size_t ll = lds_large(n, 0, 0);
   if (!team && ll > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "AIRY: N=%d exceeds the inversion limit", n);
   const unsigned ne = nblk256((size_t)ninst * nn);

@@ -15,7 +15,7 @@
 //   k_stebz  every eigenvalue of T by bisection on the Sturm count, one
 //            thread per eigenvalue, which yields them in ascending order.
 //   k_stein  the eigenvectors of T by inverse iteration with partial
-//            pivoting, one lane per cluster of eigenvalues closer than
+//            pivoting, one wave per cluster of eigenvalues closer than
 //            1e-3 |T|_1 to their neighbour, each vector orthogonalised
 //            against the cluster's earlier ones (the scheme of LAPACK's
 //            DSTEIN); the largest component made positive.
@@ -46,7 +46,6 @@ constexpr int kERows = kET / 64;     // rows per lane in k_sytrd
 constexpr int kEMaxN = 64 * kERows;  // largest n of this solver
 constexpr int kOGroups = 16;         // lanes per column in k_ormtr
 constexpr int kORows = kEMaxN / kOGroups;
-constexpr int kSArrays = 6;          // per-lane work arrays of k_stein
 
 // every memory counter drained, then the barrier: global stores of one wave
 // are complete when another wave of the workgroup reads them
@@ -281,10 +280,15 @@ __global__ void __launch_bounds__(kET) k_stebz(int n, const double* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// eigenvectors of the tridiagonal by inverse iteration; one wave per matrix,
-// one lane per cluster.  work: kSArrays * n * 64 doubles per matrix, element
-// k of array q of lane l at ((q n + k) 64 + l) (lanes step through k
-// together, so every access is one coalesced row)
+// eigenvectors of the tridiagonal by inverse iteration; one workgroup per
+// matrix, thread j for eigenvalue j.  Every iteration solves all n systems
+// at once, then orthogonalises each cluster's iterates by modified
+// Gram-Schmidt in eigenvalue order (step s: every later member of a cluster
+// removes its component along the cluster's member s), so the last solve of
+// each vector is followed by its orthogonalisation against the cluster's
+// earlier vectors as in DSTEIN.  Interleaved storage: element k of
+// eigenvalue j at [k n + j] (the threads step through k together); the
+// iterates in x, the factors in work (5 n^2 per matrix).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double stein_start(int j, int k) {
   unsigned h = (unsigned)(j + 1) * 2654435761u ^ (unsigned)(k + 1) * 2246822519u;
@@ -294,144 +298,165 @@ __device__ __forceinline__ double stein_start(int j, int k) {
   return (double)(h & 0xFFFFFFu) / 8388608.0 - 1.0;  // (-1, 1)
 }
 
-__global__ void __launch_bounds__(64) k_stein(int n, const double* __restrict__ dg, const double* __restrict__ eg,
-                                              const double* __restrict__ evg, double* __restrict__ zg,
+__global__ void __launch_bounds__(kET) k_stein(int n, const double* __restrict__ dg, const double* __restrict__ eg,
+                                              const double* __restrict__ evg, double* __restrict__ xg,
                                               double* __restrict__ work) {
-  __shared__ int cstart[kEMaxN + 1];
-  __shared__ int ncl;
-  const int lane = threadIdx.x;
-  const int mat = blockIdx.x;
-  const double* d = dg + (size_t)mat * n;
-  const double* e = eg + (size_t)mat * n;
-  const double* ev = evg + (size_t)mat * n;
-  double* Z = zg + (size_t)mat * n * n;
-  double* W = work + (size_t)mat * kSArrays * n * 64;
+  __shared__ int cst[kEMaxN];       // first eigenvalue of each eigenvalue's cluster
+  __shared__ double lamv[kEMaxN];   // the shifts (DSTEIN's perturbation of close eigenvalues)
+  __shared__ double snorm[kEMaxN];  // squared norms of the iterates
+  __shared__ double sv[kET / 64];
+  __shared__ int maxcl;
+  const int j = threadIdx.x;
+  const size_t mat = blockIdx.x, nn = (size_t)n * n;
+  const double* d = dg + mat * n;
+  const double* e = eg + mat * n;
+  const double* ev = evg + mat * n;
+  double* X = xg + mat * nn;
+  double* F = work + mat * 5 * nn;  // multipliers
+  double* UI = F + nn;              // reciprocal pivots of U
+  double* U1 = UI + nn;             // first superdiagonal of U
+  double* U2 = U1 + nn;             // second superdiagonal of U
+  double* PV = U2 + nn;             // 1 where rows k, k+1 were interchanged
   double tn = 0.0;
-  for (int i = lane; i < n; i += 64)
-    tn = fmax(tn, fabs(d[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < n ? fabs(e[i]) : 0.0));
-  tn = wave_max(tn);
+  if (j < n) tn = fabs(d[j]) + (j > 0 ? fabs(e[j - 1]) : 0.0) + (j + 1 < n ? fabs(e[j]) : 0.0);
+  tn = block_max<kET>(tn, sv);
   const double ortol = 1e-3 * tn;
-  if (lane == 0) {
-    int c = 0;
-    for (int i = 0; i < n; ++i)
-      if (i == 0 || ev[i] - ev[i - 1] > ortol) cstart[c++] = i;
-    cstart[c] = n;
-    ncl = c;
-  }
-  __syncthreads();
   const double eps = DBL_EPSILON;
   const double tiny = fmax(eps * tn, DBL_MIN);
-  // arrays: 0 multiplier, 1 U diagonal, 2 U first superdiagonal, 3 second
-  // superdiagonal, 4 interchange flag, 5 the iterate
-  auto at = [&](int q, int k) -> double& { return W[((size_t)q * n + k) * 64 + lane]; };
-  for (int cl = lane; cl < ncl; cl += 64) {
-    const int c0 = cstart[cl], c1 = cstart[cl + 1];
+  if (j == 0) {
+    int c0 = 0, mx = 1;
     double lprev = 0.0;
-    for (int jv = c0; jv < c1; ++jv) {
-      double lam = ev[jv];
-      if (jv > c0) {
+    for (int i = 0; i < n; ++i) {
+      double lam = ev[i];
+      if (i > 0 && ev[i] - ev[i - 1] > ortol) c0 = i;
+      if (i > c0) {
         const double pert = 10.0 * fabs(eps * lam);
         if (lam - lprev < pert) lam = lprev + pert;
       }
       lprev = lam;
-      // T - lam I = P L U, row interchanges by partial pivoting
-      double dk = d[0] - lam, duk = n > 1 ? e[0] : 0.0;
-      for (int k = 0; k + 1 < n; ++k) {
-        const double sub = e[k], dn = d[k + 1] - lam, cn = k + 2 < n ? e[k + 1] : 0.0;
-        if (fabs(dk) >= fabs(sub)) {
-          const double f = dk != 0.0 ? sub / dk : 0.0;
-          at(0, k) = f;
-          at(1, k) = dk;
-          at(2, k) = duk;
-          at(3, k) = 0.0;
-          at(4, k) = 0.0;
-          dk = dn - f * duk;
-          duk = cn;
-        } else {
-          const double f = dk / sub;
-          at(0, k) = f;
-          at(1, k) = sub;
-          at(2, k) = dn;
-          at(3, k) = cn;
-          at(4, k) = 1.0;
-          dk = duk - f * dn;
-          duk = -f * cn;
-        }
-      }
-      at(0, n - 1) = 0.0;
-      at(1, n - 1) = dk;
-      at(2, n - 1) = 0.0;
-      at(3, n - 1) = 0.0;
-      at(4, n - 1) = 0.0;
-      double ss = 0.0;
-      for (int k = 0; k < n; ++k) {
-        const double v = stein_start(jv, k);
-        at(5, k) = v;
-        ss += v * v;
-      }
-      for (int it = 0; it < 3; ++it) {
-        const double sc = 1.0 / sqrt(ss);
-        // b <- L^-1 P^T (sc b)
-        double cur = sc * at(5, 0);
-        for (int k = 0; k + 1 < n; ++k) {
-          const double nxt = sc * at(5, k + 1), f = at(0, k);
-          if (at(4, k) != 0.0) {
-            at(5, k) = nxt;
-            cur = cur - f * nxt;
-          } else {
-            at(5, k) = cur;
-            cur = nxt - f * cur;
-          }
-        }
-        at(5, n - 1) = cur;
-        // x <- U^-1 b, pivots below tiny raised to tiny
-        double x1 = 0.0, x2 = 0.0;
-        ss = 0.0;
-        for (int k = n - 1; k >= 0; --k) {
-          double u = at(1, k);
-          if (fabs(u) < tiny) u = u < 0.0 ? -tiny : tiny;
-          const double xk = (at(5, k) - at(2, k) * x1 - at(3, k) * x2) / u;
-          at(5, k) = xk;
-          ss += xk * xk;
-          x2 = x1;
-          x1 = xk;
-        }
-        // orthogonal to the cluster's earlier vectors (modified Gram-Schmidt)
-        if (jv > c0) {
-          for (int p = c0; p < jv; ++p) {
-            const double* zp = Z + (size_t)p * n;
-            double dt = 0.0;
-            for (int k = 0; k < n; ++k) dt += at(5, k) * zp[k];
-            ss = 0.0;
-            for (int k = 0; k < n; ++k) {
-              const double v = at(5, k) - dt * zp[k];
-              at(5, k) = v;
-              ss += v * v;
-            }
-          }
-        }
-      }
-      // unit 2-norm, largest component positive
-      int kmax = 0;
-      double vmax = -1.0;
-      for (int k = 0; k < n; ++k) {
-        const double v = fabs(at(5, k));
-        if (v > vmax) {
-          vmax = v;
-          kmax = k;
-        }
-      }
-      double sc = 1.0 / sqrt(ss);
-      if (at(5, kmax) < 0.0) sc = -sc;
-      double* zj = Z + (size_t)jv * n;
-      for (int k = 0; k < n; ++k) zj[k] = sc * at(5, k);
+      cst[i] = c0;
+      lamv[i] = lam;
+      mx = max(mx, i - c0 + 1);
     }
+    maxcl = mx;
+  }
+  __syncthreads();
+  double ss = 0.0;
+  if (j < n) {
+    const double lam = lamv[j];
+    // T - lam I = P L U by partial pivoting; pivots below tiny raised to tiny
+    double dk = d[0] - lam, duk = n > 1 ? e[0] : 0.0;
+    for (int k = 0; k + 1 < n; ++k) {
+      const size_t o = (size_t)k * n + j;
+      const double sub = e[k], dn = d[k + 1] - lam, cn = k + 2 < n ? e[k + 1] : 0.0;
+      double u;
+      if (fabs(dk) >= fabs(sub)) {
+        const double f = dk != 0.0 ? sub / dk : 0.0;
+        F[o] = f;
+        u = dk;
+        U1[o] = duk;
+        U2[o] = 0.0;
+        PV[o] = 0.0;
+        dk = dn - f * duk;
+        duk = cn;
+      } else {
+        const double f = dk / sub;
+        F[o] = f;
+        u = sub;
+        U1[o] = dn;
+        U2[o] = cn;
+        PV[o] = 1.0;
+        dk = duk - f * dn;
+        duk = -f * cn;
+      }
+      if (fabs(u) < tiny) u = u < 0.0 ? -tiny : tiny;
+      UI[o] = 1.0 / u;
+    }
+    if (fabs(dk) < tiny) dk = dk < 0.0 ? -tiny : tiny;
+    const size_t o = (size_t)(n - 1) * n + j;
+    F[o] = 0.0;
+    UI[o] = 1.0 / dk;
+    U1[o] = 0.0;
+    U2[o] = 0.0;
+    PV[o] = 0.0;
+    for (int k = 0; k < n; ++k) {
+      const double v = stein_start(j, k);
+      X[(size_t)k * n + j] = v;
+      ss += v * v;
+    }
+  }
+  for (int it = 0; it < 3; ++it) {
+    if (j < n) {
+      const double sc = 1.0 / sqrt(ss);
+      // b <- L^-1 P^T (sc b)
+      double cur = sc * X[j];
+      for (int k = 0; k + 1 < n; ++k) {
+        const size_t o = (size_t)k * n + j;
+        const double nxt = sc * X[o + n], f = F[o];
+        if (PV[o] != 0.0) {
+          X[o] = nxt;
+          cur = cur - f * nxt;
+        } else {
+          X[o] = cur;
+          cur = nxt - f * cur;
+        }
+      }
+      X[(size_t)(n - 1) * n + j] = cur;
+      // x <- U^-1 b
+      double x1 = 0.0, x2 = 0.0;
+      ss = 0.0;
+      for (int k = n - 1; k >= 0; --k) {
+        const size_t o = (size_t)k * n + j;
+        const double xk = (X[o] - U1[o] * x1 - U2[o] * x2) * UI[o];
+        X[o] = xk;
+        ss += xk * xk;
+        x2 = x1;
+        x1 = xk;
+      }
+      snorm[j] = ss;
+    }
+    sync_mem();
+    // modified Gram-Schmidt inside the clusters, all clusters together
+    for (int st = 0; st + 1 < maxcl; ++st) {
+      // the pivot of this thread's cluster; none for threads beyond n
+      const int m = j < n ? cst[j] + st : j;
+      if (m < j) {
+        double dt = 0.0;
+        for (int k = 0; k < n; ++k) dt += X[(size_t)k * n + j] * X[(size_t)k * n + m];
+        const double f = dt / snorm[m];
+        double s2 = 0.0;
+        for (int k = 0; k < n; ++k) {
+          const size_t o = (size_t)k * n + j;
+          const double v = X[o] - f * X[(size_t)k * n + m];
+          X[o] = v;
+          s2 += v * v;
+        }
+        ss = s2;
+        snorm[j] = s2;
+      }
+      sync_mem();
+    }
+  }
+  if (j < n) {
+    // unit 2-norm, largest component (first of equals) positive
+    double vmax = -1.0, vk = 0.0;
+    for (int k = 0; k < n; ++k) {
+      const double v = X[(size_t)k * n + j];
+      if (fabs(v) > vmax) {
+        vmax = fabs(v);
+        vk = v;
+      }
+    }
+    double sc = 1.0 / sqrt(ss);
+    if (vk < 0.0) sc = -sc;
+    for (int k = 0; k < n; ++k) X[(size_t)k * n + j] *= sc;
   }
 }
 
 // ---------------------------------------------------------------------------
 // V = Q Z; grid (ceil(n/16), nmat), 256 threads: lane l of wave w holds rows
-// l%16 + 16 r of column 16 bx + 4 w + l/16
+// l%16 + 16 r of column 16 bx + 4 w + l/16.  Z as k_stein leaves it
+// (element i of vector c at [i n + c]); V column-major.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_ormtr(int n, const double* __restrict__ a, const double* __restrict__ taug,
                                                const double* __restrict__ zg, double* __restrict__ vg) {
@@ -447,7 +472,7 @@ __global__ void __launch_bounds__(256) k_ormtr(int n, const double* __restrict__
 #pragma unroll
   for (int r = 0; r < kORows; ++r) {
     const int i = g + kOGroups * r;
-    z[r] = (live && i < n) ? zg[mat * nn + (size_t)col * n + i] : 0.0;
+    z[r] = (live && i < n) ? zg[mat * nn + (size_t)i * n + col] : 0.0;
   }
   for (int j = n - 3; j >= 0; --j) {
     const double t = tau[j];
