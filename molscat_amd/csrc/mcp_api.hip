@@ -359,6 +359,102 @@ int tree_gemm(mcp_ctx* c, int n, int nbatch, int per, mcp::SegRef A, mcp::SegRef
 }
 
 // Device bytes the parallel-in-R MDPROP needs for ninst instances.
+
+// Symmetric inversions of one tree level: nb matrices at zb, inertias to kc.
+// Small batches, where one workgroup per matrix leaves the device idle, go
+// to workgroup teams (unless a negative team size was set).
+constexpr int kTreeTeamMax = 64;
+int tree_invert(mcp_ctx* c, int n, int nb, double* zb, int* kc, int bk) {
+  if (nb <= kTreeTeamMax && !c->no_reg && team_inverse_fits(c, n)) return team_inverse(c, n, nb, zb, kc, bk);
+  hipLaunchKernelGGL(mcp::k_syminv_large, dim3(nb), dim3(kNT), lds_large(n, 0, 0), c->stream, n, zb, kc, bk);
+  HIPCHK(c, hipGetLastError());
+  return MCP_OK;
+}
+
+// Buffers of a tree over nleaf segments per instance: two levels of
+// segments, the merge pivots and inverses, the products, the inertias.
+struct TreeBufs {
+  double *in, *out, *zb, *pq;
+  int *kc, *acc, *bad;
+};
+
+int tree_alloc(mcp_ctx* c, int n, int ninst, int nleaf, TreeBufs& t) {
+  const size_t nn = (size_t)n * n, sz = 3 * nn;
+  const size_t k1 = (size_t)std::max((nleaf + 1) / 2, 1);
+  HIPCHK(c, c->s_seg0.ensure((size_t)ninst * nleaf * sz * sizeof(double)));
+  HIPCHK(c, c->s_seg1.ensure((size_t)ninst * k1 * sz * sizeof(double)));
+  HIPCHK(c, c->s_zb.ensure((size_t)ninst * nleaf * nn * sizeof(double)));
+  HIPCHK(c, c->s_pq.ensure((size_t)ninst * k1 * 2 * nn * sizeof(double)));
+  HIPCHK(c, c->s_kc.ensure((size_t)2 * ninst * nleaf * sizeof(int)));
+  HIPCHK(c, c->s_acc.ensure((size_t)2 * ninst * sizeof(int)));
+  t.in = c->s_seg0.as<double>();
+  t.out = c->s_seg1.as<double>();
+  t.zb = c->s_zb.as<double>();
+  t.pq = c->s_pq.as<double>();
+  t.kc = c->s_kc.as<int>();
+  t.acc = c->s_acc.as<int>();
+  t.bad = t.acc + ninst;
+  hipLaunchKernelGGL(mcp::k_tree_zero, dim3(nblk256(2 * ninst)), dim3(256), 0, c->stream, t.acc, 2 * ninst);
+  HIPCHK(c, hipGetLastError());
+  return MCP_OK;
+}
+
+// The levels of a tree: the nleaf segments per instance at t.in merged
+// pairwise, (2m, 2m+1) -> m, until one is left (at t.in); the merge pivots'
+// negative-pivot counts summed into t.acc.
+int tree_levels(mcp_ctx* c, int n, int ninst, int nleaf, TreeBufs& t, int bk) {
+  const size_t nn = (size_t)n * n, sz = 3 * nn;
+  int rc;
+  for (int kl = nleaf; kl > 1;) {
+    const int per = kl / 2, kn = (kl + 1) / 2;
+    const int nb = ninst * per;
+    hipLaunchKernelGGL(mcp::k_tree_w, dim3(nblk256((size_t)nb * nn)), dim3(256), 0, c->stream, n, ninst, kl, per,
+                       t.in, t.zb);
+    HIPCHK(c, hipGetLastError());
+    if ((rc = tree_invert(c, n, nb, t.zb, t.kc, bk))) return rc;
+    hipLaunchKernelGGL(mcp::k_tree_count, dim3(nblk256(nb)), dim3(256), 0, c->stream, n, nb, per, t.kc, t.acc, t.bad);
+    HIPCHK(c, hipGetLastError());
+    const size_t si = (size_t)kl * sz, sm = 2 * sz;
+    const mcp::SegRef Z{t.zb, (size_t)per * nn, nn};
+    const mcp::SegRef L1{t.in, si, sm}, L2{t.in + nn, si, sm};
+    const mcp::SegRef R2{t.in + sz + nn, si, sm}, R4{t.in + sz + 2 * nn, si, sm};
+    const mcp::SegRef P{t.pq, (size_t)per * 2 * nn, 2 * nn}, Q{t.pq + nn, (size_t)per * 2 * nn, 2 * nn};
+    const mcp::SegOut Po{t.pq, (size_t)per * 2 * nn, 2 * nn}, Qo{t.pq + nn, (size_t)per * 2 * nn, 2 * nn};
+    const mcp::SegOut O1{t.out, (size_t)kn * sz, sz}, O2{t.out + nn, (size_t)kn * sz, sz};
+    const mcp::SegOut O4{t.out + 2 * nn, (size_t)kn * sz, sz};
+    // P = Z l2^T; Y1 = l1 - l2 P; Y2 = P^T r2; Q = Z r2; Y4 = r4 - r2^T Q
+    if ((rc = tree_gemm<false, true>(c, n, nb, per, Z, L2, Z, Po, 1.0, 0.0))) return rc;
+    if ((rc = tree_gemm<false, false>(c, n, nb, per, L2, P, L1, O1, -1.0, 1.0))) return rc;
+    if ((rc = tree_gemm<true, false>(c, n, nb, per, P, R2, P, O2, 1.0, 0.0))) return rc;
+    if ((rc = tree_gemm<false, false>(c, n, nb, per, Z, R2, Z, Qo, 1.0, 0.0))) return rc;
+    if ((rc = tree_gemm<true, false>(c, n, nb, per, R2, Q, R4, O4, -1.0, 1.0))) return rc;
+    if (kl & 1) {
+      hipLaunchKernelGGL(mcp::k_tree_pass, dim3(nblk256((size_t)ninst * sz)), dim3(256), 0, c->stream, n, ninst, kl,
+                         kn, t.in, t.out);
+      HIPCHK(c, hipGetLastError());
+    }
+    std::swap(t.in, t.out);
+    kl = kn;
+  }
+  return MCP_OK;
+}
+
+// The whole map applied to the starting matrices: t.zb holds A0 + Y1 of each
+// instance's last segment (at t.in); res = Y4 - Y2^T (A0 + Y1)^-1 Y2.
+int tree_final(mcp_ctx* c, int n, int ninst, TreeBufs& t, int bk, double* res) {
+  const size_t nn = (size_t)n * n, sz = 3 * nn;
+  int rc;
+  if ((rc = tree_invert(c, n, ninst, t.zb, t.kc, bk))) return rc;
+  hipLaunchKernelGGL(mcp::k_tree_count, dim3(nblk256(ninst)), dim3(256), 0, c->stream, n, ninst, 1, t.kc, t.acc,
+                     t.bad);
+  HIPCHK(c, hipGetLastError());
+  const mcp::SegRef Z{t.zb, nn, 0}, Y2{t.in + nn, sz, 0}, Y4{t.in + 2 * nn, sz, 0}, Q{t.pq, nn, 0};
+  const mcp::SegOut Qo{t.pq, nn, 0}, Ao{res, nn, 0};
+  if ((rc = tree_gemm<false, false>(c, n, ninst, 1, Z, Y2, Z, Qo, 1.0, 0.0))) return rc;
+  if ((rc = tree_gemm<true, false>(c, n, ninst, 1, Y2, Q, Y4, Ao, -1.0, 1.0))) return rc;
+  return MCP_OK;
+}
+
 size_t tree_bytes(int n, int ninst, int nstep) {
   const size_t nn = (size_t)n * n, k1 = (size_t)(nstep + 1) / 2;
   return sizeof(double) * nn * ninst * (3 * (size_t)nstep + 3 * k1 + (size_t)nstep + 2 * k1) +
@@ -372,91 +468,41 @@ size_t tree_bytes(int n, int ninst, int nstep) {
 int tree_mdprop(mcp_ctx* c, const mcp::BlockArgs& B, const mcp::GridArgs& G, const mcp::InstArgs& I, int ninst,
                 int nstep, double h, bool fused, const double* uc, const int* uck, const double* diag, int bk) {
   const int n = B.n;
-  const size_t nn = (size_t)n * n, sz = 3 * nn;
-  const int k1 = (nstep + 1) / 2;
-  HIPCHK(c, c->s_seg0.ensure((size_t)ninst * nstep * sz * sizeof(double)));
-  HIPCHK(c, c->s_seg1.ensure((size_t)ninst * std::max(k1, 1) * sz * sizeof(double)));
-  HIPCHK(c, c->s_zb.ensure((size_t)ninst * nstep * nn * sizeof(double)));
-  HIPCHK(c, c->s_pq.ensure((size_t)ninst * std::max(k1, 1) * 2 * nn * sizeof(double)));
-  HIPCHK(c, c->s_kc.ensure((size_t)2 * ninst * nstep * sizeof(int)));
-  HIPCHK(c, c->s_acc.ensure((size_t)2 * ninst * sizeof(int)));
-  double* zb = c->s_zb.as<double>();
-  double* pq = c->s_pq.as<double>();
-  int* kc = c->s_kc.as<int>();
-  int* acc = c->s_acc.as<int>();
-  int* bad = acc + ninst;
-  const size_t ll = lds_large(n, 0, 0);
-  hipLaunchKernelGGL(mcp::k_tree_zero, dim3(nblk256(2 * ninst)), dim3(256), 0, c->stream, acc, 2 * ninst);
+  const size_t nn = (size_t)n * n;
+  TreeBufs t;
+  int rc;
+  if ((rc = tree_alloc(c, n, ninst, nstep, t))) return rc;
   // leaves: each sector's two half-steps merged, Z = (UC + 2 A14)^-1
   const size_t nleaf = (size_t)ninst * nstep;
   if (fused) {
-    int* epos = kc + nleaf;
+    int* epos = t.kc + nleaf;
     hipLaunchKernelGGL(mcp::k_tree_zero, dim3(nblk256(nleaf)), dim3(256), 0, c->stream, epos, (int)nleaf);
     hipLaunchKernelGGL(mcp::k_tree_m, dim3(nblk256(nleaf * nn)), dim3(256), 0, c->stream, B, G, I, ninst, nstep, h,
-                       diag, zb, epos);
-    hipLaunchKernelGGL(mcp::k_syminv_large, dim3((unsigned)nleaf), dim3(kNT), ll, c->stream, n, zb, kc, bk);
-    hipLaunchKernelGGL(mcp::k_tree_count_leaf, dim3(nblk256(nleaf)), dim3(256), 0, c->stream, n, (int)nleaf, nstep, kc,
-                       epos, acc, bad);
-    hipLaunchKernelGGL(mcp::k_tree_z, dim3(nblk256(nleaf * nn)), dim3(256), 0, c->stream, n, ninst, nstep, diag, zb);
+                       diag, t.zb, epos);
+    if ((rc = tree_invert(c, n, (int)nleaf, t.zb, t.kc, bk))) return rc;
+    hipLaunchKernelGGL(mcp::k_tree_count_leaf, dim3(nblk256(nleaf)), dim3(256), 0, c->stream, n, (int)nleaf, nstep,
+                       t.kc, epos, t.acc, t.bad);
+    hipLaunchKernelGGL(mcp::k_tree_z, dim3(nblk256(nleaf * nn)), dim3(256), 0, c->stream, n, ninst, nstep, diag, t.zb);
   } else {
     hipLaunchKernelGGL(mcp::k_tree_c1, dim3(nblk256(nleaf * nn)), dim3(256), 0, c->stream, n, ninst, nstep, uc, diag,
-                       zb);
-    hipLaunchKernelGGL(mcp::k_syminv_large, dim3((unsigned)nleaf), dim3(kNT), ll, c->stream, n, zb, kc, bk);
-    hipLaunchKernelGGL(mcp::k_tree_count, dim3(nblk256(nleaf)), dim3(256), 0, c->stream, n, (int)nleaf, nstep, kc,
-                       acc, bad);
+                       t.zb);
+    if ((rc = tree_invert(c, n, (int)nleaf, t.zb, t.kc, bk))) return rc;
+    hipLaunchKernelGGL(mcp::k_tree_count, dim3(nblk256(nleaf)), dim3(256), 0, c->stream, n, (int)nleaf, nstep, t.kc,
+                       t.acc, t.bad);
   }
-  double* in = c->s_seg0.as<double>();
-  double* out = c->s_seg1.as<double>();
   hipLaunchKernelGGL(mcp::k_tree_leaf, dim3(nblk256(nleaf * nn)), dim3(256), 0, c->stream, B, G, I, ninst, nstep, h,
-                     diag, zb, in);
+                     diag, t.zb, t.in);
   HIPCHK(c, hipGetLastError());
-  int rc;
-  // levels: segments (2m, 2m+1) -> m
-  for (int kl = nstep; kl > 1;) {
-    const int per = kl / 2, kn = (kl + 1) / 2;
-    const int nb = ninst * per;
-    hipLaunchKernelGGL(mcp::k_tree_w, dim3(nblk256((size_t)nb * nn)), dim3(256), 0, c->stream, n, ninst, kl, per, in,
-                       zb);
-    hipLaunchKernelGGL(mcp::k_syminv_large, dim3(nb), dim3(kNT), ll, c->stream, n, zb, kc, bk);
-    hipLaunchKernelGGL(mcp::k_tree_count, dim3(nblk256(nb)), dim3(256), 0, c->stream, n, nb, per, kc, acc, bad);
-    HIPCHK(c, hipGetLastError());
-    const size_t si = (size_t)kl * sz, sm = 2 * sz;
-    const mcp::SegRef Z{zb, (size_t)per * nn, nn};
-    const mcp::SegRef L1{in, si, sm}, L2{in + nn, si, sm};
-    const mcp::SegRef R2{in + sz + nn, si, sm}, R4{in + sz + 2 * nn, si, sm};
-    const mcp::SegRef P{pq, (size_t)per * 2 * nn, 2 * nn}, Q{pq + nn, (size_t)per * 2 * nn, 2 * nn};
-    const mcp::SegOut Po{pq, (size_t)per * 2 * nn, 2 * nn}, Qo{pq + nn, (size_t)per * 2 * nn, 2 * nn};
-    const mcp::SegOut O1{out, (size_t)kn * sz, sz}, O2{out + nn, (size_t)kn * sz, sz}, O4{out + 2 * nn, (size_t)kn * sz, sz};
-    // P = Z l2^T; Y1 = l1 - l2 P; Y2 = P^T r2; Q = Z r2; Y4 = r4 - r2^T Q
-    if ((rc = tree_gemm<false, true>(c, n, nb, per, Z, L2, Z, Po, 1.0, 0.0))) return rc;
-    if ((rc = tree_gemm<false, false>(c, n, nb, per, L2, P, L1, O1, -1.0, 1.0))) return rc;
-    if ((rc = tree_gemm<true, false>(c, n, nb, per, P, R2, P, O2, 1.0, 0.0))) return rc;
-    if ((rc = tree_gemm<false, false>(c, n, nb, per, Z, R2, Z, Qo, 1.0, 0.0))) return rc;
-    if ((rc = tree_gemm<true, false>(c, n, nb, per, R2, Q, R4, O4, -1.0, 1.0))) return rc;
-    if (kl & 1) {
-      hipLaunchKernelGGL(mcp::k_tree_pass, dim3(nblk256((size_t)ninst * sz)), dim3(256), 0, c->stream, n, ninst, kl, kn,
-                         in, out);
-      HIPCHK(c, hipGetLastError());
-    }
-    std::swap(in, out);
-    kl = kn;
-  }
+  if ((rc = tree_levels(c, n, ninst, nstep, t, bk))) return rc;
   // the starting matrix: Y = Y4 - Y2^T (A0 + Y1)^-1 Y2
   hipLaunchKernelGGL(mcp::k_tree_final_w, dim3(nblk256((size_t)ninst * nn)), dim3(256), 0, c->stream, B, G, I, ninst, h,
-                     in, zb);
-  hipLaunchKernelGGL(mcp::k_syminv_large, dim3(ninst), dim3(kNT), ll, c->stream, n, zb, kc, bk);
-  hipLaunchKernelGGL(mcp::k_tree_count, dim3(nblk256(ninst)), dim3(256), 0, c->stream, n, ninst, 1, kc, acc, bad);
+                     t.in, t.zb);
   HIPCHK(c, hipGetLastError());
-  {
-    const mcp::SegRef Z{zb, nn, 0}, Y2{in + nn, sz, 0}, Y4{in + 2 * nn, sz, 0}, Q{pq, nn, 0};
-    const mcp::SegOut Qo{pq, nn, 0}, Ao{out, nn, 0};
-    if ((rc = tree_gemm<false, false>(c, n, ninst, 1, Z, Y2, Z, Qo, 1.0, 0.0))) return rc;
-    if ((rc = tree_gemm<true, false>(c, n, ninst, 1, Y2, Q, Y4, Ao, -1.0, 1.0))) return rc;
-  }
-  hipLaunchKernelGGL(mcp::k_tree_out, dim3(nblk256((size_t)ninst * nn)), dim3(256), 0, c->stream, n, ninst, h, out,
+  if ((rc = tree_final(c, n, ninst, t, bk, t.out))) return rc;
+  hipLaunchKernelGGL(mcp::k_tree_out, dim3(nblk256((size_t)ninst * nn)), dim3(256), 0, c->stream, n, ninst, h, t.out,
                      I.y);
   hipLaunchKernelGGL(mcp::k_tree_nodes, dim3(nblk256(ninst)), dim3(256), 0, c->stream, ninst, nstep,
-                     fused ? nullptr : uck, acc, bad, I.nodes, I.status);
+                     fused ? nullptr : uck, t.acc, t.bad, I.nodes, I.status);
   HIPCHK(c, hipGetLastError());
   return MCP_OK;
 }
@@ -703,10 +749,46 @@ int airy_prop_global(mcp_ctx* c, Block& b, Grid& g, int ninst, const double* d_e
   hipLaunchKernelGGL(mcp::k_airy_y, dim3(nblk256((size_t)ninst * nstep * n)), dim3(256), 0, c->stream, n, nstep, ninst,
                      ae, ah, w, d_ered, ay);
   HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipMemsetAsync(d_nodes, 0, ninst * sizeof(int), c->stream));
-  HIPCHK(c, hipMemsetAsync(d_status, 0, ninst * sizeof(int), c->stream));
   const int outward = g.h_w[0] > 0.0 ? 1 : 0;
   const int bk = n > 30 ? 1 : 0;
+  const mcp::SegRef Vl{av + (size_t)(nstep - 1) * nn, 0, 0};
+  if (c->large_mode == 0) {
+    // in parallel over the steps where the device memory holds the tree
+    size_t freeb = 0, totalb = 0;
+    (void)hipMemGetInfo(&freeb, &totalb);
+    const size_t held = c->s_seg0.bytes + c->s_seg1.bytes + c->s_zb.bytes + c->s_pq.bytes;
+    if (freeb + held > tree_bytes(n, ninst, nstep) + (size_t(2) << 30)) {
+      TreeBufs t;
+      if ((rc = tree_alloc(c, n, ninst, nstep, t))) return rc;
+      const size_t nl = (size_t)ninst * nstep;
+      hipLaunchKernelGGL(mcp::k_airy_leaf, dim3(nblk256(nl * nn)), dim3(256), 0, c->stream, n, ninst, nstep, at, ay,
+                         t.zb, t.in);
+      HIPCHK(c, hipGetLastError());
+      // Y1 = (T diag(y1)) T^T
+      if ((rc = tree_gemm<false, true>(c, n, (int)nl, nstep, mcp::SegRef{t.zb, (size_t)nstep * nn, nn},
+                                       mcp::SegRef{at, 0, nn}, mcp::SegRef{t.zb, 0, 0},
+                                       mcp::SegOut{t.in, (size_t)nstep * 3 * nn, 3 * nn}, 1.0, 0.0)))
+        return rc;
+      if ((rc = tree_levels(c, n, ninst, nstep, t, bk))) return rc;
+      hipLaunchKernelGGL(mcp::k_airy_final_w, dim3(nblk256((size_t)ninst * nn)), dim3(256), 0, c->stream, n, ninst,
+                         dy, t.in, t.zb);
+      HIPCHK(c, hipGetLastError());
+      if ((rc = tree_final(c, n, ninst, t, bk, dy))) return rc;
+      hipLaunchKernelGGL(mcp::k_airy_tree_nodes, dim3(nblk256(ninst)), dim3(256), 0, c->stream, ninst, nstep, n, outward,
+                         t.acc, t.bad, d_nodes, d_status);
+      HIPCHK(c, hipGetLastError());
+      // back to the primitive basis: Z <- sym(V Z V^T)
+      const mcp::SegRef Z{dy, nn, 0}, X{ax, nn, 0};
+      if ((rc = tree_gemm<false, false>(c, n, ninst, 1, Vl, Z, Vl, mcp::SegOut{ax, nn, 0}, 1.0, 0.0))) return rc;
+      if ((rc = tree_gemm<false, true>(c, n, ninst, 1, X, Vl, X, mcp::SegOut{am, nn, 0}, 1.0, 0.0))) return rc;
+      hipLaunchKernelGGL(mcp::k_airy_out, dim3(nblk256((size_t)ninst * nn)), dim3(256), 0, c->stream, n, ninst, am, dy);
+      HIPCHK(c, hipGetLastError());
+      HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+      return MCP_OK;
+    }
+  }
+  HIPCHK(c, hipMemsetAsync(d_nodes, 0, ninst * sizeof(int), c->stream));
+  HIPCHK(c, hipMemsetAsync(d_status, 0, ninst * sizeof(int), c->stream));
   // a negative team size keeps the inversions on one workgroup per instance
   const bool team = !c->no_reg && team_inverse_fits(c, n);
   const size_t ll = lds_large(n, 0, 0);
@@ -731,7 +813,6 @@ int airy_prop_global(mcp_ctx* c, Block& b, Grid& g, int ninst, const double* d_e
     HIPCHK(c, hipGetLastError());
   }
   // back to the primitive basis: Z <- sym(V Z V^T), V of the last step
-  const mcp::SegRef Vl{av + (size_t)(nstep - 1) * nn, 0, 0};
   if ((rc = tree_gemm<false, false>(c, n, ninst, 1, Vl, Z, Vl, mcp::SegOut{ax, nn, 0}, 1.0, 0.0))) return rc;
   if ((rc = tree_gemm<false, true>(c, n, ninst, 1, X, Vl, X, mcp::SegOut{am, nn, 0}, 1.0, 0.0))) return rc;
   hipLaunchKernelGGL(mcp::k_airy_out, dim3(ne), dim3(256), 0, c->stream, n, ninst, am, dy);

@@ -620,4 +620,50 @@ __global__ void k_airy_out(int n, int ninst, const double* __restrict__ x, doubl
   zout[idx] = 0.5 * (xi[i + (size_t)j * n] + xi[j + (size_t)i * n]);
 }
 
+// ---------------------------------------------------------------------------
+// AIRY solved in parallel over the radial grid.  Step k maps Z of basis k-1
+// to Z of basis k as Z -> y4 - y2 (y1 + T^T Z T)^-1 y2 (diagonal y's,
+// T = T_k orthogonal), which is Z -> Y4 - Y2^T (Z + Y1)^-1 Y2 with
+// Y1 = T y1 T^T, Y2 = T y2, Y4 = y4: the map form the merges of mcp_tree.hpp
+// compose.  The pivots of the merges are another elimination order of the
+// block-tridiagonal matrix whose sequential pivots are y1 + T^T Z T, so
+// their negative-pivot counts have the same sum (KOUNT outward; N - KOUNT
+// per step inward).
+// ---------------------------------------------------------------------------
+// leaf segment (Y1 pending, Y2, Y4) of step k for each instance, and
+// B = T_k diag(y1) in zb for the product Y1 = B T_k^T
+__global__ void k_airy_leaf(int n, int ninst, int nstep, const double* __restrict__ tmat,
+                            const double* __restrict__ y, double* __restrict__ zb, double* __restrict__ seg) {
+  const size_t nn = (size_t)n * n;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nn * nstep * ninst) return;
+  const size_t ik = idx / nn, e = idx - ik * nn;  // ik = inst * nstep + k
+  const int k = (int)(ik % nstep);
+  const int j = (int)(e / n), i = (int)(e - (size_t)j * n);
+  const double t = tmat[(size_t)k * nn + e];
+  const double* yy = y + ik * 3 * n;
+  zb[idx] = t * yy[j];
+  double* sg = seg + ik * 3 * nn;
+  sg[nn + e] = t * yy[n + j];
+  sg[2 * nn + e] = i == j ? yy[2 * n + i] : 0.0;
+}
+
+// A0 + Y1 of the whole map, A0 the starting Z
+__global__ void k_airy_final_w(int n, int ninst, const double* __restrict__ z0, const double* __restrict__ seg,
+                               double* __restrict__ zb) {
+  const size_t nn = (size_t)n * n;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nn * ninst) return;
+  const size_t inst = idx / nn, e = idx - inst * nn;
+  zb[idx] = z0[idx] + seg[inst * 3 * nn + e];
+}
+
+__global__ void k_airy_tree_nodes(int ninst, int nstep, int n, int outward, const int* __restrict__ acc,
+                                  const int* __restrict__ bad, int* __restrict__ nodes, int* __restrict__ status) {
+  const int inst = blockIdx.x * blockDim.x + threadIdx.x;
+  if (inst >= ninst) return;
+  nodes[inst] = outward ? acc[inst] : nstep * n - acc[inst];
+  status[inst] = bad[inst] ? 1 : 0;
+}
+
 }  // namespace mcp
