@@ -61,6 +61,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-sectors", type=int, default=600, help="sectors in the CPU baseline sample (0: skip)")
     ap.add_argument("--team", type=int, default=0, help="workgroups per instance on the large-N path (0: auto)")
+    ap.add_argument("--plain-launch", action="store_true",
+                    help="ordinary launches for the team kernels (for profilers that cannot trace cooperative ones)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -86,6 +88,8 @@ def main() -> int:
 
     eng = Engine(0)
     eng.set_team_size(args.team)
+    if args.plain_launch:
+        eng.set_team_launch(False)
     b = eng.block(blk.vl, blk.eint, blk.cent, mxlam=blk.mxlam)
     g = eng.grid(r, blk.ptab(r), seg.dr)
     ga = eng.airy_grid(ra, wa, blk.ptab(ra))

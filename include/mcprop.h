@@ -179,6 +179,13 @@ int mcp_bound_match(mcp_ctx *ctx, int32_t block_id, int32_t grid_out, int32_t gr
  * reference counterpart. */
 int mcp_set_team_size(mcp_ctx *ctx, int32_t g);
 
+/* How the workgroup-team kernels (the inversions on teams, the sequential
+ * large-N propagators, the matching's LU) are launched: cooperative != 0
+ * (default) through hipLaunchCooperativeKernel; 0 as ordinary launches of
+ * the same grids after the same residency check, for tools that cannot
+ * trace cooperative launches.  No reference counterpart. */
+int mcp_set_team_launch(mcp_ctx *ctx, int32_t cooperative);
+
 /* How MDPROP runs for large N: 0 (default) solves the recurrence in parallel
  * over the radial grid, as a tree of merged sector maps, where the device
  * memory holds it, else sequentially; 1 always sequentially on the

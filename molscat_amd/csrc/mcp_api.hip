@@ -169,6 +169,7 @@ struct mcp_ctx {
   bool no_reg = false;  // keep the large-N MDPROP on the global-memory kernel
   int large_mode = 0;   // large-N MDPROP: 0 tree where memory allows, 1 sequential matrix-core, 2 global memory
   bool eig_global = false;  // DIAGVC and AIRY on the global-memory eigen-solver for every N
+  bool coop = true;         // team kernels through the cooperative launch API
   DevBuf s_u;  // LDPROP sector-matrix workspace (large N)
   DevBuf s_av, s_at, s_ae, s_ah;  // AIRY step bases, basis changes, eigenvalues, slopes
   DevBuf s_x;                     // column exchange of the register-resident MDPROP
@@ -316,6 +317,24 @@ int prep_team(mcp_ctx* c, int ninst) {
 
 unsigned nblk256(size_t cnt) { return (unsigned)((cnt + 255) / 256); }
 
+// Team kernels need all their workgroups resident at once.  By default they
+// go through hipLaunchCooperativeKernel; with c->coop off they are ordinary
+// launches of the same grids after the same residency check (every team is
+// sized to fit the device, and a team barrier that waits too long sets the
+// abort flag instead of hanging).
+int launch_team(mcp_ctx* c, const void* kern, dim3 grid, dim3 block, void** args, size_t lds) {
+  if (c->coop) {
+    HIPCHK(c, hipLaunchCooperativeKernel(kern, grid, block, args, lds, c->stream));
+    return MCP_OK;
+  }
+  int per_cu = 0;
+  HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, (int)(block.x * block.y * block.z), lds));
+  if ((size_t)grid.x * grid.y * grid.z > (size_t)per_cu * c->num_cu)
+    return c->fail(MCP_ERR_UNSUPPORTED, "team of %u workgroups does not fit the device", grid.x * grid.y * grid.z);
+  HIPCHK(c, hipLaunchKernel(kern, grid, block, args, lds, c->stream));
+  return MCP_OK;
+}
+
 // Full symmetric inverses and inertias of nb n x n matrices at a (stride
 // n^2) in place, each on a team of ceil(n/16) workgroups on the matrix cores
 // (for a few matrices: latency-bound, the whole device on them).  False when
@@ -342,8 +361,8 @@ int team_inverse(mcp_ctx* c, int n, int nb, double* a, int* kount, int bk) {
     int* kb = kount + b0;
     int n_ = n, bk_ = bk, g_ = g;
     void* args[] = {&n_, &ab, &kb, &bk_, &g_, &ctrs, &abort_flag, &xg};
-    HIPCHK(c, hipLaunchCooperativeKernel((const void*)mcp::k_syminv_mfma<1>, dim3(m * g), dim3(mcp::kMT), args, lr,
-                                         c->stream));
+    const int rt = launch_team(c, (const void*)mcp::k_syminv_mfma<1>, dim3(m * g), dim3(mcp::kMT), args, lr);
+    if (rt) return rt;
   }
   return MCP_OK;
 }
@@ -632,12 +651,11 @@ int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const do
         void* args[] = {&B,  &G, &I, (void*)&nstep, (void*)&h, &ucp, &uck, (void*)&bk, (void*)&g, &ctrs, &abort_flag,
                         &xg, &dgp};
         const void* kern = rb == 1 ? (const void*)mcp::k_mdprop_mfma<1> : (const void*)mcp::k_mdprop_mfma<2>;
-        HIPCHK(c, hipLaunchCooperativeKernel(kern, dim3(ninst * g), dim3(mcp::kMT), args, lr, c->stream));
+        if ((rc = launch_team(c, kern, dim3(ninst * g), dim3(mcp::kMT), args, lr))) return rc;
       } else {
         const int g = team_size(c, ninst);
         void* args[] = {&B, &G, &I, (void*)&nstep, (void*)&h, &ucp, &uck, (void*)&bk, (void*)&g, &ctrs, &abort_flag};
-        HIPCHK(c, hipLaunchCooperativeKernel((const void*)mcp::k_mdprop_large, dim3(ninst * g), dim3(kNT), args, lm,
-                                             c->stream));
+        if ((rc = launch_team(c, (const void*)mcp::k_mdprop_large, dim3(ninst * g), dim3(kNT), args, lm))) return rc;
       }
     }
   } else {
@@ -657,8 +675,7 @@ int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const do
       unsigned* ctrs = c->s_bar.as<unsigned>();
       int* abort_flag = (int*)(ctrs + ninst);
       void* args[] = {&B, &G, &I, (void*)&nstep, (void*)&h, &uw, (void*)&bk, (void*)&g, &ctrs, &abort_flag};
-      HIPCHK(c, hipLaunchCooperativeKernel((const void*)mcp::k_ldprop_large, dim3(ninst * g), dim3(kNT), args, ll,
-                                           c->stream));
+      if ((rc = launch_team(c, (const void*)mcp::k_ldprop_large, dim3(ninst * g), dim3(kNT), args, ll))) return rc;
     }
   }
   HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
@@ -1285,8 +1302,7 @@ static int match_device(mcp_ctx* c, int n, int ninst, const double* d_y, const d
     } else {
       int g_ = g;
       void* args[] = {&M, &g_, &ctrs, &abort_flag};
-      HIPCHK(c, hipLaunchCooperativeKernel((const void*)mcp::k_match_lu, dim3(ninst * g), dim3(kNT), args, lu_lds,
-                                           c->stream));
+      if ((rc = launch_team(c, (const void*)mcp::k_match_lu, dim3(ninst * g), dim3(kNT), args, lu_lds))) return rc;
     }
   }
   hipLaunchKernelGGL(mcp::k_match, dim3(ninst), dim3(kNT), 0, c->stream, M, mcp::kMatchFinish);
@@ -1512,6 +1528,12 @@ int mcp_set_large_mode(mcp_ctx* c, int32_t mode) {
     return c ? c->fail(MCP_ERR_ARG, "large mode %d: 0, 1 or 2, plus 4", mode) : MCP_ERR_ARG;
   c->large_mode = mode & 3;
   c->eig_global = (mode & 4) != 0;
+  return MCP_OK;
+}
+
+int mcp_set_team_launch(mcp_ctx* c, int32_t cooperative) {
+  if (!c) return MCP_ERR_ARG;
+  c->coop = cooperative != 0;
   return MCP_OK;
 }
 

@@ -15,10 +15,11 @@
 //   k_stebz  every eigenvalue of T by bisection on the Sturm count, one
 //            thread per eigenvalue, which yields them in ascending order.
 //   k_stein  the eigenvectors of T by inverse iteration with partial
-//            pivoting, one wave per cluster of eigenvalues closer than
-//            1e-3 |T|_1 to their neighbour, each vector orthogonalised
-//            against the cluster's earlier ones (the scheme of LAPACK's
-//            DSTEIN); the largest component made positive.
+//            pivoting, all eigenvalues of a matrix at once (one thread
+//            each); clusters of eigenvalues closer than 1e-3 |T|_1 to their
+//            neighbour orthogonalised in eigenvalue order after every solve
+//            (the thresholds of LAPACK's DSTEIN); the largest component
+//            made positive.
 //   k_ormtr  V = Q Z: the reflectors, last first, applied to columns of Z
 //            held in registers (16 lanes per column).
 //
@@ -27,7 +28,8 @@
 // sqrt(3)(W(RB)-W(RA))/DRNOW of every step are built at once (k_airy_wd),
 // diagonalised as one batch, and the slopes HP = diag(V^T D V) and basis
 // changes T_k = V_{k-1}^T V_k formed with the batched matrix-core products.
-// Per energy the steps then run in sequence: Z <- sym(T^T Z T) (two
+// The steps of all energies then run as a tree of merged step maps (the
+// kernels at the end of this file), or in sequence: Z <- sym(T^T Z T) (two
 // products), the inversion of Z + Y1 on a team of workgroups, and
 // Z <- Y4 - Y2 (Z + Y1)^-1 Y2 with the node count (k_airy_m, k_airy_z).
 #pragma once

@@ -290,6 +290,11 @@ class Engine:
         N."""
         self._check(self.lib.mcp_set_large_mode(self.ctx, int(mode)), "mcp_set_large_mode")
 
+    def set_team_launch(self, cooperative: bool):
+        """Team kernels through the cooperative launch API (default) or as
+        ordinary launches of the same grids (for tracing tools)."""
+        self._check(self.lib.mcp_set_team_launch(self.ctx, 1 if cooperative else 0), "mcp_set_team_launch")
+
     def last_timing(self) -> dict:
         v = [ctypes.c_double() for _ in range(3)]
         self._check(self.lib.mcp_last_timing(self.ctx, *[ctypes.byref(x) for x in v]), "mcp_last_timing")

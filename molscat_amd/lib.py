@@ -52,6 +52,7 @@ SIGNATURES = {
                                         _pi]),
     "mcp_set_team_size": (ctypes.c_int, [_vp, _i32]),
     "mcp_set_large_mode": (ctypes.c_int, [_vp, _i32]),
+    "mcp_set_team_launch": (ctypes.c_int, [_vp, _i32]),
     "mcp_last_timing": (ctypes.c_int, [_vp, _pd, _pd, _pd]),
     "mcp_last_segment_timing": (ctypes.c_int, [_vp, _i32, _pd, _pd]),
 }

@@ -150,3 +150,27 @@ def test_tree_matches_sequential(eng, nstep):
         err = np.abs(yt[m] - ys[m]).max() / np.abs(ys[m]).max()
         print(f"  E={ered[m] / blk.cm2ru:.0f} nodes={int(nt[m])} rel|dY|={err:.2e}", flush=True)
         assert err < 1e-10
+
+
+def test_plain_team_launch_bitwise(eng):
+    """Ordinary launches of the team kernels (for tracing tools) run the same
+    grids as the cooperative launches: Y(RSTOP) and the node counts of a
+    tree whose small levels invert on teams are bitwise the same."""
+    blk = _big_block()
+    rstart, rstop, nstep = 2.0, 8.0, 40
+    dr = (rstop - rstart) / nstep
+    r = S.ld_radii(rstart, dr, nstep)
+    b = eng.block(blk.vl, blk.eint, blk.cent, mxlam=blk.mxlam)
+    g = eng.grid(r, blk.ptab(r), dr)
+    ered = np.array([300.0, 2000.0]) * blk.cm2ru
+    y0 = eng.yinit(b, g, ered)
+    print("team launches: cooperative", flush=True)
+    y1, n1, s1 = eng.prop_ld(b, g, 6, ered, y0)
+    eng.set_team_launch(False)
+    print("team launches: ordinary", flush=True)
+    y0b = eng.yinit(b, g, ered)
+    y2, n2, s2 = eng.prop_ld(b, g, 6, ered, y0b)
+    eng.set_team_launch(True)
+    assert np.array_equal(y0, y0b)
+    assert np.array_equal(s1, s2) and np.array_equal(n1, n2)
+    assert np.array_equal(y1, y2)
