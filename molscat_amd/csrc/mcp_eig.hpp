@@ -300,9 +300,12 @@ __device__ __forceinline__ double stein_start(int j, int k) {
   return (double)(h & 0xFFFFFFu) / 8388608.0 - 1.0;  // (-1, 1)
 }
 
+constexpr int kSU = 8;  // elements per batch of loads in k_stein's sweeps
+
 __global__ void __launch_bounds__(kET) k_stein(int n, const double* __restrict__ dg, const double* __restrict__ eg,
                                               const double* __restrict__ evg, double* __restrict__ xg,
                                               double* __restrict__ work) {
+  __shared__ double sd[kEMaxN], se[kEMaxN];  // the tridiagonal
   __shared__ int cst[kEMaxN];       // first eigenvalue of each eigenvalue's cluster
   __shared__ double lamv[kEMaxN];   // the shifts (DSTEIN's perturbation of close eigenvalues)
   __shared__ double snorm[kEMaxN];  // squared norms of the iterates
@@ -310,8 +313,6 @@ __global__ void __launch_bounds__(kET) k_stein(int n, const double* __restrict__
   __shared__ int maxcl;
   const int j = threadIdx.x;
   const size_t mat = blockIdx.x, nn = (size_t)n * n;
-  const double* d = dg + mat * n;
-  const double* e = eg + mat * n;
   const double* ev = evg + mat * n;
   double* X = xg + mat * nn;
   double* F = work + mat * 5 * nn;  // multipliers
@@ -320,7 +321,13 @@ __global__ void __launch_bounds__(kET) k_stein(int n, const double* __restrict__
   double* U2 = U1 + nn;             // second superdiagonal of U
   double* PV = U2 + nn;             // 1 where rows k, k+1 were interchanged
   double tn = 0.0;
-  if (j < n) tn = fabs(d[j]) + (j > 0 ? fabs(e[j - 1]) : 0.0) + (j + 1 < n ? fabs(e[j]) : 0.0);
+  if (j < n) {
+    const double* d = dg + mat * n;
+    const double* e = eg + mat * n;
+    sd[j] = d[j];
+    se[j] = j + 1 < n ? e[j] : 0.0;
+    tn = fabs(d[j]) + (j > 0 ? fabs(e[j - 1]) : 0.0) + (j + 1 < n ? fabs(e[j]) : 0.0);
+  }
   tn = block_max<kET>(tn, sv);
   const double ortol = 1e-3 * tn;
   const double eps = DBL_EPSILON;
@@ -347,10 +354,11 @@ __global__ void __launch_bounds__(kET) k_stein(int n, const double* __restrict__
   if (j < n) {
     const double lam = lamv[j];
     // T - lam I = P L U by partial pivoting; pivots below tiny raised to tiny
-    double dk = d[0] - lam, duk = n > 1 ? e[0] : 0.0;
+    double dk = sd[0] - lam, duk = n > 1 ? se[0] : 0.0;
+#pragma unroll 4
     for (int k = 0; k + 1 < n; ++k) {
       const size_t o = (size_t)k * n + j;
-      const double sub = e[k], dn = d[k + 1] - lam, cn = k + 2 < n ? e[k + 1] : 0.0;
+      const double sub = se[k], dn = sd[k + 1] - lam, cn = se[k + 1];
       double u;
       if (fabs(dk) >= fabs(sub)) {
         const double f = dk != 0.0 ? sub / dk : 0.0;
@@ -390,9 +398,32 @@ __global__ void __launch_bounds__(kET) k_stein(int n, const double* __restrict__
   for (int it = 0; it < 3; ++it) {
     if (j < n) {
       const double sc = 1.0 / sqrt(ss);
-      // b <- L^-1 P^T (sc b)
+      // b <- L^-1 P^T (sc b); every batch's loads issued before its
+      // recurrence (the loads of b read ahead of the stores)
       double cur = sc * X[j];
-      for (int k = 0; k + 1 < n; ++k) {
+      int k = 0;
+      for (; k + kSU < n; k += kSU) {
+        double nx[kSU], f[kSU], pv[kSU];
+#pragma unroll
+        for (int u = 0; u < kSU; ++u) {
+          const size_t o = (size_t)(k + u) * n + j;
+          nx[u] = sc * X[o + n];
+          f[u] = F[o];
+          pv[u] = PV[o];
+        }
+#pragma unroll
+        for (int u = 0; u < kSU; ++u) {
+          const size_t o = (size_t)(k + u) * n + j;
+          if (pv[u] != 0.0) {
+            X[o] = nx[u];
+            cur = cur - f[u] * nx[u];
+          } else {
+            X[o] = cur;
+            cur = nx[u] - f[u] * cur;
+          }
+        }
+      }
+      for (; k + 1 < n; ++k) {
         const size_t o = (size_t)k * n + j;
         const double nxt = sc * X[o + n], f = F[o];
         if (PV[o] != 0.0) {
@@ -404,10 +435,30 @@ __global__ void __launch_bounds__(kET) k_stein(int n, const double* __restrict__
         }
       }
       X[(size_t)(n - 1) * n + j] = cur;
-      // x <- U^-1 b
+      // x <- U^-1 b, from the last row up
       double x1 = 0.0, x2 = 0.0;
       ss = 0.0;
-      for (int k = n - 1; k >= 0; --k) {
+      k = n - 1;
+      for (; k + 1 >= kSU; k -= kSU) {
+        double b[kSU], a1[kSU], a2[kSU], ri[kSU];
+#pragma unroll
+        for (int u = 0; u < kSU; ++u) {
+          const size_t o = (size_t)(k - u) * n + j;
+          b[u] = X[o];
+          a1[u] = U1[o];
+          a2[u] = U2[o];
+          ri[u] = UI[o];
+        }
+#pragma unroll
+        for (int u = 0; u < kSU; ++u) {
+          const double xk = (b[u] - a1[u] * x1 - a2[u] * x2) * ri[u];
+          X[(size_t)(k - u) * n + j] = xk;
+          ss += xk * xk;
+          x2 = x1;
+          x1 = xk;
+        }
+      }
+      for (; k >= 0; --k) {
         const size_t o = (size_t)k * n + j;
         const double xk = (X[o] - U1[o] * x1 - U2[o] * x2) * UI[o];
         X[o] = xk;
@@ -423,16 +474,51 @@ __global__ void __launch_bounds__(kET) k_stein(int n, const double* __restrict__
       // the pivot of this thread's cluster; none for threads beyond n
       const int m = j < n ? cst[j] + st : j;
       if (m < j) {
+        double acc[kSU];
+#pragma unroll
+        for (int u = 0; u < kSU; ++u) acc[u] = 0.0;
+        int k = 0;
+        for (; k + kSU <= n; k += kSU) {
+          double a[kSU], b[kSU];
+#pragma unroll
+          for (int u = 0; u < kSU; ++u) {
+            a[u] = X[(size_t)(k + u) * n + j];
+            b[u] = X[(size_t)(k + u) * n + m];
+          }
+#pragma unroll
+          for (int u = 0; u < kSU; ++u) acc[u] += a[u] * b[u];
+        }
+        for (; k < n; ++k) acc[0] += X[(size_t)k * n + j] * X[(size_t)k * n + m];
         double dt = 0.0;
-        for (int k = 0; k < n; ++k) dt += X[(size_t)k * n + j] * X[(size_t)k * n + m];
+#pragma unroll
+        for (int u = 0; u < kSU; ++u) dt += acc[u];
         const double f = dt / snorm[m];
-        double s2 = 0.0;
-        for (int k = 0; k < n; ++k) {
+#pragma unroll
+        for (int u = 0; u < kSU; ++u) acc[u] = 0.0;
+        k = 0;
+        for (; k + kSU <= n; k += kSU) {
+          double a[kSU], b[kSU];
+#pragma unroll
+          for (int u = 0; u < kSU; ++u) {
+            a[u] = X[(size_t)(k + u) * n + j];
+            b[u] = X[(size_t)(k + u) * n + m];
+          }
+#pragma unroll
+          for (int u = 0; u < kSU; ++u) {
+            const double v = a[u] - f * b[u];
+            X[(size_t)(k + u) * n + j] = v;
+            acc[u] += v * v;
+          }
+        }
+        for (; k < n; ++k) {
           const size_t o = (size_t)k * n + j;
           const double v = X[o] - f * X[(size_t)k * n + m];
           X[o] = v;
-          s2 += v * v;
+          acc[0] += v * v;
         }
+        double s2 = 0.0;
+#pragma unroll
+        for (int u = 0; u < kSU; ++u) s2 += acc[u];
         ss = s2;
         snorm[j] = s2;
       }
