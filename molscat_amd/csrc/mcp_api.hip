@@ -367,11 +367,13 @@ int team_inverse(mcp_ctx* c, int n, int nb, double* a, int* kount, int bk) {
   return MCP_OK;
 }
 
-template <bool TA, bool TB>
+// LOW: a symmetric result, only its tiles on and below the diagonal
+template <bool TA, bool TB, bool LOW = false>
 int tree_gemm(mcp_ctx* c, int n, int nbatch, int per, mcp::SegRef A, mcp::SegRef Bm, mcp::SegRef Ci, mcp::SegOut Co,
               double alpha, double beta) {
   const int nt = (n + mcp::kGT - 1) / mcp::kGT;
-  hipLaunchKernelGGL((mcp::k_gemm_seg<TA, TB>), dim3(nbatch, nt * nt), dim3(256), 0, c->stream, n, per, A, Bm, Ci, Co,
+  const unsigned ny = LOW ? (unsigned)(nt * (nt + 1) / 2) : (unsigned)(nt * nt);
+  hipLaunchKernelGGL((mcp::k_gemm_seg<TA, TB, LOW>), dim3(nbatch, ny), dim3(256), 0, c->stream, n, per, A, Bm, Ci, Co,
                      alpha, beta);
   HIPCHK(c, hipGetLastError());
   return MCP_OK;
@@ -442,11 +444,12 @@ int tree_levels(mcp_ctx* c, int n, int ninst, int nleaf, TreeBufs& t, int bk) {
     const mcp::SegOut O1{t.out, (size_t)kn * sz, sz}, O2{t.out + nn, (size_t)kn * sz, sz};
     const mcp::SegOut O4{t.out + 2 * nn, (size_t)kn * sz, sz};
     // P = Z l2^T; Y1 = l1 - l2 P; Y2 = P^T r2; Q = Z r2; Y4 = r4 - r2^T Q
+    // (Y1, Y4 symmetric: their lower tiles only)
     if ((rc = tree_gemm<false, true>(c, n, nb, per, Z, L2, Z, Po, 1.0, 0.0))) return rc;
-    if ((rc = tree_gemm<false, false>(c, n, nb, per, L2, P, L1, O1, -1.0, 1.0))) return rc;
+    if ((rc = tree_gemm<false, false, true>(c, n, nb, per, L2, P, L1, O1, -1.0, 1.0))) return rc;
     if ((rc = tree_gemm<true, false>(c, n, nb, per, P, R2, P, O2, 1.0, 0.0))) return rc;
     if ((rc = tree_gemm<false, false>(c, n, nb, per, Z, R2, Z, Qo, 1.0, 0.0))) return rc;
-    if ((rc = tree_gemm<true, false>(c, n, nb, per, R2, Q, R4, O4, -1.0, 1.0))) return rc;
+    if ((rc = tree_gemm<true, false, true>(c, n, nb, per, R2, Q, R4, O4, -1.0, 1.0))) return rc;
     if (kl & 1) {
       hipLaunchKernelGGL(mcp::k_tree_pass, dim3(nblk256((size_t)ninst * sz)), dim3(256), 0, c->stream, n, ninst, kl,
                          kn, t.in, t.out);
@@ -459,7 +462,8 @@ int tree_levels(mcp_ctx* c, int n, int ninst, int nleaf, TreeBufs& t, int bk) {
 }
 
 // The whole map applied to the starting matrices: t.zb holds A0 + Y1 of each
-// instance's last segment (at t.in); res = Y4 - Y2^T (A0 + Y1)^-1 Y2.
+// instance's last segment (at t.in); res = Y4 - Y2^T (A0 + Y1)^-1 Y2, its
+// tiles on and below the diagonal.
 int tree_final(mcp_ctx* c, int n, int ninst, TreeBufs& t, int bk, double* res) {
   const size_t nn = (size_t)n * n, sz = 3 * nn;
   int rc;
@@ -470,7 +474,7 @@ int tree_final(mcp_ctx* c, int n, int ninst, TreeBufs& t, int bk, double* res) {
   const mcp::SegRef Z{t.zb, nn, 0}, Y2{t.in + nn, sz, 0}, Y4{t.in + 2 * nn, sz, 0}, Q{t.pq, nn, 0};
   const mcp::SegOut Qo{t.pq, nn, 0}, Ao{res, nn, 0};
   if ((rc = tree_gemm<false, false>(c, n, ninst, 1, Z, Y2, Z, Qo, 1.0, 0.0))) return rc;
-  if ((rc = tree_gemm<true, false>(c, n, ninst, 1, Y2, Q, Y4, Ao, -1.0, 1.0))) return rc;
+  if ((rc = tree_gemm<true, false, true>(c, n, ninst, 1, Y2, Q, Y4, Ao, -1.0, 1.0))) return rc;
   return MCP_OK;
 }
 
@@ -791,6 +795,7 @@ int airy_prop_global(mcp_ctx* c, Block& b, Grid& g, int ninst, const double* d_e
                          dy, t.in, t.zb);
       HIPCHK(c, hipGetLastError());
       if ((rc = tree_final(c, n, ninst, t, bk, dy))) return rc;
+      hipLaunchKernelGGL(mcp::k_tree_mirror, dim3(nblk256((size_t)ninst * nn)), dim3(256), 0, c->stream, n, ninst, dy);
       hipLaunchKernelGGL(mcp::k_airy_tree_nodes, dim3(nblk256(ninst)), dim3(256), 0, c->stream, ninst, nstep, n, outward,
                          t.acc, t.bad, d_nodes, d_status);
       HIPCHK(c, hipGetLastError());

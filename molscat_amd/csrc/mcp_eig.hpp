@@ -743,7 +743,8 @@ __global__ void k_airy_final_w(int n, int ninst, const double* __restrict__ z0, 
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= nn * ninst) return;
   const size_t inst = idx / nn, e = idx - inst * nn;
-  zb[idx] = z0[idx] + seg[inst * 3 * nn + e];
+  const int j = (int)(e / n), i = (int)(e - (size_t)j * n);
+  zb[idx] = z0[idx] + sym_tile(seg + inst * 3 * nn, n, i, j);
 }
 
 __global__ void k_airy_tree_nodes(int ninst, int nstep, int n, int outward, const int* __restrict__ acc,

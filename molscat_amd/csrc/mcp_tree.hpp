@@ -42,6 +42,12 @@ struct SegOut {
   __device__ __forceinline__ double* at(int inst, int m) const { return base + inst * si + m * sm; }
 };
 
+// element (i, j) of a symmetric matrix of which only the 64 x 64 tiles on
+// and below the diagonal are current (the products with LOW)
+__device__ __forceinline__ double sym_tile(const double* a, int n, int i, int j) {
+  return (i / 64 >= j / 64) ? a[i + (size_t)j * n] : a[j + (size_t)i * n];
+}
+
 // W(i, j) at grid point k (HAMMAT: sumlam plus the diagonal terms)
 __device__ __forceinline__ double w_elem(const BlockArgs& B, const GridArgs& G, int k, const double* pconst, int i,
                                          int j, double ered) {
@@ -98,7 +104,9 @@ __device__ __forceinline__ void gemm_stash(double (*S)[kGT + kGPad], const doubl
   }
 }
 
-template <bool TA, bool TB>
+// LOW: only the tiles on and below the diagonal of Co (a symmetric result),
+// grid y = nt (nt + 1) / 2
+template <bool TA, bool TB, bool LOW = false>
 __global__ void __launch_bounds__(256) k_gemm_seg(int n, int per, SegRef A, SegRef Bm, SegRef Ci, SegOut Co,
                                                   double alpha, double beta) {
   __shared__ double As[kGK][kGT + kGPad];
@@ -106,7 +114,15 @@ __global__ void __launch_bounds__(256) k_gemm_seg(int n, int per, SegRef A, SegR
   const int bt = blockIdx.x;
   const int inst = bt / per, m = bt - inst * per;
   const int nt = (n + kGT - 1) / kGT;
-  const int m0 = (int)(blockIdx.y % nt) * kGT, n0 = (int)(blockIdx.y / nt) * kGT;
+  int ti = (int)(blockIdx.y % nt), tj = (int)(blockIdx.y / nt);
+  if (LOW) {
+    // tile t = ti (ti + 1) / 2 + tj, tj <= ti
+    const int t = (int)blockIdx.y;
+    ti = 0;
+    while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+    tj = t - ti * (ti + 1) / 2;
+  }
+  const int m0 = ti * kGT, n0 = tj * kGT;
   const double* a = A.at(inst, m);
   const double* bb = Bm.at(inst, m);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -296,7 +312,8 @@ __global__ void k_tree_w(int n, int ninst, int kl, int per, const double* __rest
   const int inst = (int)(mat / per), m = (int)(mat - (size_t)inst * per);
   const double* L = seg + ((size_t)inst * kl + 2 * m) * 3 * nn;
   const double* R = L + 3 * nn;
-  zb[e] = L[2 * nn + r] + R[r];
+  const int j = (int)(r / n), i = (int)(r - (size_t)j * n);
+  zb[e] = sym_tile(L + 2 * nn, n, i, j) + sym_tile(R, n, i, j);
 }
 
 // the odd last segment of a level passes to the next unchanged
@@ -322,7 +339,7 @@ __global__ void k_tree_final_w(BlockArgs B, GridArgs G, InstArgs I, int ninst, d
   const double* pconst = I.pconst ? I.pconst + (size_t)inst * B.nconst : nullptr;
   const double d1 = h * h / 3.0;
   const double a0 = h * I.y[e] + d1 * w_elem(B, G, 0, pconst, i, j, I.ered[inst]);
-  zb[e] = a0 + seg[(size_t)inst * 3 * nn + r];
+  zb[e] = a0 + sym_tile(seg + (size_t)inst * 3 * nn, n, i, j);
 }
 
 // Y(RSTOP) = A / h
@@ -330,7 +347,19 @@ __global__ void k_tree_out(int n, int ninst, double h, const double* __restrict_
   const size_t nn = (size_t)n * n;
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nn * ninst) return;
-  y[e] = a[e] / h;
+  const size_t inst = e / nn, r = e - inst * nn;
+  const int j = (int)(r / n), i = (int)(r - (size_t)j * n);
+  y[e] = sym_tile(a + inst * nn, n, i, j) / h;
+}
+
+// the tiles above the diagonal of each instance's matrix from those below
+__global__ void k_tree_mirror(int n, int ninst, double* __restrict__ a) {
+  const size_t nn = (size_t)n * n;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nn * ninst) return;
+  const size_t inst = e / nn, r = e - inst * nn;
+  const int j = (int)(r / n), i = (int)(r - (size_t)j * n);
+  if (i / 64 < j / 64) a[e] = a[inst * nn + j + (size_t)i * n];
 }
 
 // negative-pivot counts of a batch of inversions (item b of instance b / per)
