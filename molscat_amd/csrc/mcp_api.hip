@@ -497,15 +497,14 @@ int tree_mdprop(mcp_ctx* c, const mcp::BlockArgs& B, const mcp::GridArgs& G, con
   if ((rc = tree_alloc(c, n, ninst, nstep, t))) return rc;
   // leaves: each sector's two half-steps merged, Z = (UC + 2 A14)^-1
   const size_t nleaf = (size_t)ninst * nstep;
+  const dim3 leaf_grid(nblk256(nn), (unsigned)(ninst * ((nstep + mcp::kLeafChunk - 1) / mcp::kLeafChunk)));
   if (fused) {
     int* epos = t.kc + nleaf;
     hipLaunchKernelGGL(mcp::k_tree_zero, dim3(nblk256(nleaf)), dim3(256), 0, c->stream, epos, (int)nleaf);
-    hipLaunchKernelGGL(mcp::k_tree_m, dim3(nblk256(nleaf * nn)), dim3(256), 0, c->stream, B, G, I, ninst, nstep, h,
-                       diag, t.zb, epos);
+    hipLaunchKernelGGL(mcp::k_tree_m, leaf_grid, dim3(256), 0, c->stream, B, G, I, ninst, nstep, h, diag, t.zb, epos);
     if ((rc = tree_invert(c, n, (int)nleaf, t.zb, t.kc, bk))) return rc;
     hipLaunchKernelGGL(mcp::k_tree_count_leaf, dim3(nblk256(nleaf)), dim3(256), 0, c->stream, n, (int)nleaf, nstep,
                        t.kc, epos, t.acc, t.bad);
-    hipLaunchKernelGGL(mcp::k_tree_z, dim3(nblk256(nleaf * nn)), dim3(256), 0, c->stream, n, ninst, nstep, diag, t.zb);
   } else {
     hipLaunchKernelGGL(mcp::k_tree_c1, dim3(nblk256(nleaf * nn)), dim3(256), 0, c->stream, n, ninst, nstep, uc, diag,
                        t.zb);
@@ -513,8 +512,8 @@ int tree_mdprop(mcp_ctx* c, const mcp::BlockArgs& B, const mcp::GridArgs& G, con
     hipLaunchKernelGGL(mcp::k_tree_count, dim3(nblk256(nleaf)), dim3(256), 0, c->stream, n, (int)nleaf, nstep, t.kc,
                        t.acc, t.bad);
   }
-  hipLaunchKernelGGL(mcp::k_tree_leaf, dim3(nblk256(nleaf * nn)), dim3(256), 0, c->stream, B, G, I, ninst, nstep, h,
-                     diag, t.zb, t.in);
+  hipLaunchKernelGGL(mcp::k_tree_leaf, leaf_grid, dim3(256), 0, c->stream, B, G, I, ninst, nstep, h, diag, t.zb,
+                     fused ? 1 : 0, t.in);
   HIPCHK(c, hipGetLastError());
   if ((rc = tree_levels(c, n, ninst, nstep, t, bk))) return rc;
   // the starting matrix: Y = Y4 - Y2^T (A0 + Y1)^-1 Y2
@@ -587,7 +586,7 @@ int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const do
         return MCP_OK;
       };
       // parallel-in-R tree where the device memory holds two of its levels
-      if (c->large_mode == 0) {
+      if (c->large_mode == 0 && b.nham <= mcp::kLeafMaxHam) {
         size_t freeb = 0, totalb = 0;
         (void)hipMemGetInfo(&freeb, &totalb);
         const size_t held = c->s_seg0.bytes + c->s_seg1.bytes + c->s_zb.bytes + c->s_pq.bytes + c->s_dg.bytes;

@@ -217,50 +217,59 @@ __global__ void k_tree_guard(int n, int ninst, int nstep, const double* __restri
   if (!(fabs(u2 - 8.0) >= 0.5)) atomicMax(flag, 1);
 }
 
-// M = V + E^-1 of every (instance, sector) into zb; epos[inst*nstep + k] += pos(E)
-__global__ void k_tree_m(BlockArgs B, GridArgs G, InstArgs I, int ninst, int nstep, double h,
-                         const double* __restrict__ diag, double* __restrict__ zb, int* __restrict__ epos) {
-  const int n = B.n;
-  const size_t nn = (size_t)n * n;
-  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nn * nstep * ninst) return;
-  const size_t mat = e / nn, r = e - mat * nn;
-  const int inst = (int)(mat / nstep), k = (int)(mat - (size_t)inst * nstep);
-  const int j = (int)(r / n), i = (int)(r - (size_t)j * n);
-  double v;
-  if (i == j) {
-    const double u2 = diag[((size_t)inst * 3 + 2) * nstep * n + (size_t)k * n + i];
-    v = 0.125 + 1.0 / (u2 - 8.0);
-    if (u2 - 8.0 > 0.0) atomicAdd(epos + mat, 1);
-  } else {
-    const double* pconst = I.pconst ? I.pconst + (size_t)inst * B.nconst : nullptr;
-    const double d4 = -(h * h / 3.0) / 16.0;
-    const int kc = 2 * (k + 1) - 1;
-    const double rsq = rsq_of(G.r[kc]);
-    const double* vl = B.vl + tri_index(i, j) * (size_t)B.nham;
-    double sw = 0.0;
-    for (int l = 0; l < B.nham; ++l) {
-      const double pl = l < B.mxlam ? G.ptab[(size_t)kc * B.mxlam + l]
+// The leaf builders: one element (i, j) per thread over kLeafChunk sectors
+// of one instance, its VL entries in registers (VL is then read once per
+// chunk instead of once per sector); grid (ceil(n^2/256), ninst * nchunk).
+constexpr int kLeafChunk = 32;
+constexpr int kLeafMaxHam = 16;  // NHAM of the tree path
+
+// sum_l VL(l, (i,j)) P_l at grid point kp
+__device__ __forceinline__ double leaf_sum(const BlockArgs& B, const GridArgs& G, const double* pconst,
+                                           const double (&vl)[kLeafMaxHam], int kp, double rsq) {
+  double s = 0.0;
+#pragma unroll
+  for (int l = 0; l < kLeafMaxHam; ++l) {
+    if (l < B.nham) {
+      const double pl = l < B.mxlam ? G.ptab[(size_t)kp * B.mxlam + l]
                                     : (l < B.mxlam + B.nconst ? pconst[l - B.mxlam] : rsq);
-      sw += vl[l] * pl;
+      s += vl[l] * pl;
     }
-    v = d4 * sw;
   }
-  zb[e] = v;
+  return s;
 }
 
-// Z = E^-1 - E^-1 X E^-1 in place, X = M^-1
-__global__ void k_tree_z(int n, int ninst, int nstep, const double* __restrict__ diag, double* __restrict__ zb) {
+// M = V + E^-1 of every (instance, sector) into zb; epos[inst*nstep + k] += pos(E)
+__global__ void __launch_bounds__(256) k_tree_m(BlockArgs B, GridArgs G, InstArgs I, int ninst, int nstep, double h,
+                                                const double* __restrict__ diag, double* __restrict__ zb,
+                                                int* __restrict__ epos) {
+  const int n = B.n;
   const size_t nn = (size_t)n * n;
-  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nn * nstep * ninst) return;
-  const size_t mat = e / nn, r = e - mat * nn;
-  const int inst = (int)(mat / nstep), k = (int)(mat - (size_t)inst * nstep);
+  const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nn) return;
+  const int nch = (nstep + kLeafChunk - 1) / kLeafChunk;
+  const int inst = (int)blockIdx.y / nch, k0 = ((int)blockIdx.y % nch) * kLeafChunk;
+  const int k1 = min(nstep, k0 + kLeafChunk);
   const int j = (int)(r / n), i = (int)(r - (size_t)j * n);
-  const double* u2 = diag + ((size_t)inst * 3 + 2) * nstep * n + (size_t)k * n;
-  const double ei = 1.0 / (u2[i] - 8.0), ej = 1.0 / (u2[j] - 8.0);
-  const double z = -ei * zb[e] * ej;
-  zb[e] = i == j ? z + ei : z;
+  const size_t mat0 = (size_t)inst * nstep;
+  if (i == j) {
+    const double* u2 = diag + ((size_t)inst * 3 + 2) * nstep * n;
+    for (int k = k0; k < k1; ++k) {
+      const double u = u2[(size_t)k * n + i];
+      if (u - 8.0 > 0.0) atomicAdd(epos + mat0 + k, 1);
+      zb[(mat0 + k) * nn + r] = 0.125 + 1.0 / (u - 8.0);
+    }
+    return;
+  }
+  const double* pconst = I.pconst ? I.pconst + (size_t)inst * B.nconst : nullptr;
+  const double d4 = -(h * h / 3.0) / 16.0;
+  const double* vp = B.vl + tri_index(i, j) * (size_t)B.nham;
+  double vl[kLeafMaxHam];
+#pragma unroll
+  for (int l = 0; l < kLeafMaxHam; ++l) vl[l] = l < B.nham ? vp[l] : 0.0;
+  for (int k = k0; k < k1; ++k) {
+    const int kc = 2 * (k + 1) - 1;
+    zb[(mat0 + k) * nn + r] = d4 * leaf_sum(B, G, pconst, vl, kc, rsq_of(G.r[kc]));
+  }
 }
 
 // leaf counts n - pos(E) - neg(M) into the instances' totals
@@ -276,30 +285,54 @@ __global__ void k_tree_count_leaf(int n, int nb, int per, const int* __restrict_
     atomicAdd(acc + inst, n - epos[b] - kc);
 }
 
-// leaf segments from Z = c1^-1: M = D Z D, Y1 = diag(Y14) - M, Y2 = M,
+// the leaf segments: Z of the first half-step from zb (fused: zb holds
+// (V + E^-1)^-1 and Z = E^-1 - E^-1 zb E^-1; else zb holds Z = c1^-1),
+// M = D Z D with D = diag(Y23), Y1 = diag(Y14) - M, Y2 = M,
 // Y4 = D2 W(B) + diag(Y14) - M
-__global__ void k_tree_leaf(BlockArgs B, GridArgs G, InstArgs I, int ninst, int nstep, double h,
-                            const double* __restrict__ diag, const double* __restrict__ zb, double* __restrict__ seg) {
+__global__ void __launch_bounds__(256) k_tree_leaf(BlockArgs B, GridArgs G, InstArgs I, int ninst, int nstep,
+                                                   double h, const double* __restrict__ diag,
+                                                   const double* __restrict__ zb, int fused, double* __restrict__ seg) {
   const int n = B.n;
   const size_t nn = (size_t)n * n;
-  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nn * nstep * ninst) return;
-  const size_t mat = e / nn, r = e - mat * nn;
-  const int inst = (int)(mat / nstep), k = (int)(mat - (size_t)inst * nstep);
+  const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nn) return;
+  const int nch = (nstep + kLeafChunk - 1) / kLeafChunk;
+  const int inst = (int)blockIdx.y / nch, k0 = ((int)blockIdx.y % nch) * kLeafChunk;
+  const int k1 = min(nstep, k0 + kLeafChunk);
   const int j = (int)(r / n), i = (int)(r - (size_t)j * n);
+  const size_t mat0 = (size_t)inst * nstep;
   const size_t per = (size_t)nstep * n;
-  const double* dg = diag + (size_t)inst * 3 * per + (size_t)k * n;
-  const double* y14 = dg;
-  const double* y23 = dg + per;
-  const double mij = y23[i] * zb[e] * y23[j];
+  const double* y14 = diag + (size_t)inst * 3 * per;
+  const double* y23 = y14 + per;
+  const double* u2 = y23 + per;
   const double* pconst = I.pconst ? I.pconst + (size_t)inst * B.nconst : nullptr;
   const double d1 = h * h / 3.0;
-  const double dd2 = (k == nstep - 1) ? d1 : 2.0 * d1;
-  double wb = dd2 * w_elem(B, G, 2 * (k + 1), pconst, i, j, I.ered[inst]);
-  double* s = seg + mat * 3 * nn;
-  s[r] = (i == j ? y14[i] : 0.0) - mij;
-  s[nn + r] = mij;
-  s[2 * nn + r] = wb + (i == j ? y14[i] : 0.0) - mij;
+  const double ered = I.ered[inst];
+  const double* vp = B.vl + tri_index(i, j) * (size_t)B.nham;
+  double vl[kLeafMaxHam];
+#pragma unroll
+  for (int l = 0; l < kLeafMaxHam; ++l) vl[l] = l < B.nham ? vp[l] : 0.0;
+  for (int k = k0; k < k1; ++k) {
+    const size_t o = (size_t)k * n;
+    double z = zb[(mat0 + k) * nn + r];
+    if (fused) {
+      const double ei = 1.0 / (u2[o + i] - 8.0), ej = 1.0 / (u2[o + j] - 8.0);
+      const double zz = -ei * z * ej;
+      z = i == j ? zz + ei : zz;
+    }
+    const double mij = y23[o + i] * z * y23[o + j];
+    const double dd2 = (k == nstep - 1) ? d1 : 2.0 * d1;
+    const int kb = 2 * (k + 1);
+    const double rb = G.r[kb], rsq = rsq_of(rb);
+    double w = leaf_sum(B, G, pconst, vl, kb, rsq);
+    if (i == j) w = hammat_diag(B, i, w, rb, rsq, ered);
+    const double wb = dd2 * w;
+    const double yd = i == j ? y14[o + i] : 0.0;
+    double* sg = seg + (mat0 + k) * 3 * nn;
+    sg[r] = yd - mij;
+    sg[nn + r] = mij;
+    sg[2 * nn + r] = wb + yd - mij;
+  }
 }
 
 // merge pivots l4 + r1 of segments (2m, 2m+1) of level kl into Zb[inst*per + m]
