@@ -118,6 +118,20 @@ int mcp_prop_ld(mcp_ctx *ctx, int32_t block_id, int32_t grid_id, int32_t iprop, 
 int mcp_prop_airy(mcp_ctx *ctx, int32_t block_id, int32_t grid_id, int32_t ninst, const double *ered, double *y,
                   int32_t *nodes, int32_t *status);
 
+/* YTRANS (source_code/ytrans.f90:1) for a block with NCONST > 0 and NRSQ = 0
+ * on the MOLSCAT path: for each of ninst instances (pconst: ninst x nconst,
+ * the constants as for mcp_prop_ld) the asymptotic operator, the sum of the
+ * constant VL terms times their constants, is diagonalised within each set
+ * of channels of equal CENT; eint (ninst x N) receives the thresholds EINT in
+ * channel order (ytrans.f90:330-350) and, when y is not NULL, y (ninst N x N)
+ * is turned into the asymptotic basis, Y <- EVEC^T Y EVEC (TRNSFM,
+ * ytrans.f90:418-422).  WVEC, NOPEN and the channel order NB then follow
+ * from eint by GTWVEC and CHNSRT, as the reference's driver does before the
+ * propagation (source_code/mol.driver.f:1336-1342).  mcp_smatrix_ld and
+ * mcp_smatrix_segments apply the same transformation to Y themselves when
+ * the block has NCONST > 0.  N up to the LDS-resident limit (≈80). */
+int mcp_ytrans(mcp_ctx *ctx, int32_t block_id, int32_t ninst, const double *pconst, double *y, double *eint);
+
 /* YTOK + KTOS (source_code/ytok.f:1, ytokg.f:1, ktos.f:1) for ninst
  * instances: y (N x N), wvec (N, GTWVEC's signed wave vectors), nopen, nb
  * (N, 0-based CHNSRT order), cent (N), rval = RSTOP.  Writes kmat, sr, si as

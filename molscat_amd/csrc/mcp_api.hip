@@ -17,6 +17,7 @@
 #include "mcp_mfmaprop.hpp"
 #include "mcp_tree.hpp"
 #include "mcp_eig.hpp"
+#include "mcp_ytrans.hpp"
 
 namespace {
 
@@ -51,6 +52,7 @@ struct DevBuf {
 
 struct Block {
   int n, mxlam, nconst, nrsq, nham;
+  bool const_offdiag = false;  // some constant term couples two channels
   double conlen4inv;
   DevBuf vl, eint, cent;
   std::vector<double> h_cent;
@@ -120,6 +122,15 @@ size_t lds_yinit(int n, int nham) {
       .b;
 }
 
+size_t lds_ytrans(int n) {
+  // same pieces, in the same order, as k_ytrans carves them
+  const size_t nn = (size_t)n * n;
+  const size_t half = (size_t)(n + 1) / 2 + 1;
+  LdsSize z;
+  z.add<double>(nn).add<double>(nn).add<double>(nn).add<double>(half).add<double>(half).add<int>(half).add<int>(half);
+  z.add<double>(kNT / 64).add<double>(n).add<int>(n).add<double>(n).add<int>(n);
+  return z.b;
+}
 size_t lds_airy_sector(int n, int nham) { return lds_eig(n) + LdsSize().add<double>(nham).add<double>(nham).b; }
 size_t lds_airy_prop(int n) {
   return LdsSize().add<double>((size_t)n * n).add<double>((size_t)n * n).add<double>((size_t)n * n)
@@ -182,6 +193,7 @@ struct mcp_ctx {
   DevBuf s_aa, s_ad, s_az;        // large-N AIRY: step matrices / reflectors, slopes' matrices, Z of T / D V
   DevBuf s_ed, s_ew;              // global-memory eigen-solver: d, e, tau; inverse-iteration factors
   DevBuf s_ay, s_ax, s_am, s_akc; // large-N AIRY per instance: Y1 Y2 Y4, products, pivots, inertias
+  DevBuf s_ye;                    // asymptotic thresholds of YTRANS
   double t_sector = 0.0, t_prop = 0.0, t_total = 0.0;
   int seg_timed = 0;                       // segments of the last mcp_smatrix_segments call
   double seg_ms[8][2] = {};                // their (energy-independent, propagation) times
@@ -219,7 +231,8 @@ int set_lds_limits(mcp_ctx* c) {
                        (const void*)mcp::k_mdprop_large,   (const void*)mcp::k_ldprop_large,
                        (const void*)mcp::k_yinit_large,    (const void*)mcp::k_airy_sector,
                        (const void*)mcp::k_airy_prop,      (const void*)mcp::k_mdprop_mfma<1>, (const void*)mcp::k_mdprop_mfma<2>,
-                       (const void*)mcp::k_syminv_mfma<1>, (const void*)mcp::k_match_lu};
+                       (const void*)mcp::k_syminv_mfma<1>, (const void*)mcp::k_match_lu,
+                       (const void*)mcp::k_ytrans};
   const size_t total = c->lds_max;
   size_t lim = total;
   for (const void* f : fns) {
@@ -555,6 +568,16 @@ int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const do
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     hipLaunchKernelGGL(mcp::k_odprop, dim3((ninst + 63) / 64), dim3(64), 0, c->stream, B, G, I, ninst, nstep, g.dr);
     HIPCHK(c, hipGetLastError());
+  } else if (iprop == 6 && b.const_offdiag && ninst > 1 && d_pconst) {
+    // constant terms off the diagonal enter W_off(C), so U(C) is particular
+    // to each instance's constants: the instances one at a time
+    const size_t nn = (size_t)n * n;
+    for (int i = 0; i < ninst; ++i) {
+      int rc = prop_ld_device(c, b, g, iprop, 1, d_ered + i, d_pconst + (size_t)i * b.nconst, dy + i * nn,
+                              d_nodes + i, d_status + i);
+      if (rc) return rc;
+    }
+    return MCP_OK;
   } else if (iprop == 6) {
     // energy-independent sector matrices (shared by all instances)
     HIPCHK(c, c->s_e.ensure((size_t)nstep * n * n * sizeof(double)));
@@ -710,6 +733,19 @@ int eig_global(mcp_ctx* c, int n, int nmat, double* a, double* evals, double* ev
                        evecs + onn);
     HIPCHK(c, hipGetLastError());
   }
+  return MCP_OK;
+}
+
+// YTRANS on the device (mcp_ytrans.hpp): the asymptotic thresholds of ninst
+// instances (d_pconst: ninst x nconst) into d_eint, and their Y (dy, may be
+// null) turned into the asymptotic basis.
+int ytrans_device(mcp_ctx* c, Block& b, int ninst, const double* d_pconst, double* dy, double* d_eint) {
+  if (b.nconst <= 0) return c->fail(MCP_ERR_ARG, "ytrans: the block has no constant terms (NCONST = 0)");
+  if (b.nrsq != 0) return c->fail(MCP_ERR_UNSUPPORTED, "ytrans: NRSQ > 0 (the centrifugal operator) is not supported");
+  const size_t lds = lds_ytrans(b.n);
+  if (lds > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "ytrans: N=%d exceeds the LDS-resident limit", b.n);
+  hipLaunchKernelGGL(mcp::k_ytrans, dim3(ninst), dim3(kNT), lds, c->stream, block_args(b), d_pconst, dy, d_eint);
+  HIPCHK(c, hipGetLastError());
   return MCP_OK;
 }
 
@@ -1029,6 +1065,12 @@ int mcp_block_create(mcp_ctx* c, int32_t n, int32_t mxlam, int32_t nconst, int32
   if ((rc = upload(c, b->eint, eint, (size_t)n))) return rc;
   if ((rc = upload(c, b->cent, cent, (size_t)n))) return rc;
   b->h_cent.assign(cent, cent + n);
+  for (int j = 0; j < n && !b->const_offdiag; ++j)
+    for (int i = 0; i < j && !b->const_offdiag; ++i) {
+      const double* v = vl + ((size_t)j * (j + 1) / 2 + i) * b->nham + mxlam;
+      for (int k = 0; k < nconst; ++k)
+        if (v[k] != 0.0) b->const_offdiag = true;
+    }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const int id = c->next_id++;
   c->blocks[id] = std::move(b);
@@ -1384,6 +1426,11 @@ int mcp_smatrix_ld(mcp_ctx* c, int32_t bid, int32_t gid, int32_t iprop, int32_t 
   const int ip = (iprop == 6 && n == 1) ? MCP_PROP_ODPROP : iprop;
   if ((rc = prop_ld_device(c, *b, *g, ip, ninst, I.ered, I.pconst, I.y, dn, dn + ninst))) return rc;
   if (!(rval > 0.0)) return c->fail(MCP_ERR_ARG, "smatrix_ld: matching radius %g must be positive", rval);
+  if (b->nconst > 0) {
+    // YTRANS before YTOK (source_code/scctrl.f:274-287)
+    HIPCHK(c, c->s_ye.ensure((size_t)ninst * n * sizeof(double)));
+    if ((rc = ytrans_device(c, *b, ninst, I.pconst, I.y, c->s_ye.as<double>()))) return rc;
+  }
   if ((rc = match_device(c, n, ninst, I.y, wvec, nopen, nb, b->cent.as<double>(), rval))) return rc;
   std::vector<int> tmp(3 * (size_t)ninst);
   HIPCHK(c, hipMemcpyAsync(tmp.data(), dn, 3 * ninst * sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -1441,6 +1488,10 @@ int mcp_smatrix_segments(mcp_ctx* c, int32_t bid, int32_t nseg, const int32_t* g
     c->seg_ms[k][0] = c->t_sector;
     c->seg_ms[k][1] = c->t_prop;
     c->seg_timed = k + 1;
+  }
+  if (b->nconst > 0) {
+    HIPCHK(c, c->s_ye.ensure((size_t)ninst * n * sizeof(double)));
+    if ((rc = ytrans_device(c, *b, ninst, I.pconst, I.y, c->s_ye.as<double>()))) return rc;
   }
   if ((rc = match_device(c, n, ninst, I.y, wvec, nopen, nb, b->cent.as<double>(), rval))) return rc;
   std::vector<int> tmp((size_t)ninst * (2 * nseg + 1));
@@ -1532,6 +1583,29 @@ int mcp_set_large_mode(mcp_ctx* c, int32_t mode) {
     return c ? c->fail(MCP_ERR_ARG, "large mode %d: 0, 1 or 2, plus 4", mode) : MCP_ERR_ARG;
   c->large_mode = mode & 3;
   c->eig_global = (mode & 4) != 0;
+  return MCP_OK;
+}
+
+int mcp_ytrans(mcp_ctx* c, int32_t bid, int32_t ninst, const double* pconst, double* y, double* eint) {
+  if (!c || ninst < 0 || !pconst || !eint) return c ? c->fail(MCP_ERR_ARG, "ytrans: bad args") : MCP_ERR_ARG;
+  Block* b;
+  int rc;
+  if ((rc = lookup(c, bid, 0, &b, nullptr))) return rc;
+  if (ninst == 0) return MCP_OK;
+  const int n = b->n;
+  if (b->nconst <= 0) return c->fail(MCP_ERR_ARG, "ytrans: the block has no constant terms (NCONST = 0)");
+  HIPCHK(c, hipSetDevice(c->device));
+  if ((rc = upload(c, c->s_h, pconst, (size_t)ninst * b->nconst))) return rc;
+  HIPCHK(c, c->s_ye.ensure((size_t)ninst * n * sizeof(double)));
+  double* dy = nullptr;
+  if (y) {
+    if ((rc = upload(c, c->s_i, y, (size_t)ninst * n * n))) return rc;
+    dy = c->s_i.as<double>();
+  }
+  if ((rc = ytrans_device(c, *b, ninst, c->s_h.as<double>(), dy, c->s_ye.as<double>()))) return rc;
+  if ((rc = download(c, eint, c->s_ye, (size_t)ninst * n))) return rc;
+  if (y && (rc = download(c, y, c->s_i, (size_t)ninst * n * n))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return MCP_OK;
 }
 

@@ -94,3 +94,86 @@ def rotor441() -> Deck:
 
 def sweep_energies() -> np.ndarray:
     return np.linspace(10.0, 500.0, 64)
+
+
+# ---------------------------------------------------------------------------
+# field-dependent deck (a synthetic BASE9 system with NCONST = 1)
+# ---------------------------------------------------------------------------
+MHZ_IN_INV_CM = 3.335640951e-11 * 1.0e6  # source_code/physical_constants_module.f
+
+
+@dataclass
+class FieldBlock:
+    """A synthetic BASE9-style block: 12 internal states x L = 0, 2, 4, 6
+    (N = 48, channel 4*alpha + l, so the sets of equal L interleave), three
+    radial terms (POTENL case 1, EPSIL = 1 cm-1, RM = 5 A): a singlet
+    4000 cm-1 and a triplet 200 cm-1 LJ 12-6 well and a dipolar d(R) =
+    0.5 (RM/R)^3, with coupling matrices P_s x I_L, (I - P_s) x I_L and
+    D x C_L (P_s a projector on four seeded internal states, D a seeded
+    symmetric internal operator, C_L coupling L and L+-2); one constant term,
+    the Zeeman operator diag(s_alpha) x I_L with slopes s ~ U[-2, 2] MHz/G,
+    whose constant is the field B in G (HAMMAT's P = VCONST * CM2RU).
+    EINT is unused (NCONST > 0: hammat.f:95-99); the thresholds are YTRANS's.
+    URED = 40 amu, RUNIT = 1 A."""
+
+    vl: np.ndarray
+    eint: np.ndarray
+    cent: np.ndarray
+    lvals: np.ndarray
+    levidx: np.ndarray
+    cm2ru: float
+    ep2ru: float
+    rscale: float
+    potl: S.PowerPotential
+    mxlam: int = 3
+    nconst: int = 1
+
+    @property
+    def n(self) -> int:
+        return len(self.eint)
+
+    def ptab(self, r: np.ndarray) -> np.ndarray:
+        out = np.empty((len(r), self.mxlam))
+        for k, rr in enumerate(r):
+            p = self.potl(rr * self.rscale)
+            p = np.where(np.abs(p) > 1.0e60, np.sign(p) * 1.0e60, p)
+            out[k] = self.ep2ru * p
+        return out
+
+    def pconst(self, field_g: float) -> np.ndarray:
+        return np.array([field_g * self.cm2ru])
+
+
+def _pack_upper(m: np.ndarray) -> np.ndarray:
+    n = m.shape[0]
+    return np.array([m[i, j] for j in range(n) for i in range(j + 1)])
+
+
+def field48(seed: int = 4, zeeman=None) -> FieldBlock:
+    """The field-dependent deck; zeeman replaces the internal Zeeman operator
+    (12 x 12 symmetric, in MHz/G) when given."""
+    rng = np.random.default_rng(seed)
+    na, lv = 12, np.array([0, 2, 4, 6])
+    q, _ = np.linalg.qr(rng.standard_normal((na, na)))
+    ps = q[:, :4] @ q[:, :4].T
+    d = rng.standard_normal((na, na))
+    d = 0.3 * (d + d.T) / 2.0
+    cl = np.zeros((4, 4))
+    for a in range(4):
+        for b in range(4):
+            if abs(int(lv[a]) - int(lv[b])) == 2:
+                cl[a, b] = 1.0 / np.sqrt(3.0)
+            elif a == b and lv[a] > 0:
+                cl[a, b] = -0.5
+    slopes = rng.uniform(-2.0, 2.0, na)
+    z = np.diag(slopes) if zeeman is None else np.asarray(zeeman, dtype=np.float64)
+    il = np.eye(4)
+    mats = [np.kron(ps, il), np.kron(np.eye(na) - ps, il), np.kron(d, cl), np.kron(z * MHZ_IN_INV_CM, il)]
+    vl = np.stack([_pack_upper(m) for m in mats])
+    ured, runit, rm = 40.0, 1.0, 5.0
+    c2r = S.cm2ru(ured, runit)
+    potl = S.PowerPotential([[(4000.0, -12), (-8000.0, -6)], [(200.0, -12), (-400.0, -6)], [(0.5, -3)]],
+                            rm=rm, epsil=1.0)
+    lvals = np.tile(lv, na)
+    return FieldBlock(vl=vl, eint=np.zeros(na * 4), cent=(lvals * (lvals + 1)).astype(np.float64), lvals=lvals,
+                      levidx=np.repeat(np.arange(na), 4), cm2ru=c2r, ep2ru=1.0 * c2r, rscale=runit / rm, potl=potl)

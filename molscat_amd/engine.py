@@ -201,6 +201,17 @@ class Engine:
                     "mcp_prop_airy")
         return _from_colmajor(y), nodes, st
 
+    def ytrans(self, b: BlockHandle, pconst, y0: Optional[np.ndarray] = None):
+        """YTRANS for a block with NCONST > 0: the asymptotic thresholds
+        (m, N) of each instance's constants pconst (m, nconst) and, with y0
+        (m, N, N), Y in the asymptotic basis.  Returns (eint, y or None)."""
+        pc = np.ascontiguousarray(np.atleast_2d(pconst), dtype=np.float64)
+        m = pc.shape[0]
+        eint = np.empty((m, b.n))
+        buf = None if y0 is None else _colmajor(np.asarray(y0, dtype=np.float64).reshape(m, b.n, b.n))
+        self._check(self.lib.mcp_ytrans(self.ctx, b.id, m, dptr(pc), dptr(buf), dptr(eint)), "mcp_ytrans")
+        return eint, (None if buf is None else _from_colmajor(buf))
+
     def smatrix_segments(self, b: BlockHandle, grids, iprops, ered, wvec, nopen, nb, rval: float, pconst=None):
         """SCCTRL over several segments (grids with their propagator codes)."""
         ered = np.ascontiguousarray(np.atleast_1d(ered), dtype=np.float64)

@@ -262,19 +262,26 @@ class Block:
     ep2ru: float
     cm2ru: float
     vl_full: np.ndarray = field(default=None)  # (NHAM, N, N)
+    nconst: int = 0  # constant terms: the last nconst VL blocks
+    pconst: Optional[np.ndarray] = None  # their P values, VCONST * CM2RU (hammat.f:57-73)
 
     def __post_init__(self):
         if self.vl_full is None:
             self.vl_full = np.stack([packed_to_full(self.vl[k], self.n) for k in range(self.vl.shape[0])])
 
     def pvec(self, r: float) -> np.ndarray:
-        """HAMMAT steps 1-6 (source_code/hammat.f:38-73) for NCONST=NRSQ=0."""
+        """HAMMAT steps 1-6 (source_code/hammat.f:38-73): the radial terms,
+        then the NCONST constants."""
         p = np.array(self.potl(r * self.rscale), dtype=np.float64)
         p = np.where(np.abs(p) > 1.0e60, np.sign(p) * 1.0e60, p)
-        return self.ep2ru * p
+        p = self.ep2ru * p
+        if self.nconst > 0:
+            p = np.concatenate([p, np.asarray(self.pconst, dtype=np.float64)])
+        return p
 
     def hammat(self, r: float, ered: float) -> Tuple[np.ndarray, np.ndarray]:
-        """HAMMAT: source_code/hammat.f:1-116 with SUMLAM source_code/sumlam.f:29-36.
+        """HAMMAT: source_code/hammat.f:1-116 with SUMLAM source_code/sumlam.f:29-36
+        (NRSQ = 0; EINT added only when NCONST = 0, hammat.f:95-99).
 
         Returns the full W(N,N) and DIAG(N) = diag(W).
         """
@@ -283,7 +290,8 @@ class Block:
         w = np.tensordot(p, self.vl_full, axes=(0, 0))
         d = np.diag(w).copy()
         d = d + rsq * self.cent
-        d = d + self.eint
+        if self.nconst == 0:
+            d = d + self.eint
         d = d - ered
         np.fill_diagonal(w, d)
         return w, d.copy()
@@ -1201,13 +1209,42 @@ def ld_grid(blk: Block, rmin: float, rmax: float, eredmx: float, steps: float = 
     return Grid(rstart, rmax, nstep, dr), rturn
 
 
+def ytrans(y: Optional[np.ndarray], blk: Block) -> Tuple[Optional[np.ndarray], np.ndarray, np.ndarray]:
+    """YTRANS (source_code/ytrans.f90:79-422) for NCONST > 0, NRSQ = 0 on the
+    MOLSCAT path (one operator, channels grouped by L): the operator
+    sum_c VL(MXLAM+c) P_c (ytrans.f90:116-140) diagonalised in each set of
+    channels of equal L (:147-215, DIAGVC through MULTOP), every eigenvector
+    with its largest component (first of equals, IDAMAX) positive
+    (:262-275), eigenvector icol of a set at the set's icol-th channel with
+    EINT its eigenvalue (:330-350), Y <- EVEC^T Y EVEC (TRNSFM,
+    source_code/trnsfm.f:20-32).  Returns (Y or None, EINT, EVEC)."""
+    n = blk.n
+    mx = blk.vl.shape[0] - blk.nconst
+    c = np.tensordot(np.asarray(blk.pconst, dtype=np.float64), blk.vl_full[mx:], axes=(0, 0))
+    evec = np.zeros((n, n))
+    eint = np.zeros(n)
+    for ll in sorted(set(int(x) for x in blk.lvals)):
+        idx = np.array([i for i in range(n) if int(blk.lvals[i]) == ll])
+        w, v = np.linalg.eigh(c[np.ix_(idx, idx)])
+        for icol in range(len(idx)):
+            col = v[:, icol]
+            if col[int(np.argmax(np.abs(col)))] < 0.0:
+                col = -col
+            evec[idx, idx[icol]] = col
+            eint[idx[icol]] = w[icol]
+    yt = None if y is None else evec.T @ y @ evec
+    return yt, eint, evec
+
+
 def smatrix_ld(blk: Block, ered: float, grid: Grid, iprop: int = 6):
-    """YINIT + MDPROP/LDPROP/ODPROP + YTOK + KTOS for one energy.
+    """YINIT + MDPROP/LDPROP/ODPROP + (YTRANS when NCONST > 0) + YTOK + KTOS
+    for one energy.
 
     Returns dict with Y (final), K, SR, SI (NB-ordered open channels), NB,
     NOPEN, WVEC, NODES."""
-    wvec, nopen = gtwvec(ered, blk.eint)
-    nb = chnsrt(blk.eint)
+    eint = ytrans(None, blk)[1] if blk.nconst > 0 else blk.eint
+    wvec, nopen = gtwvec(ered, eint)
+    nb = chnsrt(eint)
     y = yinit(blk, ered, grid.rstart)
     hm = lambda r: blk.hammat(r, ered)
     if iprop == 6 and blk.n > 1:
@@ -1219,6 +1256,8 @@ def smatrix_ld(blk: Block, ered: float, grid: Grid, iprop: int = 6):
         y, nodes = ldprop(y, hm, grid.rstart, grid.nstep, grid.dr)
     else:
         raise ValueError(iprop)
+    if blk.nconst > 0:
+        y = ytrans(y, blk)[0]
     out = dict(Y=y, NB=nb, NOPEN=nopen, WVEC=wvec, NODES=nodes)
     if nopen > 0:
         k = ytok(nb, wvec, blk.n, nopen, y, grid.rstop, blk.cent)
