@@ -101,3 +101,34 @@ def test_field_smatrix_vs_oracle(eng, zeeman):
         assert d < 5e-10
         s = out["SR"][m] + 1j * out["SI"][m]
         assert np.abs(s @ s.conj().T - np.eye(len(s))).max() < 1e-11
+
+
+def test_field_bound_match_vs_oracle(eng):
+    """BOUND matching on the field deck (rotated Zeeman operator, so each
+    instance propagates with its own U(C)): two fields x two energies below
+    the lowest threshold in one call; NODEC exact, EIGMIN and the matching
+    matrix's eigenvalues against the oracle's."""
+    fb = decks.field48(zeeman=_random_zeeman(5))
+    b = _block(eng, fb)
+    (r0, dro, no), (r1, dri, ni) = S.bound_segments(3.5, 8.0, 40.0, 0.02)
+    ro = S.ld_radii(r0, dro, no)
+    ri = S.ld_radii(r1, dri, ni)
+    go = eng.grid(ro, fb.ptab(ro), dro)
+    gi = eng.grid(ri, fb.ptab(ri), dri)
+    og = O.BoundGrid(3.5, 8.0, 40.0, no, dro, ni, dri)
+    fields = [200.0, 200.0, 700.0, 700.0]
+    below = [3.0, 40.0, 3.0, 40.0]
+    pc = np.stack([fb.pconst(f) for f in fields])
+    eint, _ = eng.ytrans(b, pc)
+    ered = np.array([eint[m].min() - below[m] * fb.cm2ru for m in range(len(fields))])
+    print(f"field BOUND N={fb.n} {no}+{ni} sectors", flush=True)
+    out = eng.bound_match(b, go, gi, ered, pconst=pc)
+    for m in range(len(fields)):
+        ref = O.bound_match(_oracle_block(fb, fields[m]), ered[m], og)
+        assert out["STATUS"][m] == 0
+        de = abs(out["EIGMIN"][m] - ref["EIGMIN"]) / np.abs(ref["EVALS"]).max()
+        dv = np.abs(out["EVALS"][m] - ref["EVALS"]).max() / np.abs(ref["EVALS"]).max()
+        print(f"  B={fields[m]} G E=thr-{below[m]} cm-1 NODEC={int(out['NODEC'][m])} (oracle {ref['NODEC']}) "
+              f"|dEIGMIN|={de:.1e} |dEVALS|={dv:.1e}", flush=True)
+        assert int(out["NODEC"][m]) == int(ref["NODEC"])
+        assert de < 1e-9 and dv < 1e-9
