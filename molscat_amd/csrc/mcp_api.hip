@@ -398,8 +398,9 @@ int tree_gemm(mcp_ctx* c, int n, int nbatch, int per, mcp::SegRef A, mcp::SegRef
 // Small batches, where one workgroup per matrix leaves the device idle, go
 // to workgroup teams (unless a negative team size was set).
 constexpr int kTreeTeamMax = 64;
+bool tree_on_teams(mcp_ctx* c, int n, int nb) { return nb <= kTreeTeamMax && !c->no_reg && team_inverse_fits(c, n); }
 int tree_invert(mcp_ctx* c, int n, int nb, double* zb, int* kc, int bk) {
-  if (nb <= kTreeTeamMax && !c->no_reg && team_inverse_fits(c, n)) return team_inverse(c, n, nb, zb, kc, bk);
+  if (tree_on_teams(c, n, nb)) return team_inverse(c, n, nb, zb, kc, bk);
   hipLaunchKernelGGL(mcp::k_syminv_large, dim3(nb), dim3(kNT), lds_large(n, 0, 0), c->stream, n, zb, kc, bk);
   HIPCHK(c, hipGetLastError());
   return MCP_OK;
@@ -443,7 +444,7 @@ int tree_levels(mcp_ctx* c, int n, int ninst, int nleaf, TreeBufs& t, int bk) {
     const int per = kl / 2, kn = (kl + 1) / 2;
     const int nb = ninst * per;
     hipLaunchKernelGGL(mcp::k_tree_w, dim3(nblk256((size_t)nb * nn)), dim3(256), 0, c->stream, n, ninst, kl, per,
-                       t.in, t.zb);
+                       t.in, t.zb, tree_on_teams(c, n, nb) ? 1 : 0);
     HIPCHK(c, hipGetLastError());
     if ((rc = tree_invert(c, n, nb, t.zb, t.kc, bk))) return rc;
     hipLaunchKernelGGL(mcp::k_tree_count, dim3(nblk256(nb)), dim3(256), 0, c->stream, n, nb, per, t.kc, t.acc, t.bad);
@@ -514,7 +515,8 @@ int tree_mdprop(mcp_ctx* c, const mcp::BlockArgs& B, const mcp::GridArgs& G, con
   if (fused) {
     int* epos = t.kc + nleaf;
     hipLaunchKernelGGL(mcp::k_tree_zero, dim3(nblk256(nleaf)), dim3(256), 0, c->stream, epos, (int)nleaf);
-    hipLaunchKernelGGL(mcp::k_tree_m, leaf_grid, dim3(256), 0, c->stream, B, G, I, ninst, nstep, h, diag, t.zb, epos);
+    hipLaunchKernelGGL(mcp::k_tree_m, leaf_grid, dim3(256), 0, c->stream, B, G, I, ninst, nstep, h, diag, t.zb, epos,
+                       tree_on_teams(c, n, (int)nleaf) ? 1 : 0);
     if ((rc = tree_invert(c, n, (int)nleaf, t.zb, t.kc, bk))) return rc;
     hipLaunchKernelGGL(mcp::k_tree_count_leaf, dim3(nblk256(nleaf)), dim3(256), 0, c->stream, n, (int)nleaf, nstep,
                        t.kc, epos, t.acc, t.bad);

@@ -241,7 +241,7 @@ __device__ __forceinline__ double leaf_sum(const BlockArgs& B, const GridArgs& G
 // M = V + E^-1 of every (instance, sector) into zb; epos[inst*nstep + k] += pos(E)
 __global__ void __launch_bounds__(256) k_tree_m(BlockArgs B, GridArgs G, InstArgs I, int ninst, int nstep, double h,
                                                 const double* __restrict__ diag, double* __restrict__ zb,
-                                                int* __restrict__ epos) {
+                                                int* __restrict__ epos, int full) {
   const int n = B.n;
   const size_t nn = (size_t)n * n;
   const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -250,6 +250,8 @@ __global__ void __launch_bounds__(256) k_tree_m(BlockArgs B, GridArgs G, InstArg
   const int inst = (int)blockIdx.y / nch, k0 = ((int)blockIdx.y % nch) * kLeafChunk;
   const int k1 = min(nstep, k0 + kLeafChunk);
   const int j = (int)(r / n), i = (int)(r - (size_t)j * n);
+  // the one-workgroup inversion reads the lower triangle only
+  if (!full && i / 64 < j / 64) return;
   const size_t mat0 = (size_t)inst * nstep;
   if (i == j) {
     const double* u2 = diag + ((size_t)inst * 3 + 2) * nstep * n;
@@ -337,15 +339,17 @@ __global__ void __launch_bounds__(256) k_tree_leaf(BlockArgs B, GridArgs G, Inst
 
 // merge pivots l4 + r1 of segments (2m, 2m+1) of level kl into Zb[inst*per + m]
 __global__ void k_tree_w(int n, int ninst, int kl, int per, const double* __restrict__ seg,
-                         double* __restrict__ zb) {
+                         double* __restrict__ zb, int full) {
   const size_t nn = (size_t)n * n;
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nn * per * ninst) return;
   const size_t mat = e / nn, r = e - mat * nn;
   const int inst = (int)(mat / per), m = (int)(mat - (size_t)inst * per);
+  const int j = (int)(r / n), i = (int)(r - (size_t)j * n);
+  // the one-workgroup inversion reads the lower triangle only
+  if (!full && i / 64 < j / 64) return;
   const double* L = seg + ((size_t)inst * kl + 2 * m) * 3 * nn;
   const double* R = L + 3 * nn;
-  const int j = (int)(r / n), i = (int)(r - (size_t)j * n);
   zb[e] = sym_tile(L + 2 * nn, n, i, j) + sym_tile(R, n, i, j);
 }
 
