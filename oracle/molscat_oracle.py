@@ -1436,3 +1436,400 @@ def aiprop(z: np.ndarray, blk: Block, ered: float, rstart: float, rstop: float, 
         vprev = v
     z = vprev @ z @ vprev.T
     return 0.5 * (z + z.T), nodes
+
+
+# ---------------------------------------------------------------------------
+# VIVS (IPROP = 4): source_code/vvprop.f with PERT2 (source_code/pert2.f) and
+# DELRD (source_code/delrd.f), the options of the reference's defaults
+# (source_code/mol.driver.f:277-349): IV = .TRUE. (constant perturbation
+# only: no DERMAT, IVP = IVPP = ISHIFT = IDIAG = .FALSE.), IPERT = .TRUE.,
+# ISYM = .TRUE., IALPHA = 6 steps per interval in geometric progression
+# with ALPHA running from ALPHA1 = 1 to ALPHA2 = 1.5 (IALFP = .FALSE.),
+# XSQMAX = 1e4.
+# ---------------------------------------------------------------------------
+VIVS_IALPHA = 6
+VIVS_ALPHA1 = 1.0
+VIVS_ALPHA2 = 1.5
+VIVS_XSQMAX = 1.0e4
+VIVS_DRMAX = 5.0
+
+
+def vivs_tolerances(tolhi: float) -> Tuple[float, float]:
+    """TLDIAG and TOFF from the segment's TOLHI (source_code/scctrl.f:157-158)."""
+    tldiag = 0.064 * math.sqrt(tolhi / 0.001)
+    return tldiag, tldiag
+
+
+def delrd(dr: float, cdiag: float, coff: float, tol: float, drmax: float, e1: float, en: float, rnow: float,
+          rmax: float) -> float:
+    """DELRD: source_code/delrd.f:1-60 (new interval size from the largest
+    diagonal and off-diagonal corrections)."""
+    rtol = 0.8 * tol
+    dfact = 2.5 if cdiag == 0.0 else (rtol / cdiag) ** 0.333
+    ofact = 2.5 if coff == 0.0 else (rtol / coff) ** 0.333
+    factor = min(dfact, ofact)
+    if en > 0.0:
+        factor = min(factor, 1.6 if rnow <= 0.1 * rmax else 2.5)
+    elif e1 > 0.0:
+        factor = min(factor, 1.20)
+    else:
+        factor = min(factor, 1.15)
+    dr = dr * factor
+    if en < 0.0:
+        dr = min(dr, 4.0 / math.sqrt(-en))
+    dr = min(dr, drmax)
+    if dr < 1.0e-6 * drmax:
+        raise RuntimeError("DELRD: STEP SIZE TOO SMALL")
+    return dr
+
+
+def pert2_v0(cx, sx, xsq, x, h, v0, a1, a2, a1p, a2p, eye11, eye12, eye22):
+    """PERT2 (source_code/pert2.f:1-230) with V1 = V2 = 0 (IV only): the step
+    integrals CC, CS, SS of the constant off-diagonal perturbation V0 between
+    the step's reference solutions, and their accumulation into EYE11,
+    EYE12, EYE22 with the step-start values A1, A1P (regular) and A2, A2P
+    (irregular).  The V1 and V2 terms of PERT2 are finite multiples of V1
+    and V2 and vanish exactly.  Returns (CC, CS, SS); the EYE are updated in
+    place.  Vectorised over the pairs I <= J; each pair takes PERT2's branch:
+    |XI - XJ| > TOL (1e-3) or one open and one closed channel: the closed
+    form; else |XSQI + XSQJ| < TOL: the series about XSQ = 0; else the
+    series about XI = XJ (both open or both closed)."""
+    n = len(x)
+    hsq = h * h
+    hinv = 1.0 / h
+    hsqinv = 1.0 / hsq
+    hf = h ** 3
+    iu, ju = np.triu_indices(n)
+    xi, xj = x[iu], x[ju]
+    sxi, sxj = sx[iu], sx[ju]
+    cxi, cxj = cx[iu], cx[ju]
+    xsqi, xsqj = xsq[iu], xsq[ju]
+    xsxi = xi * sxi
+    xsxj = xsxj0 = xj * sxj
+    diag = iu == ju
+    f5 = xsqi * xsqj
+    f6 = xsxi * xsxj
+    f7 = cxi * cxj
+    sd = xsqi - xsqj
+    total = xsqi + xsqj
+    coef = hf / (2.0 * f6) * v0[iu, ju]
+    d = xi - xj
+    tol = 1.0e-3
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        near = np.abs(d) <= tol
+        b150 = near & (np.abs(total) < tol)
+        bothc = (xsqi < 0.0) & (xsqj < 0.0)
+        b140 = near & ~b150 & (bothc | (f5 >= 0.0))
+        b130 = ~(b150 | b140)
+        # closed form (label 130)
+        f25 = 1.0 / sd
+        f8 = xsxj * cxi
+        f9 = xsxi * cxj
+        f12 = 2.0 * coef * f25
+        r1_130 = f12 * (f9 - f8)
+        r2_130 = f12 * (xsxi - xsxj)
+        # series about XI = XJ (labels 110/120/140)
+        s = xi + xj
+        ssq = s * s
+        dsq = d * d
+        shalf = s * 0.5
+        c2 = np.where(bothc, np.cosh(shalf), np.cos(shalf))
+        s2 = np.where(bothc, -np.sinh(shalf), np.sin(shalf))
+        ssq = np.where(bothc, -ssq, ssq)
+        dsq = np.where(bothc, -dsq, dsq)
+        ss2 = s * s2
+        f14 = ss2 / ssq
+        f16 = dsq * (1.0 / 24.0)
+        f17 = dsq / 80.0
+        f15 = ss2 * c2
+        f36 = dsq * (1.0 / 168.0)
+        f39 = 1.0 - dsq * (1.0 / 8.0) * (1.0 - dsq * (1.0 / 48.0) * (1.0 - dsq * (1.0 / 120.0)))
+        f40 = 1.0 / ssq
+        r1_140 = coef * (1.0 - dsq * (1.0 / 6.0) * (1.0 - dsq / 20.0 * (1.0 - dsq * (1.0 / 42.0))) + 2.0 * f15 * f40)
+        r2_140 = coef * (f14 * f39 + 0.5 * c2 * (1.0 - f16 * (1.0 - f17 * (1.0 - f36)))) * 2.0
+        # series about XSQ = 0 (label 150)
+        g3 = xsqi * xsqi
+        f20 = total * total
+        f22 = 4.0 * xsqi * xsqj
+        f23 = xsqj * xsqj
+        f29 = total * (1.0 / 6.0)
+        f28 = 14.0 * f5
+        f30 = coef * 2.0
+        r1_150 = f30 * (1.0 - f29 + (f20 + f22) * (1.0 / 120.0) - total * (g3 + f28 + f23) * (1.0 / 5040.0))
+        r2_150 = f30 * (1.0 - f29 + (xsqi * total + f23) * (1.0 / 120.0) - (g3 + f23) * total * (1.0 / 5040.0))
+        # diagonal (label 160)
+        g1 = xsxi / xsqi
+        r1_d = coef * (1.0 + cxi * g1)
+        r2_d = coef * (g1 + cxi)
+    r1 = np.where(diag, r1_d, np.where(b130, r1_130, np.where(b140, r1_140, r1_150)))
+    r2 = np.where(diag, r2_d, np.where(b130, r2_130, np.where(b140, r2_140, r2_150)))
+    # label 170
+    term1 = -cxj * r1
+    ccij = f6 * r1 * hsqinv
+    csij = xsxi * (r2 + term1) * hinv
+    ssij = r1 - cxj * r2 - cxi * r2 + f7 * r1
+    csji = xsxj0 * (r2 - cxi * r1) * hinv
+    cc = np.zeros((n, n))
+    cs = np.zeros((n, n))
+    ss = np.zeros((n, n))
+    cc[iu, ju] = ccij
+    cc[ju, iu] = ccij
+    cs[iu, ju] = csij
+    cs[ju, iu] = csji
+    ss[iu, ju] = ssij
+    ss[ju, iu] = ssij
+    t1 = a1p[ju] * ssij + a1[ju] * csji
+    t2 = a1p[ju] * csij + a1[ju] * ccij
+    t3 = a2p[ju] * ssij + a2[ju] * csji
+    t4 = a2p[ju] * csij + a2[ju] * ccij
+    e11 = a1p[iu] * t1 + a1[iu] * t2 + eye11[iu, ju]
+    e22 = a2p[iu] * t3 + a2[iu] * t4 + eye22[iu, ju]
+    e12 = a1p[iu] * t3 + a1[iu] * t4 + eye12[iu, ju]
+    off = ~diag
+    e21 = a2p[iu[off]] * t1[off] + a2[iu[off]] * t2[off] + eye12[ju[off], iu[off]]
+    eye11[iu, ju] = e11
+    eye11[ju, iu] = e11
+    eye22[iu, ju] = e22
+    eye22[ju, iu] = e22
+    eye12[iu, ju] = e12
+    eye12[ju[off], iu[off]] = e21
+    return cc, cs, ss
+
+
+@dataclass
+class VivsStep:
+    """One VVPROP step as the reference's scratch file records it for the
+    energies after the first (source_code/vvprop.f:350-352): its end RNOW,
+    width DRNOW, whether it opens an interval (ICRMAT, with the interval's
+    diagonalisation point RMIDI) and whether it is the last."""
+    rnow: float
+    drnow: float
+    newint: bool
+    rmidi: float
+    last: bool
+
+
+def vvprop(rmat: np.ndarray, blk: Block, ered: float, rstart: float, rstop: float, drnow: float,
+           tldiag: float, toff: float, drmax: float = VIVS_DRMAX,
+           replay: Optional[List[VivsStep]] = None) -> Tuple[np.ndarray, List[VivsStep]]:
+    """VVPROP (source_code/vvprop.f:1-750) for the default options (IV,
+    IPERT, ISYM, IALPHA = 6, IALFP = .FALSE.): propagates the R-matrix (the
+    inverse of Y) from RSTART to RSTOP.  Per step: W at the step centre in
+    the interval's local basis (the eigenvectors of W at the interval's
+    centre RMIDI, ascending, DIAGVC), its diagonal the reference potential
+    and its off-diagonal part the perturbation; sine/cosine (sinh/cosh)
+    reference solutions G1 (regular) and G2 (irregular) of each channel;
+    PERT2's step integrals accumulated over the interval.  At each interval
+    end the R-matrix R <- (G1' + D1' + R (G2' + D2'))^-1 (G1 + D1 + R (G2 +
+    D2)), symmetrised, D the first-order corrections (DSYMM + DGESV,
+    vvprop.f:510-618), and the next interval size from DELRD on the
+    corrections' estimates.  With replay (the steps of an earlier energy),
+    the steps are the recorded ones, as with the reference's scratch file
+    (the energies after the first, vvprop.f:155-160, 410, 590).
+    Returns (R at RSTOP in the primitive basis, the steps taken)."""
+    n = blk.n
+    rmat = np.array(rmat, dtype=np.float64)
+    ialpha = VIVS_IALPHA
+    g1 = np.zeros(n)
+    g1p = np.ones(n)
+    g2 = np.ones(n)
+    g2p = np.zeros(n)
+    vecnew = np.eye(n)
+    eye11 = np.zeros((n, n))
+    eye12 = np.zeros((n, n))
+    eye22 = np.zeros((n, n))
+    steps: List[VivsStep] = []
+    icrmat = True
+    last = False
+    coffl = 0.0
+    soff1 = sdiag1 = 0.0
+    istep = 1
+    drint = drnow
+    diagi = rstart + 0.5 * drint
+    rmid = rstart
+    balpha = (VIVS_ALPHA2 - VIVS_ALPHA1) / (rstop - rstart)
+    alpha = VIVS_ALPHA1 + balpha * (diagi - rstart)
+    drnow = drint / ialpha if alpha == 1.0 else drint * (alpha - 1.0) / (alpha ** ialpha - 1.0)
+    rnow = rstart + drnow
+    rmidi = diagi
+    k = 0
+    while True:
+        if replay is not None:
+            st = replay[k]
+            rnow, drnow, last = st.rnow, st.drnow, st.last
+            if st.newint:
+                rmidi = st.rmidi
+            k += 1
+        rcent = rnow - 0.5 * drnow
+        w0, _ = blk.hammat(rcent, ered)
+        newint = icrmat
+        if icrmat:
+            if replay is None:
+                rmidi = diagi
+            wm = w0 if rmidi == rcent else blk.hammat(rmidi, ered)[0]
+            vecold = vecnew
+            _, vecnew = np.linalg.eigh(wm)
+        w0 = vecnew.T @ w0 @ vecnew
+        w0 = 0.5 * (w0 + w0.T)
+        if icrmat:
+            tstore = vecold.T @ vecnew
+            rmat = tstore.T @ rmat @ tstore
+            rmat = 0.5 * (rmat + rmat.T)
+        icrmat = False
+        diag = -np.diag(w0).copy()
+        np.fill_diagonal(w0, 0.0)
+        # zeroth-order solutions over the step (vvprop.f:360-395)
+        dif = diag
+        xsq = dif * drnow * drnow
+        xl = np.sqrt(np.abs(dif))
+        xk = xl * drnow
+        op = dif >= 0.0
+        with np.errstate(over="ignore"):
+            sx = np.where(op, np.sin(xk), np.sinh(xk)) / xl
+            cx = np.where(op, np.cos(xk), np.cosh(xk))
+        sinx = sx
+        sine = np.where(op, sx * xl, -(sx * xl))
+        cosx = cx
+        a1, a1p, b1, b1p = g1.copy(), g1p.copy(), g2.copy(), g2p.copy()
+        g1 = a1p * sx + a1 * cx
+        g1p = a1p * cx - dif * a1 * sx
+        g2 = b1p * sx + b1 * cx
+        g2p = b1p * cx - dif * b1 * sx
+        cc, cs, ss = pert2_v0(cosx, sine, xsq, xk, drnow, w0, a1, b1, a1p, b1p, eye11, eye12, eye22)
+        # first-order corrections and their size estimates (vvprop.f:410-470)
+        prt1 = g1[None, :] * eye12 - g2[None, :] * eye11
+        prt2 = g1[:, None] * eye22 - g2[:, None] * eye12
+        prt1p = g1p[None, :] * eye12 - g2p[None, :] * eye11
+        prt2p = g1p[:, None] * eye22 - g2p[:, None] * eye12
+        if replay is None:
+            an = 1.0 / np.sqrt(a1p * a1p / np.abs(diag) + a1 * a1)
+            bn = 1.0 / np.sqrt(b1p * b1p / np.abs(diag) + b1 * b1)
+            anp = drnow * an / xk
+            bnp = drnow * bn / xk
+            expx = np.where(diag > 0.0, 1.0, np.exp(-xk * drint / drnow))
+            expx = np.where((diag <= 0.0) & (diag < -VIVS_XSQMAX), 0.0, expx)
+            wgt = expx
+            an, bn, anp, bnp = an * expx, bn * expx, anp * expx, bnp * expx
+            e = np.maximum.reduce([np.abs(prt1) * an[None, :], np.abs(prt2) * bn[:, None],
+                                   np.abs(prt1p) * anp[None, :], np.abs(prt2p) * bnp[:, None]])
+            eye_n = np.eye(n, dtype=bool)
+            coff = float(e[~eye_n].max()) if n > 1 else 0.0
+            cdiag = float(np.diag(e).max())
+            coff = max(coff, 0.0)
+            sj = sinx[None, :]
+            si = sinx[:, None]
+            cj = cosx[None, :]
+            ci = cosx[:, None]
+            dj = diag[None, :]
+            di = diag[:, None]
+            wj = wgt[None, :]
+            wi = wgt[:, None]
+            kj = xk[None, :]
+            ki = xk[:, None]
+            ccij, ccji = cc, cc.T
+            csij, csji = cs, cs.T
+            ssij, ssji = ss, ss.T
+            es = np.maximum.reduce([
+                np.abs(sj * csji - cj * ssij) * wj * kj / drnow,
+                np.abs(si * ccij - ci * csji) * wi,
+                np.abs(cj * csji + dj * sj * ssij) * wj,
+                np.abs(ci * ccij + di * si * csji) * wi * drnow / ki,
+                np.abs(si * csij - ci * ssji) * wi * ki / drnow,
+                np.abs(sj * ccji - cj * csij) * wj,
+                np.abs(ci * csij + di * si * ssji) * wi,
+                np.abs(cj * ccji + dj * sj * csij) * wj * drnow / kj])
+            lowr = np.tril(np.ones((n, n), dtype=bool), -1)
+            soff = float(es[lowr].max()) if n > 1 else 0.0
+            sdiag = float(np.diag(es).max())
+            if soff == 0.0:
+                soff = 1.0e-30
+        steps.append(VivsStep(rnow, drnow, newint, rmidi, last))
+        endint = last or (istep % ialpha == 0)
+        if endint:
+            if replay is None:
+                coffl = 0.0
+                soff1 = soff
+                sdiag1 = sdiag
+            icrmat = True
+            # vvprop.f:510-618
+            w0m = prt1p + np.diag(g1p)
+            w1m = prt2p.T + np.diag(g2p)
+            w2m = prt1 + np.diag(g1)
+            wm = prt2.T + np.diag(g2)
+            ts = w0m + rmat @ w1m
+            vo = w2m + rmat @ wm
+            vo = np.linalg.solve(ts, vo)
+            rmat = 0.5 * (vo + vo.T)
+            g1 = np.zeros(n)
+            g2 = np.ones(n)
+            g1p = np.ones(n)
+            g2p = np.zeros(n)
+            eye11 = np.zeros((n, n))
+            eye12 = np.zeros((n, n))
+            eye22 = np.zeros((n, n))
+        if not last:
+            istep += 1
+        if last:
+            break
+        if replay is not None:
+            continue
+        # next step and interval (vvprop.f:600-665)
+        drnow = drnow * alpha
+        if icrmat:
+            drint = rnow - rmid
+            drint1 = drint
+            tmxx = max(toff, tldiag)
+            if drint != drnow:
+                coff = 0.8 * tmxx * (coff / (0.8 * tmxx)) ** 1.5
+            drint = delrd(drint, cdiag, coff, tmxx, drmax, diag[0], diag[n - 1], rnow, rstop)
+            if drint1 != drnow:
+                soff1 = tldiag * (2.0 * soff1 / tldiag) ** 1.5
+            drint1 = delrd(drint1, sdiag1, soff1, tldiag, drmax, diag[0], diag[n - 1], rnow, rstop)
+            if abs(drint1) < abs(drint):
+                drint = drint1
+            if drint < drnow:
+                drint = drnow
+            if abs(rstop - rnow - drint) < abs(0.01 * drint):
+                drint = rstop - rnow
+            if (rstop - rnow - drint) * drint < 0.0:
+                drint = rstop - rnow
+            rmid = rnow
+            diagi = rnow + 0.5 * drint
+            alpha = VIVS_ALPHA1 + balpha * (diagi - rstart)
+            drnow = drint / ialpha if alpha == 1.0 else drint * (alpha - 1.0) / (alpha ** ialpha - 1.0)
+        if abs(rstop - rnow - drnow) < abs(0.01 * drnow):
+            drnow = rstop - rnow
+        if (rstop - rnow - drnow) * drnow < 0.0:
+            drnow = rstop - rnow
+        rnow = rnow + drnow
+        if abs((rnow - rstop) / drnow) < 0.005:
+            last = True
+    # back to the primitive basis (vvprop.f:670-682)
+    rmat = vecnew @ rmat @ vecnew.T
+    return 0.5 * (rmat + rmat.T), steps
+
+
+def smatrix_ld_vivs(blk: Block, ered: float, ld: Grid, rmax: float, tolhi: float = 1.0e-4,
+                    replay: Optional[List[VivsStep]] = None):
+    """The two-segment run IPROPS = 5, IPROPL = 4 (source_code/scctrl.f:
+    99-208): YINIT and LDPROP on the constant-step grid RSTART -> RMID, Y
+    inverted to R, VVPROP RMID -> RMAX with the LD segment's step as its
+    first step (DRSET returns at once for IPROP = 4, source_code/drset.f:
+    37-39) and TLDIAG = TOFF from TOLHI, R inverted to Y, YTOK + KTOS.
+    Returns the smatrix_ld dict plus STEPS (the VVPROP steps; pass them as
+    replay for a later energy, as the reference does with its scratch
+    file)."""
+    wvec, nopen = gtwvec(ered, blk.eint)
+    nb = chnsrt(blk.eint)
+    y = yinit(blk, ered, ld.rstart)
+    y, nodes = ldprop(y, lambda r: blk.hammat(r, ered), ld.rstart, ld.nstep, ld.dr)
+    rm, _ = syminv(y)
+    tldiag, toff = vivs_tolerances(tolhi)
+    rm, steps = vvprop(rm, blk, ered, ld.rstop, rmax, ld.dr, tldiag, toff, replay=replay)
+    y, _ = syminv(rm)
+    out = dict(Y=y, NB=nb, NOPEN=nopen, WVEC=wvec, NODES=nodes, STEPS=steps)
+    if nopen > 0:
+        k = ytok(nb, wvec, blk.n, nopen, y, rmax, blk.cent)
+        sr, si = ktos(k)
+        out.update(K=k, SR=sr, SI=si)
+    return out
