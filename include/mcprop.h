@@ -118,6 +118,36 @@ int mcp_prop_ld(mcp_ctx *ctx, int32_t block_id, int32_t grid_id, int32_t iprop, 
 int mcp_prop_airy(mcp_ctx *ctx, int32_t block_id, int32_t grid_id, int32_t ninst, const double *ered, double *y,
                   int32_t *nodes, int32_t *status);
 
+/* POTENL's radial terms (source_code/potenl.f:160-200, the &POTL LAMBDA /
+ * NTERM / NPOWER / A / E form) for the propagators that choose their radii as
+ * they go: term t adds a[t] * x^npower[t] (npower != 0; x below 1e-8 raised
+ * to 1e-8 for negative powers) or a[t] * exp(e[t] * x) (npower == 0) to
+ * radial term lam[t] (0-based, < mxlam) at x = R * rscale; P(lam, R) = ep2ru
+ * * the sum, clamped at 1e60 first (HAMMAT, source_code/hammat.f:38-56). */
+int mcp_block_set_potl(mcp_ctx *ctx, int32_t block_id, int32_t nterm, const int32_t *lam, const int32_t *npower,
+                       const double *a, const double *e, double rscale, double ep2ru);
+
+/* VVPROP (source_code/vvprop.f:1-4; VIVS, IPROP = 4) for ninst energies of
+ * one block (NCONST = NRSQ = 0, potential from mcp_block_set_potl) with the
+ * reference's default options (IV, IPERT, ISYM, IALPHA = 6, IALFP off,
+ * source_code/mol.driver.f:277-349): rmat (ninst N x N) holds the R-matrix
+ * (the inverse of Y, source_code/scctrl.f:150-166) at rstart on entry and at
+ * rstop on exit; drnow = the first step, drmax = DRMAX, tldiag / toff = TLDIAG
+ * / TOFF (scctrl.f:157-158: 0.064 sqrt(TOLHI / 0.001) for both).  The step
+ * record steps (max_steps x 4: RNOW, DRNOW, RMIDI, flags with bit 0 = the
+ * step opens an interval, bit 1 = the last step) stands for the reference's
+ * scratch file: replay 0 = every instance its own step control (steps, if
+ * not NULL, receives instance 0's); replay 1 = every instance follows steps;
+ * replay 2 = instance 0 chooses the steps, written to steps when not NULL,
+ * and the others follow them, as the energies after the first do with
+ * ISCRU (vvprop.f:155-160).  nstep = NSTEP; status = 0, 1 (DELRD: step size
+ * too small, delrd.f:57), 2 (more than max_steps steps), 3 (an exactly zero
+ * pivot in the interval-end DGESV, vvprop.f:600) or 4 (the record ended
+ * before its last step). */
+int mcp_prop_vivs(mcp_ctx *ctx, int32_t block_id, int32_t ninst, const double *ered, double *rmat, double rstart,
+                  double rstop, double drnow, double drmax, double tldiag, double toff, int32_t replay,
+                  int32_t max_steps, double *steps, int32_t *nstep, int32_t *status);
+
 /* YTRANS (source_code/ytrans.f90:1) for a block with NCONST > 0 and NRSQ = 0
  * on the MOLSCAT path: for each of ninst instances (pconst: ninst x nconst,
  * the constants as for mcp_prop_ld) the asymptotic operator, the sum of the

@@ -18,6 +18,7 @@
 #include "mcp_tree.hpp"
 #include "mcp_eig.hpp"
 #include "mcp_ytrans.hpp"
+#include "mcp_vivs.hpp"
 
 namespace {
 
@@ -56,6 +57,10 @@ struct Block {
   double conlen4inv;
   DevBuf vl, eint, cent;
   std::vector<double> h_cent;
+  // POTENL's terms for the propagators that choose their radii as they go
+  int potl_nterm = -1;  // -1: not set
+  double potl_rscale = 1.0, potl_ep2ru = 1.0;
+  DevBuf potl_lam, potl_npow, potl_a, potl_e;
 };
 
 struct Grid {
@@ -131,6 +136,16 @@ size_t lds_ytrans(int n) {
   z.add<double>(kNT / 64).add<double>(n).add<int>(n).add<double>(n).add<int>(n);
   return z.b;
 }
+// same pieces, in the same order, as mcp::k_vivs carves them
+size_t lds_vivs(int n, int nham, bool mats) {
+  const size_t half = (size_t)(n + 1) / 2 + 1;
+  LdsSize z;
+  for (int v = 0; v < 14; ++v) z.add<double>(n);
+  z.add<double>(half).add<double>(half).add<int>(half).add<int>(half).add<double>(kNT / 64);
+  z.add<double>(n).add<double>(n).add<int>(n).add<double>(nham > 0 ? nham : 1);
+  if (mats) z.add<double>((size_t)mcp::kVivsMats * n * n);
+  return z.b;
+}
 size_t lds_airy_sector(int n, int nham) { return lds_eig(n) + LdsSize().add<double>(nham).add<double>(nham).b; }
 size_t lds_airy_prop(int n) {
   return LdsSize().add<double>((size_t)n * n).add<double>((size_t)n * n).add<double>((size_t)n * n)
@@ -194,6 +209,7 @@ struct mcp_ctx {
   DevBuf s_ed, s_ew;              // global-memory eigen-solver: d, e, tau; inverse-iteration factors
   DevBuf s_ay, s_ax, s_am, s_akc; // large-N AIRY per instance: Y1 Y2 Y4, products, pivots, inertias
   DevBuf s_ye;                    // asymptotic thresholds of YTRANS
+  DevBuf s_vr, s_vw, s_vs, s_ve, s_vn, s_vst;  // VIVS: R, work matrices, step record, ERED, NSTEP, status
   double t_sector = 0.0, t_prop = 0.0, t_total = 0.0;
   int seg_timed = 0;                       // segments of the last mcp_smatrix_segments call
   double seg_ms[8][2] = {};                // their (energy-independent, propagation) times
@@ -232,7 +248,7 @@ int set_lds_limits(mcp_ctx* c) {
                        (const void*)mcp::k_yinit_large,    (const void*)mcp::k_airy_sector,
                        (const void*)mcp::k_airy_prop,      (const void*)mcp::k_mdprop_mfma<1>, (const void*)mcp::k_mdprop_mfma<2>,
                        (const void*)mcp::k_syminv_mfma<1>, (const void*)mcp::k_match_lu,
-                       (const void*)mcp::k_ytrans};
+                       (const void*)mcp::k_ytrans,         (const void*)mcp::k_vivs};
   const size_t total = c->lds_max;
   size_t lim = total;
   for (const void* f : fns) {
@@ -1607,6 +1623,104 @@ int mcp_ytrans(mcp_ctx* c, int32_t bid, int32_t ninst, const double* pconst, dou
   if ((rc = ytrans_device(c, *b, ninst, c->s_h.as<double>(), dy, c->s_ye.as<double>()))) return rc;
   if ((rc = download(c, eint, c->s_ye, (size_t)ninst * n))) return rc;
   if (y && (rc = download(c, y, c->s_i, (size_t)ninst * n * n))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MCP_OK;
+}
+
+int mcp_block_set_potl(mcp_ctx* c, int32_t bid, int32_t nterm, const int32_t* lam, const int32_t* npower,
+                       const double* a, const double* e, double rscale, double ep2ru) {
+  if (!c || nterm < 0 || (nterm > 0 && (!lam || !npower || !a || !e)))
+    return c ? c->fail(MCP_ERR_ARG, "block_set_potl: bad args") : MCP_ERR_ARG;
+  Block* b;
+  int rc;
+  if ((rc = lookup(c, bid, 0, &b, nullptr))) return rc;
+  for (int t = 0; t < nterm; ++t)
+    if (lam[t] < 0 || lam[t] >= b->mxlam)
+      return c->fail(MCP_ERR_ARG, "block_set_potl: term %d belongs to radial term %d of %d", t, lam[t], b->mxlam);
+  HIPCHK(c, hipSetDevice(c->device));
+  if ((rc = upload(c, b->potl_lam, lam, (size_t)nterm))) return rc;
+  if ((rc = upload(c, b->potl_npow, npower, (size_t)nterm))) return rc;
+  if ((rc = upload(c, b->potl_a, a, (size_t)nterm))) return rc;
+  if ((rc = upload(c, b->potl_e, e, (size_t)nterm))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  b->potl_nterm = nterm;
+  b->potl_rscale = rscale;
+  b->potl_ep2ru = ep2ru;
+  return MCP_OK;
+}
+
+int mcp_prop_vivs(mcp_ctx* c, int32_t bid, int32_t ninst, const double* ered, double* rmat, double rstart,
+                  double rstop, double drnow, double drmax, double tldiag, double toff, int32_t replay,
+                  int32_t max_steps, double* steps, int32_t* nstep, int32_t* status) {
+  if (!c || ninst < 0 || !ered || !rmat || !nstep || !status || replay < 0 || replay > 2 || max_steps <= 0)
+    return c ? c->fail(MCP_ERR_ARG, "prop_vivs: bad args") : MCP_ERR_ARG;
+  if (replay == 1 && !steps) return c->fail(MCP_ERR_ARG, "prop_vivs: replay without a step record");
+  if (!(rstop > rstart) || !(drnow > 0.0) || !(drmax > 0.0) || !(tldiag > 0.0) || !(toff > 0.0))
+    return c->fail(MCP_ERR_ARG, "prop_vivs: needs RSTOP > RSTART and positive DRNOW, DRMAX, TLDIAG, TOFF");
+  Block* b;
+  int rc;
+  if ((rc = lookup(c, bid, 0, &b, nullptr))) return rc;
+  if (ninst == 0) return MCP_OK;
+  if (b->nconst != 0 || b->nrsq != 0)
+    return c->fail(MCP_ERR_UNSUPPORTED, "prop_vivs: blocks with NCONST or NRSQ terms are not supported");
+  if (b->potl_nterm < 0) return c->fail(MCP_ERR_ARG, "prop_vivs: the block has no potential (mcp_block_set_potl)");
+  const int n = b->n;
+  const size_t nn = (size_t)n * n;
+  HIPCHK(c, hipSetDevice(c->device));
+  const bool lds_mats = lds_vivs(n, b->nham, true) <= c->lds_max;
+  const size_t lds = lds_vivs(n, b->nham, lds_mats);
+  if (lds > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "prop_vivs: N=%d exceeds the LDS limit", n);
+  if ((rc = upload(c, c->s_ve, ered, (size_t)ninst))) return rc;
+  if ((rc = upload(c, c->s_vr, rmat, (size_t)ninst * nn))) return rc;
+  HIPCHK(c, c->s_vs.ensure((size_t)max_steps * 4 * sizeof(double)));
+  HIPCHK(c, c->s_vn.ensure((size_t)ninst * sizeof(int)));
+  HIPCHK(c, c->s_vst.ensure((size_t)ninst * sizeof(int)));
+  if (!lds_mats) HIPCHK(c, c->s_vw.ensure((size_t)ninst * mcp::kVivsMats * nn * sizeof(double)));
+  if (replay == 1 && (rc = upload(c, c->s_vs, steps, (size_t)max_steps * 4))) return rc;
+  mcp::PotlArgs P;
+  P.nterm = b->potl_nterm;
+  P.lam = b->potl_lam.as<int>();
+  P.npow = b->potl_npow.as<int>();
+  P.a = b->potl_a.as<double>();
+  P.e = b->potl_e.as<double>();
+  P.rscale = b->potl_rscale;
+  P.ep2ru = b->potl_ep2ru;
+  mcp::VivsArgs A;
+  A.rstart = rstart;
+  A.rstop = rstop;
+  A.drnow = drnow;
+  A.drmax = drmax;
+  A.tldiag = tldiag;
+  A.toff = toff;
+  A.max_steps = max_steps;
+  A.steps = c->s_vs.as<double>();
+  A.ered = c->s_ve.as<double>();
+  A.rmat = c->s_vr.as<double>();
+  A.work = lds_mats ? nullptr : c->s_vw.as<double>();
+  A.lds_mats = lds_mats ? 1 : 0;
+  A.nstep = c->s_vn.as<int>();
+  A.status = c->s_vst.as<int>();
+  const mcp::BlockArgs B = block_args(*b);
+  // replay 2: instance 0 chooses and records the steps, the others follow
+  // them (the reference's scratch-file sequence over energies)
+  const bool record = replay == 2 || (replay == 0 && steps);
+  A.inst0 = 0;
+  A.replay = replay == 1 ? 1 : 0;
+  A.record = record ? 1 : 0;
+  const int first = replay == 2 ? 1 : (record ? 1 : ninst);
+  hipLaunchKernelGGL(mcp::k_vivs, dim3(first), dim3(kNT), lds, c->stream, B, P, A);
+  HIPCHK(c, hipGetLastError());
+  if (first < ninst) {
+    A.inst0 = first;
+    A.replay = replay == 0 ? 0 : 1;
+    A.record = 0;
+    hipLaunchKernelGGL(mcp::k_vivs, dim3(ninst - first), dim3(kNT), lds, c->stream, B, P, A);
+    HIPCHK(c, hipGetLastError());
+  }
+  if ((rc = download(c, rmat, c->s_vr, (size_t)ninst * nn))) return rc;
+  if ((rc = download(c, nstep, c->s_vn, (size_t)ninst))) return rc;
+  if ((rc = download(c, status, c->s_vst, (size_t)ninst))) return rc;
+  if (steps && replay != 1 && (rc = download(c, steps, c->s_vs, (size_t)max_steps * 4))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return MCP_OK;
 }

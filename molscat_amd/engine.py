@@ -301,6 +301,49 @@ class Engine:
         N."""
         self._check(self.lib.mcp_set_large_mode(self.ctx, int(mode)), "mcp_set_large_mode")
 
+    def set_potl(self, b: BlockHandle, potl, rscale: float, ep2ru: float):
+        """POTENL's radial terms for the propagators that choose their radii
+        as they go (VIVS): potl.terms[lam] = [(A, n) | (A, 0, E), ...] (A x^n
+        or A exp(E x) at x = R * rscale), P = ep2ru * the sum."""
+        lam, npw, a, e = [], [], [], []
+        for l, tl in enumerate(potl.terms):
+            for t in tl:
+                lam.append(l)
+                npw.append(int(t[1]))
+                a.append(float(t[0]))
+                e.append(float(t[2]) if int(t[1]) == 0 else 0.0)
+        lam = np.ascontiguousarray(lam, dtype=np.int32)
+        npw = np.ascontiguousarray(npw, dtype=np.int32)
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        e = np.ascontiguousarray(e, dtype=np.float64)
+        self._check(self.lib.mcp_block_set_potl(self.ctx, b.id, len(lam), iptr(lam), iptr(npw), dptr(a), dptr(e),
+                                                float(rscale), float(ep2ru)), "mcp_block_set_potl")
+
+    def prop_vivs(self, b: BlockHandle, ered, rmat0: np.ndarray, rstart: float, rstop: float, drnow: float,
+                  tldiag: float, toff: float, drmax: float = 5.0, replay: int = 2, steps=None,
+                  max_steps: int = 200000):
+        """VVPROP for ninst = len(ered) instances: R(RSTART) -> R(RSTOP).
+        replay 0: each instance its own steps; 1: every instance follows the
+        record `steps` (max_steps x 4); 2: the first instance chooses the
+        steps, the others follow them.  Returns (R, nstep, status, record)
+        with the record of the steps chosen (replay 0 and 2) or given."""
+        ered = np.ascontiguousarray(np.atleast_1d(ered), dtype=np.float64)
+        m = len(ered)
+        r = _colmajor(np.asarray(rmat0, dtype=np.float64).reshape(m, b.n, b.n))
+        if steps is None:
+            rec = np.zeros((max_steps, 4))
+        else:
+            rec = np.zeros((max_steps, 4))
+            s = np.asarray(steps, dtype=np.float64).reshape(-1, 4)
+            rec[: len(s)] = s
+        nstep = np.empty(m, dtype=np.int32)
+        st = np.empty(m, dtype=np.int32)
+        self._check(self.lib.mcp_prop_vivs(self.ctx, b.id, m, dptr(ered), dptr(r), float(rstart), float(rstop),
+                                           float(drnow), float(drmax), float(tldiag), float(toff), int(replay),
+                                           int(max_steps), dptr(rec), iptr(nstep), iptr(st)), "mcp_prop_vivs")
+        k = int(nstep.max()) if replay != 1 else len(rec)
+        return _from_colmajor(r), nstep, st, rec[:k].copy()
+
     def set_team_launch(self, cooperative: bool):
         """Team kernels through the cooperative launch API (default) or as
         ordinary launches of the same grids (for tracing tools)."""

@@ -56,6 +56,9 @@ SIGNATURES = {
     "mcp_ytrans": (ctypes.c_int, [_vp, _i32, _i32, _pd, _pd, _pd]),
     "mcp_last_timing": (ctypes.c_int, [_vp, _pd, _pd, _pd]),
     "mcp_last_segment_timing": (ctypes.c_int, [_vp, _i32, _pd, _pd]),
+    "mcp_block_set_potl": (ctypes.c_int, [_vp, _i32, _i32, _pi, _pi, _pd, _pd, _dbl, _dbl]),
+    "mcp_prop_vivs": (ctypes.c_int, [_vp, _i32, _i32, _pd, _pd, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _i32, _i32,
+                                      _pd, _pi, _pi]),
 }
 
 
