@@ -148,6 +148,27 @@ int mcp_prop_vivs(mcp_ctx *ctx, int32_t block_id, int32_t ninst, const double *e
                   double rstop, double drnow, double drmax, double tldiag, double toff, int32_t replay,
                   int32_t max_steps, double *steps, int32_t *nstep, int32_t *status);
 
+/* AIPROP with TOLHI > 0 (source_code/aiprop.f:1-4, the steps chosen as the
+ * propagation goes, aiprop.f:288-316) for ninst energies of one block
+ * (NCONST = NRSQ = 0, potential from mcp_block_set_potl): z (ninst N x N)
+ * holds Z, the log-derivative matrix, at rstart on entry and at rstop on
+ * exit; dr0 = the first step (DRSET's DRFIT, source_code/drset.f:41-84),
+ * powr = POWR, tolhi = TOLHI (0 < TOLHI < 1).  The step record steps
+ * (max_steps x 2: RNOW, DRNOW of each step) stands for the reference's
+ * scratch file: replay 0 = every instance its own steps (instance 0's
+ * written to steps when not NULL, their count to *nrec); replay 1 = every
+ * instance follows the *nrec steps of steps; replay 2 = instance 0 chooses
+ * the steps (written back as for 0) and the others follow them, as the
+ * energies after the first do with ISCRU.  nodes = NODES (KOUNT outward,
+ * N - KOUNT inward); nstep = NSTEP; status = 0, the 1-based step of a
+ * SYMINV instability ("INSTABILITY IN SYMINV IN AIPROP", aiprop.f:211-215),
+ * -1 (not finished in max_steps steps: Z at the last point reached, as
+ * aiprop.f:331-333) or -2 (a replay past the end of its record).  N up to
+ * the LDS-resident limit (about 56 on gfx950), MCP_ERR_UNSUPPORTED beyond. */
+int mcp_prop_airy_tolhi(mcp_ctx *ctx, int32_t block_id, int32_t ninst, const double *ered, double *z, double rstart,
+                        double rstop, double dr0, double powr, double tolhi, int32_t replay, int32_t max_steps,
+                        double *steps, int32_t *nrec, int32_t *nodes, int32_t *nstep, int32_t *status);
+
 /* YTRANS (source_code/ytrans.f90:1) for a block with NCONST > 0 and NRSQ = 0
  * on the MOLSCAT path: for each of ninst instances (pconst: ninst x nconst,
  * the constants as for mcp_prop_ld) the asymptotic operator, the sum of the

@@ -19,6 +19,7 @@
 #include "mcp_eig.hpp"
 #include "mcp_ytrans.hpp"
 #include "mcp_vivs.hpp"
+#include "mcp_airytol.hpp"
 
 namespace {
 
@@ -146,6 +147,16 @@ size_t lds_vivs(int n, int nham, bool mats) {
   if (mats) z.add<double>((size_t)mcp::kVivsMats * n * n);
   return z.b;
 }
+// same pieces, in the same order, as mcp::k_airy_tolhi carves them
+size_t lds_airytol(int n, int nham, bool mats) {
+  const size_t half = (size_t)(n + 1) / 2 + 1;
+  LdsSize z;
+  for (int v = 0; v < 6; ++v) z.add<double>(n);
+  z.add<double>(half).add<double>(half).add<int>(half).add<int>(half).add<double>(kNT / 64);
+  z.add<int>(n).add<double>(n).add<double>(nham > 0 ? nham : 1).add<double>(nham > 0 ? nham : 1).sweep(n);
+  if (mats) z.add<double>((size_t)mcp::kAiryTolMats * n * n);
+  return z.b;
+}
 size_t lds_airy_sector(int n, int nham) { return lds_eig(n) + LdsSize().add<double>(nham).add<double>(nham).b; }
 size_t lds_airy_prop(int n) {
   return LdsSize().add<double>((size_t)n * n).add<double>((size_t)n * n).add<double>((size_t)n * n)
@@ -210,6 +221,7 @@ struct mcp_ctx {
   DevBuf s_ay, s_ax, s_am, s_akc; // large-N AIRY per instance: Y1 Y2 Y4, products, pivots, inertias
   DevBuf s_ye;                    // asymptotic thresholds of YTRANS
   DevBuf s_vr, s_vw, s_vs, s_ve, s_vn, s_vst;  // VIVS: R, work matrices, step record, ERED, NSTEP, status
+  DevBuf s_vnd;                                // AIRY with TOLHI > 0: NODES
   double t_sector = 0.0, t_prop = 0.0, t_total = 0.0;
   int seg_timed = 0;                       // segments of the last mcp_smatrix_segments call
   double seg_ms[8][2] = {};                // their (energy-independent, propagation) times
@@ -248,7 +260,7 @@ int set_lds_limits(mcp_ctx* c) {
                        (const void*)mcp::k_yinit_large,    (const void*)mcp::k_airy_sector,
                        (const void*)mcp::k_airy_prop,      (const void*)mcp::k_mdprop_mfma<1>, (const void*)mcp::k_mdprop_mfma<2>,
                        (const void*)mcp::k_syminv_mfma<1>, (const void*)mcp::k_match_lu,
-                       (const void*)mcp::k_ytrans,         (const void*)mcp::k_vivs};
+                       (const void*)mcp::k_ytrans,         (const void*)mcp::k_vivs,           (const void*)mcp::k_airy_tolhi};
   const size_t total = c->lds_max;
   size_t lim = total;
   for (const void* f : fns) {
@@ -1722,6 +1734,98 @@ int mcp_prop_vivs(mcp_ctx* c, int32_t bid, int32_t ninst, const double* ered, do
   if ((rc = download(c, status, c->s_vst, (size_t)ninst))) return rc;
   if (steps && replay != 1 && (rc = download(c, steps, c->s_vs, (size_t)max_steps * 4))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MCP_OK;
+}
+
+int mcp_prop_airy_tolhi(mcp_ctx* c, int32_t bid, int32_t ninst, const double* ered, double* z, double rstart,
+                        double rstop, double dr0, double powr, double tolhi, int32_t replay, int32_t max_steps,
+                        double* steps, int32_t* nrec, int32_t* nodes, int32_t* nstep, int32_t* status) {
+  if (!c || ninst < 0 || !ered || !z || !nodes || !nstep || !status || replay < 0 || replay > 2 || max_steps <= 0)
+    return c ? c->fail(MCP_ERR_ARG, "prop_airy_tolhi: bad args") : MCP_ERR_ARG;
+  if (replay == 1 && (!steps || !nrec || *nrec <= 0 || *nrec > max_steps))
+    return c->fail(MCP_ERR_ARG, "prop_airy_tolhi: replay needs a record of 1..max_steps steps");
+  if (!(tolhi > 0.0 && tolhi < 1.0) || !(powr > 0.0) || dr0 == 0.0 || rstop == rstart ||
+      (rstop > rstart) != (dr0 > 0.0))
+    return c->fail(MCP_ERR_ARG, "prop_airy_tolhi: needs 0 < TOLHI < 1, POWR > 0 and DR0 pointing to RSTOP");
+  Block* b;
+  int rc;
+  if ((rc = lookup(c, bid, 0, &b, nullptr))) return rc;
+  if (ninst == 0) return MCP_OK;
+  if (b->nconst != 0 || b->nrsq != 0)
+    return c->fail(MCP_ERR_UNSUPPORTED, "prop_airy_tolhi: blocks with NCONST or NRSQ terms are not supported");
+  if (b->potl_nterm < 0)
+    return c->fail(MCP_ERR_ARG, "prop_airy_tolhi: the block has no potential (mcp_block_set_potl)");
+  const int n = b->n;
+  const size_t nn = (size_t)n * n;
+  HIPCHK(c, hipSetDevice(c->device));
+  // the matrices in LDS only (N up to about 56 on gfx950)
+  const size_t lds = lds_airytol(n, b->nham, true);
+  if (lds > c->lds_max)
+    return c->fail(MCP_ERR_UNSUPPORTED, "prop_airy_tolhi: N=%d exceeds the LDS-resident limit", n);
+  if ((rc = upload(c, c->s_ve, ered, (size_t)ninst))) return rc;
+  if ((rc = upload(c, c->s_vr, z, (size_t)ninst * nn))) return rc;
+  HIPCHK(c, c->s_vs.ensure((size_t)max_steps * 2 * sizeof(double)));
+  HIPCHK(c, c->s_vn.ensure((size_t)ninst * sizeof(int)));
+  HIPCHK(c, c->s_vnd.ensure((size_t)ninst * sizeof(int)));
+  HIPCHK(c, c->s_vst.ensure((size_t)ninst * sizeof(int)));
+  if (replay == 1 && (rc = upload(c, c->s_vs, steps, (size_t)(*nrec) * 2))) return rc;
+  mcp::PotlArgs P;
+  P.nterm = b->potl_nterm;
+  P.lam = b->potl_lam.as<int>();
+  P.npow = b->potl_npow.as<int>();
+  P.a = b->potl_a.as<double>();
+  P.e = b->potl_e.as<double>();
+  P.rscale = b->potl_rscale;
+  P.ep2ru = b->potl_ep2ru;
+  mcp::AiryTolArgs A;
+  A.rstart = rstart;
+  A.rstop = rstop;
+  A.dr0 = dr0;
+  A.powr = powr;
+  A.tolhi = tolhi;
+  A.max_steps = max_steps;
+  A.nrec = replay == 1 ? *nrec : 0;
+  A.steps = c->s_vs.as<double>();
+  A.ered = c->s_ve.as<double>();
+  A.z = c->s_vr.as<double>();
+  A.nodes = c->s_vnd.as<int>();
+  A.nstep = c->s_vn.as<int>();
+  A.status = c->s_vst.as<int>();
+  const mcp::BlockArgs B = block_args(*b);
+  // replay 2: instance 0 chooses and records the steps, the others follow
+  const bool record = replay == 2 || (replay == 0 && steps);
+  A.inst0 = 0;
+  A.replay = replay == 1 ? 1 : 0;
+  A.record = record ? 1 : 0;
+  const int first = record ? 1 : ninst;
+  hipLaunchKernelGGL(mcp::k_airy_tolhi, dim3(first), dim3(kNT), lds, c->stream, B, P, A);
+  HIPCHK(c, hipGetLastError());
+  if (first < ninst) {
+    int n0 = 0, st0 = 0;
+    if (replay == 2) {
+      // the record's length is instance 0's step count, if it finished
+      HIPCHK(c, hipMemcpyAsync(&n0, c->s_vn.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipMemcpyAsync(&st0, c->s_vst.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      A.nrec = st0 == 0 ? n0 : 0;
+    }
+    A.inst0 = first;
+    A.replay = replay == 0 ? 0 : 1;
+    A.record = 0;
+    hipLaunchKernelGGL(mcp::k_airy_tolhi, dim3(ninst - first), dim3(kNT), lds, c->stream, B, P, A);
+    HIPCHK(c, hipGetLastError());
+  }
+  if ((rc = download(c, z, c->s_vr, (size_t)ninst * nn))) return rc;
+  if ((rc = download(c, nodes, c->s_vnd, (size_t)ninst))) return rc;
+  if ((rc = download(c, nstep, c->s_vn, (size_t)ninst))) return rc;
+  if ((rc = download(c, status, c->s_vst, (size_t)ninst))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (steps && replay != 1) {
+    const int k = std::min(nstep[0], (int)max_steps);
+    if ((rc = download(c, steps, c->s_vs, (size_t)k * 2))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (nrec) *nrec = k;
+  }
   return MCP_OK;
 }
 

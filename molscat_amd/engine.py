@@ -344,6 +344,31 @@ class Engine:
         k = int(nstep.max()) if replay != 1 else len(rec)
         return _from_colmajor(r), nstep, st, rec[:k].copy()
 
+    def prop_airy_tolhi(self, b: BlockHandle, ered, z0: np.ndarray, rstart: float, rstop: float, dr0: float,
+                        powr: float, tolhi: float, replay: int = 2, steps=None, max_steps: int = 500000):
+        """AIPROP with TOLHI > 0 for ninst = len(ered) instances: Z(RSTART)
+        -> Z(RSTOP).  replay 0: each instance its own steps; 1: every
+        instance follows `steps` ((k, 2) RNOW, DRNOW); 2: the first instance
+        chooses the steps, the others follow them.  Returns (Z, nodes,
+        nstep, status, the (k, 2) record of the steps chosen or given)."""
+        ered = np.ascontiguousarray(np.atleast_1d(ered), dtype=np.float64)
+        m = len(ered)
+        z = _colmajor(np.asarray(z0, dtype=np.float64).reshape(m, b.n, b.n))
+        rec = np.zeros((max_steps, 2))
+        nrec = ctypes.c_int32(0)
+        if steps is not None:
+            s = np.asarray(steps, dtype=np.float64).reshape(-1, 2)
+            rec[: len(s)] = s
+            nrec.value = len(s)
+        nodes = np.empty(m, dtype=np.int32)
+        nstep = np.empty(m, dtype=np.int32)
+        st = np.empty(m, dtype=np.int32)
+        self._check(self.lib.mcp_prop_airy_tolhi(self.ctx, b.id, m, dptr(ered), dptr(z), float(rstart),
+                                                 float(rstop), float(dr0), float(powr), float(tolhi), int(replay),
+                                                 int(max_steps), dptr(rec), ctypes.byref(nrec), iptr(nodes),
+                                                 iptr(nstep), iptr(st)), "mcp_prop_airy_tolhi")
+        return _from_colmajor(z), nodes, nstep, st, rec[: nrec.value].copy()
+
     def set_team_launch(self, cooperative: bool):
         """Team kernels through the cooperative launch API (default) or as
         ordinary launches of the same grids (for tracing tools)."""

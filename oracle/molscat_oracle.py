@@ -1438,6 +1438,109 @@ def aiprop(z: np.ndarray, blk: Block, ered: float, rstart: float, rstop: float, 
     return 0.5 * (z + z.T), nodes
 
 
+def aiprop_tolhi(z: np.ndarray, blk: Block, ered: float, rstart: float, rstop: float, dr0: float, powr: float,
+                 tolhi: float, replay: Optional[List[Tuple[float, float]]] = None):
+    """AIPROP with TOLHI > 0 (source_code/aiprop.f:86-327): the steps chosen
+    as the propagation goes.  EIGOLD starts as the eigenvalues of -W(RSTART)
+    (WAVEIG, source_code/waveig.f:38-46); per step POTENT, CORR's estimates
+    (source_code/corr.f:43-60: CDIAG from the mean |W''| implied by the
+    eigenvalue change and slope, COFF from the mean local wavevector times
+    XLARGE, the largest off-diagonal element of the derivative matrix in the
+    local basis, potent.f:108-116), the step as for TOLHI = 0, then DRNOW <-
+    DRNOW * min(2, (TOLHI / max(CDIAG, COFF))^(1/POWR)) with the last step
+    cut at RSTOP (aiprop.f:288-316).  With replay (the (RNOW, DRNOW) of an
+    earlier energy) the steps are those, as with the reference's scratch
+    file for the energies after the first.  Returns (Z at RSTOP in the
+    primitive basis, NODES, the steps taken)."""
+    n = blk.n
+    nodes = 0
+    eigold = np.linalg.eigvalsh(-blk.hammat(rstart, ered)[0])
+    drnow = dr0
+    drmid = drnow * 0.5
+    rlast = rstart
+    rnow = rlast + drmid
+    rnext = rlast + drnow
+    last = False
+    vprev = None
+    steps: List[Tuple[float, float]] = []
+    k = 0
+    while True:
+        if replay is not None:
+            rnow, drnow = replay[k]
+            k += 1
+            last = k == len(replay)
+        ra = rnow - 0.5 * drnow / AIRY_SQ3
+        rb = rnow + 0.5 * drnow / AIRY_SQ3
+        wa, _ = blk.hammat(ra, ered)
+        wb, _ = blk.hammat(rb, ered)
+        eig, v = np.linalg.eigh(-0.5 * (wa + wb))
+        wdl = v.T @ ((AIRY_SQ3 / drnow) * (wb - wa)) @ v
+        hp = np.diag(wdl).copy()
+        xlarge = float(np.abs(np.tril(wdl, -1)).max()) if n > 1 else 0.0
+        fac = 2.0 / (drmid ** 2)
+        w2p = -fac * (eig - eigold + drmid * hp)
+        cdiag = abs(drnow ** 3) * (np.sum(np.abs(w2p)) / n) / 12.0
+        cay = np.sum(np.sqrt(np.abs(eig))) / n
+        coff = cay * xlarge * abs(drnow ** 3) / 12.0
+        eigold = eig
+        t = v if vprev is None else vprev.T @ v
+        z = t.T @ z @ t
+        z = 0.5 * (z + z.T)
+        y1, y2, y4 = airy_sector(drnow, eig, hp)
+        zi, kount = syminv(z + np.diag(y1))
+        if kount > n:
+            raise RuntimeError("AIPROP: instability in SYMINV")
+        nodes += kount if rstop > rstart else n - kount
+        z = -(y2[:, None] * zi * y2[None, :]) + np.diag(y4)
+        vprev = v
+        steps.append((rnow, drnow))
+        if last:
+            break
+        if replay is not None:
+            continue
+        cmax = max(abs(cdiag), abs(coff))
+        factor = (tolhi / cmax) ** (1.0 / powr) if cmax > 0.0 else math.inf
+        if drnow < 0.0 and rnext + drnow > 0.0:
+            factor = factor * (1.0 + drnow / rnext) ** 3
+        factor = min(factor, 2.0)
+        drnow = drnow * factor
+        rlast = rnext
+        rnext = rnext + drnow
+        last = (rstop - rstart) * (rstop - rnext) < 0.0
+        if last:
+            rnext = rstop
+            drnow = rnext - rlast
+        rnew = rlast + 0.5 * drnow
+        drmid = rnew - rnow
+        rnow = rnew
+        if len(steps) >= 500000:
+            raise RuntimeError("AIRY PROPAGATION NOT FINISHED IN 500000 STEPS")
+    z = vprev @ z @ vprev.T
+    return 0.5 * (z + z.T), nodes, steps
+
+
+def smatrix_md_airy(blk: Block, ered: float, ld: Grid, rmax: float, dr0: float, tolhi: float = 1.0e-4,
+                    powr: float = 3.0, replay: Optional[List[Tuple[float, float]]] = None):
+    """The two-segment run IPROPS = 6, IPROPL = 9 with TOLHIL > 0
+    (source_code/scctrl.f:99-208): YINIT and MDPROP on the constant-step
+    grid RSTART -> RMID, AIPROP with its own step control RMID -> RMAX (the
+    first step DRSET's DRFIT, source_code/drset.f:41-84; POWRL = POWRX = 3,
+    source_code/mol.prpini.f:108-116), YTOK + KTOS.  Returns the smatrix_ld
+    dict plus STEPS (the AIRY steps; pass them as replay for a later
+    energy)."""
+    wvec, nopen = gtwvec(ered, blk.eint)
+    nb = chnsrt(blk.eint)
+    y = yinit(blk, ered, ld.rstart)
+    y, nodes = mdprop(y, lambda r: blk.hammat(r, ered), ld.rstart, ld.nstep, ld.dr)
+    y, n2, steps = aiprop_tolhi(y, blk, ered, ld.rstop, rmax, dr0, powr, tolhi, replay=replay)
+    out = dict(Y=y, NB=nb, NOPEN=nopen, WVEC=wvec, NODES=nodes + n2, STEPS=steps)
+    if nopen > 0:
+        k = ytok(nb, wvec, blk.n, nopen, y, rmax, blk.cent)
+        sr, si = ktos(k)
+        out.update(K=k, SR=sr, SI=si)
+    return out
+
+
 # ---------------------------------------------------------------------------
 # VIVS (IPROP = 4): source_code/vvprop.f with PERT2 (source_code/pert2.f) and
 # DELRD (source_code/delrd.f), the options of the reference's defaults
