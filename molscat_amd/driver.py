@@ -1,0 +1,100 @@
+"""MOLSCAT's scattering loop over the engine for the atom + rigid-rotor
+(ITYPE=1) basis: the partial-wave and energy loop of the driver
+(source_code/mol.driver.f:1181-1856) with one log-derivative segment
+(IPROPS = 6 or 5, no long-range segment).
+
+For each JTOT = JTOTL, JTOTL + JSTEP, ..., JTOTU and each parity block IB:
+the block's channels and couplings (BASE + COUPLE), FINDRM's start radius
+and DRSET's constant-step grid, both at the largest energy as the driver
+sets them up (mol.driver.f:1490-1550), then every energy of the block in one
+batched S-matrix call on the device (YINIT, MDPROP / LDPROP, YTOK, KTOS),
+and SPROC's state-to-state cross sections (source_code/sproc.f:380-430)
+accumulated over the blocks (ACCSIG, sproc.f:524-530).  The totals are
+returned multiplied by JSTEP, as SPROC prints them (sproc.f:1190-1239).
+
+The lowest adiabat that FINDRM follows is taken from the device eigen-
+solver; W at its trial radii is built on the host (a handful of N x N
+matrices per block).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import List, Sequence
+
+import numpy as np
+
+from . import system as S
+
+
+@dataclass
+class BlockRun:
+    """What one (JTOT, IB) block did: its size, grid and per-energy NOPEN,
+    NODES and status."""
+    jtot: int
+    ib: int
+    n: int
+    rstart: float
+    nstep: int
+    dr: float
+    nopen: List[int]
+    nodes: List[int]
+    status: List[int]
+
+
+@dataclass
+class CrossSections:
+    energies: np.ndarray          # cm-1
+    sigma: np.ndarray             # [energy][F][I], Angstrom^2, x JSTEP
+    blocks: List[BlockRun] = field(default_factory=list)
+
+
+def _lowest_on_device(eng, blk, ered):
+    def lowest(r: float) -> float:
+        w = S.hammat_host(blk, r, ered)
+        ev, _ = eng.eigh(w[None], vectors=False)
+        return float(np.asarray(ev).reshape(-1)[0])
+    return lowest
+
+
+def cross_sections(eng, jlevels: Sequence[int], be: float, lambdas: Sequence[int], potl: S.PowerPotential,
+                   ured: float, energies: Sequence[float], jtotl: int, jtotu: int, jstep: int, rmin: float,
+                   rmax: float, dr: float, iprop: int = 6, irmset: int = 9, ibs: Sequence[int] = (1, 2),
+                   runit=None) -> CrossSections:
+    """Integral cross sections SIG(F, I) over the open levels at each
+    energy, summed over JTOT (step JSTEP) and the parity blocks ibs; rmin,
+    rmax and dr in RUNIT (RM by default), energies in cm-1."""
+    if iprop not in (5, 6):
+        raise ValueError("cross_sections: one LD segment, IPROPS = 5 or 6")
+    energies = np.asarray(energies, dtype=np.float64)
+    elev = np.array([be * float(j * (j + 1)) for j in jlevels])
+    nsig = int(np.sum(elev < energies.max()))
+    acc = np.zeros((len(energies), nsig, nsig))
+    runs: List[BlockRun] = []
+    for jtot in range(jtotl, jtotu + 1, jstep):
+        for ib in ibs:
+            blk = S.rotor_block(jtot, ib, jlevels, be, lambdas, potl, ured, runit=runit)
+            if blk.n == 0:
+                continue
+            ered = energies * blk.cm2ru
+            eredmx = float(ered.max())
+            if irmset > 0:
+                rstart, _ = S.findrm(_lowest_on_device(eng, blk, eredmx), rmin, blk.eint, blk.cent, eredmx, irmset)
+            else:
+                rstart = rmin
+            nstep, h = S.drset_constant(rstart, rmax, dr)
+            r = S.ld_radii(rstart, h, nstep)
+            b = eng.block(blk.vl, blk.eint, blk.cent, mxlam=blk.mxlam)
+            g = eng.grid(r, blk.ptab(r), h)
+            wv, nops = zip(*[S.gtwvec(e, blk.eint) for e in ered])
+            nb = S.chnsrt(blk.eint)
+            out = eng.smatrix_ld(b, g, iprop, ered, np.array(wv), np.array(nops), nb, rmax)
+            eng.destroy_block(b)
+            eng.destroy_grid(g)
+            for m in range(len(energies)):
+                if out["STATUS"][m] != 0 or nops[m] == 0:
+                    continue
+                acc[m] += S.partial_sigma(out["SR"][m], out["SI"][m], nb, nops[m], blk.levidx, jlevels, wv[m],
+                                          jtot, blk.runit, nsig, wght=1.0)
+            runs.append(BlockRun(jtot, ib, blk.n, rstart, nstep, h, list(nops), [int(x) for x in out["NODES"]],
+                                 [int(x) for x in out["STATUS"]]))
+    return CrossSections(energies=energies, sigma=acc * float(jstep), blocks=runs)

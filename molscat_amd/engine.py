@@ -301,6 +301,12 @@ class Engine:
         N."""
         self._check(self.lib.mcp_set_large_mode(self.ctx, int(mode)), "mcp_set_large_mode")
 
+    def destroy_block(self, b: BlockHandle):
+        self._check(self.lib.mcp_block_destroy(self.ctx, b.id), "mcp_block_destroy")
+
+    def destroy_grid(self, g: GridHandle):
+        self._check(self.lib.mcp_grid_destroy(self.ctx, g.id), "mcp_grid_destroy")
+
     def set_potl(self, b: BlockHandle, potl, rscale: float, ep2ru: float):
         """POTENL's radial terms for the propagators that choose their radii
         as they go (VIVS): potl.terms[lam] = [(A, n) | (A, 0, E), ...] (A x^n

@@ -315,6 +315,112 @@ def bound_segments(rmin: float, rmatch: float, rmax: float, drs: float, drl: Opt
     return (rmin, dro, nout), (rmax, -dri, nin)
 
 
+def hammat_host(blk: "RotorBlock", r: float, ered: float) -> np.ndarray:
+    """W(R) of a block on the host (HAMMAT, source_code/hammat.f:28-113 with
+    NCONST = NRSQ = 0): the driver's set-up needs it at a few trial radii
+    (FINDRM); the propagators build W on the device."""
+    n = blk.n
+    p = blk.ptab(np.array([r]))[0]
+    packed = np.tensordot(p, blk.vl, axes=(0, 0))
+    w = np.empty((n, n))
+    k = 0
+    for j in range(n):
+        for i in range(j + 1):
+            w[i, j] = w[j, i] = packed[k]
+            k += 1
+    rsq = min(1.0 / (r * r), 1.0e16)
+    w[np.diag_indices(n)] += rsq * blk.cent + blk.eint - ered
+    return w
+
+
+def findrm(lowest, rmin: float, eint: np.ndarray, cent: np.ndarray, ered: float, irmset: int):
+    """FINDRM (source_code/findrm.f, the MOLSCAT driver, not ITYPE=8): the
+    inner classical turning point of the lowest adiabat by secant steps from
+    the innermost centrifugal turning point of the open channels, then, for
+    IRMSET > 0, inwards in 3 + IRMSET/3 steps until the WKB phase integral
+    under the barrier reaches 2.303 IRMSET.  lowest(R) returns the lowest
+    eigenvalue of W(R) (DIAGVL).  Returns (RSTART, RTURN)."""
+    rturn = 1.0e30
+    for e, c in zip(eint, cent):
+        dif = ered - e
+        if dif < 0.0:
+            continue
+        rturn = min(rturn, max(math.sqrt(c / dif), rmin))
+    itry = 1 if rturn == rmin else 0
+    dv1 = 0.0
+    while True:
+        itry += 1
+        rstart = rturn
+        if itry > 25:
+            return rmin, rturn
+        d_prev = lowest(rstart)
+        rturn = 0.999 * rstart
+        found = retry = False
+        for _ in range(100):
+            v2 = lowest(rturn)
+            dv1 = (v2 - d_prev) / (rturn - rstart)
+            d_prev = v2
+            if dv1 >= 0.0:
+                if itry != 1:
+                    return rmin, 2.0 * rmin
+                rturn, retry = rmin, True
+                break
+            rstart = rturn
+            dr = -v2 / dv1
+            if dr < -0.3 * rturn:
+                dr = -0.3 * rturn
+            rturn = rturn + dr
+            if dr > 1.0e3 or rturn <= 0.0:
+                break
+            if abs(dr / rturn) <= 1.0e-3:
+                found = True
+                break
+        if retry:
+            continue
+        if not found:
+            return rmin, rturn
+        break
+    if irmset <= 0:
+        return rmin, rturn
+    nstep = 3 + irmset // 3
+    target = 2.303 * float(irmset)
+    dr = (1.5 * target / math.sqrt(abs(dv1)) / float(nstep)) ** (2.0 / 3.0)
+    rstart = rturn
+    while True:
+        phase = 0.0
+        xk_prev = 0.0
+        again = False
+        for istep in range(1, nstep + 1):
+            rnext = rstart - dr
+            if rnext < 0.0:
+                return 0.0, rturn
+            d1 = lowest(rnext)
+            if d1 <= 0.0:
+                raise NotImplementedError("FINDRM: classically allowed region inside the turning point")
+            xk = math.sqrt(d1)
+            phase = (2.0 / 3.0) * xk * dr if istep == 1 else phase + dr * 0.5 * (xk + xk_prev)
+            drnext = max(0.0, (target - phase) / xk)
+            xk_prev = xk
+            rstart = rnext
+            if istep < nstep:
+                dr = drnext / float(nstep - istep)
+            if drnext <= 0.0:
+                break
+            if istep < nstep and dr > 0.5 * rstart and dr > 0.5 * rmin:
+                dr = 0.02 * rstart
+                again = True
+                break
+        if not again:
+            return rstart, rturn
+
+
+def calck(ep: float, ered: float, eint: np.ndarray) -> float:
+    """CALCK (source_code/calck.f): the largest open-channel wavevector."""
+    dif = ep + ered - np.asarray(eint)
+    dif = dif[dif > 0.0]
+    return float(np.sqrt(dif).max()) if len(dif) else 0.0
+
+
 def gtwvec(ered: float, eint: np.ndarray) -> Tuple[np.ndarray, int]:
     """GTWVEC: source_code/gtwvec.f:18-30."""
     dif = ered - eint
