@@ -110,6 +110,25 @@ int mcp_yinit(mcp_ctx *ctx, int32_t block_id, int32_t grid_id, int32_t k0, int32
 int mcp_prop_ld(mcp_ctx *ctx, int32_t block_id, int32_t grid_id, int32_t iprop, int32_t ninst, const double *ered,
                 const double *pconst, double *y, int32_t *nodes, int32_t *status);
 
+/* MGPROP's symplectic schemes (SYMSEL, source_code/mgprop.f:103-150):
+ * imgsel = 4 (CS4, 5 substeps) or 5 (MA5, 6 substeps); a, b receive the
+ * scheme's A (position advances) and B (potential weights), nsub entries. */
+int mcp_mg_scheme(int32_t imgsel, int32_t *nsub, double *a, double *b);
+
+/* MGPROP (source_code/mgprop.f:1-4; IPROP = 8, the symplectic log-derivative
+ * propagators) for ninst instances of one block over nstep sectors of width
+ * dr.  The grid (mcp_grid_create, its dr unused) holds the radius of every W
+ * evaluation in the reference's order: per sector R = RSTART + (ISTEP-1)*DR,
+ * then R += A_k*DR after substep k, one evaluation per substep except that
+ * the first substep of sectors 2.. reuses the last W when the scheme's last
+ * A is zero (CS4): m + (nstep-1)(m-1) radii for CS4, nstep*m for MA5.  y
+ * holds Y(RSTART) on entry and Y(RSTOP) on exit; nodes = NODES (the inertia
+ * of I + A_k DR Y, subtracted when A_k < 0); status = 0 or the 1-based
+ * sector of an inversion failure.  N up to the LDS-resident limit. */
+int mcp_prop_mg(mcp_ctx *ctx, int32_t block_id, int32_t grid_id, int32_t imgsel, int32_t nstep, double dr,
+                int32_t ninst, const double *ered, const double *pconst, double *y, int32_t *nodes,
+                int32_t *status);
+
 /* AIPROP (source_code/aiprop.f:1-4, TOLHI = 0, NCONST = 0) over an AIRY grid
  * for ninst energies of one block: y holds Z (the log-derivative matrix) at
  * the grid's RSTART on entry and at its end on exit; nodes = NODES (KOUNT

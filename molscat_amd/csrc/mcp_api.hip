@@ -20,6 +20,7 @@
 #include "mcp_ytrans.hpp"
 #include "mcp_vivs.hpp"
 #include "mcp_airytol.hpp"
+#include "mcp_mgprop.hpp"
 
 namespace {
 
@@ -260,7 +261,8 @@ int set_lds_limits(mcp_ctx* c) {
                        (const void*)mcp::k_yinit_large,    (const void*)mcp::k_airy_sector,
                        (const void*)mcp::k_airy_prop,      (const void*)mcp::k_mdprop_mfma<1>, (const void*)mcp::k_mdprop_mfma<2>,
                        (const void*)mcp::k_syminv_mfma<1>, (const void*)mcp::k_match_lu,
-                       (const void*)mcp::k_ytrans,         (const void*)mcp::k_vivs,           (const void*)mcp::k_airy_tolhi};
+                       (const void*)mcp::k_ytrans,         (const void*)mcp::k_vivs,           (const void*)mcp::k_airy_tolhi,
+                       (const void*)mcp::k_mgprop};
   const size_t total = c->lds_max;
   size_t lim = total;
   for (const void* f : fns) {
@@ -1826,6 +1828,78 @@ int mcp_prop_airy_tolhi(mcp_ctx* c, int32_t bid, int32_t ninst, const double* er
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (nrec) *nrec = k;
   }
+  return MCP_OK;
+}
+
+// SYMSEL's schemes (source_code/mgprop.f:103-150): IMGSEL = 4 -> CS4, 5 -> MA5
+static bool mg_scheme(int imgsel, mcp::MgScheme& S) {
+  static const double a4[5] = {0.20517766154229, 0.40302128160421, -0.12092087633891, 0.51272193319241, 0.0};
+  static const double b4[5] = {0.061758858135626, 0.33897802655364, 0.61479130717558, -0.14054801465937,
+                               0.12501982279453};
+  static const double a5[6] = {0.33983962583911, -0.088601336903027, 0.58585647682596, -0.60303935653649,
+                               0.32358079655470, 0.44236379421975};
+  static const double b5[6] = {0.11939002928757, 0.69892737038248, -0.17131235827160, 0.40126950225135,
+                               0.010705081848236, -0.058979625498031};
+  const double* a = imgsel == 4 ? a4 : imgsel == 5 ? a5 : nullptr;
+  const double* b = imgsel == 4 ? b4 : b5;
+  if (!a) return false;
+  S.m = imgsel == 4 ? 5 : 6;
+  for (int k = 0; k < mcp::kMgMaxSub; ++k) {
+    S.a[k] = k < S.m ? a[k] : 0.0;
+    S.b[k] = k < S.m ? b[k] : 0.0;
+  }
+  return true;
+}
+
+int mcp_mg_scheme(int32_t imgsel, int32_t* nsub, double* a, double* b) {
+  mcp::MgScheme S;
+  if (!nsub || !a || !b || !mg_scheme(imgsel, S)) return MCP_ERR_ARG;
+  *nsub = S.m;
+  for (int k = 0; k < S.m; ++k) {
+    a[k] = S.a[k];
+    b[k] = S.b[k];
+  }
+  return MCP_OK;
+}
+
+int mcp_prop_mg(mcp_ctx* c, int32_t bid, int32_t gid, int32_t imgsel, int32_t nstep, double dr, int32_t ninst,
+                const double* ered, const double* pconst, double* y, int32_t* nodes, int32_t* status) {
+  if (!c || nstep <= 0 || ninst < 0 || !ered || !y || !nodes || !status)
+    return c ? c->fail(MCP_ERR_ARG, "prop_mg: bad args") : MCP_ERR_ARG;
+  mcp::MgScheme S;
+  if (!mg_scheme(imgsel, S)) return c->fail(MCP_ERR_ARG, "prop_mg: IMGSEL must be 4 (CS4) or 5 (MA5)");
+  Block* b;
+  Grid* g;
+  int rc;
+  if ((rc = lookup(c, bid, gid, &b, &g))) return rc;
+  if (ninst == 0) return MCP_OK;
+  const bool reuse = S.a[S.m - 1] == 0.0;
+  const long long need = reuse ? (long long)S.m + (long long)(nstep - 1) * (S.m - 1) : (long long)nstep * S.m;
+  if ((long long)g->npts != need)
+    return c->fail(MCP_ERR_ARG, "prop_mg: the grid has %d radii, the scheme needs %lld for %d sectors", g->npts,
+                   need, nstep);
+  if (b->nconst > 0 && !pconst) return c->fail(MCP_ERR_ARG, "prop_mg: the block needs its constants (pconst)");
+  const int n = b->n;
+  const size_t lds = LdsSize().add<double>((size_t)n * n).add<double>(b->nham > 0 ? b->nham : 1).sweep(n).b;
+  if (lds > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "prop_mg: N=%d exceeds the LDS-resident limit", n);
+  HIPCHK(c, hipSetDevice(c->device));
+  if ((rc = upload_inst(c, *b, ninst, ered, pconst))) return rc;
+  if ((rc = upload(c, c->s_i, y, (size_t)ninst * n * n))) return rc;
+  HIPCHK(c, c->s_vnd.ensure((size_t)ninst * sizeof(int)));
+  HIPCHK(c, c->s_vst.ensure((size_t)ninst * sizeof(int)));
+  mcp::InstArgs I;
+  I.ered = c->s_g.as<double>();
+  I.pconst = b->nconst > 0 ? c->s_h.as<double>() : nullptr;
+  I.y = c->s_i.as<double>();
+  I.nodes = c->s_vnd.as<int>();
+  I.status = c->s_vst.as<int>();
+  hipLaunchKernelGGL(mcp::k_mgprop, dim3(ninst), dim3(kNT), lds, c->stream, block_args(*b), grid_args(*g), I, S,
+                     nstep, dr, n > 30 ? 1 : 0);
+  HIPCHK(c, hipGetLastError());
+  if ((rc = download(c, y, c->s_i, (size_t)ninst * n * n))) return rc;
+  if ((rc = download(c, nodes, c->s_vnd, (size_t)ninst))) return rc;
+  if ((rc = download(c, status, c->s_vst, (size_t)ninst))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return MCP_OK;
 }
 

@@ -375,6 +375,43 @@ class Engine:
                                                  iptr(nstep), iptr(st)), "mcp_prop_airy_tolhi")
         return _from_colmajor(z), nodes, nstep, st, rec[: nrec.value].copy()
 
+    def mg_scheme(self, imgsel: int):
+        """(A, B) of MGPROP's scheme IMGSEL (4: CS4, 5: MA5)."""
+        m = ctypes.c_int32()
+        a = np.empty(6)
+        bb = np.empty(6)
+        if self.lib.mcp_mg_scheme(int(imgsel), ctypes.byref(m), dptr(a), dptr(bb)) != 0:
+            raise ValueError(f"IMGSEL {imgsel}: 4 (CS4) or 5 (MA5)")
+        return a[: m.value].copy(), bb[: m.value].copy()
+
+    def mg_radii(self, rstart: float, dr: float, nstep: int, imgsel: int) -> np.ndarray:
+        """The radius of every W evaluation of MGPROP, in its order
+        (source_code/mgprop.f:24-58)."""
+        a, _ = self.mg_scheme(imgsel)
+        m = len(a)
+        out = []
+        for istep in range(1, nstep + 1):
+            r = rstart + (istep - 1) * dr
+            for k in range(m):
+                if k != 0 or istep == 1 or a[m - 1] != 0.0:
+                    out.append(r)
+                r = r + a[k] * dr
+        return np.array(out)
+
+    def prop_mg(self, b: BlockHandle, g: GridHandle, imgsel: int, nstep: int, dr: float, ered, y0: np.ndarray,
+                pconst=None):
+        """MGPROP over a grid of mg_radii for ninst = len(ered) instances;
+        returns (Y(RSTOP), nodes, status)."""
+        ered = np.ascontiguousarray(np.atleast_1d(ered), dtype=np.float64)
+        m = len(ered)
+        y = _colmajor(np.asarray(y0, dtype=np.float64).reshape(m, b.n, b.n))
+        pc = None if pconst is None else np.ascontiguousarray(pconst, dtype=np.float64)
+        nodes = np.empty(m, dtype=np.int32)
+        st = np.empty(m, dtype=np.int32)
+        self._check(self.lib.mcp_prop_mg(self.ctx, b.id, g.id, int(imgsel), int(nstep), float(dr), m, dptr(ered),
+                                         dptr(pc), dptr(y), iptr(nodes), iptr(st)), "mcp_prop_mg")
+        return _from_colmajor(y), nodes, st
+
     def set_team_launch(self, cooperative: bool):
         """Team kernels through the cooperative launch API (default) or as
         ordinary launches of the same grids (for tracing tools)."""

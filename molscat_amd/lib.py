@@ -59,6 +59,8 @@ SIGNATURES = {
     "mcp_block_set_potl": (ctypes.c_int, [_vp, _i32, _i32, _pi, _pi, _pd, _pd, _dbl, _dbl]),
     "mcp_prop_vivs": (ctypes.c_int, [_vp, _i32, _i32, _pd, _pd, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _i32, _i32,
                                       _pd, _pi, _pi]),
+    "mcp_mg_scheme": (ctypes.c_int, [_i32, _pi, _pd, _pd]),
+    "mcp_prop_mg": (ctypes.c_int, [_vp, _i32, _i32, _i32, _i32, _dbl, _i32, _pd, _pd, _pd, _pi, _pi]),
     "mcp_prop_airy_tolhi": (ctypes.c_int, [_vp, _i32, _i32, _pd, _pd, _dbl, _dbl, _dbl, _dbl, _dbl, _i32, _i32,
                                             _pd, _pi, _pi, _pi, _pi]),
 }

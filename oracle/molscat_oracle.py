@@ -646,6 +646,53 @@ def ldprop(z: np.ndarray, hammat, rstart: float, nstep: int, dr: float) -> Tuple
     return z, nodes
 
 
+# MGPROP's symplectic schemes (source_code/mgprop.f:103-150, SYMSEL): IMGSEL
+# = 4 selects CS4 (5 substeps, the last with A = 0), 5 selects MA5 (6 substeps)
+MG_SCHEMES = {
+    4: ((0.20517766154229, 0.40302128160421, -0.12092087633891, 0.51272193319241, 0.0),
+        (0.061758858135626, 0.33897802655364, 0.61479130717558, -0.14054801465937, 0.12501982279453)),
+    5: ((0.33983962583911, -0.088601336903027, 0.58585647682596, -0.60303935653649, 0.32358079655470,
+         0.44236379421975),
+        (0.11939002928757, 0.69892737038248, -0.17131235827160, 0.40126950225135, 0.010705081848236,
+         -0.058979625498031)),
+}
+
+
+def mgprop(y: np.ndarray, hammat, rstart: float, nstep: int, dr: float, imgsel: int = 4) -> Tuple[np.ndarray, int]:
+    """MGPROP (symplectic log-derivative, IPROP=8): source_code/mgprop.f:
+    20-78.  Per sector of width H = DR and each substep k: Y += B_k H W(R),
+    then, for A_k != 0, Y <- (I - (I + A_k H Y)^-1) / (A_k H) with NODES
+    +/-= the inertia of I + A_k H Y (the sign of A_k), and R += A_k H; W is
+    not re-evaluated at the first substep of a sector when the scheme's last
+    A is zero (R has not moved)."""
+    a, b = MG_SCHEMES[imgsel]
+    m = len(a)
+    n = y.shape[0]
+    h = dr
+    nodes = 0
+    idx = np.arange(n)
+    u = None
+    y = np.array(y, dtype=np.float64)
+    for istep in range(1, nstep + 1):
+        r = rstart + (istep - 1) * h
+        for k in range(m):
+            if k != 0 or istep == 1 or a[m - 1] != 0.0:
+                u, _ = hammat(r)
+            y = y + (b[k] * h) * u
+            if a[k] != 0.0:
+                y = y * (a[k] * h)
+                y[idx, idx] += 1.0
+                yi, ncu = syminv(y)
+                if ncu > n or ncu < 0:
+                    raise RuntimeError("MGPROP: ERROR IN SYMINV")
+                nodes += ncu if a[k] >= 0.0 else -ncu
+                y = -yi
+                y[idx, idx] += 1.0
+                y = y * (1.0 / (a[k] * h))
+            r = r + a[k] * h
+    return _sym_lower(y), nodes
+
+
 def odprop(y0: float, blk: Block, ered: float, rstart: float, nstep: int, dr: float) -> Tuple[float, int]:
     """ODPROP (single-channel LDMD): source_code/odprop.f (NCONST=NRSQ=0,
     CONLEN=0, no scratch replay).  NODES is always 0 there."""
@@ -1236,7 +1283,7 @@ def ytrans(y: Optional[np.ndarray], blk: Block) -> Tuple[Optional[np.ndarray], n
     return yt, eint, evec
 
 
-def smatrix_ld(blk: Block, ered: float, grid: Grid, iprop: int = 6):
+def smatrix_ld(blk: Block, ered: float, grid: Grid, iprop: int = 6, imgsel: int = 4):
     """YINIT + MDPROP/LDPROP/ODPROP + (YTRANS when NCONST > 0) + YTOK + KTOS
     for one energy.
 
@@ -1254,6 +1301,8 @@ def smatrix_ld(blk: Block, ered: float, grid: Grid, iprop: int = 6):
         y = np.array([[yy]])
     elif iprop == 5:
         y, nodes = ldprop(y, hm, grid.rstart, grid.nstep, grid.dr)
+    elif iprop == 8:
+        y, nodes = mgprop(y, hm, grid.rstart, grid.nstep, grid.dr, imgsel)
     else:
         raise ValueError(iprop)
     if blk.nconst > 0:

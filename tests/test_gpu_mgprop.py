@@ -1,0 +1,82 @@
+"""GPU parity of MGPROP (mcp_prop_mg, the symplectic log-derivative
+propagators, IPROP = 8) through the C ABI on the reference's 'IPROPS=8'
+run (examples/output/molscat-all_propagators.output; ITYPE=1 rotor, N =
+12, IMGSEL = 5, 8072 sectors from FINDRM's RSTART to 20 RM): YINIT + MGPROP
+(MA5) + YTOK/KTOS on the device against the printed S-matrices at both
+energies, and CS4 (the driver's default, IMGSEL = 4) against the oracle's
+on the same grid, node counts exact."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from molscat_amd import system as S
+from oracle import molscat_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+LABEL = "model system: ITYPE=1; IPROPS=8"
+
+
+@pytest.fixture(scope="module")
+def eng():
+    from molscat_amd.engine import Engine
+
+    e = Engine(0)
+    yield e
+    e.close()
+
+
+@pytest.fixture(scope="module")
+def deck(golden_dir):
+    with open(os.path.join(golden_dir, "ref_all_propagators_smat.json")) as f:
+        sec = json.load(f)["sections"][LABEL]
+    pot = O.PowerPotential([[(1.0, -12), (-2.0, -6)], [(-0.2, -6)]], rm=3.5, epsil=50.0)
+    blk, _ = O.make_itype1_block(10, 1, [0, 2, 4, 6], 30.0, [0, 2], pot, 20.0)
+    grid, _ = O.ld_grid(blk, 0.5, 20.0, 1000.0 * blk.cm2ru, steps=12.0, irmset=9)
+    return sec, blk, grid
+
+
+def _smatrices(eng, blk, grid, imgsel, ered):
+    b = eng.block(blk.vl, blk.eint, blk.cent, mxlam=blk.vl.shape[0])
+    r = eng.mg_radii(grid.rstart, grid.dr, grid.nstep, imgsel)
+    ptab = np.stack([blk.ep2ru * np.clip(blk.potl(x * blk.rscale), -1e60, 1e60) for x in r])
+    g = eng.grid(r, ptab, grid.dr)
+    y0 = eng.yinit(b, g, ered)
+    y, nodes, st = eng.prop_mg(b, g, imgsel, grid.nstep, grid.dr, ered, y0)
+    assert np.all(st == 0)
+    out = []
+    for m in range(len(ered)):
+        wvec, nopen = S.gtwvec(ered[m], blk.eint)
+        nb = S.chnsrt(blk.eint)
+        _, sr, si, sst = eng.ytok_ktos(y[m:m + 1], wvec[None], [nopen], nb, blk.cent, grid.rstop)
+        assert sst[0] == 0
+        out.append((sr[0], si[0], int(nodes[m])))
+    return out
+
+
+def test_mgprop_ma5_fixture_smatrices(eng, deck):
+    sec, blk, grid = deck
+    assert grid.nstep == sec["props"][0][3] == 8072
+    ered = np.array([1000.0, 900.0]) * blk.cm2ru
+    print(f"MGPROP MA5 N={blk.n} {grid.nstep} sectors", flush=True)
+    res = _smatrices(eng, blk, grid, 5, ered)
+    for m, (sr, si, nodes) in enumerate(res):
+        sm = sec["smat"][m]
+        d = max(np.abs(sr - np.array(sm["sr"])).max(), np.abs(si - np.array(sm["si"])).max())
+        print(f"  E={[1000, 900][m]} cm-1 max|dS| vs printed {d:.2e} nodes={nodes}", flush=True)
+        # the oracle sits 4.7e-12 and 2.1e-11 from the printed values
+        assert d < 5e-11
+
+
+def test_mgprop_cs4_vs_oracle(eng, deck):
+    sec, blk, grid = deck
+    ered = np.array([1000.0]) * blk.cm2ru
+    print(f"MGPROP CS4 N={blk.n} {grid.nstep} sectors", flush=True)
+    sr, si, nodes = _smatrices(eng, blk, grid, 4, ered)[0]
+    ref = O.smatrix_ld(blk, ered[0], grid, iprop=8, imgsel=4)
+    d = max(np.abs(sr - ref["SR"]).max(), np.abs(si - ref["SI"]).max())
+    print(f"  max|dS| vs oracle {d:.2e} nodes={nodes} (oracle {ref['NODES']})", flush=True)
+    assert nodes == int(ref["NODES"])
+    assert d < 1e-11

@@ -56,14 +56,35 @@ def test_open_channel_order_and_wavevectors(system, ref):
     # LDPROP: the oracle agrees with an extended-precision evaluation of the
     # same recurrence to 2e-12; the printed reference values sit 1.5e-8 away.
     (5, "model system: ITYPE=1; IPROPS=5", 3e-8),
+    # MGPROP with IMGSEL = 5 (MA5); measured 4.7e-12 and 2.1e-11, while CS4
+    # lands 1e-3 away
+    (8, "model system: ITYPE=1; IPROPS=8", 5e-11),
 ])
 @pytest.mark.parametrize("ie,energy", [(0, 1000.0), (1, 900.0)])
 def test_smatrix_matches_reference(system, ref, iprop, label, tol, ie, energy):
     blk, runit, grid, _ = system
-    out = O.smatrix_ld(blk, energy * blk.cm2ru, grid, iprop=iprop)
+    out = O.smatrix_ld(blk, energy * blk.cm2ru, grid, iprop=iprop, imgsel=5)
     sm = ref[label]["smat"][ie]
     assert np.abs(out["SR"] - np.array(sm["sr"])).max() < tol
     assert np.abs(out["SI"] - np.array(sm["si"])).max() < tol
     s = out["SR"] + 1j * out["SI"]
     assert np.abs(s @ s.conj().T - np.eye(s.shape[0])).max() < 1e-12
     assert np.abs(s - s.T).max() < 1e-13
+
+
+@pytest.mark.parametrize("imgsel,order,nsteps", [(4, 4, (20, 40, 80)), (5, 5, (20, 40))])
+def test_mgprop_order_on_a_constant_potential(imgsel, order, nsteps):
+    """A single open channel with W = -k^2 constant: Y(R) = -k tan(k(R-R0) -
+    atan(Y0/k)) exactly.  CS4 is of fourth order and MA5 of fifth (halving
+    the step divides the error by about 16 and 32 until rounding takes
+    over); a wrong coefficient breaks the order."""
+    k, r0, r1, y0 = 1.3, 0.0, 2.0, 0.4
+    hm = lambda r: (np.array([[-k * k]]), None)
+    exact = -k * np.tan(k * (r1 - r0) - np.arctan(y0 / k))
+    errs = []
+    for nstep in nsteps:
+        y, _ = O.mgprop(np.array([[y0]]), hm, r0, nstep, (r1 - r0) / nstep, imgsel)
+        errs.append(abs(y[0, 0] - exact))
+    want = 2.0 ** order
+    for e1, e2 in zip(errs[:-1], errs[1:]):
+        assert 0.75 * want < e1 / e2 < 1.25 * want, errs
