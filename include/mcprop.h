@@ -101,7 +101,9 @@ int mcp_yinit(mcp_ctx *ctx, int32_t block_id, int32_t grid_id, int32_t k0, int32
 
 /* Log-derivative propagation over the whole grid for ninst instances of one
  * block: iprop = 6 replaces MDPROP (source_code/mdprop.f:1-4), iprop = 5
- * replaces LDPROP (source_code/ldprop.f:1-4), iprop = MCP_PROP_ODPROP
+ * replaces LDPROP (source_code/ldprop.f:1-4), iprop = 7 replaces MAPROP
+ * (source_code/maprop.f:1-4; NCONST = 0, N up to the LDS-resident limit),
+ * iprop = MCP_PROP_ODPROP
  * replaces ODPROP (source_code/odprop.f:1-4; N = 1, NRSQ = 0; it takes the
  * sector ends and midpoints from the LD grid).  y holds the starting log-
  * derivative matrices on entry and Y(RSTOP) on exit; nodes = NODES; status =

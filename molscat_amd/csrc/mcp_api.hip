@@ -21,6 +21,7 @@
 #include "mcp_vivs.hpp"
 #include "mcp_airytol.hpp"
 #include "mcp_mgprop.hpp"
+#include "mcp_maprop.hpp"
 
 namespace {
 
@@ -158,6 +159,24 @@ size_t lds_airytol(int n, int nham, bool mats) {
   if (mats) z.add<double>((size_t)mcp::kAiryTolMats * n * n);
   return z.b;
 }
+// same pieces, in the same order, as the MAPROP kernels carve them
+size_t lds_ma_basis(int n, int nham) {
+  const size_t half = (size_t)(n + 1) / 2 + 1;
+  LdsSize z;
+  z.add<double>((size_t)n * n).add<double>((size_t)n * n);
+  z.add<double>(half).add<double>(half).add<int>(half).add<int>(half).add<double>(kNT / 64);
+  z.add<double>(n).add<int>(n).add<double>(n).add<double>(nham > 0 ? nham : 1);
+  return z.b;
+}
+size_t lds_ma_mats(int n, int nham) {
+  const size_t p = nham > 0 ? nham : 1;
+  return LdsSize().add<double>((size_t)n * n).add<double>((size_t)n * n).add<double>((size_t)n * n)
+      .add<double>(p).add<double>(p).add<double>(p).b;
+}
+size_t lds_ma_prop(int n) {
+  return LdsSize().add<double>((size_t)n * n).add<double>((size_t)n * n).add<double>((size_t)n * n)
+      .add<double>(n).add<double>(n).add<double>(kNT / 64).sweep(n).b;
+}
 size_t lds_airy_sector(int n, int nham) { return lds_eig(n) + LdsSize().add<double>(nham).add<double>(nham).b; }
 size_t lds_airy_prop(int n) {
   return LdsSize().add<double>((size_t)n * n).add<double>((size_t)n * n).add<double>((size_t)n * n)
@@ -223,6 +242,7 @@ struct mcp_ctx {
   DevBuf s_ye;                    // asymptotic thresholds of YTRANS
   DevBuf s_vr, s_vw, s_vs, s_ve, s_vn, s_vst;  // VIVS: R, work matrices, step record, ERED, NSTEP, status
   DevBuf s_vnd;                                // AIRY with TOLHI > 0: NODES
+  DevBuf s_mt, s_mc, s_mp, s_me, s_mq;         // MAPROP: bases, corrections, basis changes, eigenvalues, end term
   double t_sector = 0.0, t_prop = 0.0, t_total = 0.0;
   int seg_timed = 0;                       // segments of the last mcp_smatrix_segments call
   double seg_ms[8][2] = {};                // their (energy-independent, propagation) times
@@ -262,7 +282,8 @@ int set_lds_limits(mcp_ctx* c) {
                        (const void*)mcp::k_airy_prop,      (const void*)mcp::k_mdprop_mfma<1>, (const void*)mcp::k_mdprop_mfma<2>,
                        (const void*)mcp::k_syminv_mfma<1>, (const void*)mcp::k_match_lu,
                        (const void*)mcp::k_ytrans,         (const void*)mcp::k_vivs,           (const void*)mcp::k_airy_tolhi,
-                       (const void*)mcp::k_mgprop};
+                       (const void*)mcp::k_mgprop,         (const void*)mcp::k_maprop_basis,
+                       (const void*)mcp::k_maprop_mats,    (const void*)mcp::k_maprop_prop};
   const size_t total = c->lds_max;
   size_t lim = total;
   for (const void* f : fns) {
@@ -577,8 +598,9 @@ int tree_mdprop(mcp_ctx* c, const mcp::BlockArgs& B, const mcp::GridArgs& G, con
 // Device-side propagation of ninst instances whose Y already sits in dy.
 int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const double* d_ered, const double* d_pconst,
                    double* dy, int* d_nodes, int* d_status) {
-  if (iprop != 5 && iprop != 6 && iprop != MCP_PROP_ODPROP)
-    return c->fail(MCP_ERR_ARG, "iprop %d: only 5 (LDPROP), 6 (MDPROP) and %d (ODPROP)", iprop, MCP_PROP_ODPROP);
+  if (iprop != 5 && iprop != 6 && iprop != 7 && iprop != MCP_PROP_ODPROP)
+    return c->fail(MCP_ERR_ARG, "iprop %d: only 5 (LDPROP), 6 (MDPROP), 7 (MAPROP) and %d (ODPROP)", iprop,
+                   MCP_PROP_ODPROP);
   if ((g.npts - 1) % 2 != 0 || g.npts < 3) return c->fail(MCP_ERR_ARG, "LD grid needs 2*NSTEP+1 points");
   const int nstep = (g.npts - 1) / 2;
   const int n = b.n;
@@ -716,6 +738,30 @@ int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const do
         if ((rc = launch_team(c, (const void*)mcp::k_mdprop_large, dim3(ninst * g), dim3(kNT), args, lm))) return rc;
       }
     }
+  } else if (iprop == 7) {
+    // MAPROP: the energy-independent bases, corrections and basis changes of
+    // every sector, then one workgroup per instance
+    if (b.nconst != 0) return c->fail(MCP_ERR_UNSUPPORTED, "MAPROP: blocks with NCONST terms are not supported");
+    const size_t lb = lds_ma_basis(n, b.nham), lm = lds_ma_mats(n, b.nham), lp = lds_ma_prop(n);
+    if (std::max(lb, std::max(lm, lp)) > c->lds_max)
+      return c->fail(MCP_ERR_UNSUPPORTED, "MAPROP: N=%d exceeds the LDS-resident limit", n);
+    const size_t nn = (size_t)n * n;
+    HIPCHK(c, c->s_mt.ensure((size_t)nstep * nn * sizeof(double)));
+    HIPCHK(c, c->s_mc.ensure((size_t)nstep * nn * sizeof(double)));
+    HIPCHK(c, c->s_mp.ensure((size_t)nstep * nn * sizeof(double)));
+    HIPCHK(c, c->s_me.ensure((size_t)nstep * n * sizeof(double)));
+    HIPCHK(c, c->s_mq.ensure(nn * sizeof(double)));
+    double* mt = c->s_mt.as<double>();
+    hipLaunchKernelGGL(mcp::k_maprop_basis, dim3(nstep), dim3(kNT), lb, c->stream, B, G, mt, c->s_me.as<double>());
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(mcp::k_maprop_mats, dim3(nstep), dim3(kNT), lm, c->stream, B, G, nstep, g.dr / 6.0, mt,
+                       c->s_mc.as<double>(), c->s_mp.as<double>(), c->s_mq.as<double>());
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    hipLaunchKernelGGL(mcp::k_maprop_prop, dim3(ninst), dim3(kNT), lp, c->stream, n, nstep, g.dr,
+                       c->s_me.as<double>(), c->s_mc.as<double>(), c->s_mp.as<double>(), mt + (size_t)(nstep - 1) * nn,
+                       c->s_mq.as<double>(), I, g.dr > 0.0 ? 1 : 0, bk);
+    HIPCHK(c, hipGetLastError());
   } else {
     const size_t need = lds_ld(n, b.nham);
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));

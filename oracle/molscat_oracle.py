@@ -646,6 +646,62 @@ def ldprop(z: np.ndarray, hammat, rstart: float, nstep: int, dr: float) -> Tuple
     return z, nodes
 
 
+def maprop(y: np.ndarray, blk: Block, ered: float, rstart: float, nstep: int, dr: float) -> Tuple[np.ndarray, int]:
+    """MAPROP (quasi-adiabatic modified log-derivative, IPROP=7):
+    source_code/maprop.f:24-186.  Per sector the adiabatic basis T_k of W at
+    the sector midpoint (ERED = 0, so energy-independent; DIAGVC ascending),
+    the correction C_k = T_{k-1}^T [(W_start - W_mid) DR/6 + Q] T_{k-1} with
+    Q = (W_end - W_mid) DR/6 of the previous sector (0 at the first), Y +=
+    C_k, Y <- P^T Y P with P = T_{k-1}^T T_k, the half-sector reference
+    terms of each adiabat (tan/tanh of sqrt|EIVAL - ERED| DR/2), the
+    inversion with its inertia added to NODES unless some adiabat exceeds
+    24/DR^2 (NCHECK), and the Y update; at the end Y <- T Y T^T + Q."""
+    n = blk.n
+    nodes = 0
+    r = rstart
+    t = np.eye(n)
+    q = np.zeros((n, n))
+    u, _ = blk.hammat(r, 0.0)
+    y = np.array(y, dtype=np.float64)
+    idx = np.arange(n)
+    for kstep in range(1, nstep + 1):
+        r = r + 0.5 * dr
+        w, _ = blk.hammat(r, 0.0)
+        cr = dr / 6.0
+        u = cr * (u - w) + q
+        u = t.T @ u @ t
+        u = 0.5 * (u + u.T)
+        y = y + u
+        eival, v = np.linalg.eigh(w)
+        q = t.T @ v
+        t = v
+        y = q.T @ y @ q
+        y = 0.5 * (y + y.T)
+        wref = eival - ered
+        flam = 0.5 * np.sqrt(np.abs(wref))
+        wmax = 24.0 / (dr * dr)
+        op = wref < 0.0
+        tn = np.tan(flam * dr)
+        th = np.tanh(flam * dr)
+        y1 = np.where(op, flam / tn - flam * tn, flam / th + flam * th)
+        y2 = np.where(op, flam / tn + flam * tn, flam / th - flam * th)
+        ncheck = int(np.sum(~op & (wref > wmax)))
+        y[idx, idx] += y1
+        y, ncount = syminv(y)
+        if ncount > n:
+            raise RuntimeError(f"MATRIX INVERSION ERROR IN MAPROP AT STEP K = {kstep}")
+        if ncheck == 0:
+            nodes += ncount
+        y = -(y2[:, None] * y * y2[None, :])
+        y[idx, idx] += y1
+        r = r + 0.5 * dr
+        u, _ = blk.hammat(r, 0.0)
+        q = cr * (u - w)
+    y = t @ y @ t.T
+    y = 0.5 * (y + y.T) + q
+    return y, nodes
+
+
 # MGPROP's symplectic schemes (source_code/mgprop.f:103-150, SYMSEL): IMGSEL
 # = 4 selects CS4 (5 substeps, the last with A = 0), 5 selects MA5 (6 substeps)
 MG_SCHEMES = {
@@ -1301,6 +1357,8 @@ def smatrix_ld(blk: Block, ered: float, grid: Grid, iprop: int = 6, imgsel: int 
         y = np.array([[yy]])
     elif iprop == 5:
         y, nodes = ldprop(y, hm, grid.rstart, grid.nstep, grid.dr)
+    elif iprop == 7:
+        y, nodes = maprop(y, blk, ered, grid.rstart, grid.nstep, grid.dr)
     elif iprop == 8:
         y, nodes = mgprop(y, hm, grid.rstart, grid.nstep, grid.dr, imgsel)
     else:

@@ -141,3 +141,42 @@ def test_odprop_free_particle_converges_to_riccati_bessel():
         errs.append(abs(y - k * ujp / uj))
     assert errs[1] < errs[0] / 10.0
     assert errs[1] < 1e-4
+
+
+def test_maprop_constant_coupled_potential_is_exact():
+    """MAPROP (LDMA) for an R-independent coupled W: the adiabatic basis is
+    W's eigenbasis at every sector, the corrections vanish, and the half-
+    sector reference terms are exact, so Y must follow the exact solution
+    in each eigenchannel (cot for open, coth-like for closed), and NODES
+    counts the zeros of the open eigenchannels' solutions."""
+    rng = np.random.default_rng(3)
+    n = 3
+    m = rng.standard_normal((n, n))
+    wc = 0.5 * (m + m.T)
+    vl = np.array([[wc[i, j] for j in range(n) for i in range(j + 1)]])
+    pot = O.PowerPotential([[(1.0, 0, 0.0)]], rm=1.0, epsil=1.0)
+    blk = O.Block(n=n, vl=vl, eint=np.array([0.0, 1.5, 12.0]), cent=np.zeros(n), lvals=np.zeros(n, dtype=int),
+                  levidx=np.arange(n), potl=pot, rscale=1.0, ep2ru=1.0, cm2ru=1.0)
+    ered = 5.0
+    lam, u = np.linalg.eigh(blk.hammat(1.0, ered)[0])
+    assert np.sum(lam < 0.0) == 2
+    rs, re, r0 = 0.5, 6.0, 0.1
+    yc0 = np.empty(n)
+    for c in range(n):
+        k = math.sqrt(abs(lam[c]))
+        yc0[c] = k / math.tan(k * (rs - r0)) if lam[c] < 0.0 else k / math.tanh(k * (rs - r0))
+    y0 = u @ np.diag(yc0) @ u.T
+    nstep = 300
+    y, nodes = O.maprop(y0, blk, ered, rs, nstep, (re - rs) / nstep)
+    yc = np.empty(n)
+    zeros = 0
+    for c in range(n):
+        k = math.sqrt(abs(lam[c]))
+        if lam[c] < 0.0:
+            yc[c] = k / math.tan(k * (re - r0))
+            zeros += int(math.floor(k * (re - r0) / math.pi)) - int(math.floor(k * (rs - r0) / math.pi))
+        else:
+            yc[c] = k / math.tanh(k * (re - r0))
+    yex = u @ np.diag(yc) @ u.T
+    assert np.abs(y - yex).max() < 1e-9 * np.abs(yex).max()
+    assert nodes == zeros

@@ -59,6 +59,8 @@ def test_open_channel_order_and_wavevectors(system, ref):
     # MGPROP with IMGSEL = 5 (MA5); measured 4.7e-12 and 2.1e-11, while CS4
     # lands 1e-3 away
     (8, "model system: ITYPE=1; IPROPS=8", 5e-11),
+    # MAPROP (LDMA): measured 3.8e-12 and 5.4e-12
+    (7, "model system: ITYPE=1; IPROPS=7", 2e-11),
 ])
 @pytest.mark.parametrize("ie,energy", [(0, 1000.0), (1, 900.0)])
 def test_smatrix_matches_reference(system, ref, iprop, label, tol, ie, energy):
