@@ -54,6 +54,132 @@ def _cpu_baseline(blk, seg, ered, nsamp: int) -> dict:
                       f"counted)"}
 
 
+def _allprop(args) -> int:
+    """Every propagator of the reference's all-propagators deck (ITYPE=1
+    rotor BE=30, j=0,2,4,6, JTOT=10, IB=1, N=12, URED=20; FINDRM's RSTART,
+    STEPS=12 at 1000 cm-1, RMAX=20 RM) for a batch of energies in [900, 1100]
+    cm-1, one workgroup per energy: MDPROP, LDPROP, MAPROP and MGPROP (MA5)
+    over 8072 sectors; MDPROP to RMID = 1.4 RTURN then AIRY with TOLHI = 1e-4
+    (the first energy choosing the steps, the others replaying them); LDPROP
+    to RMID then VIVS (likewise).  Wall time of the whole batch through the C
+    ABI (host buffers in and out), after one warm-up; beside it the oracle's
+    rate for one energy on one host thread."""
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    import math
+
+    from molscat_amd import system as S
+    from molscat_amd.engine import Engine
+
+    pot = S.PowerPotential([[(1.0, -12), (-2.0, -6)], [(-0.2, -6)]], rm=3.5, epsil=50.0)
+    blk = S.rotor_block(10, 1, [0, 2, 4, 6], 30.0, [0, 2], pot, 20.0)
+    eng = Engine(0)
+    eredmx = 1000.0 * blk.cm2ru
+    lowest = lambda r: float(np.asarray(eng.eigh(S.hammat_host(blk, r, eredmx)[None], vectors=False)[0]).reshape(-1)[0])
+    rstart, rturn = S.findrm(lowest, 0.5, blk.eint, blk.cent, eredmx, 9)
+    rmid = 1.4 * rturn
+    cays = S.calck(0.0, eredmx, blk.eint)
+    drfit = math.pi / (cays * 12.0)
+    nfull, drf = S.drset_constant(rstart, 20.0, drfit)
+    nmid, drm = S.drset_constant(rstart, rmid, drfit)
+    k = args.energies
+    ered = np.linspace(900.0, 1100.0, k) * blk.cm2ru
+    wv, nops = zip(*[S.gtwvec(e, blk.eint) for e in ered])
+    wv, nops = np.array(wv), np.array(nops)
+    nb = S.chnsrt(blk.eint)
+    b = eng.block(blk.vl, blk.eint, blk.cent, mxlam=blk.mxlam)
+    eng.set_potl(b, blk.potl, blk.rscale, blk.ep2ru)
+
+    def grid_of(r, dr):
+        return eng.grid(r, blk.ptab(r), dr)
+
+    gfull = grid_of(S.ld_radii(rstart, drf, nfull), drf)
+    gmid = grid_of(S.ld_radii(rstart, drm, nmid), drm)
+    rmg = eng.mg_radii(rstart, drf, nfull, 5)
+    gmg = grid_of(rmg, drf)
+    tl = 0.064 * math.sqrt(1e-4 / 0.001)
+
+    def match(y, rval):
+        _, _, _, st = eng.ytok_ktos(y, wv, nops, nb, blk.cent, rval)
+        if np.any(st != 0):
+            raise RuntimeError("matching failed")
+
+    def run_ld(iprop):
+        out = eng.smatrix_ld(b, gfull, iprop, ered, wv, nops, nb, 20.0)
+        if np.any(out["STATUS"] != 0):
+            raise RuntimeError(f"iprop {iprop} failed")
+
+    def run_mg():
+        y0 = eng.yinit(b, gmg, ered)
+        y, _, st = eng.prop_mg(b, gmg, 5, nfull, drf, ered, y0)
+        if np.any(st != 0):
+            raise RuntimeError("MGPROP failed")
+        match(y, 20.0)
+
+    def run_airy():
+        y0 = eng.yinit(b, gmid, ered)
+        y, _, st = eng.prop_ld(b, gmid, 6, ered, y0)
+        z, _, _, st2, _ = eng.prop_airy_tolhi(b, ered, y, rmid, 20.0, drfit, 3.0, 1e-4, replay=2)
+        if np.any(st != 0) or np.any(st2 != 0):
+            raise RuntimeError("MDPROP + AIRY failed")
+        match(z, 20.0)
+
+    def run_vivs():
+        y0 = eng.yinit(b, gmid, ered)
+        y, _, st = eng.prop_ld(b, gmid, 5, ered, y0)
+        rm, _ = eng.syminv(y)
+        rr, _, st2, _ = eng.prop_vivs(b, ered, rm, rmid, 20.0, drm, tl, tl, replay=2)
+        if np.any(st != 0) or np.any(st2 != 0):
+            raise RuntimeError("LDPROP + VIVS failed")
+        yv, _ = eng.syminv(rr)
+        match(yv, 20.0)
+
+    runs = {"MDPROP": lambda: run_ld(6), "LDPROP": lambda: run_ld(5), "MAPROP": lambda: run_ld(7),
+            "MGPROP_MA5": run_mg, "MDPROP+AIRY_TOLHI": run_airy, "LDPROP+VIVS": run_vivs}
+    res = {}
+    for name, fn in runs.items():
+        fn()
+        t0 = time.perf_counter()
+        fn()
+        dt = time.perf_counter() - t0
+        res[name] = {"ms": dt * 1e3, "s_matrices_per_s": k / dt}
+        print(f"# {name}: {dt * 1e3:.1f} ms for {k} energies", file=sys.stderr, flush=True)
+    cpu = {}
+    if args.cpu_sectors > 0:
+        from oracle import molscat_oracle as O
+
+        ob, _ = O.make_itype1_block(10, 1, [0, 2, 4, 6], 30.0, [0, 2],
+                                    O.PowerPotential(pot.terms, rm=pot.rm, epsil=pot.epsil), 20.0)
+        e0 = 1000.0 * ob.cm2ru
+        full = O.Grid(rstart, 20.0, nfull, drf)
+        mid = O.Grid(rstart, rmid, nmid, drm)
+        cpu_runs = {"MDPROP": lambda: O.smatrix_ld(ob, e0, full, iprop=6),
+                    "LDPROP": lambda: O.smatrix_ld(ob, e0, full, iprop=5),
+                    "MAPROP": lambda: O.smatrix_ld(ob, e0, full, iprop=7),
+                    "MGPROP_MA5": lambda: O.smatrix_ld(ob, e0, full, iprop=8, imgsel=5),
+                    "MDPROP+AIRY_TOLHI": lambda: O.smatrix_md_airy(ob, e0, mid, 20.0, drfit),
+                    "LDPROP+VIVS": lambda: O.smatrix_ld_vivs(ob, e0, mid, 20.0)}
+        for name, fn in cpu_runs.items():
+            t0 = time.perf_counter()
+            fn()
+            cpu[name] = 1.0 / (time.perf_counter() - t0)
+            res[name]["cpu_s_matrices_per_s"] = cpu[name]
+    line = {
+        "metric": "S-matrices/s per propagator, all-propagators deck (N=12)", "value": res["MDPROP"]["s_matrices_per_s"],
+        "unit": "S-matrices/s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": res["MDPROP"]["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"ITYPE=1 rotor N=12 JTOT=10 IB=1, {k} energies in [900, 1100] cm-1, "
+                               f"{nfull} LD sectors (RSTART {rstart:.4f} to 20 RM) or {nmid} to RMID {rmid:.4f} "
+                               f"then AIRY TOLHI=1e-4 / VIVS to 20 RM", "energies": k},
+        "propagators": res,
+        "cpu_baseline": None if not cpu else {
+            "value": cpu["MDPROP"], "unit": "S-matrices/s", "cores": 1, "kind": "port",
+            "sample": "the oracle, one energy per propagator, single thread (its AIRY sector propagator "
+                      "evaluates the Airy functions with mpmath, which dominates that line)"},
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,12 +189,17 @@ def main() -> int:
     ap.add_argument("--team", type=int, default=0, help="workgroups per instance on the large-N path (0: auto)")
     ap.add_argument("--plain-launch", action="store_true",
                     help="ordinary launches for the team kernels (for profilers that cannot trace cooperative ones)")
+    ap.add_argument("--workload", choices=["rotor441", "allprop"], default="rotor441",
+                    help="rotor441: the N=441 deck (default); allprop: every propagator on the N=12 deck")
+    ap.add_argument("--energies", type=int, default=256, help="energies batched per call for --workload allprop")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               f"(independent (J, parity, energy) instances would shard across GPUs with no exchange "
               f"but the final cross-section sum)", file=sys.stderr)
         return 2
+    if args.workload == "allprop":
+        return _allprop(args)
 
     from molscat_amd import decks
     from molscat_amd import system as S
