@@ -133,12 +133,9 @@ class FieldBlock:
         return len(self.eint)
 
     def ptab(self, r: np.ndarray) -> np.ndarray:
-        out = np.empty((len(r), self.mxlam))
-        for k, rr in enumerate(r):
-            p = self.potl(rr * self.rscale)
-            p = np.where(np.abs(p) > 1.0e60, np.sign(p) * 1.0e60, p)
-            out[k] = self.ep2ru * p
-        return out
+        p = self.potl.table(np.asarray(r, dtype=np.float64) * self.rscale)
+        p = np.where(np.abs(p) > 1.0e60, np.sign(p) * 1.0e60, p)
+        return self.ep2ru * p
 
     def pconst(self, field_g: float) -> np.ndarray:
         return np.array([field_g * self.cm2ru])

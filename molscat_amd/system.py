@@ -151,6 +151,23 @@ class PowerPotential:
             out[i] = s
         return out
 
+    def table(self, x: np.ndarray) -> np.ndarray:
+        """The same terms at many radii at once: (len(x), number of terms),
+        each term evaluated as __call__ does (pow, exp), summed in the same
+        order."""
+        x = np.asarray(x, dtype=np.float64)
+        out = np.zeros((len(x), len(self.terms)))
+        for i, tl in enumerate(self.terms):
+            s = np.zeros(len(x))
+            for t in tl:
+                if t[1] != 0:
+                    xx = np.where(np.abs(x) < 1.0e-8, 1.0e-8, x) if t[1] < 0 else x
+                    s = s + t[0] * np.power(xx, float(t[1]))
+                else:
+                    s = s + t[0] * np.exp(t[2] * x)
+            out[:, i] = s
+        return out
+
 
 def lj_rotor_potential(cvals: Sequence[float], rm: float, epsil: float) -> PowerPotential:
     return PowerPotential([[(c, -12), (-2.0 * c, -6)] for c in cvals], rm=rm, epsil=epsil)
@@ -184,12 +201,9 @@ class RotorBlock:
 
     def ptab(self, r: np.ndarray) -> np.ndarray:
         """P(lambda, R) * EP2RU with HAMMAT's clamp (hammat.f:38-73)."""
-        out = np.empty((len(r), self.mxlam))
-        for k, rr in enumerate(r):
-            p = self.potl(rr * self.rscale)
-            p = np.where(np.abs(p) > 1.0e60, np.sign(p) * 1.0e60, p)
-            out[k] = self.ep2ru * p
-        return out
+        p = self.potl.table(np.asarray(r, dtype=np.float64) * self.rscale)
+        p = np.where(np.abs(p) > 1.0e60, np.sign(p) * 1.0e60, p)
+        return self.ep2ru * p
 
 
 def rotor_block(jtot, ib, jlevels, be, lambdas, potl: PowerPotential, ured, runit=None) -> RotorBlock:

@@ -80,3 +80,20 @@ def test_transfer_matrix_is_unimodular_and_matches_airy(airy_lib):
         b = mp.airyai(x1 + dl), mp.airyai(x1 + dl, 1)
         got = p11 * float(a[0]) + p12 * float(a[1])
         assert abs(got - float(b[0])) < 1e-12 * max(abs(float(b[0])), abs(p11 * float(a[0])))
+
+
+def test_potential_table_is_the_scalar_evaluation():
+    """PowerPotential.table (the host P tables of every grid) gives the
+    per-radius evaluation, powers, exponentials and the 1e-8 floor for
+    negative powers included, to 2 ulp (numpy's vectorised pow rounds the
+    last bit differently from the scalar libm pow on some arguments)."""
+    from molscat_amd import decks
+    from molscat_amd import system as S
+
+    pots = [decks.rotor441().potential(), decks.field48().potl,
+            S.PowerPotential([[(2.0, -6), (0.5, 0, -1.3)], [(1.5, 1), (-0.25, -12)]], rm=1.0, epsil=1.0)]
+    x = np.concatenate([[0.0, 1e-9, 2e-8], np.linspace(0.3, 40.0, 997)])
+    for pot in pots:
+        tab = pot.table(x)
+        ref = np.stack([pot(v) for v in x])
+        assert np.all(np.abs(tab - ref) <= 4.5e-16 * np.abs(ref))
