@@ -54,6 +54,79 @@ def _cpu_baseline(blk, seg, ered, nsamp: int) -> dict:
                       f"counted)"}
 
 
+def _field(args) -> int:
+    """The field scan of the synthetic field-dependent deck (N = 48, one
+    constant term, the field B): 4096 fields uniform in [0, 1000] G in one
+    batched call, the collision energy 1e-4 cm-1 above each field's lowest
+    threshold; YTRANS's thresholds for all fields (mcp_ytrans), then YINIT
+    at 3.5 A, MDPROP to 25 A at DR = 0.02 A (1075 sectors), YTRANS and
+    YTOK/KTOS (mcp_smatrix_ld), timed from the first of the two device
+    calls to the last; CALCA's a(B) on the host afterwards.  Beside it the
+    oracle's rate for one field on one host thread."""
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    from molscat_amd import decks
+    from molscat_amd import system as S
+    from molscat_amd.engine import Engine
+
+    fb = decks.field48()
+    eng = Engine(0)
+    nf = args.fields
+    fields = np.linspace(0.0, 1000.0, nf)
+    b = eng.block(fb.vl, fb.eint, fb.cent, mxlam=fb.mxlam, nconst=fb.nconst)
+    nstep, h = S.drset_constant(3.5, 25.0, 0.02)
+    r = S.ld_radii(3.5, h, nstep)
+    g = eng.grid(r, fb.ptab(r), h)
+    pc = np.stack([fb.pconst(f) for f in fields])
+    eint, _ = eng.ytrans(b, pc)
+    ered = eint.min(axis=1) + 1.0e-4 * fb.cm2ru
+    wv = np.empty_like(eint)
+    nops = np.empty(nf, dtype=np.int32)
+    nb = np.empty(eint.shape, dtype=np.int32)
+    for m in range(nf):
+        wv[m], nops[m] = S.gtwvec(ered[m], eint[m])
+        nb[m] = S.chnsrt(eint[m])
+
+    def solve():
+        eng.ytrans(b, pc)
+        out = eng.smatrix_ld(b, g, 6, ered, wv, nops, nb, 25.0, pconst=pc)
+        if np.any(out["STATUS"] != 0):
+            raise RuntimeError("field scan failed")
+        return out
+
+    for _ in range(args.warmup):
+        solve()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = solve()
+    dt = (time.perf_counter() - t0) / args.steps
+    a = [S.scattering_length(out["SR"][m][0, 0], out["SI"][m][0, 0], wv[m][int(nb[m][0])], 0) for m in range(nf)]
+    cpu = None
+    if args.cpu_sectors > 0:
+        from oracle import molscat_oracle as O
+
+        ob = O.Block(n=fb.n, vl=fb.vl, eint=fb.eint, cent=fb.cent, lvals=fb.lvals, levidx=fb.levidx,
+                     potl=O.PowerPotential(fb.potl.terms, rm=fb.potl.rm, epsil=fb.potl.epsil), rscale=fb.rscale,
+                     ep2ru=fb.ep2ru, cm2ru=fb.cm2ru, nconst=1, pconst=fb.pconst(fields[1]))
+        t1 = time.perf_counter()
+        O.smatrix_ld(ob, ered[1], O.Grid(3.5, 25.0, nstep, h), iprop=6)
+        tc = time.perf_counter() - t1
+        cpu = {"value": 1.0 / tc, "unit": "S-matrices/s", "cores": 1, "kind": "port",
+               "sample": f"the oracle, one field ({fields[1]:.3f} G), YINIT + MDPROP + YTRANS + YTOK/KTOS, "
+                         f"single thread ({tc:.2f} s)"}
+    line = {
+        "metric": "S-matrices/s over a field scan (N=48, 4096 fields, scattering length a(B))",
+        "value": nf / dt, "unit": "S-matrices/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"field deck N=48, {nf} fields in [0, 1000] G, E = threshold + 1e-4 cm-1, "
+                               f"MDPROP 3.5-25 A ({nstep} sectors), one batched call", "fields": nf},
+        "a_range": [float(np.nanmin(np.real(a))), float(np.nanmax(np.real(a)))],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def _allprop(args) -> int:
     """Every propagator of the reference's all-propagators deck (ITYPE=1
     rotor BE=30, j=0,2,4,6, JTOT=10, IB=1, N=12, URED=20; FINDRM's RSTART,
@@ -189,8 +262,10 @@ def main() -> int:
     ap.add_argument("--team", type=int, default=0, help="workgroups per instance on the large-N path (0: auto)")
     ap.add_argument("--plain-launch", action="store_true",
                     help="ordinary launches for the team kernels (for profilers that cannot trace cooperative ones)")
-    ap.add_argument("--workload", choices=["rotor441", "allprop"], default="rotor441",
-                    help="rotor441: the N=441 deck (default); allprop: every propagator on the N=12 deck")
+    ap.add_argument("--workload", choices=["rotor441", "allprop", "field"], default="rotor441",
+                    help="rotor441: the N=441 deck (default); allprop: every propagator on the N=12 deck; "
+                         "field: the N=48 field scan")
+    ap.add_argument("--fields", type=int, default=4096, help="fields per call for --workload field")
     ap.add_argument("--energies", type=int, default=256, help="energies batched per call for --workload allprop")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -200,6 +275,8 @@ def main() -> int:
         return 2
     if args.workload == "allprop":
         return _allprop(args)
+    if args.workload == "field":
+        return _field(args)
 
     from molscat_amd import decks
     from molscat_amd import system as S

@@ -15,6 +15,10 @@ returned multiplied by JSTEP, as SPROC prints them (sproc.f:1190-1239).
 The lowest adiabat that FINDRM follows is taken from the device eigen-
 solver; W at its trial radii is built on the host (a handful of N x N
 matrices per block).
+
+field_scan is the same loop over an external field for a block with one
+constant term: the fields are the instances of one batched call, and the
+output is CALCA's scattering length of the s-wave entrance channel.
 """
 from __future__ import annotations
 
@@ -98,3 +102,49 @@ def cross_sections(eng, jlevels: Sequence[int], be: float, lambdas: Sequence[int
             runs.append(BlockRun(jtot, ib, blk.n, rstart, nstep, h, list(nops), [int(x) for x in out["NODES"]],
                                  [int(x) for x in out["STATUS"]]))
     return CrossSections(energies=energies, sigma=acc * float(jstep), blocks=runs)
+
+
+@dataclass
+class FieldScan:
+    fields: np.ndarray        # G
+    ered: np.ndarray          # reduced collision energies used
+    a: np.ndarray             # complex scattering length of the entrance channel, RUNIT
+    nopen: np.ndarray
+    status: np.ndarray
+
+
+def field_scan(eng, fb, fields: Sequence[float], rstart: float, rstop: float, dr: float,
+               ecoll_cm: float = 1.0e-4, iprop: int = 6) -> FieldScan:
+    """Scattering length a(B) over many fields for a block with one constant
+    term (the field), every field one instance of one batched call: YTRANS's
+    thresholds for all fields (mcp_ytrans), the collision energy ecoll_cm
+    above each field's lowest threshold, YINIT + MDPROP + YTRANS + YTOK/KTOS
+    on the device (mcp_smatrix_ld with the fields as the instances'
+    constants), and CALCA (source_code/calca.f:42-59) on the s-wave
+    entrance channel, the lowest threshold's channel of the L = 0 set."""
+    fields = np.asarray(fields, dtype=np.float64)
+    b = eng.block(fb.vl, fb.eint, fb.cent, mxlam=fb.mxlam, nconst=fb.nconst)
+    nstep, h = S.drset_constant(rstart, rstop, dr)
+    r = S.ld_radii(rstart, h, nstep)
+    g = eng.grid(r, fb.ptab(r), h)
+    pc = np.stack([fb.pconst(f) for f in fields])
+    eint, _ = eng.ytrans(b, pc)
+    ered = eint.min(axis=1) + ecoll_cm * fb.cm2ru
+    wv = np.empty_like(eint)
+    nops = np.empty(len(fields), dtype=np.int32)
+    nb = np.empty(eint.shape, dtype=np.int32)
+    for m in range(len(fields)):
+        wv[m], nops[m] = S.gtwvec(ered[m], eint[m])
+        nb[m] = S.chnsrt(eint[m])
+    out = eng.smatrix_ld(b, g, iprop, ered, wv, nops, nb, rstop, pconst=pc)
+    a = np.full(len(fields), np.nan + 0j)
+    for m in range(len(fields)):
+        if out["STATUS"][m] != 0 or nops[m] == 0:
+            continue
+        e0 = int(nb[m][0])
+        if int(fb.lvals[e0]) != 0:
+            continue
+        a[m] = S.scattering_length(out["SR"][m][0, 0], out["SI"][m][0, 0], wv[m][e0], 0)
+    eng.destroy_block(b)
+    eng.destroy_grid(g)
+    return FieldScan(fields=fields, ered=ered, a=a, nopen=nops, status=np.asarray(out["STATUS"]))
