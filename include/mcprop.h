@@ -282,6 +282,12 @@ int mcp_set_team_launch(mcp_ctx *ctx, int32_t cooperative);
  * No reference counterpart. */
 int mcp_set_large_mode(mcp_ctx *ctx, int32_t mode);
 
+/* Threads per instance of the LDS-resident MDPROP kernel: 64 (one wave) or
+ * 256 (four waves, also what 0, the default, gives).  The results do not
+ * depend on it (same arithmetic per element, same pivots).  No reference
+ * counterpart: a tuning knob for the host program. */
+int mcp_set_lds_threads(mcp_ctx *ctx, int32_t nt);
+
 /* Wall time in milliseconds of the last mcp_prop_ld / mcp_smatrix_ld call's
  * device work, measured with HIP events on the library's stream, split into
  * the energy-independent sector matrices and the propagation loop. */

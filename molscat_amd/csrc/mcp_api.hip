@@ -227,6 +227,7 @@ struct mcp_ctx {
   int large_mode = 0;   // large-N MDPROP: 0 tree where memory allows, 1 sequential matrix-core, 2 global memory
   bool eig_global = false;  // DIAGVC and AIRY on the global-memory eigen-solver for every N
   bool coop = true;         // team kernels through the cooperative launch API
+  int lds_threads = 0;      // threads per instance of the LDS-resident MDPROP (0: 256)
   DevBuf s_u;  // LDPROP sector-matrix workspace (large N)
   DevBuf s_av, s_at, s_ae, s_ah;  // AIRY step bases, basis changes, eigenvalues, slopes
   DevBuf s_x;                     // column exchange of the register-resident MDPROP
@@ -274,7 +275,7 @@ namespace {
 // its static LDS; lds_max becomes the smallest such limit.
 int set_lds_limits(mcp_ctx* c) {
   const void* fns[] = {(const void*)mcp::k_syminv,         (const void*)mcp::k_mdprop_uc,
-                       (const void*)mcp::k_mdprop_lds,     (const void*)mcp::k_ldprop_lds,
+                       (const void*)mcp::k_mdprop_lds<kNT>, (const void*)mcp::k_mdprop_lds<64>,     (const void*)mcp::k_ldprop_lds,
                        (const void*)mcp::k_eigh,           (const void*)mcp::k_bound_match,           (const void*)mcp::k_yinit,
                        (const void*)mcp::k_syminv_large,   (const void*)mcp::k_mdprop_uc_large,
                        (const void*)mcp::k_mdprop_large,   (const void*)mcp::k_ldprop_large,
@@ -642,8 +643,15 @@ int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const do
                          nstep, h, bk, c->s_e.as<double>(), c->s_f.as<int>());
       HIPCHK(c, hipGetLastError());
       HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-      hipLaunchKernelGGL(mcp::k_mdprop_lds, dim3(ninst), dim3(kNT), need, c->stream, B, G, I, nstep, h,
-                         c->s_e.as<double>(), c->s_f.as<int>(), bk);
+      // four waves per instance by default; one wave (mcp_set_lds_threads)
+      // measured slower at N = 12 and 48 even with 16 instances per CU
+      const int nt = c->lds_threads ? c->lds_threads : kNT;
+      if (nt == 64)
+        hipLaunchKernelGGL(mcp::k_mdprop_lds<64>, dim3(ninst), dim3(64), need, c->stream, B, G, I, nstep, h,
+                           c->s_e.as<double>(), c->s_f.as<int>(), bk);
+      else
+        hipLaunchKernelGGL(mcp::k_mdprop_lds<kNT>, dim3(ninst), dim3(kNT), need, c->stream, B, G, I, nstep, h,
+                           c->s_e.as<double>(), c->s_f.as<int>(), bk);
       HIPCHK(c, hipGetLastError());
     } else {
       const size_t lu = lds_large(n, b.nham, 0), lm = lds_large(n, b.nham, 3);
@@ -1946,6 +1954,12 @@ int mcp_prop_mg(mcp_ctx* c, int32_t bid, int32_t gid, int32_t imgsel, int32_t ns
   if ((rc = download(c, nodes, c->s_vnd, (size_t)ninst))) return rc;
   if ((rc = download(c, status, c->s_vst, (size_t)ninst))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MCP_OK;
+}
+
+int mcp_set_lds_threads(mcp_ctx* c, int32_t nt) {
+  if (!c || (nt != 0 && nt != 64 && nt != kNT)) return c ? c->fail(MCP_ERR_ARG, "lds_threads: 0, 64 or 256") : MCP_ERR_ARG;
+  c->lds_threads = nt;
   return MCP_OK;
 }
 

@@ -216,7 +216,11 @@ struct InstArgs {
   int* status;           // [ninst]
 };
 
-__global__ void __launch_bounds__(kNT) k_mdprop_lds(BlockArgs B, GridArgs G, InstArgs I, int nstep, double h,
+// NT threads per instance: 256 (four waves) or 64 (one wave, no cross-wave
+// barriers, more instances resident per CU); the arithmetic per element and
+// the pivot decisions do not depend on NT.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_mdprop_lds(BlockArgs B, GridArgs G, InstArgs I, int nstep, double h,
                                                     const double* __restrict__ uc,
                                                     const int* __restrict__ uc_kount, int bk) {
   extern __shared__ double smem[];
@@ -229,7 +233,7 @@ __global__ void __launch_bounds__(kNT) k_mdprop_lds(BlockArgs B, GridArgs G, Ins
   double* y23 = c.take<double>(n);
   double* u2 = c.take<double>(n);
   SweepScratch s = carve_sweep(c, n);
-  const Tile2D<kNT> t2;
+  const Tile2D<NT> t2;
   const double ered = I.ered[inst];
   const double* pconst = I.pconst ? I.pconst + (size_t)inst * B.nconst : nullptr;
   const double d1 = h * h / 3.0;
@@ -241,13 +245,13 @@ __global__ void __launch_bounds__(kNT) k_mdprop_lds(BlockArgs B, GridArgs G, Ins
   int nsave = 0;
   {
     double ps = 0.0, pbad = 0.0;
-    for (int k = threadIdx.x; k < nstep; k += kNT) {
+    for (int k = threadIdx.x; k < nstep; k += NT) {
       const int kc = uc_kount[k];
       ps += (double)kc;
       if (kc > n || kc < 0) pbad += 1.0;
     }
-    ps = block_sum<kNT>(ps, s.sv);
-    pbad = block_sum<kNT>(pbad, s.sv);
+    ps = block_sum<NT>(ps, s.sv);
+    pbad = block_sum<NT>(pbad, s.sv);
     if (pbad > 0.0) {
       if (threadIdx.x == 0) {
         I.nodes[inst] = 0;
@@ -263,7 +267,7 @@ __global__ void __launch_bounds__(kNT) k_mdprop_lds(BlockArgs B, GridArgs G, Ins
   __syncthreads();
   {
     const double r0 = G.r[0], rsq0 = rsq_of(r0);
-    for (int j = t2.tj; j < n; j += Tile2D<kNT>::kWaves)
+    for (int j = t2.tj; j < n; j += Tile2D<NT>::kWaves)
       for (int i = t2.ti; i < n; i += 64) {
         if (i < j) continue;
         double w = sumlam(B, i, j, pv);
@@ -281,7 +285,7 @@ __global__ void __launch_bounds__(kNT) k_mdprop_lds(BlockArgs B, GridArgs G, Ins
     __syncthreads();
     {
       const double rc = G.r[2 * ks - 1], rsqc = rsq_of(rc);
-      for (int i = threadIdx.x; i < n; i += kNT) {
+      for (int i = threadIdx.x; i < n; i += NT) {
         const double wref = hammat_diag(B, i, sumlam(B, i, i, pv), rc, rsqc, ered);
         const double arg = half * sqrt(fabs(wref));
         double a14, a23;
@@ -302,7 +306,7 @@ __global__ void __launch_bounds__(kNT) k_mdprop_lds(BlockArgs B, GridArgs G, Ins
       }
     }
     __syncthreads();
-    int kc = sweep_inv<kNT>(Y, n, lda, bk != 0, s);
+    int kc = sweep_inv<NT>(Y, n, lda, bk != 0, s);
     if (kc > n || kc < 0) {
       status = ks;
       break;
@@ -311,7 +315,7 @@ __global__ void __launch_bounds__(kNT) k_mdprop_lds(BlockArgs B, GridArgs G, Ins
     // Y <- U(C) + 2 Y14 - Y23 Y Y23   (mdprop.f:181-185)
     {
       const double* U = uc + (size_t)(ks - 1) * n * n;
-      for (int j = t2.tj; j < n; j += Tile2D<kNT>::kWaves)
+      for (int j = t2.tj; j < n; j += Tile2D<NT>::kWaves)
         for (int i = t2.ti; i < n; i += 64) {
           if (i < j) continue;
           double uij = U[i + (size_t)j * n];
@@ -322,7 +326,7 @@ __global__ void __launch_bounds__(kNT) k_mdprop_lds(BlockArgs B, GridArgs G, Ins
         }
     }
     __syncthreads();
-    kc = sweep_inv<kNT>(Y, n, lda, bk != 0, s);
+    kc = sweep_inv<NT>(Y, n, lda, bk != 0, s);
     if (kc > n || kc < 0) {
       status = ks;
       break;
@@ -334,7 +338,7 @@ __global__ void __launch_bounds__(kNT) k_mdprop_lds(BlockArgs B, GridArgs G, Ins
     __syncthreads();
     {
       const double rb = G.r[2 * ks], rsqb = rsq_of(rb);
-      for (int j = t2.tj; j < n; j += Tile2D<kNT>::kWaves)
+      for (int j = t2.tj; j < n; j += Tile2D<NT>::kWaves)
         for (int i = t2.ti; i < n; i += 64) {
           if (i < j) continue;
           double w = sumlam(B, i, j, pv);
@@ -348,7 +352,7 @@ __global__ void __launch_bounds__(kNT) k_mdprop_lds(BlockArgs B, GridArgs G, Ins
     __syncthreads();
   }
   const double hi = 1.0 / h;
-  for (int j = t2.tj; j < n; j += Tile2D<kNT>::kWaves)
+  for (int j = t2.tj; j < n; j += Tile2D<NT>::kWaves)
     for (int i = t2.ti; i < n; i += 64) yg[i + (size_t)j * n] = hi * Y[i + j * lda];
   if (threadIdx.x == 0) {
     I.nodes[inst] = nodes - nsave;

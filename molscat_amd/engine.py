@@ -412,6 +412,10 @@ class Engine:
                                          dptr(pc), dptr(y), iptr(nodes), iptr(st)), "mcp_prop_mg")
         return _from_colmajor(y), nodes, st
 
+    def set_lds_threads(self, nt: int):
+        """Threads per instance of the LDS-resident MDPROP: 64, 256 or 0 (256)."""
+        self._check(self.lib.mcp_set_lds_threads(self.ctx, int(nt)), "mcp_set_lds_threads")
+
     def set_team_launch(self, cooperative: bool):
         """Team kernels through the cooperative launch API (default) or as
         ordinary launches of the same grids (for tracing tools)."""

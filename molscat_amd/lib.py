@@ -53,6 +53,7 @@ SIGNATURES = {
     "mcp_set_team_size": (ctypes.c_int, [_vp, _i32]),
     "mcp_set_large_mode": (ctypes.c_int, [_vp, _i32]),
     "mcp_set_team_launch": (ctypes.c_int, [_vp, _i32]),
+    "mcp_set_lds_threads": (ctypes.c_int, [_vp, _i32]),
     "mcp_ytrans": (ctypes.c_int, [_vp, _i32, _i32, _pd, _pd, _pd]),
     "mcp_last_timing": (ctypes.c_int, [_vp, _pd, _pd, _pd]),
     "mcp_last_segment_timing": (ctypes.c_int, [_vp, _i32, _pd, _pd]),
