@@ -218,9 +218,11 @@ struct InstArgs {
 
 // NT threads per instance: 256 (four waves) or 64 (one wave, no cross-wave
 // barriers, more instances resident per CU); the arithmetic per element and
-// the pivot decisions do not depend on NT.
+// the pivot decisions do not depend on NT.  Four waves per SIMD for NT =
+// 256 (at most 128 VGPRs, no spills): the field scans run many instances
+// per CU.
 template <int NT>
-__global__ void __launch_bounds__(NT) k_mdprop_lds(BlockArgs B, GridArgs G, InstArgs I, int nstep, double h,
+__global__ void __launch_bounds__(NT, NT == 256 ? 4 : 1) k_mdprop_lds(BlockArgs B, GridArgs G, InstArgs I, int nstep, double h,
                                                     const double* __restrict__ uc,
                                                     const int* __restrict__ uc_kount, int bk) {
   extern __shared__ double smem[];
