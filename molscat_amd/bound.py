@@ -81,3 +81,107 @@ def find_levels(eng, b, g_out, g_in, cm2ru: float, emin: float, emax: float, ipr
                     fb[n] *= 0.5
                 side[n] = 1
     return [dict(STATE=n, ENERGY=best[n][0], EIGMIN=best[n][1], NODES_EMIN=nlo, NODES_EMAX=nhi) for n in states]
+
+
+def field_levels(eng, fb, delta_cm: float, fields, rmin: float, rmatch: float, rmax: float, dr: float,
+                 width: float = 1e-6, btol: float = 1e-10, maxit: int = 60) -> List[Dict]:
+    """The fields at which a bound state lies delta_cm below the lowest
+    threshold, for a block with one constant term (the field): FIELD's
+    search (source_code/fld.driver.f:657-1318) over B at an energy fixed
+    relative to each field's threshold.  The whole scan of `fields` is one
+    batched BOUND matching (YTRANS thresholds, YINIT, MDPROP out to RMATCH
+    and in from RMAX, NODEC and EIGMIN per field); wherever NODEC steps
+    between neighbouring fields a level crosses (a step of k is split into
+    k brackets by halving), and each bracket is narrowed by bisection on
+    NODEC and then converged by Illinois regula falsi on EIGMIN(B), all
+    brackets together in each batched call.
+    Returns one dict per crossing: FIELD (G), EIGMIN there, NODEC on both
+    sides of the crossing (NODEC_LO at the lower field)."""
+    from . import system as S
+
+    b = eng.block(fb.vl, fb.eint, fb.cent, mxlam=fb.mxlam, nconst=fb.nconst)
+    (r0, dro, no), (r1, dri, ni) = S.bound_segments(rmin, rmatch, rmax, dr)
+    ro = S.ld_radii(r0, dro, no)
+    ri = S.ld_radii(r1, dri, ni)
+    go = eng.grid(ro, fb.ptab(ro), dro)
+    gi = eng.grid(ri, fb.ptab(ri), dri)
+
+    def evaluate(bs):
+        bs = np.asarray(bs, dtype=np.float64)
+        pc = np.stack([fb.pconst(x) for x in bs])
+        eint, _ = eng.ytrans(b, pc)
+        ered = eint.min(axis=1) - delta_cm * fb.cm2ru
+        r = eng.bound_match(b, go, gi, ered, pconst=pc)
+        if np.any(r["STATUS"] != 0):
+            raise RuntimeError(f"bound_match failed: {r['STATUS']}")
+        return r["NODEC"].astype(int), r["EIGMIN"]
+
+    try:
+        fields = np.asarray(fields, dtype=np.float64)
+        nod, _ = evaluate(fields)
+        brackets = [[fields[k], fields[k + 1], int(nod[k]), int(nod[k + 1])] for k in range(len(fields) - 1)
+                    if nod[k] != nod[k + 1]]
+        # brackets over which NODEC steps by more than one hold several
+        # crossings: halve them until each holds one step
+        for _ in range(maxit):
+            multi = [br for br in brackets if abs(br[3] - br[2]) > 1 and br[1] - br[0] > width]
+            if not multi:
+                break
+            mids = [0.5 * (br[0] + br[1]) for br in multi]
+            nm, _ = evaluate(mids)
+            keep = [br for br in brackets if not (abs(br[3] - br[2]) > 1 and br[1] - br[0] > width)]
+            for br, m, n in zip(multi, mids, nm):
+                for sub in ([br[0], m, br[2], int(n)], [m, br[1], int(n), br[3]]):
+                    if sub[2] != sub[3]:
+                        keep.append(sub)
+            brackets = sorted(keep, key=lambda x: x[0])
+        # bisection on NODEC: keep the sub-interval that still has the step
+        for _ in range(maxit):
+            active = [br for br in brackets if br[1] - br[0] > width]
+            if not active:
+                break
+            mids = [0.5 * (br[0] + br[1]) for br in active]
+            nm, _ = evaluate(mids)
+            for br, m, n in zip(active, mids, nm):
+                if n != br[2]:
+                    br[1], br[3] = m, int(n)
+                else:
+                    br[0] = m
+        out = []
+        if not brackets:
+            return out
+        _, f = evaluate(np.concatenate([[br[0], br[1]] for br in brackets]))
+        fa = [f[2 * i] for i in range(len(brackets))]
+        fbv = [f[2 * i + 1] for i in range(len(brackets))]
+        best = [(br[0], fa[i]) if abs(fa[i]) < abs(fbv[i]) else (br[1], fbv[i]) for i, br in enumerate(brackets)]
+        side = [0] * len(brackets)
+        for _ in range(maxit):
+            act = [i for i, br in enumerate(brackets) if br[1] - br[0] > btol * max(1.0, abs(br[1]))
+                   and fa[i] * fbv[i] < 0.0]
+            if not act:
+                break
+            trial = [(fbv[i] * brackets[i][0] - fa[i] * brackets[i][1]) / (fbv[i] - fa[i]) for i in act]
+            _, ft = evaluate(trial)
+            for i, x, fx in zip(act, trial, ft):
+                if abs(fx) <= abs(best[i][1]):
+                    best[i] = (x, fx)
+                if fx == 0.0:
+                    brackets[i][0] = brackets[i][1] = x
+                    continue
+                if fx * fbv[i] > 0.0:
+                    brackets[i][1], fbv[i] = x, fx
+                    if side[i] == -1:
+                        fa[i] *= 0.5
+                    side[i] = -1
+                else:
+                    brackets[i][0], fa[i] = x, fx
+                    if side[i] == 1:
+                        fbv[i] *= 0.5
+                    side[i] = 1
+        for i, br in enumerate(brackets):
+            out.append(dict(FIELD=best[i][0], EIGMIN=best[i][1], NODEC_LO=br[2], NODEC_HI=br[3]))
+    finally:
+        eng.destroy_block(b)
+        eng.destroy_grid(go)
+        eng.destroy_grid(gi)
+    return out
