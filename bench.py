@@ -54,6 +54,21 @@ def _cpu_baseline(blk, seg, ered, nsamp: int) -> dict:
                       f"counted)"}
 
 
+def _copy(args) -> int:
+    """A plain device copy of known size (256 MiB, float64) for calibrating
+    the HBM counters FETCH_SIZE / WRITE_SIZE against bytes moved."""
+    import torch
+
+    n = 32 * 1024 * 1024
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": "copy", "bytes_read": n * 8, "bytes_written": n * 8}))
+    return 0
+
+
 def _field(args) -> int:
     """The field scan of the synthetic field-dependent deck (N = 48, one
     constant term, the field B): 4096 fields uniform in [0, 1000] G in one
@@ -262,7 +277,7 @@ def main() -> int:
     ap.add_argument("--team", type=int, default=0, help="workgroups per instance on the large-N path (0: auto)")
     ap.add_argument("--plain-launch", action="store_true",
                     help="ordinary launches for the team kernels (for profilers that cannot trace cooperative ones)")
-    ap.add_argument("--workload", choices=["rotor441", "allprop", "field"], default="rotor441",
+    ap.add_argument("--workload", choices=["rotor441", "allprop", "field", "copy"], default="rotor441",
                     help="rotor441: the N=441 deck (default); allprop: every propagator on the N=12 deck; "
                          "field: the N=48 field scan")
     ap.add_argument("--fields", type=int, default=4096, help="fields per call for --workload field")
@@ -277,6 +292,8 @@ def main() -> int:
         return _allprop(args)
     if args.workload == "field":
         return _field(args)
+    if args.workload == "copy":
+        return _copy(args)
 
     from molscat_amd import decks
     from molscat_amd import system as S

@@ -437,9 +437,11 @@ template <bool TA, bool TB, bool LOW = false>
 int tree_gemm(mcp_ctx* c, int n, int nbatch, int per, mcp::SegRef A, mcp::SegRef Bm, mcp::SegRef Ci, mcp::SegOut Co,
               double alpha, double beta) {
   const int nt = (n + mcp::kGT - 1) / mcp::kGT;
-  const unsigned ny = LOW ? (unsigned)(nt * (nt + 1) / 2) : (unsigned)(nt * nt);
-  hipLaunchKernelGGL((mcp::k_gemm_seg<TA, TB, LOW>), dim3(nbatch, ny), dim3(256), 0, c->stream, n, per, A, Bm, Ci, Co,
-                     alpha, beta);
+  const int ny = LOW ? nt * (nt + 1) / 2 : nt * nt;
+  const long long grid = (long long)mcp::kXcd * ((nbatch + mcp::kXcd - 1) / mcp::kXcd) * ny;
+  if (grid > 0x7fffffffLL) return c->fail(MCP_ERR_UNSUPPORTED, "tree_gemm: %d products of N=%d", nbatch, n);
+  hipLaunchKernelGGL((mcp::k_gemm_seg<TA, TB, LOW>), dim3((unsigned)grid), dim3(256), 0, c->stream, n, per, nbatch, ny,
+                     A, Bm, Ci, Co, alpha, beta);
   HIPCHK(c, hipGetLastError());
   return MCP_OK;
 }

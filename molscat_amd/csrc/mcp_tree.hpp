@@ -105,19 +105,30 @@ __device__ __forceinline__ void gemm_stash(double (*S)[kGT + kGPad], const doubl
 }
 
 // LOW: only the tiles on and below the diagonal of Co (a symmetric result),
-// grid y = nt (nt + 1) / 2
+// ny = nt (nt + 1) / 2 tiles per product, else nt^2.
+//
+// One-dimensional grid of kXcd * ceil(nbatch / kXcd) * ny workgroups, mapped
+// for the XCDs: workgroups are dispatched round-robin over the eight XCDs
+// in order of their index, so workgroup L runs on XCD L % 8; the L / 8-th
+// workgroup of an XCD is tile (L / 8) % ny of product XCD + 8 (L / 8 / ny).
+// Each product's tiles therefore run on one XCD one after another, and the
+// panels of its operands are re-read from that XCD's L2 instead of memory.
+constexpr int kXcd = 8;
 template <bool TA, bool TB, bool LOW = false>
-__global__ void __launch_bounds__(256) k_gemm_seg(int n, int per, SegRef A, SegRef Bm, SegRef Ci, SegOut Co,
-                                                  double alpha, double beta) {
+__global__ void __launch_bounds__(256) k_gemm_seg(int n, int per, int nbatch, int ny, SegRef A, SegRef Bm,
+                                                  SegRef Ci, SegOut Co, double alpha, double beta) {
   __shared__ double As[kGK][kGT + kGPad];
   __shared__ double Bs[kGK][kGT + kGPad];
-  const int bt = blockIdx.x;
+  const int lx = (int)(blockIdx.x % kXcd), kx = (int)(blockIdx.x / kXcd);
+  const int pi = kx / ny, tile = kx - pi * ny;
+  const int bt = lx + kXcd * pi;
+  if (bt >= nbatch) return;
   const int inst = bt / per, m = bt - inst * per;
   const int nt = (n + kGT - 1) / kGT;
-  int ti = (int)(blockIdx.y % nt), tj = (int)(blockIdx.y / nt);
+  int ti = tile % nt, tj = tile / nt;
   if (LOW) {
     // tile t = ti (ti + 1) / 2 + tj, tj <= ti
-    const int t = (int)blockIdx.y;
+    const int t = tile;
     ti = 0;
     while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
     tj = t - ti * (ti + 1) / 2;
