@@ -31,6 +31,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "S-matrices/s and sector-steps/s at N=441, mixed vs FP64 at max|ΔS|≤1e-8"
 PEAK_FP64 = 78.6e12  # MI355X dense FP64 (vector and matrix), FLOP/s
+# HBM bytes the LD stage of one solve moves (FETCH_SIZE + WRITE_SIZE over the
+# stage's dispatches, each scaled by the counter's reading for a plain 256 MiB
+# copy), measured by the counter passes of profiles/rotor_N441_ld_stage_traffic.txt;
+# it changes only when the stage's kernels do
+LD_STAGE_TRAFFIC_BYTES = 561.777e9
 
 
 def _cpu_baseline(blk, seg, ered, nsamp: int) -> dict:
@@ -383,7 +388,7 @@ def main() -> int:
                      "kernel": "LDMD propagation stage: k_mdprop_diag + tree (k_tree_m, k_syminv_large, "
                                "k_gemm_seg, k_tree_*)",
                      "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64, "traffic": None,
+                     "frac": achieved / PEAK_FP64, "traffic": LD_STAGE_TRAFFIC_BYTES,
                      "flops_per_launch": flops_alg, "ms_per_launch": ms_stage,
                      "executed_flops": flops_exec,
                      "executed_tflops": flops_exec / (ms_stage * 1e-3) / 1e12},
