@@ -1,7 +1,8 @@
 """MOLSCAT's scattering loop over the engine for the atom + rigid-rotor
 (ITYPE=1) basis: the partial-wave and energy loop of the driver
-(source_code/mol.driver.f:1181-1856) with one log-derivative segment
-(IPROPS = 6 or 5, no long-range segment).
+(source_code/mol.driver.f:1181-1856): one log-derivative segment (IPROPS =
+5, 6 or 7), or that segment to RMID followed by AIRY with TOLHI > 0
+(IPROPL = 9) or VIVS (IPROPL = 4) to RMAX.
 
 For each JTOT = JTOTL, JTOTL + JSTEP, ..., JTOTU and each parity block IB:
 the block's channels and couplings (BASE + COUPLE), FINDRM's start radius
@@ -22,6 +23,7 @@ output is CALCA's scattering length of the s-wave entrance channel.
 """
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass, field
 from typing import List, Sequence
 
@@ -43,6 +45,8 @@ class BlockRun:
     nopen: List[int]
     nodes: List[int]
     status: List[int]
+    sr: list = field(default_factory=list)    # per energy, NOPEN x NOPEN in CHNSRT order
+    si: list = field(default_factory=list)
 
 
 @dataclass
@@ -60,15 +64,39 @@ def _lowest_on_device(eng, blk, ered):
     return lowest
 
 
+def _long_range(eng, b, blk, ered, y, iprop_long, rmid, rmax, dr_ld, drfit, tolhi, powr):
+    """The long-range segment RMID -> RMAX for every energy of a block, the
+    first energy choosing the steps and the others replaying them (the
+    scratch-file sequence): AIRY (IPROPL = 9, TOLHI > 0, first step DRFIT,
+    source_code/drset.f:41-84) or VIVS (IPROPL = 4, R = Y^-1 and back,
+    source_code/scctrl.f:150-166, the LD segment's step first)."""
+    if iprop_long == 9:
+        z, _, _, st, _ = eng.prop_airy_tolhi(b, ered, y, rmid, rmax, drfit, powr, tolhi, replay=2)
+        return z, st
+    tl = 0.064 * math.sqrt(tolhi / 0.001)
+    rm, _ = eng.syminv(y)
+    rr, _, st, _ = eng.prop_vivs(b, ered, rm, rmid, rmax, dr_ld, tl, tl, replay=2)
+    yv, _ = eng.syminv(rr)
+    return yv, st
+
+
 def cross_sections(eng, jlevels: Sequence[int], be: float, lambdas: Sequence[int], potl: S.PowerPotential,
                    ured: float, energies: Sequence[float], jtotl: int, jtotu: int, jstep: int, rmin: float,
-                   rmax: float, dr: float, iprop: int = 6, irmset: int = 9, ibs: Sequence[int] = (1, 2),
-                   runit=None) -> CrossSections:
+                   rmax: float, dr: float = None, iprop: int = 6, irmset: int = 9, ibs: Sequence[int] = (1, 2),
+                   runit=None, steps: float = 0.0, iprop_long: int = 0, rvfac: float = 0.0, rmid: float = 0.0,
+                   tolhi: float = 1.0e-4, powr: float = 3.0) -> CrossSections:
     """Integral cross sections SIG(F, I) over the open levels at each
     energy, summed over JTOT (step JSTEP) and the parity blocks ibs; rmin,
-    rmax and dr in RUNIT (RM by default), energies in cm-1."""
-    if iprop not in (5, 6):
-        raise ValueError("cross_sections: one LD segment, IPROPS = 5 or 6")
+    rmax and dr in RUNIT (RM by default), energies in cm-1.  The LD step is
+    dr, or pi / (CAY * steps) when steps > 0 (DRSET, source_code/drset.f:
+    41-64; CAY the largest open wavevector at the largest energy).  With
+    iprop_long = 9 (AIRY, TOLHI > 0) or 4 (VIVS) the LD segment ends at RMID
+    (rmid, or rvfac x FINDRM's turning point, source_code/mol.driver.f:
+    1504-1506) and the long-range propagator carries Y on to RMAX."""
+    if iprop not in (5, 6, 7):
+        raise ValueError("cross_sections: the LD segment's IPROPS must be 5, 6 or 7")
+    if iprop_long not in (0, 4, 9):
+        raise ValueError("cross_sections: IPROPL must be 0 (none), 4 (VIVS) or 9 (AIRY, TOLHI > 0)")
     energies = np.asarray(energies, dtype=np.float64)
     elev = np.array([be * float(j * (j + 1)) for j in jlevels])
     nsig = int(np.sum(elev < energies.max()))
@@ -82,16 +110,30 @@ def cross_sections(eng, jlevels: Sequence[int], be: float, lambdas: Sequence[int
             ered = energies * blk.cm2ru
             eredmx = float(ered.max())
             if irmset > 0:
-                rstart, _ = S.findrm(_lowest_on_device(eng, blk, eredmx), rmin, blk.eint, blk.cent, eredmx, irmset)
+                rstart, rturn = S.findrm(_lowest_on_device(eng, blk, eredmx), rmin, blk.eint, blk.cent, eredmx,
+                                         irmset)
             else:
-                rstart = rmin
-            nstep, h = S.drset_constant(rstart, rmax, dr)
+                rstart, rturn = rmin, rmin
+            drfit = math.pi / (S.calck(0.0, eredmx, blk.eint) * steps) if steps > 0.0 else dr
+            rend = rmax
+            if iprop_long:
+                rend = rmid if rmid > 0.0 else rvfac * rturn
+            nstep, h = S.drset_constant(rstart, rend, drfit)
             r = S.ld_radii(rstart, h, nstep)
             b = eng.block(blk.vl, blk.eint, blk.cent, mxlam=blk.mxlam)
             g = eng.grid(r, blk.ptab(r), h)
             wv, nops = zip(*[S.gtwvec(e, blk.eint) for e in ered])
             nb = S.chnsrt(blk.eint)
-            out = eng.smatrix_ld(b, g, iprop, ered, np.array(wv), np.array(nops), nb, rmax)
+            if not iprop_long:
+                out = eng.smatrix_ld(b, g, iprop, ered, np.array(wv), np.array(nops), nb, rmax)
+            else:
+                eng.set_potl(b, blk.potl, blk.rscale, blk.ep2ru)
+                y0 = eng.yinit(b, g, ered)
+                y, nodes_ld, st = eng.prop_ld(b, g, iprop, ered, y0)
+                y, st2 = _long_range(eng, b, blk, ered, y, iprop_long, rend, rmax, h, drfit, tolhi, powr)
+                _, sr, si, st3 = eng.ytok_ktos(y, np.array(wv), np.array(nops), nb, blk.cent, rmax)
+                status = np.where(st != 0, st, np.where(st2 != 0, st2, st3))
+                out = dict(SR=sr, SI=si, NODES=nodes_ld, STATUS=status)
             eng.destroy_block(b)
             eng.destroy_grid(g)
             for m in range(len(energies)):
@@ -100,7 +142,7 @@ def cross_sections(eng, jlevels: Sequence[int], be: float, lambdas: Sequence[int
                 acc[m] += S.partial_sigma(out["SR"][m], out["SI"][m], nb, nops[m], blk.levidx, jlevels, wv[m],
                                           jtot, blk.runit, nsig, wght=1.0)
             runs.append(BlockRun(jtot, ib, blk.n, rstart, nstep, h, list(nops), [int(x) for x in out["NODES"]],
-                                 [int(x) for x in out["STATUS"]]))
+                                 [int(x) for x in out["STATUS"]], sr=list(out["SR"]), si=list(out["SI"])))
     return CrossSections(energies=energies, sigma=acc * float(jstep), blocks=runs)
 
 

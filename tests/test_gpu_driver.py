@@ -77,3 +77,27 @@ def test_batched_energies(eng):
                 lhs = g[i] * (e - elev[i]) * sig[f, i]
                 rhs = g[f] * (e - elev[f]) * sig[i, f]
                 assert abs(lhs - rhs) <= 1e-10 * max(abs(lhs), abs(rhs))
+
+
+@pytest.mark.parametrize("iprops,ipropl,label,tol", [
+    (6, 9, "model system: ITYPE=1; IPROPS=6, IPROPL=9", 5e-10),
+    (5, 4, "model system: ITYPE=1; IPROPS=5, IPROPL=4", 1e-7),
+])
+def test_two_segment_runs_match_reference(eng, golden_dir, iprops, ipropl, label, tol):
+    """The all-propagators deck's two-segment runs through the driver: the
+    LD segment with STEPS = 12 to RMID = 1.4 x FINDRM's turning point, then
+    AIRY with the default TOLHIL = 1e-4 (POWRL = 3) or VIVS to 20 RM, at 1000
+    and 900 cm-1 (the second energy replaying the first's long-range steps);
+    the printed S-matrices (tolerances as for the propagators alone)."""
+    with open(os.path.join(golden_dir, "ref_all_propagators_smat.json")) as f:
+        sec = json.load(f)["sections"][label]
+    print(f"driver IPROPS={iprops} IPROPL={ipropl}", flush=True)
+    cs = D.cross_sections(eng, JLEV, 30.0, [0, 2], POT, 20.0, [1000.0, 900.0], 10, 10, 1, 0.5, 20.0,
+                          iprop=iprops, ibs=(1,), steps=12.0, iprop_long=ipropl, rvfac=1.4)
+    br = cs.blocks[0]
+    assert br.status == [0, 0] and br.nstep == sec["props"][0][3]
+    for m in range(2):
+        sm = sec["smat"][m]
+        d = max(np.abs(br.sr[m] - np.array(sm["sr"])).max(), np.abs(br.si[m] - np.array(sm["si"])).max())
+        print(f"  E={[1000, 900][m]} cm-1 max|dS| vs printed {d:.2e}", flush=True)
+        assert d < tol
