@@ -9,7 +9,8 @@ ABI:
   steps against the oracle's AIPROP from the same Z(RMID);
 - the replay modes: the first instance's run and record are the same alone
   and in a batch, and a replay of the record is bitwise the batch's
-  replaying instance."""
+  replaying instance;
+- inward propagation from RMAX (BOUND's outer segment) against the oracle."""
 import json
 import math
 import os
@@ -118,3 +119,22 @@ def test_airy_tolhi_replay_modes_bitwise(eng, deck):
     _, _, kshort, sshort, _ = eng.prop_airy_tolhi(b, ered[1:], z[1:], ld.rstop, 20.0, dr2, 3.0, 1e-4, replay=1,
                                                   steps=rec2[:10])
     assert np.all(sshort == 0) and np.all(kshort == 10)
+
+
+def test_airy_tolhi_inward_vs_oracle(eng, deck):
+    """Inward from RMAX, as BOUND propagates its outer segment: negative
+    steps (the growth factor's (1 + DRNOW/RNEXT)^3 term) and NODES = N -
+    KOUNT per step, at an energy below every threshold, against the
+    oracle's AIPROP on the same start: steps, node count and Z."""
+    sec, blk, ld, dr2 = deck
+    b = _upload(eng, blk)
+    ered = -3.0 * blk.cm2ru
+    z0 = O.yinit(blk, ered, 20.0, zout=False)
+    print(f"AIRY TOLHI=1e-4 inward 20 -> 1.2 RM, N={blk.n}", flush=True)
+    z, nodes, nstep, st, rec = eng.prop_airy_tolhi(b, [ered], z0[None], 20.0, 1.2, -0.05, 3.0, 1e-4, replay=0)
+    zo, no, steps = O.aiprop_tolhi(z0, blk, ered, 20.0, 1.2, -0.05, 3.0, 1e-4)
+    print(f"  steps {nstep[0]} (oracle {len(steps)}) nodes {nodes[0]} (oracle {no})", flush=True)
+    assert st[0] == 0 and nstep[0] == len(steps) and nodes[0] == no
+    dz = np.abs(z[0] - zo).max() / np.abs(zo).max()
+    print(f"  Z vs oracle rel {dz:.2e}", flush=True)
+    assert dz < 1e-9
