@@ -690,6 +690,8 @@ def maprop(y: np.ndarray, blk: Block, ered: float, rstart: float, nstep: int, dr
         y, ncount = syminv(y)
         if ncount > n:
             raise RuntimeError(f"MATRIX INVERSION ERROR IN MAPROP AT STEP K = {kstep}")
+        if dr < 0.0:  # RSTART > RSTOP (maprop.f:145)
+            ncount = n - ncount
         if ncheck == 0:
             nodes += ncount
         y = -(y2[:, None] * y * y2[None, :])
@@ -1405,10 +1407,12 @@ def bound_match(blk: Block, ered: float, g: BoundGrid, iprop_out: int = 6, iprop
     (DIAGVC, ascending), NEGCNT = #{eig <= 0}, EIGMIN = the eigenvalue of
     smallest modulus (the last such in ascending order), PSIMCH = its
     eigenvector with the largest component made positive, NODEC = nodes of
-    both propagations + NEGCNT (source_code/bdctrl.f:221-257).  Returns a
+    both propagations + NEGCNT (source_code/bdctrl.f:221-257).  iprop 6
+    (MDPROP), 5 (LDPROP) or 7 (MAPROP) in either direction.  Returns a
     dict with NODEC, EIGMIN, IMIN (0-based), PSIMCH, EVALS, M."""
     hm = lambda r: blk.hammat(r, ered)
-    props = {6: mdprop, 5: ldprop}
+    props = {6: mdprop, 5: ldprop,
+             7: lambda y, hm_, r0, ns, dr: maprop(y, blk, ered, r0, ns, dr)}
     y_out = yinit(blk, ered, g.rmin, zout=True)
     y_out, n_out = props[iprop_out](y_out, hm, g.rmin, g.nout, g.drout)
     y_in = yinit(blk, ered, g.rmax, zout=False)

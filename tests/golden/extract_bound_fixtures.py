@@ -3,8 +3,8 @@ propagators as a small JSON fixture.
 
 Reads examples/output/bound-all_propagators.output and the matching
 IBDSUM summary files (bound-all_propagators.ibdsum.606 / .505) under the
-reference tree and keeps only numbers: per deck (IPROPS=IPROPL=6 and
-IPROPS=IPROPL=5) and JTOT, the node counts at EMIN and EMAX, and for each
+reference tree and keeps only numbers: per deck (IPROPS=IPROPL=6, 5
+and 7) and JTOT, the node counts at EMIN and EMAX, and for each
 converged state its number, its energy (12 digits from the summary file),
 the smallest matching-matrix eigenvalue printed at that energy, its index,
 and the principal eigenvector components.  Run from the repository root:
@@ -17,7 +17,8 @@ import re
 import sys
 
 DECKS = {6: ("model system: ITYPE=1; IPROPS=IPROPL=6", "bound-all_propagators.ibdsum.606"),
-         5: ("model system: ITYPE=1; IPROPS=IPROPL=5", "bound-all_propagators.ibdsum.505")}
+         5: ("model system: ITYPE=1; IPROPS=IPROPL=5", "bound-all_propagators.ibdsum.505"),
+         7: ("model system: ITYPE=1; IPROPS=IPROPL=7", "bound-all_propagators.ibdsum.707")}
 
 
 def parse_section(body):
@@ -65,7 +66,7 @@ def main(ref):
                 k += 1
         result[str(iprop)] = {"label": label, "blocks": recs}
     with open(os.path.join(outdir, "ref_bound_all_propagators.json"), "w") as f:
-        json.dump({"source": "examples/output/bound-all_propagators.output (+ .ibdsum.606, .ibdsum.505)",
+        json.dump({"source": "examples/output/bound-all_propagators.output (+ .ibdsum.606, .ibdsum.505, .ibdsum.707)",
                    "deck": "examples/input/bound-all_propagators.input",
                    "decks": result}, f, indent=1)
     print("wrote", sum(len(v["blocks"]) for v in result.values()), "JTOT blocks")

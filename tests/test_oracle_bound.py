@@ -1,12 +1,14 @@
 """Pins the oracle's BOUND matching (BDCTRL: outward and inward
 log-derivative propagation, matching matrix, NEGCNT/EIGMIN/PSIMCH, node
 count) against the reference's own example results,
-examples/output/bound-all_propagators.output and its .ibdsum.606/.505
+examples/output/bound-all_propagators.output and its .ibdsum.606/.505/.707
 summaries (written as JSON by tests/golden/extract_bound_fixtures.py).
 
 System: examples/input/bound-all_propagators.input, ITYPE=1 rotor (BE=30,
 j=0,2,4,6), LJ 12-6 + P2 term, URED=20, IB=2, RMIN=0.5, RMATCH=1.0,
-RMAX=20 (units of RM), DR=0.001 (DRL=0.005 for the MDPROP deck).
+RMAX=20 (units of RM), DR=0.001 (DRL=0.005 for the MDPROP deck); the
+decks propagate with MDPROP (6/6), LDPROP (5/5) and MAPROP (7/7) in both
+directions.
 
 What is pinned: the integer node counts at EMIN and EMAX (bit-exact), the
 reference's converged level energies (12 digits) as zeros of EIGMIN(E) to
@@ -21,7 +23,7 @@ import pytest
 from oracle import molscat_oracle as O
 
 POT = O.PowerPotential([[(1.0, -12), (-2.0, -6)], [(-0.2, -6)]], rm=3.5, epsil=50.0)
-CASES = [(6, 0), (6, 1), (6, 2), (5, 0)]
+CASES = [(6, 0), (6, 1), (6, 2), (5, 0), (7, 0)]
 
 
 @pytest.fixture(scope="module")
