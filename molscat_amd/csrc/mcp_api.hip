@@ -26,6 +26,8 @@
 namespace {
 
 using mcp::kNT;
+// threads per workgroup of the batched one-workgroup-per-matrix inversions
+constexpr int kInvNT = 512;
 
 struct DevBuf {
   void* p = nullptr;
@@ -197,11 +199,11 @@ size_t lds_large(int n, int nham, int nvec) {
   // same pieces, in the same order, as mcp::carve_large
   const size_t ld = mcp::panel_ld(n);
   z.add<double>(mcp::kPB * ld).add<double>(mcp::kPB * ld).add<double>(mcp::kPB * mcp::kPB);
-  z.add<double>(mcp::kPB * mcp::kPB).add<double>(mcp::kPB * mcp::kPB).add<double>((kNT / 64) * mcp::kPB);
+  z.add<double>(mcp::kPB * mcp::kPB).add<double>(mcp::kPB * mcp::kPB).add<double>(mcp::kMaxW * mcp::kPB);
   z.add<double>(n).add<double>(n).add<double>(2 * mcp::kPB).add<double>(mcp::kPB);
   z.add<int>(n).add<int>(n).add<int>(n).add<int>(mcp::kPB);
   z.add<int>(mcp::kPB).add<int>(mcp::kPB).add<int>(8);
-  z.add<double>(kNT / 64).add<int>(kNT / 64);
+  z.add<double>(mcp::kMaxW).add<int>(mcp::kMaxW);
   if (nham > 0) z.add<double>(nham);
   for (int v = 0; v < nvec; ++v) z.add<double>(n);
   return z.b;
@@ -277,7 +279,7 @@ int set_lds_limits(mcp_ctx* c) {
   const void* fns[] = {(const void*)mcp::k_syminv,         (const void*)mcp::k_mdprop_uc,
                        (const void*)mcp::k_mdprop_lds<kNT>, (const void*)mcp::k_mdprop_lds<64>,     (const void*)mcp::k_ldprop_lds,
                        (const void*)mcp::k_eigh,           (const void*)mcp::k_bound_match,           (const void*)mcp::k_yinit,
-                       (const void*)mcp::k_syminv_large,   (const void*)mcp::k_mdprop_uc_large,
+                       (const void*)mcp::k_syminv_large<kInvNT>,   (const void*)mcp::k_mdprop_uc_large,
                        (const void*)mcp::k_mdprop_large,   (const void*)mcp::k_ldprop_large,
                        (const void*)mcp::k_yinit_large,    (const void*)mcp::k_airy_sector,
                        (const void*)mcp::k_airy_prop,      (const void*)mcp::k_mdprop_mfma<1>, (const void*)mcp::k_mdprop_mfma<2>,
@@ -455,7 +457,7 @@ constexpr int kTreeTeamMax = 64;
 bool tree_on_teams(mcp_ctx* c, int n, int nb) { return nb <= kTreeTeamMax && !c->no_reg && team_inverse_fits(c, n); }
 int tree_invert(mcp_ctx* c, int n, int nb, double* zb, int* kc, int bk) {
   if (tree_on_teams(c, n, nb)) return team_inverse(c, n, nb, zb, kc, bk);
-  hipLaunchKernelGGL(mcp::k_syminv_large, dim3(nb), dim3(kNT), lds_large(n, 0, 0), c->stream, n, zb, kc, bk);
+  hipLaunchKernelGGL(mcp::k_syminv_large<kInvNT>, dim3(nb), dim3(kInvNT), lds_large(n, 0, 0), c->stream, n, zb, kc, bk);
   HIPCHK(c, hipGetLastError());
   return MCP_OK;
 }
@@ -950,7 +952,7 @@ int airy_prop_global(mcp_ctx* c, Block& b, Grid& g, int ninst, const double* d_e
     if (team) {
       if ((rc = team_inverse(c, n, ninst, am, akc, bk))) return rc;
     } else {
-      hipLaunchKernelGGL(mcp::k_syminv_large, dim3(ninst), dim3(kNT), ll, c->stream, n, am, akc, bk);
+      hipLaunchKernelGGL(mcp::k_syminv_large<kInvNT>, dim3(ninst), dim3(kInvNT), ll, c->stream, n, am, akc, bk);
       HIPCHK(c, hipGetLastError());
     }
     hipLaunchKernelGGL(mcp::k_airy_z, dim3(ne), dim3(256), 0, c->stream, n, ninst, k, nstep, am, akc, ay, dy, d_nodes,
@@ -1083,7 +1085,7 @@ int mcp_syminv_batched(mcp_ctx* c, int32_t n, int32_t nmat, const double* a, dou
     if (ll > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "syminv: N=%d exceeds the large-N panel limit", n);
     if ((rc = upload(c, c->s_b, a, cnt))) return rc;
     HIPCHK(c, c->s_c.ensure((size_t)nmat * sizeof(int)));
-    hipLaunchKernelGGL(mcp::k_syminv_large, dim3(nmat), dim3(kNT), ll, c->stream, n, c->s_b.as<double>(),
+    hipLaunchKernelGGL(mcp::k_syminv_large<kInvNT>, dim3(nmat), dim3(kInvNT), ll, c->stream, n, c->s_b.as<double>(),
                        c->s_c.as<int>(), n > 30 ? 1 : 0);
     HIPCHK(c, hipGetLastError());
     if ((rc = download(c, ainv, c->s_b, cnt))) return rc;

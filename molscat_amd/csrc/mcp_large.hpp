@@ -37,6 +37,7 @@ __device__ __forceinline__ dvec4 mma(double a, double b, dvec4 c) {
 }
 
 constexpr int kPB = 16;    // pivots per panel
+constexpr int kMaxW = 8;   // wavefronts per workgroup the scratch is laid out for (512 threads)
 constexpr int kTile = 64;  // ownership / update tile edge
 
 __device__ __forceinline__ void wave_sync() {
@@ -130,14 +131,14 @@ __device__ __forceinline__ bool owns(const Team& t, int r, int c) {
 
 // f(i, j) for every lower element (i >= j) of this workgroup's tiles; within
 // a tile consecutive threads take consecutive rows of one column
-template <class F>
+template <int NT = kNT, class F>
 __device__ __forceinline__ void for_owned_lower(const Team& t, int n, F&& f) {
   const int nt = (n + kTile - 1) / kTile;
   const int ntt = nt * (nt + 1) / 2;
   for (int tau = t.rank; tau < ntt; tau += t.g) {
     const int bi = tile_row(tau);
     const int bj = tau - bi * (bi + 1) / 2;
-    for (int e = threadIdx.x; e < kTile * kTile; e += kNT) {
+    for (int e = threadIdx.x; e < kTile * kTile; e += NT) {
       const int i = bi * kTile + (e & (kTile - 1));
       const int j = bj * kTile + e / kTile;
       if (i < n && j < n && i >= j) f(i, j);
@@ -155,7 +156,7 @@ struct LargeScratch {
   double* Dinv;  // [kPB*kPB] inverse of the panel pivot block
   double* Lb;    // [kPB*kPB] LDL^T of the candidate block (D on the diagonal)
   double* Li;    // [kPB*kPB] L^-1
-  double* cmax;  // [kNT/64][kPB] per-wave partial column maxima
+  double* cmax;  // [kMaxW][kPB] per-wave partial column maxima
   double* mc;    // [n] current column of the candidate
   double* mci;   // [n] current column of the row candidate
   double* wv;    // [2*kPB]
@@ -179,7 +180,7 @@ __device__ inline LargeScratch carve_large(LdsCarve& c, int n) {
   s.Dinv = c.take<double>(kPB * kPB);
   s.Lb = c.take<double>(kPB * kPB);
   s.Li = c.take<double>(kPB * kPB);
-  s.cmax = c.take<double>((kNT / 64) * kPB);
+  s.cmax = c.take<double>(kMaxW * kPB);
   s.mc = c.take<double>(n);
   s.mci = c.take<double>(n);
   s.wv = c.take<double>(2 * kPB);
@@ -191,8 +192,8 @@ __device__ inline LargeScratch carve_large(LdsCarve& c, int n) {
   s.pslot = c.take<int>(kPB);
   s.slotpos = c.take<int>(kPB);
   s.ctl = c.take<int>(8);
-  s.sv = c.take<double>(kNT / 64);
-  s.si = c.take<int>(kNT / 64);
+  s.sv = c.take<double>(kMaxW);
+  s.si = c.take<int>(kMaxW);
   return s;
 }
 
@@ -212,6 +213,7 @@ __device__ __forceinline__ double bcast(double x, int src) {
 // number np of accepted pivots (0: use the pivot-by-pivot path).  On np > 0:
 // X holds A(:,P) S^-1 (rows outside P), Dinv holds S^-1, pidx/pslot/inP the
 // pivots, ctl[1] the inertia of the accepted pivots.
+template <int NT = kNT>
 __device__ __attribute__((always_inline)) int block_panel(int n, int kstart, int npre, bool bk,
                                                           const LargeScratch& s) {
   const int tid = threadIdx.x;
@@ -272,7 +274,7 @@ __device__ __attribute__((always_inline)) int block_panel(int n, int kstart, int
   double cm[kPB];
 #pragma unroll
   for (int k = 0; k < kPB; ++k) cm[k] = 0.0;
-  for (int i = tid; i < n; i += kNT) {
+  for (int i = tid; i < n; i += NT) {
     const int rk = s.swept[i] ? 0 : (s.inP[i] ? s.inP[i] - 1 : kPB);
     double g[kPB];
 #pragma unroll
@@ -299,7 +301,7 @@ __device__ __attribute__((always_inline)) int block_panel(int n, int kstart, int
     if (lane < f) {
       double colmax = 0.0;
 #pragma unroll
-      for (int wq = 0; wq < kNT / 64; ++wq) colmax = fmax(colmax, s.cmax[wq * kPB + lane]);
+      for (int wq = 0; wq < NT / 64; ++wq) colmax = fmax(colmax, s.cmax[wq * kPB + lane]);
       d = s.Lb[lane + lane * kPB];
       pass = !bk || (fabs(d) >= kBkAlpha * colmax);
       neg = d < 0.0;
@@ -332,7 +334,7 @@ __device__ __attribute__((always_inline)) int block_panel(int n, int kstart, int
   const int np = s.ctl[0];
   if (np == 0) return 0;
   // X = A(:,P) S^-1 for the rows outside P (S^-1 is zero beyond np)
-  for (int i = tid; i < n; i += kNT) {
+  for (int i = tid; i < n; i += NT) {
     if (s.inP[i]) continue;
     double g[kPB];
 #pragma unroll
@@ -563,6 +565,7 @@ __device__ __attribute__((always_inline)) void wave_panel(const double* A, int n
 // A(i,j) -= sum_q X(i,q) A(p_q,j) for i, j outside the panel (64 x 64 tiles on
 // the matrix cores), then the panel rows/columns
 // A(i,p_q) = X(i,q) and the pivot block -Dinv.  X rows beyond np are zero.
+template <int NT = kNT>
 __device__ __attribute__((always_inline)) void apply_panel(double* A, int n, int np, const LargeScratch& s,
                                                            const Team& t) {
   if (np == 0) return;
@@ -573,7 +576,11 @@ __device__ __attribute__((always_inline)) void apply_panel(double* A, int n, int
   // is A(row0 + l%16, col0 + l/16 + 4v) and loads and stores run down
   // columns.  Tile values are fetched two tiles ahead of the products (one
   // workgroup per CU: the load latency has no other wave to hide behind).
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // NT = 512 (one workgroup per matrix only): the two 4-wave halves take
+  // alternate tiles
+  static_assert(NT == 256 || NT == 512, "apply_panel: 4 or 8 wavefronts");
+  constexpr int kSub = NT / 256;
+  const int lane = threadIdx.x & 63, w = (threadIdx.x >> 6) & 3, sub = threadIdx.x >> 8;
   const int q4 = lane >> 4, r16 = lane & 15;
   const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
   const int nt = (n + kTile - 1) / kTile;
@@ -605,20 +612,24 @@ __device__ __attribute__((always_inline)) void apply_panel(double* A, int n, int
         }
   };
   // two tiles ahead: av the current tile, an the next, the one after that
-  // fetched at the top of each iteration
-  int tau = t.rank;
+  // fetched at the top of each iteration.  With 8 wavefronts (two per SIMD)
+  // one tile ahead: the other wavefront covers the latency, and the
+  // registers of a third tile would not fit the halved register budget.
+  constexpr bool kDeep = kSub == 1;
+  const int step = t.g * kSub;
+  int tau = t.rank * kSub + sub;
   dvec4 av[2][2], an[2][2];
   int r0 = 0, c0 = 0, rn = 0, cn = 0;
   if (tau < ntt) {
     tile_at(tau, r0, c0);
     fetch(r0, c0, av);
   }
-  if (tau + t.g < ntt) {
-    tile_at(tau + t.g, rn, cn);
+  if (kDeep && tau + step < ntt) {
+    tile_at(tau + step, rn, cn);
     fetch(rn, cn, an);
   }
   while (tau < ntt) {
-    const int tn2 = tau + 2 * t.g;
+    const int tn2 = tau + (kDeep ? 2 : 1) * step;
     int r2 = 0, c2 = 0;
     dvec4 a2[2][2];
     if (tn2 < ntt) {
@@ -652,28 +663,37 @@ __device__ __attribute__((always_inline)) void apply_panel(double* A, int n, int
           const int j = c0 + wn + 16 * b + q4 + 4 * v;
           if (i < n && j < n && i >= j && !s.inP[i] && !s.inP[j]) A[i + (size_t)j * n] = av[a][b][v] - acc[a][b][v];
         }
-    tau += t.g;
-    r0 = rn;
-    c0 = cn;
-    rn = r2;
-    cn = c2;
+    tau += step;
+    if constexpr (kDeep) {
+      r0 = rn;
+      c0 = cn;
+      rn = r2;
+      cn = c2;
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+      for (int a = 0; a < 2; ++a)
 #pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        av[a][b] = an[a][b];
-        an[a][b] = a2[a][b];
-      }
+        for (int b = 0; b < 2; ++b) {
+          av[a][b] = an[a][b];
+          an[a][b] = a2[a][b];
+        }
+    } else {
+      r0 = r2;
+      c0 = c2;
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) av[a][b] = a2[a][b];
+    }
   }
   for (int q = 0; q < np; ++q) {
     const int p = s.pidx[q];
-    for (int i = threadIdx.x; i < n; i += kNT) {
+    for (int i = threadIdx.x; i < n; i += NT) {
       if (s.inP[i]) continue;
       const int lo = i < p ? i : p, hi = i < p ? p : i;
       if (owns(t, hi, lo)) A[hi + (size_t)lo * n] = s.X[q * ld + i];
     }
   }
-  for (int e = threadIdx.x; e < np * np; e += kNT) {
+  for (int e = threadIdx.x; e < np * np; e += NT) {
     const int a = e % np, b = e / np;
     const int pa = s.pidx[a], pb = s.pidx[b];
     if (pa >= pb && owns(t, pa, pb)) A[pa + (size_t)pb * n] = -s.Dinv[a + b * kPB];
@@ -686,9 +706,10 @@ __device__ __attribute__((always_inline)) void apply_panel(double* A, int n, int
 // its next pass).  Returns the inertia (as sweep_inv), -(k+1) for an exactly
 // zero candidate column, n+1 for an exactly zero pivot, or n+2 when a team
 // barrier gave up.  Must be entered after a team barrier.
+template <int NT = kNT>
 __device__ __attribute__((always_inline)) int sweep_neginv_lower(double* A, int n, bool bk, const LargeScratch& s, Team& tm) {
   const int tid = threadIdx.x;
-  for (int i = tid; i < n; i += kNT) {
+  for (int i = tid; i < n; i += NT) {
     s.perm[i] = i;
     s.inP[i] = 0;
     s.swept[i] = 0;
@@ -703,12 +724,12 @@ __device__ __attribute__((always_inline)) int sweep_neginv_lower(double* A, int 
     // in i) and the row segment A(p, i < p) (coalesced in p for a natural run)
     constexpr int kU = 8;  // loads in flight per thread
     const int ld = s.ld;
-    for (int base = tid; base < kPB * ld; base += kU * kNT) {
+    for (int base = tid; base < kPB * ld; base += kU * NT) {
       double v[kU];
       bool st[kU];
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
-        const int e = base + u * kNT;
+        const int e = base + u * NT;
         const int sl = e / ld, i = e - sl * ld;
         const int p = (e < kPB * ld && sl < npre) ? s.perm[kstart + sl] : 0;
         st[u] = e < kPB * ld && (sl >= npre || i >= n || i >= p);
@@ -716,14 +737,14 @@ __device__ __attribute__((always_inline)) int sweep_neginv_lower(double* A, int 
       }
 #pragma unroll
       for (int u = 0; u < kU; ++u)
-        if (st[u]) s.G[base + u * kNT] = v[u];
+        if (st[u]) s.G[base + u * NT] = v[u];
     }
-    for (int base = tid; base < npre * n; base += kU * kNT) {
+    for (int base = tid; base < npre * n; base += kU * NT) {
       double v[kU];
       int dst[kU];
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
-        const int e = base + u * kNT;
+        const int e = base + u * NT;
         const int i = e / npre, sl = e - i * npre;
         const int p = e < npre * n ? s.perm[kstart + sl] : 0;
         dst[u] = (e < npre * n && i < p) ? sl * ld + i : -1;
@@ -735,9 +756,9 @@ __device__ __attribute__((always_inline)) int sweep_neginv_lower(double* A, int 
     }
     if (tid < kPB) s.slotpos[tid] = tid;
     __syncthreads();
-    int np = block_panel(n, kstart, npre, bk, s);
+    int np = block_panel<NT>(n, kstart, npre, bk, s);
     if (np == 0) {
-      for (int e = tid; e < kPB * ld; e += kNT) s.X[e] = 0.0;
+      for (int e = tid; e < kPB * ld; e += NT) s.X[e] = 0.0;
       __syncthreads();
       if (tid < 64) wave_panel(A, n, kstart, bk, s);
       __syncthreads();
@@ -747,7 +768,7 @@ __device__ __attribute__((always_inline)) int sweep_neginv_lower(double* A, int 
     const int err = s.ctl[2];
     if (err != 0) return err;
     k = s.ctl[3];
-    apply_panel(A, n, np, s, tm);
+    apply_panel<NT>(A, n, np, s, tm);
     __syncthreads();
     if (tid < np) {
       s.inP[s.pidx[tid]] = 0;
@@ -777,17 +798,20 @@ __global__ void k_zero_u32(unsigned* p, int n) {
   if (i < n) p[i] = 0u;
 }
 
-__global__ void __launch_bounds__(kNT) k_syminv_large(int n, double* __restrict__ a, int* __restrict__ kount, int bk) {
+// one workgroup of NT threads per matrix; NT = 512 puts two wavefronts on
+// each SIMD of the CU the panels' LDS confines it to
+template <int NT>
+__global__ void __launch_bounds__(NT) k_syminv_large(int n, double* __restrict__ a, int* __restrict__ kount, int bk) {
   extern __shared__ double smem[];
   LdsCarve c(smem);
   LargeScratch s = carve_large(c, n);
   Team tm = make_team(1, nullptr, 0, nullptr);
   double* A = a + (size_t)blockIdx.x * n * n;
-  const int k = sweep_neginv_lower(A, n, bk != 0, s, tm);
+  const int k = sweep_neginv_lower<NT>(A, n, bk != 0, s, tm);
   // full symmetric inverse out (SYMINL hands back the lower triangle)
-  for_owned_lower(tm, n, [&](int i, int j) { A[i + (size_t)j * n] = -A[i + (size_t)j * n]; });
+  for_owned_lower<NT>(tm, n, [&](int i, int j) { A[i + (size_t)j * n] = -A[i + (size_t)j * n]; });
   __syncthreads();
-  for_owned_lower(tm, n, [&](int i, int j) {
+  for_owned_lower<NT>(tm, n, [&](int i, int j) {
     if (i != j) A[j + (size_t)i * n] = A[i + (size_t)j * n];
   });
   if (threadIdx.x == 0) kount[blockIdx.x] = k;
