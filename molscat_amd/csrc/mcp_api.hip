@@ -452,8 +452,10 @@ int tree_gemm(mcp_ctx* c, int n, int nbatch, int per, mcp::SegRef A, mcp::SegRef
 
 // Symmetric inversions of one tree level: nb matrices at zb, inertias to kc.
 // Small batches, where one workgroup per matrix leaves the device idle, go
-// to workgroup teams (unless a negative team size was set).
-constexpr int kTreeTeamMax = 64;
+// to workgroup teams (unless a negative team size was set).  At N=441 a
+// team launch inverts 9 matrices in ~0.5 ms and a batched launch of up to
+// 256 matrices takes one inversion's ~2.5 ms: teams win up to 45.
+constexpr int kTreeTeamMax = 45;
 bool tree_on_teams(mcp_ctx* c, int n, int nb) { return nb <= kTreeTeamMax && !c->no_reg && team_inverse_fits(c, n); }
 int tree_invert(mcp_ctx* c, int n, int nb, double* zb, int* kc, int bk) {
   if (tree_on_teams(c, n, nb)) return team_inverse(c, n, nb, zb, kc, bk);
