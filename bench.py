@@ -35,7 +35,7 @@ PEAK_FP64 = 78.6e12  # MI355X dense FP64 (vector and matrix), FLOP/s
 # stage's dispatches, each scaled by the counter's reading for a plain 256 MiB
 # copy), measured by the counter passes of profiles/rotor_N441_ld_stage_traffic.txt;
 # it changes only when the stage's kernels do
-LD_STAGE_TRAFFIC_BYTES = 565.769e9
+LD_STAGE_TRAFFIC_BYTES = 567.264e9
 
 
 def _cpu_baseline(blk, seg, ered, nsamp: int) -> dict:
