@@ -241,8 +241,10 @@ int mcp_smatrix_segments(mcp_ctx *ctx, int32_t block_id, int32_t nseg, const int
  * fields, through pconst) of one block, two log-derivative segments as
  * BDPSET lays them out for RMID = RMATCH (source_code/bdpset.f:26-42):
  * YINIT at the first point of grid_out and propagation outward (DR > 0,
- * iprop_out 6 = MDPROP / 5 = LDPROP) to RMATCH; YINIT with RSTART > RSTOP
- * at the first point of grid_in and propagation inward (DR < 0) to RMATCH;
+ * iprop_out 6 = MDPROP / 5 = LDPROP / 7 = MAPROP, as in mcp_prop_ld) to
+ * RMATCH; YINIT with RSTART > RSTOP at the first point of grid_in and
+ * propagation inward (DR < 0, iprop_in likewise; MAPROP counts N - KOUNT
+ * nodes per sector inward, source_code/maprop.f:145) to RMATCH;
  * matching matrix M = Y_out - Y_in and DIAGVC.  Per instance: nodec = NODEC
  * (the nodes of both propagations plus NEGCNT = #{eig(M) <= 0}), eigmin =
  * EIGMIN (the eigenvalue of smallest modulus), imin = its 0-based index in
