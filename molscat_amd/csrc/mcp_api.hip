@@ -1767,6 +1767,7 @@ int mcp_prop_vivs(mcp_ctx* c, int32_t bid, int32_t ninst, const double* ered, do
   A.tldiag = tldiag;
   A.toff = toff;
   A.max_steps = max_steps;
+  A.nrec = replay == 1 ? max_steps : 0;
   A.steps = c->s_vs.as<double>();
   A.ered = c->s_ve.as<double>();
   A.rmat = c->s_vr.as<double>();
@@ -1785,6 +1786,16 @@ int mcp_prop_vivs(mcp_ctx* c, int32_t bid, int32_t ninst, const double* ered, do
   hipLaunchKernelGGL(mcp::k_vivs, dim3(first), dim3(kNT), lds, c->stream, B, P, A);
   HIPCHK(c, hipGetLastError());
   if (first < ninst) {
+    if (replay == 2) {
+      // the record holds instance 0's steps only if it finished; a failed
+      // recording leaves nrec = 0, so the followers stop with kVivsReplayShort
+      // instead of reading steps left in the buffer by an earlier call
+      int n0 = 0, st0 = 0;
+      HIPCHK(c, hipMemcpyAsync(&n0, c->s_vn.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipMemcpyAsync(&st0, c->s_vst.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      A.nrec = st0 == 0 ? std::min(n0, (int)max_steps) : 0;
+    }
     A.inst0 = first;
     A.replay = replay == 0 ? 0 : 1;
     A.record = 0;

@@ -65,6 +65,7 @@ struct VivsArgs {
   int replay;            // 0: own step control; 1: steps from the record
   int record;            // own control: write the steps to the record (one instance)
   int max_steps;         // record capacity in steps
+  int nrec;              // replay: steps the record holds (0 when the recording instance failed)
   double* steps;         // [max_steps][4]
   const double* ered;    // [ninst]
   double* rmat;          // [ninst][n*n] R(RSTART) in, R(RSTOP) out
@@ -416,7 +417,7 @@ __global__ void __launch_bounds__(kNT) k_vivs(BlockArgs B, PotlArgs P, VivsArgs 
   __syncthreads();
   while (true) {
     if (!own && tid == 0) {
-      if (st.k >= A.max_steps) {
+      if (st.k >= A.nrec) {
         st.status = kVivsReplayShort;
         st.stop = 1;
       } else {

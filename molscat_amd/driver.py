@@ -74,9 +74,14 @@ def _long_range(eng, b, blk, ered, y, iprop_long, rmid, rmax, dr_ld, drfit, tolh
         z, _, _, st, _ = eng.prop_airy_tolhi(b, ered, y, rmid, rmax, drfit, powr, tolhi, replay=2)
         return z, st
     tl = 0.064 * math.sqrt(tolhi / 0.001)
-    rm, _ = eng.syminv(y)
+    n = y.shape[-1]
+    rm, k1 = eng.syminv(y)
     rr, _, st, _ = eng.prop_vivs(b, ered, rm, rmid, rmax, dr_ld, tl, tl, replay=2)
-    yv, _ = eng.syminv(rr)
+    yv, k2 = eng.syminv(rr)
+    # SYMINV's failure codes (IFAIL = N+1 for a zero pivot, negative for
+    # LAPACK's INFO): the instance fails like a failed propagation
+    bad = (np.asarray(k1) > n) | (np.asarray(k1) < 0) | (np.asarray(k2) > n) | (np.asarray(k2) < 0)
+    st = np.where((st == 0) & bad, -1, st)
     return yv, st
 
 
@@ -136,8 +141,15 @@ def cross_sections(eng, jlevels: Sequence[int], be: float, lambdas: Sequence[int
                 out = dict(SR=sr, SI=si, NODES=nodes_ld, STATUS=status)
             eng.destroy_block(b)
             eng.destroy_grid(g)
+            failed = [m for m in range(len(energies)) if out["STATUS"][m] != 0 and nops[m] > 0]
+            if failed:
+                # the reference stops the run on a failed propagation
+                # (source_code/mdprop.f:282-287); so does the driver
+                raise RuntimeError(f"cross_sections: JTOT={jtot} IB={ib}: propagation failed at energies "
+                                   f"{[float(energies[m]) for m in failed]} (status "
+                                   f"{[int(out['STATUS'][m]) for m in failed]})")
             for m in range(len(energies)):
-                if out["STATUS"][m] != 0 or nops[m] == 0:
+                if nops[m] == 0:
                     continue
                 acc[m] += S.partial_sigma(out["SR"][m], out["SI"][m], nb, nops[m], blk.levidx, jlevels, wv[m],
                                           jtot, blk.runit, nsig, wght=1.0)

@@ -166,3 +166,27 @@ def test_vivs_beyond_lds_vs_oracle(eng):
     # oracle alone moves by 4.6e-10 when its DIAGVC switches from LAPACK's
     # dsyevd to dsyevr (same steps); the device's Jacobi bases measured 1.7e-9
     assert d < 5e-9
+
+
+def test_vivs_replay_after_failed_recording(eng, deck):
+    """replay = 2 when the recording instance stops early (here: more steps
+    than the record holds, status 2): the followers must not follow a record
+    that was never finished, nor one an earlier call left in the buffer.
+    They stop at once with status 4 (the record ends before its last step)
+    and take no step."""
+    sec, blk, ld = deck
+    b = _upload(eng, blk)
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((3, blk.n, blk.n))
+    rmat = 0.05 * (a + np.transpose(a, (0, 2, 1)))
+    ered = np.array([1000.0, 900.0, 950.0]) * blk.cm2ru
+    tl, to = O.vivs_tolerances(1e-4)
+    print("VIVS replay after a failed recording", flush=True)
+    # a complete run first, so the device's step buffer holds a valid record
+    _, n_ok, s_ok, _ = eng.prop_vivs(b, ered, rmat, 1.1, 12.0, 0.003, tl, to, replay=2)
+    assert np.all(s_ok == 0) and n_ok[0] > 10
+    _, n_bad, s_bad, _ = eng.prop_vivs(b, ered, rmat, 1.1, 12.0, 0.003, tl, to, replay=2, max_steps=10)
+    print(f"  statuses {list(s_bad)} steps {list(n_bad)}", flush=True)
+    assert s_bad[0] == 2
+    assert list(s_bad[1:]) == [4, 4]
+    assert list(n_bad[1:]) == [0, 0]
