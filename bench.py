@@ -282,6 +282,8 @@ def main() -> int:
     ap.add_argument("--team", type=int, default=0, help="workgroups per instance on the large-N path (0: auto)")
     ap.add_argument("--plain-launch", action="store_true",
                     help="ordinary launches for the team kernels (for profilers that cannot trace cooperative ones)")
+    ap.add_argument("--inverse-kernel", type=int, default=1,
+                    help="batched large-N inversions: 1 32-pivot passes (default), 0 16-pivot passes")
     ap.add_argument("--workload", choices=["rotor441", "allprop", "field", "copy"], default="rotor441",
                     help="rotor441: the N=441 deck (default); allprop: every propagator on the N=12 deck; "
                          "field: the N=48 field scan")
@@ -317,6 +319,7 @@ def main() -> int:
     nairy = len(wa)
 
     eng = Engine(0)
+    eng.set_inverse_kernel(args.inverse_kernel)
     eng.set_team_size(args.team)
     if args.plain_launch:
         eng.set_team_launch(False)

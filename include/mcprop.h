@@ -284,6 +284,12 @@ int mcp_set_team_launch(mcp_ctx *ctx, int32_t cooperative);
  * No reference counterpart. */
 int mcp_set_large_mode(mcp_ctx *ctx, int32_t mode);
 
+/* Batched inversions of the large-N paths (N beyond the LDS-resident
+ * sizes): 1 (default) passes of 32 pivots for N <= 448, one workgroup per
+ * matrix; 0 passes of 16 pivots.  Same inverse to rounding, same inertia.
+ * No reference counterpart (SYMINV's blocking is LAPACK's). */
+int mcp_set_inverse_kernel(mcp_ctx *ctx, int32_t mode);
+
 /* Threads per instance of the LDS-resident MDPROP kernel: 64 (one wave) or
  * 256 (four waves, also what 0, the default, gives).  The results do not
  * depend on it (same arithmetic per element, same pivots).  No reference

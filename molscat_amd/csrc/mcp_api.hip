@@ -13,6 +13,7 @@
 #include "../../include/mcprop.h"
 #include "mcp_kernels.hpp"
 #include "mcp_large.hpp"
+#include "mcp_wide.hpp"
 #include "mcp_airyprop.hpp"
 #include "mcp_mfmaprop.hpp"
 #include "mcp_tree.hpp"
@@ -209,6 +210,17 @@ size_t lds_large(int n, int nham, int nvec) {
   return z.b;
 }
 
+// same pieces, in the same order, as mcp::carve_wide
+size_t lds_wide(int n) {
+  LdsSize z;
+  const size_t ld = mcp::panel_ld(n);
+  z.add<double>(mcp::kWP * ld).add<double>(mcp::kWP * mcp::kWP);
+  z.add<double>(mcp::kWP).add<double>(mcp::kWW * mcp::kWP);
+  z.add<int>(n).add<int>(n).add<int>(n).add<int>(n).add<int>(mcp::kWP).add<int>(8);
+  z.add<double>(mcp::kWW).add<int>(mcp::kWW);
+  return z.b;
+}
+
 }  // namespace
 
 struct mcp_ctx {
@@ -230,6 +242,7 @@ struct mcp_ctx {
   bool eig_global = false;  // DIAGVC and AIRY on the global-memory eigen-solver for every N
   bool coop = true;         // team kernels through the cooperative launch API
   int lds_threads = 0;      // threads per instance of the LDS-resident MDPROP (0: 256)
+  int inv_kernel = 1;       // batched large-N inversions: 1 32-pivot passes (N <= 448), 0 16-pivot passes
   DevBuf s_u;  // LDPROP sector-matrix workspace (large N)
   DevBuf s_av, s_at, s_ae, s_ah;  // AIRY step bases, basis changes, eigenvalues, slopes
   DevBuf s_x;                     // column exchange of the register-resident MDPROP
@@ -279,7 +292,7 @@ int set_lds_limits(mcp_ctx* c) {
   const void* fns[] = {(const void*)mcp::k_syminv,         (const void*)mcp::k_mdprop_uc,
                        (const void*)mcp::k_mdprop_lds<kNT>, (const void*)mcp::k_mdprop_lds<64>,     (const void*)mcp::k_ldprop_lds,
                        (const void*)mcp::k_eigh,           (const void*)mcp::k_bound_match,           (const void*)mcp::k_yinit,
-                       (const void*)mcp::k_syminv_large<kInvNT>,   (const void*)mcp::k_mdprop_uc_large,
+                       (const void*)mcp::k_syminv_large<kInvNT>,   (const void*)mcp::k_mdprop_uc_large, (const void*)mcp::k_syminv_wide,
                        (const void*)mcp::k_mdprop_large,   (const void*)mcp::k_ldprop_large,
                        (const void*)mcp::k_yinit_large,    (const void*)mcp::k_airy_sector,
                        (const void*)mcp::k_airy_prop,      (const void*)mcp::k_mdprop_mfma<1>, (const void*)mcp::k_mdprop_mfma<2>,
@@ -457,11 +470,26 @@ int tree_gemm(mcp_ctx* c, int n, int nbatch, int per, mcp::SegRef A, mcp::SegRef
 // 256 matrices takes one inversion's ~2.5 ms: teams win up to 45.
 constexpr int kTreeTeamMax = 45;
 bool tree_on_teams(mcp_ctx* c, int n, int nb) { return nb <= kTreeTeamMax && !c->no_reg && team_inverse_fits(c, n); }
-int tree_invert(mcp_ctx* c, int n, int nb, double* zb, int* kc, int bk) {
-  if (tree_on_teams(c, n, nb)) return team_inverse(c, n, nb, zb, kc, bk);
-  hipLaunchKernelGGL(mcp::k_syminv_large<kInvNT>, dim3(nb), dim3(kInvNT), lds_large(n, 0, 0), c->stream, n, zb, kc, bk);
+// nb full symmetric inverses with inertias, one workgroup per matrix: 32
+// pivots per pass where the candidate window fits the LDS, else 16
+bool wide_fits(mcp_ctx* c, int n) {
+  return c->inv_kernel == 1 && n <= mcp::kWideMaxN && lds_wide(n) <= c->lds_max;
+}
+int batched_inverse(mcp_ctx* c, int n, int nb, double* a, int* kc, int bk) {
+  if (nb <= 0) return MCP_OK;
+  if (wide_fits(c, n)) {
+    hipLaunchKernelGGL(mcp::k_syminv_wide, dim3(nb), dim3(mcp::kWNT), lds_wide(n), c->stream, n, a, kc, bk);
+  } else {
+    const size_t ll = lds_large(n, 0, 0);
+    if (ll > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "inverse: N=%d exceeds the large-N panel limit", n);
+    hipLaunchKernelGGL(mcp::k_syminv_large<kInvNT>, dim3(nb), dim3(kInvNT), ll, c->stream, n, a, kc, bk);
+  }
   HIPCHK(c, hipGetLastError());
   return MCP_OK;
+}
+int tree_invert(mcp_ctx* c, int n, int nb, double* zb, int* kc, int bk) {
+  if (tree_on_teams(c, n, nb)) return team_inverse(c, n, nb, zb, kc, bk);
+  return batched_inverse(c, n, nb, zb, kc, bk);
 }
 
 // Buffers of a tree over nleaf segments per instance: two levels of
@@ -954,8 +982,7 @@ int airy_prop_global(mcp_ctx* c, Block& b, Grid& g, int ninst, const double* d_e
     if (team) {
       if ((rc = team_inverse(c, n, ninst, am, akc, bk))) return rc;
     } else {
-      hipLaunchKernelGGL(mcp::k_syminv_large<kInvNT>, dim3(ninst), dim3(kInvNT), ll, c->stream, n, am, akc, bk);
-      HIPCHK(c, hipGetLastError());
+      if ((rc = batched_inverse(c, n, ninst, am, akc, bk))) return rc;
     }
     hipLaunchKernelGGL(mcp::k_airy_z, dim3(ne), dim3(256), 0, c->stream, n, ninst, k, nstep, am, akc, ay, dy, d_nodes,
                        d_status, outward);
@@ -1087,9 +1114,7 @@ int mcp_syminv_batched(mcp_ctx* c, int32_t n, int32_t nmat, const double* a, dou
     if (ll > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "syminv: N=%d exceeds the large-N panel limit", n);
     if ((rc = upload(c, c->s_b, a, cnt))) return rc;
     HIPCHK(c, c->s_c.ensure((size_t)nmat * sizeof(int)));
-    hipLaunchKernelGGL(mcp::k_syminv_large<kInvNT>, dim3(nmat), dim3(kInvNT), ll, c->stream, n, c->s_b.as<double>(),
-                       c->s_c.as<int>(), n > 30 ? 1 : 0);
-    HIPCHK(c, hipGetLastError());
+    if ((rc = batched_inverse(c, n, nmat, c->s_b.as<double>(), c->s_c.as<int>(), n > 30 ? 1 : 0))) return rc;
     if ((rc = download(c, ainv, c->s_b, cnt))) return rc;
     if ((rc = download(c, kount, c->s_c, (size_t)nmat))) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1675,6 +1700,12 @@ int mcp_set_large_mode(mcp_ctx* c, int32_t mode) {
     return c ? c->fail(MCP_ERR_ARG, "large mode %d: 0, 1 or 2, plus 4", mode) : MCP_ERR_ARG;
   c->large_mode = mode & 3;
   c->eig_global = (mode & 4) != 0;
+  return MCP_OK;
+}
+
+int mcp_set_inverse_kernel(mcp_ctx* c, int32_t mode) {
+  if (!c || mode < 0 || mode > 1) return c ? c->fail(MCP_ERR_ARG, "inverse kernel %d: 0 or 1", mode) : MCP_ERR_ARG;
+  c->inv_kernel = mode;
   return MCP_OK;
 }
 

@@ -416,6 +416,10 @@ class Engine:
         """Threads per instance of the LDS-resident MDPROP: 64, 256 or 0 (256)."""
         self._check(self.lib.mcp_set_lds_threads(self.ctx, int(nt)), "mcp_set_lds_threads")
 
+    def set_inverse_kernel(self, mode: int):
+        """Batched large-N inversions: 1 (default) 32-pivot passes for N <= 448, 0 16-pivot passes."""
+        self._check(self.lib.mcp_set_inverse_kernel(self.ctx, int(mode)), "mcp_set_inverse_kernel")
+
     def set_team_launch(self, cooperative: bool):
         """Team kernels through the cooperative launch API (default) or as
         ordinary launches of the same grids (for tracing tools)."""

@@ -52,6 +52,7 @@ SIGNATURES = {
                                         _pi]),
     "mcp_set_team_size": (ctypes.c_int, [_vp, _i32]),
     "mcp_set_large_mode": (ctypes.c_int, [_vp, _i32]),
+    "mcp_set_inverse_kernel": (ctypes.c_int, [_vp, _i32]),
     "mcp_set_team_launch": (ctypes.c_int, [_vp, _i32]),
     "mcp_set_lds_threads": (ctypes.c_int, [_vp, _i32]),
     "mcp_ytrans": (ctypes.c_int, [_vp, _i32, _i32, _pd, _pd, _pd]),
