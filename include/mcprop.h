@@ -16,6 +16,7 @@
 #ifndef MCPROP_H
 #define MCPROP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -212,6 +213,29 @@ int mcp_ytrans(mcp_ctx *ctx, int32_t block_id, int32_t ninst, const double *pcon
 int mcp_ytok_ktos(mcp_ctx *ctx, int32_t n, int32_t ninst, const double *y, const double *wvec,
                   const int32_t *nopen, const int32_t *nb, const double *cent, double rval, double *kmat,
                   double *sr, double *si, int32_t *status);
+
+/* mcp_ytok_ktos without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Every array is a device pointer with mcp_ytok_ktos's layout:
+ * inputs d_y [ninst][N*N], d_wvec [ninst][N], d_nopen [ninst], d_nb
+ * [ninst][N], d_cent [N]; outputs d_kmat, d_sr, d_si [ninst][N*N] (nopen x
+ * nopen, leading dimension nopen) and d_status [ninst] (as mcp_ytok_ktos,
+ * or 99 for an instance whose NOPEN or NB is out of range).  The work
+ * matrices and team barrier counters live in a fixed 32 MiB device array of
+ * the library, one per GPU, held until the process ends: calls needing more
+ * (mcp_ytok_ktos_noalloc_bytes(N, ninst) bytes; e.g. 10 instances at N=441)
+ * fail with MCP_ERR_UNSUPPORTED.  Every call on a GPU shares that array, so
+ * calls that overlap on one GPU (two contexts, two host threads) race.  The
+ * inputs must be complete on the device before the call; it runs on the
+ * context's stream and returns after that stream has finished. */
+int mcp_ytok_ktos_noalloc(mcp_ctx *ctx, int32_t n, int32_t ninst, const double *d_y, const double *d_wvec,
+                          const int32_t *d_nopen, const int32_t *d_nb, const double *d_cent, double rval,
+                          double *d_kmat, double *d_sr, double *d_si, int32_t *d_status);
+
+/* Scratch bytes mcp_ytok_ktos_noalloc needs for ninst instances of N
+ * channels (the limit is 32 MiB).  No counterpart in the reference. */
+size_t mcp_ytok_ktos_noalloc_bytes(int32_t n, int32_t ninst);
 
 /* Whole MOLSCAT solve for one block on one LD grid (SCCTRL with one segment,
  * source_code/scctrl.f:80-289): YINIT at R0, propagation, YTOK, KTOS, with

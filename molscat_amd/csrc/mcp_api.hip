@@ -223,6 +223,13 @@ size_t lds_wide(int n) {
 
 }  // namespace
 
+// Device memory of the allocation-free matching entry point
+// (mcp_ytok_ktos_noalloc): the team barrier counters and YTOKG's work
+// matrices.  One copy per GPU, held until the process ends; host code uses
+// the address hipGetSymbolAddress gives on each device, never this name.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 struct mcp_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -243,6 +250,7 @@ struct mcp_ctx {
   bool coop = true;         // team kernels through the cooperative launch API
   int lds_threads = 0;      // threads per instance of the LDS-resident MDPROP (0: 256)
   int inv_kernel = 1;       // batched large-N inversions: 1 32-pivot passes (N <= 448), 0 16-pivot passes
+  unsigned char* scratch_dev = nullptr;  // this device's address of scratch (mcp_ytok_ktos_noalloc)
   DevBuf s_u;  // LDPROP sector-matrix workspace (large N)
   DevBuf s_av, s_at, s_ae, s_ah;  // AIRY step bases, basis changes, eigenvalues, slopes
   DevBuf s_x;                     // column exchange of the register-resident MDPROP
@@ -1073,6 +1081,12 @@ int mcp_ctx_create(int device, mcp_ctx** out) {
   for (auto& e : c->ev)
     if (hipEventCreate(&e) != hipSuccess) return MCP_ERR_HIP;
   set_lds_limits(c.get());
+  // the allocation-free entry point's scratch, at its address on this device
+  if (hipGetSymbolAddress((void**)&c->scratch_dev, HIP_SYMBOL(scratch)) != hipSuccess || !c->scratch_dev) {
+    (void)hipGetLastError();
+    fprintf(stderr, "mcprop: no address for the scratch array on device %d\n", device);
+    return MCP_ERR_HIP;
+  }
   *out = c.release();
   return MCP_OK;
 }
@@ -1424,6 +1438,40 @@ int mcp_prop_ld(mcp_ctx* c, int32_t bid, int32_t gid, int32_t iprop, int32_t nin
   return record_timing(c);
 }
 
+// YTOK + KTOS for ninst instances whose MatchArgs point at device memory:
+// the setup pass, the forward elimination on workgroup teams (counters:
+// ninst + 1 zeroed words, the last the abort flag), the finishing pass.
+// Allocates nothing.
+static int match_run(mcp_ctx* c, int n, int ninst, const mcp::MatchArgs& M, unsigned* ctrs) {
+  hipLaunchKernelGGL(mcp::k_match, dim3(ninst), dim3(kNT), 0, c->stream, M, mcp::kMatchSetup);
+  HIPCHK(c, hipGetLastError());
+  {
+    // the forward elimination on teams of workgroups: g ranks per instance
+    // where the CUs hold them (a single workgroup for many instances); a
+    // team size set on the context caps it
+    const int gmax = c->team > 0 ? c->team : 32;
+    const int g = std::max(1, std::min({gmax, (n + 7) / 8, c->num_cu / std::max(1, ninst)}));
+    const size_t lu_lds = (size_t)5 * n * sizeof(double);
+    if (lu_lds > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "matching: N=%d exceeds the LU's LDS", n);
+    int rc;
+    hipLaunchKernelGGL(mcp::k_zero_u32, dim3((ninst + 1 + 255) / 256), dim3(256), 0, c->stream, ctrs, ninst + 1);
+    HIPCHK(c, hipGetLastError());
+    int* abort_flag = (int*)(ctrs + ninst);
+    mcp::MatchArgs Mc = M;
+    if (g == 1) {
+      hipLaunchKernelGGL(mcp::k_match_lu, dim3(ninst), dim3(kNT), lu_lds, c->stream, Mc, 1, ctrs, abort_flag);
+      HIPCHK(c, hipGetLastError());
+    } else {
+      int g_ = g;
+      void* args[] = {&Mc, &g_, &ctrs, &abort_flag};
+      if ((rc = launch_team(c, (const void*)mcp::k_match_lu, dim3(ninst * g), dim3(kNT), args, lu_lds))) return rc;
+    }
+  }
+  hipLaunchKernelGGL(mcp::k_match, dim3(ninst), dim3(kNT), 0, c->stream, M, mcp::kMatchFinish);
+  HIPCHK(c, hipGetLastError());
+  return MCP_OK;
+}
+
 static int match_device(mcp_ctx* c, int n, int ninst, const double* d_y, const double* wvec, const int32_t* nopen,
                         const int32_t* nb, const double* d_cent, double rval) {
   int rc;
@@ -1432,6 +1480,7 @@ static int match_device(mcp_ctx* c, int n, int ninst, const double* d_y, const d
   if ((rc = upload(c, c->s_d, nb, (size_t)ninst * n))) return rc;
   HIPCHK(c, c->s_b.ensure((size_t)ninst * (2 * (size_t)n * n + 4 * (size_t)n) * sizeof(double)));
   HIPCHK(c, c->s_k.ensure((size_t)ninst * 3 * n * n * sizeof(double) + (size_t)ninst * sizeof(int)));
+  HIPCHK(c, c->s_bar.ensure((size_t)(ninst + 1) * sizeof(unsigned)));
   mcp::MatchArgs M;
   M.n = n;
   M.y = d_y;
@@ -1445,32 +1494,7 @@ static int match_device(mcp_ctx* c, int n, int ninst, const double* d_y, const d
   M.sr = M.kmat + (size_t)ninst * n * n;
   M.si = M.sr + (size_t)ninst * n * n;
   M.status = (int*)(M.si + (size_t)ninst * n * n);
-  hipLaunchKernelGGL(mcp::k_match, dim3(ninst), dim3(kNT), 0, c->stream, M, mcp::kMatchSetup);
-  HIPCHK(c, hipGetLastError());
-  {
-    // the forward elimination on teams of workgroups: g ranks per instance
-    // where the CUs hold them (a single workgroup for many instances); a
-    // team size set on the context caps it
-    const int gmax = c->team > 0 ? c->team : 32;
-    const int g = std::max(1, std::min({gmax, (n + 7) / 8, c->num_cu / std::max(1, ninst)}));
-    const size_t lu_lds = (size_t)5 * n * sizeof(double);
-    if (lu_lds > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "matching: N=%d exceeds the LU's LDS", n);
-    int rc;
-    if ((rc = prep_team(c, ninst))) return rc;
-    unsigned* ctrs = c->s_bar.as<unsigned>();
-    int* abort_flag = (int*)(ctrs + ninst);
-    if (g == 1) {
-      hipLaunchKernelGGL(mcp::k_match_lu, dim3(ninst), dim3(kNT), lu_lds, c->stream, M, 1, ctrs, abort_flag);
-      HIPCHK(c, hipGetLastError());
-    } else {
-      int g_ = g;
-      void* args[] = {&M, &g_, &ctrs, &abort_flag};
-      if ((rc = launch_team(c, (const void*)mcp::k_match_lu, dim3(ninst * g), dim3(kNT), args, lu_lds))) return rc;
-    }
-  }
-  hipLaunchKernelGGL(mcp::k_match, dim3(ninst), dim3(kNT), 0, c->stream, M, mcp::kMatchFinish);
-  HIPCHK(c, hipGetLastError());
-  return MCP_OK;
+  return match_run(c, n, ninst, M, c->s_bar.as<unsigned>());
 }
 
 static int fetch_match(mcp_ctx* c, int n, int ninst, const int32_t* nopen, double* kmat, double* sr, double* si,
@@ -1510,6 +1534,48 @@ int mcp_ytok_ktos(mcp_ctx* c, int32_t n, int32_t ninst, const double* y, const d
   if ((rc = upload(c, c->s_e, cent, (size_t)n))) return rc;
   if ((rc = match_device(c, n, ninst, c->s_i.as<double>(), wvec, nopen, nb, c->s_e.as<double>(), rval))) return rc;
   return fetch_match(c, n, ninst, nopen, kmat, sr, si, status);
+}
+
+size_t mcp_ytok_ktos_noalloc_bytes(int32_t n, int32_t ninst) {
+  if (n <= 0 || ninst <= 0) return 0;
+  const size_t ctr = ((size_t)(ninst + 1) * sizeof(unsigned) + 255) / 256 * 256;
+  return ctr + (size_t)ninst * (2 * (size_t)n * n + 4 * (size_t)n) * sizeof(double);
+}
+
+int mcp_ytok_ktos_noalloc(mcp_ctx* c, int32_t n, int32_t ninst, const double* d_y, const double* d_wvec,
+                          const int32_t* d_nopen, const int32_t* d_nb, const double* d_cent, double rval,
+                          double* d_kmat, double* d_sr, double* d_si, int32_t* d_status) {
+  if (!c || n <= 0 || ninst < 0 || !d_y || !d_wvec || !d_nopen || !d_nb || !d_cent || !d_kmat || !d_sr || !d_si ||
+      !d_status)
+    return c ? c->fail(MCP_ERR_ARG, "ytok_ktos_noalloc: bad args") : MCP_ERR_ARG;
+  if (!(rval > 0.0)) return c->fail(MCP_ERR_ARG, "ytok_ktos_noalloc: matching radius %g must be positive", rval);
+  if (ninst == 0) return MCP_OK;
+  const size_t need = mcp_ytok_ktos_noalloc_bytes(n, ninst);
+  if (need > kNoAllocScratchBytes)
+    return c->fail(MCP_ERR_UNSUPPORTED,
+                   "ytok_ktos_noalloc: N=%d with %d instances needs %zu bytes of scratch, the maximum is %zu", n, ninst,
+                   need, kNoAllocScratchBytes);
+  if (!c->scratch_dev) return c->fail(MCP_ERR_HIP, "ytok_ktos_noalloc: no scratch address on this device");
+  HIPCHK(c, hipSetDevice(c->device));
+  unsigned* ctrs = (unsigned*)c->scratch_dev;
+  const size_t ctr = ((size_t)(ninst + 1) * sizeof(unsigned) + 255) / 256 * 256;
+  mcp::MatchArgs M;
+  M.n = n;
+  M.y = d_y;
+  M.wvec = d_wvec;
+  M.nopen = d_nopen;
+  M.nb = d_nb;
+  M.cent = d_cent;
+  M.rval = rval;
+  M.work = (double*)(c->scratch_dev + ctr);
+  M.kmat = d_kmat;
+  M.sr = d_sr;
+  M.si = d_si;
+  M.status = d_status;
+  int rc;
+  if ((rc = match_run(c, n, ninst, M, ctrs))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MCP_OK;
 }
 
 int mcp_smatrix_ld(mcp_ctx* c, int32_t bid, int32_t gid, int32_t iprop, int32_t ninst, const double* ered,

@@ -759,12 +759,29 @@ struct MatchArgs {
 constexpr int kMatchSetup = 0;
 constexpr int kMatchFinish = 1;
 
+constexpr int kMatchBadArg = 99;  // NOPEN or an NB entry out of range
+
 __global__ void __launch_bounds__(kNT) k_match(MatchArgs M, int phase) {
   __shared__ int bad;
   const int inst = blockIdx.x;
   const int n = M.n;
   const int nop = M.nopen[inst];
   const int* nb = M.nb + (size_t)inst * n;
+  if (phase != kMatchFinish) {
+    // the arguments of callers that pass device arrays are checked here:
+    // an instance with NOPEN outside [0, N] or an NB entry outside [0, N)
+    // gets status kMatchBadArg and is skipped by every later pass
+    if (threadIdx.x == 0) bad = (nop < 0 || nop > n) ? 1 : 0;
+    __syncthreads();
+    for (int ii = threadIdx.x; ii < n && !bad; ii += kNT)
+      if (nb[ii] < 0 || nb[ii] >= n) atomicMax(&bad, 1);
+    __syncthreads();
+    if (bad) {
+      if (threadIdx.x == 0) M.status[inst] = kMatchBadArg;
+      return;
+    }
+    __syncthreads();
+  }
   const double* y = M.y + (size_t)inst * n * n;
   const double* wv = M.wvec + (size_t)inst * n;
   double* km = M.work + (size_t)inst * (2 * (size_t)n * n + 4 * (size_t)n);

@@ -42,6 +42,8 @@ SIGNATURES = {
     "mcp_yinit": (ctypes.c_int, [_vp, _i32, _i32, _i32, _i32, _dbl, _i32, _pd, _pd, _pd]),
     "mcp_prop_ld": (ctypes.c_int, [_vp, _i32, _i32, _i32, _i32, _pd, _pd, _pd, _pi, _pi]),
     "mcp_ytok_ktos": (ctypes.c_int, [_vp, _i32, _i32, _pd, _pd, _pi, _pi, _pd, _dbl, _pd, _pd, _pd, _pi]),
+    "mcp_ytok_ktos_noalloc": (ctypes.c_int, [_vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _dbl, _vp, _vp, _vp, _vp]),
+    "mcp_ytok_ktos_noalloc_bytes": (ctypes.c_size_t, [_i32, _i32]),
     "mcp_smatrix_ld": (ctypes.c_int, [_vp, _i32, _i32, _i32, _i32, _pd, _pd, _pd, _pi, _pi, _dbl, _pd, _pd, _pd,
                                        _pi, _pi]),
     "mcp_grid_create_airy": (ctypes.c_int, [_vp, _i32, _i32, _pd, _pd, _pd, _pi]),

@@ -14,7 +14,7 @@ LIB = os.path.join(ROOT, "molscat_amd", "libmcprop.so")
 
 def declared_symbols():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*\*?\s*(mcp_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|size_t|const char \*)\s*\*?\s*(mcp_\w+)\s*\(", text, re.M)))
 
 
 def test_header_declares_entry_points():
