@@ -113,6 +113,27 @@ int mcp_yinit(mcp_ctx *ctx, int32_t block_id, int32_t grid_id, int32_t k0, int32
 int mcp_prop_ld(mcp_ctx *ctx, int32_t block_id, int32_t grid_id, int32_t iprop, int32_t ninst, const double *ered,
                 const double *pconst, double *y, int32_t *nodes, int32_t *status);
 
+/* MDPROP (iprop 6, source_code/mdprop.f:1-4) for the instances of nblk
+ * blocks of uneven N on one LD grid, as one batched call: the reference's
+ * partial-wave loop (source_code/mol.driver.f:1181-1856) calls MDPROP once
+ * per (JTOT, IB, energy); here every block's energy-independent sector
+ * matrices U(C) are built once and all (block, energy) instances go through
+ * one persistent launch that takes them longest (N^3) first from a device
+ * queue.  ninst[b] instances of block block_ids[b]; ered, nodes and status
+ * hold the instances block after block, y their N_b x N_b matrices block
+ * after block (Y(RSTART) in, Y(RSTOP) out).  Every block needs NCONST = 0,
+ * N >= 2 and N within the LDS-resident MDPROP limit (mcp_ctx_info).  With
+ * mcp_set_graph(ctx, 1) the launch sequence is captured as a HIP graph per
+ * (grid, block set, instance counts) and replayed by later calls with the
+ * same key. */
+int mcp_prop_ld_multi(mcp_ctx *ctx, int32_t nblk, const int32_t *block_ids, int32_t grid_id, const int32_t *ninst,
+                      const double *ered, double *y, int32_t *nodes, int32_t *status);
+
+/* 1: mcp_prop_ld_multi captures its launches as a HIP graph and replays it
+ * for later calls with the same grid, blocks and instance counts; 0 (the
+ * default): ordinary launches.  No reference counterpart. */
+int mcp_set_graph(mcp_ctx *ctx, int32_t on);
+
 /* MGPROP's symplectic schemes (SYMSEL, source_code/mgprop.f:103-150):
  * imgsel = 4 (CS4, 5 substeps) or 5 (MA5, 6 substeps); a, b receive the
  * scheme's A (position advances) and B (potential weights), nsub entries. */

@@ -251,6 +251,16 @@ struct mcp_ctx {
   int lds_threads = 0;      // threads per instance of the LDS-resident MDPROP (0: 256)
   int inv_kernel = 1;       // batched large-N inversions: 1 32-pivot passes (N <= 448), 0 16-pivot passes
   unsigned char* scratch_dev = nullptr;  // this device's address of scratch (mcp_ytok_ktos_noalloc)
+  // multi-block MDPROP (mcp_prop_ld_multi): sector matrices, inertias, block
+  // table, item list, ERED, Y, nodes/status, queue head; and its graph
+  DevBuf s_qu, s_qk, s_qb, s_qi, s_qe, s_qy, s_qn, s_qh;
+  bool use_graph = false;
+  struct MultiGraph {
+    std::vector<int> key;            // grid id, nblk, then (block id, ninst) pairs
+    std::vector<void*> bufs;         // device buffers the graph was captured on
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  } mgraph;
   DevBuf s_u;  // LDPROP sector-matrix workspace (large N)
   DevBuf s_av, s_at, s_ae, s_ah;  // AIRY step bases, basis changes, eigenvalues, slopes
   DevBuf s_x;                     // column exchange of the register-resident MDPROP
@@ -301,6 +311,7 @@ int set_lds_limits(mcp_ctx* c) {
                        (const void*)mcp::k_mdprop_lds<kNT>, (const void*)mcp::k_mdprop_lds<64>,     (const void*)mcp::k_ldprop_lds,
                        (const void*)mcp::k_eigh,           (const void*)mcp::k_bound_match,           (const void*)mcp::k_yinit,
                        (const void*)mcp::k_syminv_large<kInvNT>,   (const void*)mcp::k_mdprop_uc_large, (const void*)mcp::k_syminv_wide,
+                       (const void*)mcp::k_mdprop_lds_queue<kNT>, (const void*)mcp::k_mdprop_lds_queue<64>,
                        (const void*)mcp::k_mdprop_large,   (const void*)mcp::k_ldprop_large,
                        (const void*)mcp::k_yinit_large,    (const void*)mcp::k_airy_sector,
                        (const void*)mcp::k_airy_prop,      (const void*)mcp::k_mdprop_mfma<1>, (const void*)mcp::k_mdprop_mfma<2>,
@@ -1095,6 +1106,8 @@ void mcp_ctx_destroy(mcp_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->mgraph.exec) (void)hipGraphExecDestroy(c->mgraph.exec);
+  if (c->mgraph.graph) (void)hipGraphDestroy(c->mgraph.graph);
   c->blocks.clear();
   c->grids.clear();
   for (auto& e : c->ev)
@@ -1253,6 +1266,164 @@ int mcp_grid_create_airy(mcp_ctx* c, int32_t mxlam, int32_t nstep, const double*
   HIPCHK(c, hipStreamSynchronize(c->stream));
   *grid_id = id;
   return MCP_OK;
+}
+
+int mcp_set_graph(mcp_ctx* c, int32_t on) {
+  if (!c || on < 0 || on > 1) return c ? c->fail(MCP_ERR_ARG, "graph mode %d: 0 or 1", on) : MCP_ERR_ARG;
+  c->use_graph = on != 0;
+  return MCP_OK;
+}
+
+int mcp_prop_ld_multi(mcp_ctx* c, int32_t nblk, const int32_t* block_ids, int32_t gid, const int32_t* ninst,
+                      const double* ered, double* y, int32_t* nodes, int32_t* status) {
+  if (!c || nblk < 0 || (nblk > 0 && (!block_ids || !ninst || !ered || !y || !nodes || !status)))
+    return c ? c->fail(MCP_ERR_ARG, "prop_ld_multi: bad args") : MCP_ERR_ARG;
+  if (nblk == 0) return MCP_OK;
+  auto ig = c->grids.find(gid);
+  if (ig == c->grids.end()) return c->fail(MCP_ERR_ARG, "prop_ld_multi: unknown grid id %d", gid);
+  Grid& g = *ig->second;
+  if (g.airy || (g.npts - 1) % 2 != 0 || g.npts < 3 || g.dr == 0.0)
+    return c->fail(MCP_ERR_ARG, "prop_ld_multi: needs an LD grid (2*NSTEP+1 points and its DR)");
+  const int nstep = (g.npts - 1) / 2;
+  const double h = g.dr / 2.0;
+  std::vector<Block*> bl(nblk);
+  size_t nitems = 0, ucsum = 0, ysum = 0, ldsmax = 0;
+  for (int b = 0; b < nblk; ++b) {
+    auto ib = c->blocks.find(block_ids[b]);
+    if (ib == c->blocks.end()) return c->fail(MCP_ERR_ARG, "prop_ld_multi: unknown block id %d", block_ids[b]);
+    Block& B = *ib->second;
+    if (B.mxlam != g.mxlam) return c->fail(MCP_ERR_ARG, "prop_ld_multi: block %d has %d radial terms, grid %d", block_ids[b], B.mxlam, g.mxlam);
+    if (B.nconst != 0) return c->fail(MCP_ERR_UNSUPPORTED, "prop_ld_multi: blocks with NCONST terms go through mcp_prop_ld");
+    if (B.n < 2) return c->fail(MCP_ERR_UNSUPPORTED, "prop_ld_multi: N=1 blocks take ODPROP (mcp_prop_ld)");
+    const size_t need = std::max(lds_md(B.n, B.nham), lds_uc(B.n, B.nham));
+    if (need > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "prop_ld_multi: N=%d beyond the LDS-resident MDPROP", B.n);
+    if (ninst[b] < 0) return c->fail(MCP_ERR_ARG, "prop_ld_multi: ninst[%d] < 0", b);
+    ldsmax = std::max(ldsmax, lds_md(B.n, B.nham));
+    bl[b] = &B;
+    nitems += ninst[b];
+    ucsum += (size_t)nstep * B.n * B.n;
+    ysum += (size_t)ninst[b] * B.n * B.n;
+  }
+  if (nitems == 0) return MCP_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  // device buffers (allocated before any capture; a graph is only replayed
+  // while the buffers it was captured on are still these)
+  HIPCHK(c, c->s_qu.ensure(ucsum * sizeof(double)));
+  HIPCHK(c, c->s_qk.ensure((size_t)nblk * nstep * sizeof(int)));
+  HIPCHK(c, c->s_qb.ensure((size_t)nblk * sizeof(mcp::QueueBlock)));
+  HIPCHK(c, c->s_qi.ensure(nitems * sizeof(int2)));
+  HIPCHK(c, c->s_qe.ensure(nitems * sizeof(double)));
+  HIPCHK(c, c->s_qy.ensure(ysum * sizeof(double)));
+  HIPCHK(c, c->s_qn.ensure(2 * nitems * sizeof(int)));
+  HIPCHK(c, c->s_qh.ensure(sizeof(unsigned)));
+  // block table and the item list, longest (N^3) first
+  std::vector<mcp::QueueBlock> qb(nblk);
+  std::vector<std::pair<long long, int2>> its;
+  its.reserve(nitems);
+  {
+    size_t uo = 0, yo = 0, io = 0;
+    for (int b = 0; b < nblk; ++b) {
+      const int n = bl[b]->n;
+      qb[b].B = block_args(*bl[b]);
+      qb[b].uc = c->s_qu.as<double>() + uo;
+      qb[b].uc_kount = c->s_qk.as<int>() + (size_t)b * nstep;
+      qb[b].ered = c->s_qe.as<double>() + io;
+      qb[b].y = c->s_qy.as<double>() + yo;
+      qb[b].nodes = c->s_qn.as<int>() + io;
+      qb[b].status = c->s_qn.as<int>() + nitems + io;
+      qb[b].bk = n > 30 ? 1 : 0;
+      for (int i = 0; i < ninst[b]; ++i) its.push_back({-(long long)n * n * n, make_int2(b, i)});
+      uo += (size_t)nstep * n * n;
+      yo += (size_t)ninst[b] * n * n;
+      io += ninst[b];
+    }
+  }
+  std::stable_sort(its.begin(), its.end(), [](const auto& a, const auto& b2) { return a.first < b2.first; });
+  std::vector<int2> items(nitems);
+  for (size_t i = 0; i < nitems; ++i) items[i] = its[i].second;
+  int rc;
+  if ((rc = upload(c, c->s_qe, ered, nitems))) return rc;
+  if ((rc = upload(c, c->s_qy, y, ysum))) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->s_qb.p, qb.data(), nblk * sizeof(mcp::QueueBlock), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->s_qi.p, items.data(), nitems * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+  // resident workgroups of the persistent launch
+  const int nt = c->lds_threads == 64 ? 64 : kNT;
+  const void* qk = nt == 64 ? (const void*)mcp::k_mdprop_lds_queue<64> : (const void*)mcp::k_mdprop_lds_queue<kNT>;
+  int per_cu = 0;
+  HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, qk, nt, ldsmax));
+  const int nwg = (int)std::min<size_t>(nitems, (size_t)std::max(1, per_cu) * c->num_cu);
+  mcp::GridArgs G = grid_args(g);
+  const int nitems_i = (int)nitems;
+  auto issue = [&]() -> int {
+    for (int b = 0; b < nblk; ++b) {
+      mcp::BlockArgs B = qb[b].B;
+      hipLaunchKernelGGL(mcp::k_mdprop_uc, dim3(nstep), dim3(kNT), lds_uc(B.n, B.nham), c->stream, B, G,
+                         (const double*)nullptr, 1, nstep, h, qb[b].bk, const_cast<double*>(qb[b].uc),
+                         const_cast<int*>(qb[b].uc_kount));
+      HIPCHK(c, hipGetLastError());
+    }
+    hipLaunchKernelGGL(mcp::k_zero_u32, dim3(1), dim3(64), 0, c->stream, c->s_qh.as<unsigned>(), 1);
+    HIPCHK(c, hipGetLastError());
+    if (nt == 64)
+      hipLaunchKernelGGL(mcp::k_mdprop_lds_queue<64>, dim3(nwg), dim3(64), ldsmax, c->stream,
+                         c->s_qb.as<const mcp::QueueBlock>(), c->s_qi.as<const int2>(), nitems_i,
+                         c->s_qh.as<unsigned>(), G, nstep, h);
+    else
+      hipLaunchKernelGGL(mcp::k_mdprop_lds_queue<kNT>, dim3(nwg), dim3(kNT), ldsmax, c->stream,
+                         c->s_qb.as<const mcp::QueueBlock>(), c->s_qi.as<const int2>(), nitems_i,
+                         c->s_qh.as<unsigned>(), G, nstep, h);
+    HIPCHK(c, hipGetLastError());
+    return MCP_OK;
+  };
+  HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+  HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  if (c->use_graph) {
+    // one graph per radial grid and block set: the sector matrices of every
+    // block and the persistent launch, replayed while the key and the
+    // buffers stay the same
+    std::vector<int> key = {gid, nblk, nt};
+    for (int b = 0; b < nblk; ++b) {
+      key.push_back(block_ids[b]);
+      key.push_back(ninst[b]);
+    }
+    std::vector<void*> bufs = {c->s_qu.p, c->s_qk.p, c->s_qb.p, c->s_qi.p, c->s_qe.p, c->s_qy.p, c->s_qn.p, c->s_qh.p,
+                               g.r.p, g.ptab.p};
+    for (int b = 0; b < nblk; ++b) {
+      bufs.push_back(bl[b]->vl.p);
+      bufs.push_back(bl[b]->eint.p);
+      bufs.push_back(bl[b]->cent.p);
+    }
+    auto& mg = c->mgraph;
+    if (!mg.exec || mg.key != key || mg.bufs != bufs) {
+      if (mg.exec) (void)hipGraphExecDestroy(mg.exec);
+      if (mg.graph) (void)hipGraphDestroy(mg.graph);
+      mg.exec = nullptr;
+      mg.graph = nullptr;
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+      const int rci = issue();
+      hipGraph_t gr = nullptr;
+      const hipError_t ec = hipStreamEndCapture(c->stream, &gr);
+      if (rci) return rci;
+      HIPCHK(c, ec);
+      mg.graph = gr;
+      HIPCHK(c, hipGraphInstantiate(&mg.exec, mg.graph, nullptr, nullptr, 0));
+      mg.key = key;
+      mg.bufs = bufs;
+    }
+    HIPCHK(c, hipGraphLaunch(mg.exec, c->stream));
+  } else {
+    if ((rc = issue())) return rc;
+  }
+  HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+  if ((rc = download(c, y, c->s_qy, ysum))) return rc;
+  std::vector<int> ns(2 * nitems);
+  HIPCHK(c, hipMemcpyAsync(ns.data(), c->s_qn.p, 2 * nitems * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < nitems; ++i) {
+    nodes[i] = ns[i];
+    status[i] = ns[nitems + i];
+  }
+  return record_timing(c);
 }
 
 int mcp_prop_airy(mcp_ctx* c, int32_t bid, int32_t gid, int32_t ninst, const double* ered, double* y, int32_t* nodes,

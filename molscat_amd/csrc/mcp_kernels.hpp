@@ -221,12 +221,14 @@ struct InstArgs {
 // the pivot decisions do not depend on NT.  Four waves per SIMD for NT =
 // 256 (at most 128 VGPRs, no spills): the field scans run many instances
 // per CU.
+// One instance's MDPROP with Y in LDS (the body of k_mdprop_lds and of the
+// persistent multi-block launch k_mdprop_lds_queue): yg is Y(RSTART) on
+// entry and Y(RSTOP) on exit, *nodes_out and *status_out the results.
 template <int NT>
-__global__ void __launch_bounds__(NT, NT == 256 ? 4 : 1) k_mdprop_lds(BlockArgs B, GridArgs G, InstArgs I, int nstep, double h,
-                                                    const double* __restrict__ uc,
-                                                    const int* __restrict__ uc_kount, int bk) {
-  extern __shared__ double smem[];
-  const int inst = blockIdx.x;
+__device__ __forceinline__ void mdprop_lds_run(const BlockArgs& B, const GridArgs& G, double ered,
+                                               const double* pconst, double* yg, int* nodes_out, int* status_out,
+                                               int nstep, double h, const double* __restrict__ uc,
+                                               const int* __restrict__ uc_kount, int bk, double* smem) {
   const int n = B.n, lda = n;
   LdsCarve c(smem);
   double* Y = c.take<double>((size_t)n * lda);
@@ -236,12 +238,9 @@ __global__ void __launch_bounds__(NT, NT == 256 ? 4 : 1) k_mdprop_lds(BlockArgs 
   double* u2 = c.take<double>(n);
   SweepScratch s = carve_sweep(c, n);
   const Tile2D<NT> t2;
-  const double ered = I.ered[inst];
-  const double* pconst = I.pconst ? I.pconst + (size_t)inst * B.nconst : nullptr;
   const double d1 = h * h / 3.0;
   const double d2 = 2.0 * d1;
   const double half = 0.5 * h;
-  double* yg = I.y + (size_t)inst * n * n;
 
   // NSAVE = sum of the sector-matrix inertias (mdprop.f:133,277-279)
   int nsave = 0;
@@ -256,8 +255,8 @@ __global__ void __launch_bounds__(NT, NT == 256 ? 4 : 1) k_mdprop_lds(BlockArgs 
     pbad = block_sum<NT>(pbad, s.sv);
     if (pbad > 0.0) {
       if (threadIdx.x == 0) {
-        I.nodes[inst] = 0;
-        I.status[inst] = -1;
+        *nodes_out = 0;
+        *status_out = -1;
       }
       return;
     }
@@ -357,8 +356,59 @@ __global__ void __launch_bounds__(NT, NT == 256 ? 4 : 1) k_mdprop_lds(BlockArgs 
   for (int j = t2.tj; j < n; j += Tile2D<NT>::kWaves)
     for (int i = t2.ti; i < n; i += 64) yg[i + (size_t)j * n] = hi * Y[i + j * lda];
   if (threadIdx.x == 0) {
-    I.nodes[inst] = nodes - nsave;
-    I.status[inst] = status;
+    *nodes_out = nodes - nsave;
+    *status_out = status;
+  }
+}
+
+template <int NT>
+__global__ void __launch_bounds__(NT, NT == 256 ? 4 : 1) k_mdprop_lds(BlockArgs B, GridArgs G, InstArgs I, int nstep, double h,
+                                                    const double* __restrict__ uc,
+                                                    const int* __restrict__ uc_kount, int bk) {
+  extern __shared__ double smem[];
+  const int inst = blockIdx.x;
+  const double* pconst = I.pconst ? I.pconst + (size_t)inst * B.nconst : nullptr;
+  mdprop_lds_run<NT>(B, G, I.ered[inst], pconst, I.y + (size_t)inst * B.n * B.n, I.nodes + inst, I.status + inst,
+                     nstep, h, uc, uc_kount, bk, smem);
+}
+
+// One block of a multi-block MDPROP launch: its arguments, its
+// energy-independent sector matrices and its instances' arrays.
+struct QueueBlock {
+  BlockArgs B;
+  const double* uc;    // [nstep][n*n] U(C)
+  const int* uc_kount; // [nstep]
+  const double* ered;  // [ninst]
+  double* y;           // [ninst][n*n]
+  int* nodes;          // [ninst]
+  int* status;         // [ninst]
+  int bk;
+};
+
+// MDPROP for the instances of several blocks of uneven N on one radial grid
+// in one persistent launch: every workgroup takes the next (block,
+// instance) item from the device counter *head (zero at launch) until the
+// list is exhausted.  Items are listed longest first (by N^3), so the long
+// ones start first and the short ones fill in behind them.
+template <int NT>
+__global__ void __launch_bounds__(NT, NT == 256 ? 4 : 1) k_mdprop_lds_queue(const QueueBlock* __restrict__ blocks,
+                                                                          const int2* __restrict__ items, int nitems,
+                                                                          unsigned* __restrict__ head, GridArgs G,
+                                                                          int nstep, double h) {
+  extern __shared__ double smem[];
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(head, 1u);
+    __syncthreads();
+    const int it = s_item;
+    __syncthreads();
+    if (it >= nitems) break;
+    const int2 w = items[it];
+    const QueueBlock qb = blocks[w.x];
+    const int nn = qb.B.n * qb.B.n;
+    mdprop_lds_run<NT>(qb.B, G, qb.ered[w.y], nullptr, qb.y + (size_t)w.y * nn, qb.nodes + w.y, qb.status + w.y,
+                       nstep, h, qb.uc, qb.uc_kount, qb.bk, smem);
+    __syncthreads();
   }
 }
 

@@ -189,6 +189,35 @@ class Engine:
                                          iptr(st)), "mcp_prop_ld")
         return _from_colmajor(y), nodes, st
 
+    def prop_ld_multi(self, blocks, g: GridHandle, ereds, y0s):
+        """MDPROP for several blocks of uneven N in one persistent launch:
+        blocks[b] with energies ereds[b] and starting matrices y0s[b]
+        (m_b, N_b, N_b).  Returns [(Y(RSTOP), nodes, status) per block]."""
+        nblk = len(blocks)
+        ids = np.ascontiguousarray([b.id for b in blocks], dtype=np.int32)
+        ninst = np.ascontiguousarray([len(np.atleast_1d(e)) for e in ereds], dtype=np.int32)
+        ered = np.ascontiguousarray(np.concatenate([np.atleast_1d(e) for e in ereds]), dtype=np.float64)
+        y = np.ascontiguousarray(np.concatenate([_colmajor(np.asarray(yy, dtype=np.float64).reshape(-1, b.n, b.n)).ravel()
+                                                 for yy, b in zip(y0s, blocks)]))
+        tot = int(ninst.sum())
+        nodes = np.empty(tot, dtype=np.int32)
+        st = np.empty(tot, dtype=np.int32)
+        self._check(self.lib.mcp_prop_ld_multi(self.ctx, nblk, iptr(ids), g.id, iptr(ninst), dptr(ered), dptr(y),
+                                               iptr(nodes), iptr(st)), "mcp_prop_ld_multi")
+        out, yo, io = [], 0, 0
+        for b, m in zip(blocks, ninst):
+            m = int(m)
+            cnt = m * b.n * b.n
+            out.append((_from_colmajor(y[yo:yo + cnt].reshape(m, b.n, b.n)), nodes[io:io + m].copy(),
+                        st[io:io + m].copy()))
+            yo += cnt
+            io += m
+        return out
+
+    def set_graph(self, on: bool):
+        """mcp_prop_ld_multi through a captured HIP graph (replayed for the same key)."""
+        self._check(self.lib.mcp_set_graph(self.ctx, 1 if on else 0), "mcp_set_graph")
+
     def prop_airy(self, b: BlockHandle, g: GridHandle, ered, y0: np.ndarray):
         """AIPROP over an AIRY grid for ninst = len(ered) instances; returns
         (Z at the grid's end, nodes, status)."""

@@ -55,6 +55,8 @@ SIGNATURES = {
     "mcp_set_team_size": (ctypes.c_int, [_vp, _i32]),
     "mcp_set_large_mode": (ctypes.c_int, [_vp, _i32]),
     "mcp_set_inverse_kernel": (ctypes.c_int, [_vp, _i32]),
+    "mcp_prop_ld_multi": (ctypes.c_int, [_vp, _i32, _pi, _i32, _pi, _pd, _pd, _pi, _pi]),
+    "mcp_set_graph": (ctypes.c_int, [_vp, _i32]),
     "mcp_set_team_launch": (ctypes.c_int, [_vp, _i32]),
     "mcp_set_lds_threads": (ctypes.c_int, [_vp, _i32]),
     "mcp_ytrans": (ctypes.c_int, [_vp, _i32, _i32, _pd, _pd, _pd]),
