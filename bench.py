@@ -147,6 +147,113 @@ def _field(args) -> int:
     return 0
 
 
+def _sweep(args) -> int:
+    """The batched partial-wave and energy sweep: the N=441 workload's rotor
+    and potential for JTOT = 0, jstep, ..., 120 (a stratified subset of the
+    121 JTOT values), both parities, 64 energies uniform in [10, 500] cm-1,
+    on one radial grid for every block (LDMD 2-20 A at DR = 0.005 A, 3600
+    sectors, then AIRY 20-200 A with TOLHIL = 0, 162 steps), YTOK/KTOS at
+    200 A and SPROC's cross sections summed over the blocks
+    (molscat_amd/driver.py:sweep).  Blocks within the LDS-resident MDPROP
+    limit share one persistent launch; the others take one batched call
+    each with the autotuner's setting for their N, measured first (not
+    timed).  One timed pass over the subset; the full sweep's time is
+    extrapolated from it by the sum of N^3 over its blocks."""
+    from molscat_amd import decks, driver
+    from molscat_amd import system as S
+    from molscat_amd.autotune import Autotuner
+    from molscat_amd.engine import Engine
+
+    deck = decks.rotor441()
+    seg = deck.ld
+    r = seg.radii()
+    ra, wa = deck.airy_grid()
+    energies = decks.sweep_energies()[: args.energies_sweep]
+    jtots = list(range(0, 121, args.jstep))
+    pot = deck.potential()
+
+    def blk_of(jt, ib):
+        return S.rotor_block(jt, ib, deck.jlevels, decks.BE, deck.lambdas, pot, decks.URED, runit=1.0)
+
+    sizes_all = [blk_of(jt, ib).n for jt in range(0, 121) for ib in (1, 2)]
+    sizes_all = [n for n in sizes_all if n > 0]
+    eng = Engine(0)
+    tuner = Autotuner(eng, path=args.tune_file) if args.tune_file else Autotuner(eng)
+    t0 = time.perf_counter()
+    for jt in jtots:
+        for ib in (1, 2):
+            b = blk_of(jt, ib)
+            if b.n > eng.info()["max_n_md"]:
+                tuner.tune_block(b, np.asarray(energies) * b.cm2ru)
+    tune_s = time.perf_counter() - t0
+    print(f"sweep: tuned in {tune_s:.1f} s: " + ", ".join(f"N={k}: {v}" for k, v in sorted(
+        tuner._entries().items(), key=lambda kv: int(kv[0]))), file=sys.stderr, flush=True)
+
+    def progress(jt, ib, n):
+        print(f"sweep: JTOT={jt} IB={ib} N={n} done ({time.perf_counter() - t1:.1f} s)", file=sys.stderr, flush=True)
+
+    t1 = time.perf_counter()
+    res = driver.sweep(eng, deck.jlevels, decks.BE, deck.lambdas, pot, decks.URED, energies, jtots, (1, 2),
+                       (r, seg.dr), (ra, wa), deck.airy_stop, tuner=tuner, progress=progress)
+    wall = time.perf_counter() - t1
+    sizes = [run.n for run in res.blocks]
+    nsmat = sum(sum(1 for q in run.nopen if q > 0) for run in res.blocks)
+    n3 = float(sum(n ** 3 for n in sizes))
+    n3_all = float(sum(n ** 3 for n in sizes_all))
+    ne = len(energies)
+    nstep = seg.nstep
+    nham = len(deck.lambdas)
+    # reference work of the LDMD stage per sector-step with B = ne energies
+    # sharing U(C) and W's off-diagonal: (2 + 1/B) N^3 + (2/B) NHAM N(N+1) + 12 N^2
+    flops_ld = sum(nstep * ne * ((2.0 + 1.0 / ne) * n ** 3 + (2.0 / ne) * nham * n * (n + 1) + 12.0 * n * n)
+                   for n in sizes)
+    cpu = None
+    if args.cpu_sectors > 0:
+        os.environ.setdefault("OMP_NUM_THREADS", "1")
+        from oracle import molscat_oracle as O
+
+        nmax = max(sizes)
+        jt, ib = next((run.jtot, run.ib) for run in res.blocks if run.n == nmax)
+        b = blk_of(jt, ib)
+        ob = O.Block(n=b.n, vl=b.vl, eint=b.eint, cent=b.cent, lvals=b.lvals, levidx=b.levidx,
+                     potl=O.PowerPotential(b.potl.terms, rm=b.potl.rm, epsil=b.potl.epsil), rscale=b.rscale,
+                     ep2ru=b.ep2ru, cm2ru=b.cm2ru)
+        ered = energies[-1] * b.cm2ru
+        y0 = np.diag(np.sqrt(np.abs(np.diag(ob.hammat(seg.rstart, ered)[0]))))
+        tc0 = time.perf_counter()
+        O.mdprop(y0, lambda rr: ob.hammat(rr, ered), seg.rstart, args.cpu_sectors, seg.dr)
+        tc = time.perf_counter() - tc0
+        per_step_n3 = tc / args.cpu_sectors / float(nmax) ** 3
+        cpu_subset = per_step_n3 * nstep * ne * n3   # the LD stage of the subset, one thread
+        cpu = {"value": nsmat / cpu_subset, "unit": "S-matrices/s", "cores": 1, "kind": "port",
+               "sample": f"oracle MDPROP at N={nmax}, {args.cpu_sectors} sectors, one energy, single thread "
+                         f"({tc:.1f} s), scaled by N^3 x sectors x energies to the subset's LD stage "
+                         f"(AIRY and matching not counted)"}
+    line = {
+        "metric": "S-matrices/s over the partial-wave x energy sweep (JTOT 0..120, both parities, 64 energies)",
+        "value": nsmat / wall, "unit": "S-matrices/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"rotor sweep, JTOT {jtots[0]}..{jtots[-1]} step {args.jstep} x IB 1,2 "
+                               f"({len(sizes)} blocks, N {min(sizes)}..{max(sizes)}) x {ne} energies in [10, 500] "
+                               f"cm-1, LDMD 2-20 A ({nstep} sectors) + AIRY 20-200 A (TOLHIL=0), one grid",
+                   "blocks": len(sizes), "energies": ne},
+        "s_matrices": nsmat,
+        "sector_steps_per_s": nstep * ne * len(sizes) / wall,
+        "full_sweep_seconds_extrapolated": wall * n3_all / n3,
+        "seconds": {k: round(v, 3) for k, v in res.seconds.items()},
+        "tuned": {str(k): v for k, v in res.tuned.items()},
+        "tune_seconds": tune_s,
+        "roofline": {"bound": "mfma", "kernel": "whole sweep, priced at the LDMD stage's reference work",
+                     "achieved": flops_ld / wall / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                     "frac": flops_ld / wall / PEAK_FP64, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    eng.close()
+    print(json.dumps(line))
+    return 0
+
+
 def _allprop(args) -> int:
     """Every propagator of the reference's all-propagators deck (ITYPE=1
     rotor BE=30, j=0,2,4,6, JTOT=10, IB=1, N=12, URED=20; FINDRM's RSTART,
@@ -284,7 +391,10 @@ def main() -> int:
                     help="ordinary launches for the team kernels (for profilers that cannot trace cooperative ones)")
     ap.add_argument("--inverse-kernel", type=int, default=1,
                     help="batched large-N inversions: 1 32-pivot passes (default), 0 16-pivot passes")
-    ap.add_argument("--workload", choices=["rotor441", "allprop", "field", "copy"], default="rotor441",
+    ap.add_argument("--jstep", type=int, default=30, help="JTOT step of the sweep's stratified subset")
+    ap.add_argument("--energies-sweep", type=int, default=64, help="energies per block for --workload sweep")
+    ap.add_argument("--tune-file", default="", help="autotuner table (default ~/.cache/molscat_amd/autotune.json)")
+    ap.add_argument("--workload", choices=["rotor441", "allprop", "field", "copy", "sweep"], default="rotor441",
                     help="rotor441: the N=441 deck (default); allprop: every propagator on the N=12 deck; "
                          "field: the N=48 field scan")
     ap.add_argument("--fields", type=int, default=4096, help="fields per call for --workload field")
@@ -301,6 +411,8 @@ def main() -> int:
         return _field(args)
     if args.workload == "copy":
         return _copy(args)
+    if args.workload == "sweep":
+        return _sweep(args)
 
     from molscat_amd import decks
     from molscat_amd import system as S

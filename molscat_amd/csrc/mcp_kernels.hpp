@@ -865,13 +865,20 @@ __global__ void __launch_bounds__(kNT) k_match(MatchArgs M, int phase) {
       snp[nx] = unp * rd;
     } else {
       const double dw = fabs(wv[nx]);
-      double ratio;
-      const int rc = riccati_k_ratio(ell, dw * M.rval, ratio);
-      if (rc) atomicMax(&bad, rc);
       sj[nx] = 0.0;
       sjp[nx] = 0.0;
       sn[nx] = 1.0;
-      snp[nx] = ratio * dw;
+      if (dw == 0.0) {
+        // a channel exactly at its threshold: the k -> 0 limit of
+        // k K'/K, the log-derivative of the decaying zero-energy solution
+        // R^-ell (RBESSK itself has no value at zero argument)
+        snp[nx] = -ell / M.rval;
+      } else {
+        double ratio;
+        const int rc = riccati_k_ratio(ell, dw * M.rval, ratio);
+        if (rc) atomicMax(&bad, rc);
+        snp[nx] = ratio * dw;
+      }
     }
   }
   __syncthreads();

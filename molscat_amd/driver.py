@@ -202,3 +202,129 @@ def field_scan(eng, fb, fields: Sequence[float], rstart: float, rstop: float, dr
     eng.destroy_block(b)
     eng.destroy_grid(g)
     return FieldScan(fields=fields, ered=ered, a=a, nopen=nops, status=np.asarray(out["STATUS"]))
+
+
+@dataclass
+class SweepResult:
+    energies: np.ndarray                 # cm-1
+    sigma: np.ndarray                    # [energy][NSIG][NSIG], summed over the blocks
+    blocks: List[BlockRun]
+    seconds: dict                        # device-call wall time by part
+    tuned: dict                          # N -> the autotuner's choice used
+
+
+def sweep(eng, jlevels: Sequence[int], be: float, lambdas: Sequence[int], potl: S.PowerPotential, ured: float,
+          energies: Sequence[float], jtots: Sequence[int], ibs: Sequence[int], ld_grid, airy_grid, rmax: float,
+          tuner=None, runit=1.0, progress=None, small_max: int = -1) -> SweepResult:
+    """The partial-wave and energy sweep on one radial grid for every block:
+    the reference's JTOT / IB / energy loops (source_code/mol.driver.f:
+    1181-1856) for a fixed LDMD grid (ld_grid = (r, dr) of mcp_grid_create)
+    and a fixed AIRY grid (airy_grid = (r, drnow), TOLHI = 0; or None), with
+    SPROC's cross sections (source_code/sproc.f:380-430, 524-530) summed over
+    the blocks.
+
+    Blocks within the LDS-resident MDPROP limit go through one persistent
+    launch together (mcp_prop_ld_multi: every block's energy-independent
+    sector matrices, then all (block, energy) instances longest first),
+    followed by AIRY and YTOK/KTOS per block; larger blocks each take one
+    batched S-matrix call (mcp_smatrix_segments) with the setting the
+    autotuner chose for their N (small_max >= 0 lowers the first group's
+    N limit, e.g. 0 sends every block the second way).  Every energy of a
+    block shares its U(C)
+    sector matrices and AIRY step bases.  A failed instance raises, as the
+    reference stops (source_code/mdprop.f:282-287)."""
+    import time as _time
+
+    energies = np.asarray(energies, dtype=np.float64)
+    elev = np.array([be * float(j * (j + 1)) for j in jlevels])
+    nsig = int(np.sum(elev < energies.max()))
+    acc = np.zeros((len(energies), nsig, nsig))
+    info = eng.info()
+    blocks = []
+    for jtot in jtots:
+        for ib in ibs:
+            blk = S.rotor_block(jtot, ib, jlevels, be, lambdas, potl, ured, runit=runit)
+            if blk.n > 0:
+                blocks.append((jtot, ib, blk))
+    r_ld, dr = ld_grid
+    proto = blocks[0][2]
+    g = eng.grid(r_ld, proto.ptab(r_ld), dr)
+    ga = None
+    if airy_grid is not None:
+        ra, wa = airy_grid
+        ga = eng.airy_grid(ra, wa, proto.ptab(ra))
+    secs = {"small_ld": 0.0, "small_airy_match": 0.0, "large": 0.0, "tune": 0.0}
+    tuned = {}
+    runs: List[BlockRun] = []
+
+    def accumulate(jtot, ib, blk, sr, si, nodes, status, wv, nops, nb):
+        failed = [m for m in range(len(energies)) if status[m] != 0 and nops[m] > 0]
+        if failed:
+            raise RuntimeError(f"sweep: JTOT={jtot} IB={ib}: propagation failed at energies "
+                               f"{[float(energies[m]) for m in failed]} (status {[int(status[m]) for m in failed]})")
+        for m in range(len(energies)):
+            if nops[m] > 0:
+                acc[m] += S.partial_sigma(sr[m], si[m], nb, nops[m], blk.levidx, jlevels, wv[m], jtot, blk.runit,
+                                          nsig, wght=1.0)
+        runs.append(BlockRun(jtot, ib, blk.n, float(r_ld[0]), (len(r_ld) - 1) // 2, dr, list(nops),
+                             [int(x) for x in nodes], [int(x) for x in status], sr=list(sr), si=list(si)))
+
+    def channels(blk):
+        ered = energies * blk.cm2ru
+        wv, nops = zip(*[S.gtwvec(e, blk.eint) for e in ered])
+        return ered, np.array(wv), np.array(nops, dtype=np.int32), S.chnsrt(blk.eint)
+
+    nsmall = info["max_n_md"] if small_max < 0 else min(small_max, info["max_n_md"])
+    small = [t for t in blocks if t[2].n >= 2 and t[2].n <= nsmall]
+    large = [t for t in blocks if t not in small]
+    # the small blocks: one persistent MDPROP launch for all of them
+    if small:
+        t0 = _time.perf_counter()
+        hs = [eng.block(b.vl, b.eint, b.cent, mxlam=b.mxlam) for _, _, b in small]
+        chans = [channels(b) for _, _, b in small]
+        y0s = [eng.yinit(h, g, c[0]) for h, c in zip(hs, chans)]
+        outs = eng.prop_ld_multi(hs, g, [c[0] for c in chans], y0s)
+        secs["small_ld"] += _time.perf_counter() - t0
+        t0 = _time.perf_counter()
+        for (jtot, ib, blk), h, (ered, wv, nops, nb), (y, nodes, st) in zip(small, hs, chans, outs):
+            nodes = nodes.copy()
+            if ga is not None:
+                y, n2, st2 = eng.prop_airy(h, ga, ered, y)
+                nodes = nodes + n2
+                st = np.where(st != 0, st, st2)
+            _, sr, si, st3 = eng.ytok_ktos(y, wv, nops, nb, blk.cent, rmax)
+            st = np.where(st != 0, st, st3)
+            accumulate(jtot, ib, blk, sr, si, nodes, st, wv, nops, nb)
+            eng.destroy_block(h)
+            if progress:
+                progress(jtot, ib, blk.n)
+        secs["small_airy_match"] += _time.perf_counter() - t0
+    # the large blocks: one batched call each, with the tuned setting
+    for jtot, ib, blk in large:
+        ered, wv, nops, nb = channels(blk)
+        if tuner is not None:
+            t0 = _time.perf_counter()
+            ch = tuner.tune_block(blk, ered)
+            secs["tune"] += _time.perf_counter() - t0
+            tuned[blk.n] = ch.__dict__.copy()
+            ch.apply(eng)
+        t0 = _time.perf_counter()
+        h = eng.block(blk.vl, blk.eint, blk.cent, mxlam=blk.mxlam)
+        grids, iprops = ([g, ga], [6, 9]) if ga is not None else ([g], [6])
+        out = eng.smatrix_segments(h, grids, iprops, ered, wv, nops, nb, rmax)
+        eng.destroy_block(h)
+        secs["large"] += _time.perf_counter() - t0
+        if tuner is not None:
+            from .autotune import reset
+
+            reset(eng)
+        accumulate(jtot, ib, blk, out["SR"], out["SI"], out["NODES"], out["STATUS"], wv, nops, nb)
+        if progress:
+            progress(jtot, ib, blk.n)
+    eng.destroy_grid(g)
+    if ga is not None:
+        eng.destroy_grid(ga)
+    if tuner is not None:
+        tuner.save()
+    runs.sort(key=lambda r: (r.jtot, r.ib))
+    return SweepResult(energies=energies, sigma=acc, blocks=runs, seconds=secs, tuned=tuned)
