@@ -254,6 +254,72 @@ def _sweep(args) -> int:
     return 0
 
 
+def _mixed_sweep(args) -> int:
+    """The mixed-precision sweep: the blocks of the sweep's stratified
+    subset with N >= 200 (JTOT = 0, jstep, ..., 120, both parities, 64
+    energies, the same grid as --workload sweep), solved twice: FP64, and
+    with the large-N tree's batched inversions in FP32 + Newton steps (FP64
+    below 3 A; YINIT, the recurrence's elementwise steps, AIRY's steps and
+    the matching FP64).  Reports both rates, their ratio, max|dS| over every
+    S-matrix and the largest relative change of the summed cross sections."""
+    from molscat_amd import decks, driver
+    from molscat_amd import system as S
+    from molscat_amd.autotune import Autotuner
+    from molscat_amd.engine import Engine
+
+    deck = decks.rotor441()
+    seg = deck.ld
+    r = seg.radii()
+    ra, wa = deck.airy_grid()
+    energies = decks.sweep_energies()[: args.energies_sweep]
+    pot = deck.potential()
+    jtots = [jt for jt in range(0, 121, args.jstep)]
+    eng = Engine(0)
+    tuner = Autotuner(eng, path=args.tune_file) if args.tune_file else Autotuner(eng)
+    info = eng.info()
+    blocks = [(jt, ib) for jt in jtots for ib in (1, 2)
+              if S.rotor_block(jt, ib, deck.jlevels, decks.BE, deck.lambdas, pot, decks.URED, runit=1.0).n >= 200]
+    jt_use = sorted({jt for jt, _ in blocks})
+
+    def progress(jt, ib, n):
+        print(f"mixed-sweep: JTOT={jt} IB={ib} N={n} done", file=sys.stderr, flush=True)
+
+    runs = {}
+    for label, mixed in (("fp64", None), ("mixed", (args.mixed_steps, 3.0, 200))):
+        t1 = time.perf_counter()
+        res = driver.sweep(eng, deck.jlevels, decks.BE, deck.lambdas, pot, decks.URED, energies, jt_use, (1, 2),
+                           (r, seg.dr), (ra, wa), deck.airy_stop, tuner=tuner, progress=progress, small_max=0,
+                           mixed=mixed)
+        runs[label] = (res, time.perf_counter() - t1)
+        print(f"mixed-sweep: {label} {runs[label][1]:.1f} s", file=sys.stderr, flush=True)
+    r64, w64 = runs["fp64"]
+    rmx, wmx = runs["mixed"]
+    nsmat = sum(sum(1 for q in run.nopen if q > 0) for run in r64.blocks if run.n >= 200)
+    ds = 0.0
+    nodes_equal = True
+    for a, b in zip(r64.blocks, rmx.blocks):
+        nodes_equal &= a.nodes == b.nodes
+        for sa, sb, ia, ib_ in zip(a.sr, b.sr, a.si, b.si):
+            if np.size(sa):
+                ds = max(ds, float(np.abs(np.asarray(sa) - np.asarray(sb)).max()),
+                         float(np.abs(np.asarray(ia) - np.asarray(ib_)).max()))
+    sig_scale = np.abs(r64.sigma).max()
+    dsig = float(np.abs(r64.sigma - rmx.sigma).max() / sig_scale)
+    line = {
+        "metric": "S-matrices/s of the N>=200 blocks of the sweep, mixed precision vs FP64 at max|dS|<=1e-8",
+        "value": nsmat / wmx, "unit": "S-matrices/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": wmx * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"rotor sweep blocks with N>=200: JTOT {jt_use} x IB 1,2 ({len(r64.blocks)} blocks) x "
+                               f"{len(energies)} energies, LDMD 2-20 A (3600 sectors) + AIRY 20-200 A",
+                   "newton_steps": args.mixed_steps, "fp64_below_A": 3.0},
+        "fp64_value": nsmat / w64, "ratio_mixed_to_fp64": w64 / wmx, "max_abs_dS": ds,
+        "max_rel_dsigma": dsig, "nodes_equal": bool(nodes_equal), "s_matrices": nsmat,
+    }
+    eng.close()
+    print(json.dumps(line))
+    return 0
+
+
 def _allprop(args) -> int:
     """Every propagator of the reference's all-propagators deck (ITYPE=1
     rotor BE=30, j=0,2,4,6, JTOT=10, IB=1, N=12, URED=20; FINDRM's RSTART,
@@ -392,9 +458,11 @@ def main() -> int:
     ap.add_argument("--inverse-kernel", type=int, default=1,
                     help="batched large-N inversions: 1 32-pivot passes (default), 0 16-pivot passes")
     ap.add_argument("--jstep", type=int, default=30, help="JTOT step of the sweep's stratified subset")
+    ap.add_argument("--mixed-steps", type=int, default=2,
+                    help="Newton steps of the mixed-precision leg (0: no mixed leg)")
     ap.add_argument("--energies-sweep", type=int, default=64, help="energies per block for --workload sweep")
     ap.add_argument("--tune-file", default="", help="autotuner table (default ~/.cache/molscat_amd/autotune.json)")
-    ap.add_argument("--workload", choices=["rotor441", "allprop", "field", "copy", "sweep"], default="rotor441",
+    ap.add_argument("--workload", choices=["rotor441", "allprop", "field", "copy", "sweep", "mixed-sweep"], default="rotor441",
                     help="rotor441: the N=441 deck (default); allprop: every propagator on the N=12 deck; "
                          "field: the N=48 field scan")
     ap.add_argument("--fields", type=int, default=4096, help="fields per call for --workload field")
@@ -413,6 +481,8 @@ def main() -> int:
         return _copy(args)
     if args.workload == "sweep":
         return _sweep(args)
+    if args.workload == "mixed-sweep":
+        return _mixed_sweep(args)
 
     from molscat_amd import decks
     from molscat_amd import system as S
@@ -508,6 +578,31 @@ def main() -> int:
                      "executed_flops": flops_exec,
                      "executed_tflops": flops_exec / (ms_stage * 1e-3) / 1e12},
     }
+    if args.mixed_steps > 0:
+        # the same solve with the tree's batched inversions in FP32 + Newton
+        # steps (FP64 below 3 A, YINIT / recurrence / AIRY steps / matching FP64)
+        s64 = out["SR"][0] + 1j * out["SI"][0]
+        eng.set_precision(args.mixed_steps, 3.0)
+        for _ in range(args.warmup):
+            solve()
+        eng.set_precision(args.mixed_steps, 3.0)  # statistics of the timed solves only
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            outm = solve()
+        wall_m = time.perf_counter() - t0
+        stats = eng.precision_stats()
+        eng.set_precision(0)
+        sm = outm["SR"][0] + 1j * outm["SI"][0]
+        line["mixed"] = {
+            "value": args.steps / wall_m, "unit": "S-matrices/s", "ms_per_step": 1e3 * wall_m / args.steps,
+            "ratio_to_fp64": (args.steps / wall_m) / smat_per_s,
+            "max_abs_dS_vs_fp64": float(np.abs(sm - s64).max()),
+            "nodes_equal": int(outm["NODES"][0]) == int(out["NODES"][0]),
+            "newton_steps": args.mixed_steps, "fp64_below_A": 3.0,
+            "inversions_mixed_per_solve": stats["mixed_matrices"] / args.steps,
+            "inversions_fp64_again_per_solve": stats["fp64_again"] / args.steps,
+            "precision": "tree inversions FP32 (matrix cores) + FP64 Newton-Schulz; everything else FP64",
+        }
     if args.cpu_sectors > 0:
         line["cpu_baseline"] = _cpu_baseline(blk, seg, ered, args.cpu_sectors)
     eng.close()

@@ -335,6 +335,23 @@ int mcp_set_large_mode(mcp_ctx *ctx, int32_t mode);
  * No reference counterpart (SYMINV's blocking is LAPACK's). */
 int mcp_set_inverse_kernel(mcp_ctx *ctx, int32_t mode);
 
+/* Mixed precision for the large-N tree's batched inversions (the MDPROP and
+ * AIRY trees, N <= 448): each inverse computed in FP32 on the matrix cores
+ * (with its inertia from the FP32 pivots) and won back to FP64 by
+ * newton_steps Newton-Schulz steps X <- X (2I - A X) in FP64; a matrix whose
+ * FP32 inverse leaves max|I - A X| > 1e-2 is inverted again in FP64, and the
+ * MDPROP sectors starting below r_fp64 (RUNIT; the short-range start where
+ * closed channels grow exponentially) stay FP64, as do YINIT, the Y
+ * recurrence's elementwise steps and the matching.  newton_steps = 0 (the
+ * default) is FP64 throughout.  SYMINV's job (source_code/syminv.f) in
+ * another precision; no reference counterpart. */
+int mcp_set_precision(mcp_ctx *ctx, int32_t newton_steps, double r_fp64);
+
+/* Since the last mcp_set_precision: matrices that took the mixed path, and
+ * how many of them were inverted in FP64 after all (short-range or
+ * untrusted FP32 inverse).  No reference counterpart. */
+int mcp_precision_stats(const mcp_ctx *ctx, int64_t *mixed_matrices, int64_t *fp64_again);
+
 /* Threads per instance of the LDS-resident MDPROP kernel: 64 (one wave) or
  * 256 (four waves, also what 0, the default, gives).  The results do not
  * depend on it (same arithmetic per element, same pivots).  No reference

@@ -14,6 +14,7 @@
 #include "mcp_kernels.hpp"
 #include "mcp_large.hpp"
 #include "mcp_wide.hpp"
+#include "mcp_mixed.hpp"
 #include "mcp_airyprop.hpp"
 #include "mcp_mfmaprop.hpp"
 #include "mcp_tree.hpp"
@@ -50,6 +51,14 @@ struct DevBuf {
     hipError_t e = hipMalloc(&p, b ? b : 16);
     if (e == hipSuccess) bytes = b;
     return e;
+  }
+  void release() {
+    if (p) {
+      (void)hipDeviceSynchronize();
+      (void)hipFree(p);
+    }
+    p = nullptr;
+    bytes = 0;
   }
   template <class T>
   T* as() const {
@@ -210,14 +219,18 @@ size_t lds_large(int n, int nham, int nvec) {
   return z.b;
 }
 
-// same pieces, in the same order, as mcp::carve_wide
+// same pieces, in the same order, as mcp::carve_wide<T>
+template <typename T = double>
 size_t lds_wide(int n) {
   LdsSize z;
   const size_t ld = mcp::panel_ld(n);
-  z.add<double>(mcp::kWP * ld).add<double>(mcp::kWP * mcp::kWP);
-  z.add<double>(mcp::kWP).add<double>(mcp::kWW * mcp::kWP);
+  z.add<T>(mcp::kWP * ld);
+  z.add<T>(mcp::kWP * mcp::kWP);
+  z.add<T>(mcp::kWP);
+  z.add<T>(mcp::kWW * mcp::kWP);
   z.add<int>(n).add<int>(n).add<int>(n).add<int>(n).add<int>(mcp::kWP).add<int>(8);
-  z.add<double>(mcp::kWW).add<int>(mcp::kWW);
+  z.add<T>(mcp::kWW);
+  z.add<int>(mcp::kWW);
   return z.b;
 }
 
@@ -255,6 +268,12 @@ struct mcp_ctx {
   // table, item list, ERED, Y, nodes/status, queue head; and its graph
   DevBuf s_qu, s_qk, s_qb, s_qi, s_qe, s_qy, s_qn, s_qh;
   bool use_graph = false;
+  // mixed precision on the large-N tree's batched inversions (0: FP64)
+  int mixed_steps = 0;       // Newton steps after the FP32 inverse
+  double mixed_r64 = 0.0;    // sectors starting below this radius stay FP64
+  long long mixed_fp64 = 0;  // matrices the last calls inverted in FP64 again (guard)
+  long long mixed_total = 0; // matrices the last calls took the mixed path for
+  DevBuf s_m32, s_mx, s_mtt, s_mx2, s_mflag, s_mcnt;
   struct MultiGraph {
     std::vector<int> key;            // grid id, nblk, then (block id, ninst) pairs
     std::vector<void*> bufs;         // device buffers the graph was captured on
@@ -292,6 +311,13 @@ struct mcp_ctx {
   }
   int hip(hipError_t e, const char* what) {
     if (e == hipSuccess) return MCP_OK;
+    if (e == hipErrorOutOfMemory) {
+      size_t freeb = 0, totalb = 0;
+      (void)hipGetLastError();
+      (void)hipMemGetInfo(&freeb, &totalb);
+      return fail(MCP_ERR_HIP, "%s: %s (device memory free %.2f of %.2f GiB)", what, hipGetErrorString(e),
+                  freeb / 1073741824.0, totalb / 1073741824.0);
+    }
     return fail(MCP_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
   }
 };
@@ -310,7 +336,7 @@ int set_lds_limits(mcp_ctx* c) {
   const void* fns[] = {(const void*)mcp::k_syminv,         (const void*)mcp::k_mdprop_uc,
                        (const void*)mcp::k_mdprop_lds<kNT>, (const void*)mcp::k_mdprop_lds<64>,     (const void*)mcp::k_ldprop_lds,
                        (const void*)mcp::k_eigh,           (const void*)mcp::k_bound_match,           (const void*)mcp::k_yinit,
-                       (const void*)mcp::k_syminv_large<kInvNT>,   (const void*)mcp::k_mdprop_uc_large, (const void*)mcp::k_syminv_wide,
+                       (const void*)mcp::k_syminv_large<kInvNT>,   (const void*)mcp::k_mdprop_uc_large, (const void*)mcp::k_syminv_wide<double>, (const void*)mcp::k_syminv_wide<float>,
                        (const void*)mcp::k_mdprop_lds_queue<kNT>, (const void*)mcp::k_mdprop_lds_queue<64>,
                        (const void*)mcp::k_mdprop_large,   (const void*)mcp::k_ldprop_large,
                        (const void*)mcp::k_yinit_large,    (const void*)mcp::k_airy_sector,
@@ -497,7 +523,8 @@ bool wide_fits(mcp_ctx* c, int n) {
 int batched_inverse(mcp_ctx* c, int n, int nb, double* a, int* kc, int bk) {
   if (nb <= 0) return MCP_OK;
   if (wide_fits(c, n)) {
-    hipLaunchKernelGGL(mcp::k_syminv_wide, dim3(nb), dim3(mcp::kWNT), lds_wide(n), c->stream, n, a, kc, bk);
+    hipLaunchKernelGGL(mcp::k_syminv_wide<double>, dim3(nb), dim3(mcp::kWNT), lds_wide(n), c->stream, n, a, kc, bk,
+                       (const int*)nullptr, 0);
   } else {
     const size_t ll = lds_large(n, 0, 0);
     if (ll > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "inverse: N=%d exceeds the large-N panel limit", n);
@@ -506,8 +533,69 @@ int batched_inverse(mcp_ctx* c, int n, int nb, double* a, int* kc, int bk) {
   HIPCHK(c, hipGetLastError());
   return MCP_OK;
 }
-int tree_invert(mcp_ctx* c, int n, int nb, double* zb, int* kc, int bk) {
+// nb inverses in FP32 won back to FP64 by Newton-Schulz steps (mcp_mixed.hpp);
+// matrix m is FP64 throughout when (m % per) < cut, and again when its FP32
+// inverse is not trusted.
+int mixed_inverse(mcp_ctx* c, int n, int nb, int per, int cut, double* a, int* kc, int bk) {
+  const size_t nn = (size_t)n * n;
+  HIPCHK(c, c->s_m32.ensure((size_t)nb * nn * sizeof(float)));
+  HIPCHK(c, c->s_mx.ensure((size_t)nb * nn * sizeof(double)));
+  HIPCHK(c, c->s_mtt.ensure((size_t)nb * nn * sizeof(double)));
+  HIPCHK(c, c->s_mx2.ensure((size_t)nb * nn * sizeof(double)));
+  HIPCHK(c, c->s_mflag.ensure((size_t)nb * sizeof(int)));
+  HIPCHK(c, c->s_mcnt.ensure(sizeof(int)));
+  float* a32 = c->s_m32.as<float>();
+  double* x = c->s_mx.as<double>();
+  double* t = c->s_mtt.as<double>();
+  double* x2 = c->s_mx2.as<double>();
+  int* flag = c->s_mflag.as<int>();
+  const unsigned ge = (unsigned)std::min<size_t>(nblk256((size_t)nb * nn), 65536);
+  hipLaunchKernelGGL(mcp::k_mixed_down, dim3(ge), dim3(256), 0, c->stream, n, nb, per, cut, a, a32, flag);
+  HIPCHK(c, hipGetLastError());
+  hipLaunchKernelGGL(mcp::k_syminv_wide<float>, dim3(nb), dim3(mcp::kWNT), lds_wide<float>(n), c->stream, n, a32, kc,
+                     bk, (const int*)flag, 0);
+  HIPCHK(c, hipGetLastError());
+  hipLaunchKernelGGL(mcp::k_mixed_up, dim3(ge), dim3(256), 0, c->stream, n, nb, (const float*)a32, x, (const int*)flag);
+  HIPCHK(c, hipGetLastError());
+  int rc;
+  const mcp::SegRef A{a, nn, 0};
+  for (int step = 0; step < c->mixed_steps; ++step) {
+    const mcp::SegRef X{x, nn, 0}, T{t, nn, 0};
+    // T = A X; X' = 2X - X T
+    if ((rc = tree_gemm<false, false>(c, n, nb, 1, A, X, A, mcp::SegOut{t, nn, 0}, 1.0, 0.0))) return rc;
+    if (step == 0) {
+      hipLaunchKernelGGL(mcp::k_mixed_resid, dim3(nb), dim3(256), 0, c->stream, n, (const double*)t, flag,
+                         mcp::kMixedTrust);
+      HIPCHK(c, hipGetLastError());
+    }
+    if ((rc = tree_gemm<false, false>(c, n, nb, 1, X, T, X, mcp::SegOut{x2, nn, 0}, -1.0, 2.0))) return rc;
+    std::swap(x, x2);
+  }
+  // the FP64 inversions: flagged beforehand (1) or by the guard (2)
+  hipLaunchKernelGGL(mcp::k_syminv_wide<double>, dim3(nb), dim3(mcp::kWNT), lds_wide(n), c->stream, n, a, kc, bk,
+                     (const int*)flag, 1);
+  HIPCHK(c, hipGetLastError());
+  hipLaunchKernelGGL(mcp::k_syminv_wide<double>, dim3(nb), dim3(mcp::kWNT), lds_wide(n), c->stream, n, a, kc, bk,
+                     (const int*)flag, 2);
+  HIPCHK(c, hipGetLastError());
+  hipLaunchKernelGGL(mcp::k_tree_zero, dim3(1), dim3(64), 0, c->stream, c->s_mcnt.as<int>(), 1);
+  hipLaunchKernelGGL(mcp::k_mixed_finish, dim3(ge), dim3(256), 0, c->stream, n, nb, (const double*)x, a,
+                     (const int*)flag, c->s_mcnt.as<int>());
+  HIPCHK(c, hipGetLastError());
+  int cnt = 0;
+  HIPCHK(c, hipMemcpyAsync(&cnt, c->s_mcnt.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->mixed_fp64 += cnt;
+  c->mixed_total += nb;
+  return MCP_OK;
+}
+
+// The tree's batched inversions: per matrices per instance, the first cut
+// of each instance's (short-range) FP64 when mixed precision is on.
+int tree_invert(mcp_ctx* c, int n, int nb, double* zb, int* kc, int bk, int per = 1, int cut = 0) {
   if (tree_on_teams(c, n, nb)) return team_inverse(c, n, nb, zb, kc, bk);
+  if (c->mixed_steps > 0 && n <= mcp::kWideMaxN && lds_wide<float>(n) <= c->lds_max && cut < per)
+    return mixed_inverse(c, n, nb, per, cut, zb, kc, bk);
   return batched_inverse(c, n, nb, zb, kc, bk);
 }
 
@@ -521,10 +609,28 @@ struct TreeBufs {
 int tree_alloc(mcp_ctx* c, int n, int ninst, int nleaf, TreeBufs& t) {
   const size_t nn = (size_t)n * n, sz = 3 * nn;
   const size_t k1 = (size_t)std::max((nleaf + 1) / 2, 1);
-  HIPCHK(c, c->s_seg0.ensure((size_t)ninst * nleaf * sz * sizeof(double)));
-  HIPCHK(c, c->s_seg1.ensure((size_t)ninst * k1 * sz * sizeof(double)));
-  HIPCHK(c, c->s_zb.ensure((size_t)ninst * nleaf * nn * sizeof(double)));
-  HIPCHK(c, c->s_pq.ensure((size_t)ninst * k1 * 2 * nn * sizeof(double)));
+  // the segment levels, and with mixed precision the FP32 copies and Newton
+  // iterates of the widest batch (the leaves).  When one of them must grow,
+  // all are given back first: the device then holds the new total, not the
+  // new sizes on top of larger ones kept from an earlier, wider call (which
+  // the caller's chunking counts as free).
+  const size_t nleafs = (size_t)ninst * nleaf;
+  const bool mixed = c->mixed_steps > 0;
+  DevBuf* bufs[] = {&c->s_seg0, &c->s_seg1, &c->s_zb, &c->s_pq, &c->s_m32, &c->s_mx, &c->s_mtt, &c->s_mx2};
+  const size_t need[] = {nleafs * sz * sizeof(double),
+                         (size_t)ninst * k1 * sz * sizeof(double),
+                         nleafs * nn * sizeof(double),
+                         (size_t)ninst * k1 * 2 * nn * sizeof(double),
+                         mixed ? nleafs * nn * sizeof(float) : 0,
+                         mixed ? nleafs * nn * sizeof(double) : 0,
+                         mixed ? nleafs * nn * sizeof(double) : 0,
+                         mixed ? nleafs * nn * sizeof(double) : 0};
+  bool grow = false;
+  for (int i = 0; i < 8; ++i) grow |= need[i] > 0 && (!bufs[i]->p || bufs[i]->bytes < need[i]);
+  if (grow)
+    for (DevBuf* b : bufs) b->release();
+  for (int i = 0; i < 8; ++i)
+    if (need[i] > 0) HIPCHK(c, bufs[i]->ensure(need[i]));
   HIPCHK(c, c->s_kc.ensure((size_t)2 * ninst * nleaf * sizeof(int)));
   HIPCHK(c, c->s_acc.ensure((size_t)2 * ninst * sizeof(int)));
   t.in = c->s_seg0.as<double>();
@@ -542,16 +648,19 @@ int tree_alloc(mcp_ctx* c, int n, int ninst, int nleaf, TreeBufs& t) {
 // The levels of a tree: the nleaf segments per instance at t.in merged
 // pairwise, (2m, 2m+1) -> m, until one is left (at t.in); the merge pivots'
 // negative-pivot counts summed into t.acc.
-int tree_levels(mcp_ctx* c, int n, int ninst, int nleaf, TreeBufs& t, int bk) {
+int tree_levels(mcp_ctx* c, int n, int ninst, int nleaf, TreeBufs& t, int bk, int cut_leaf = 0) {
   const size_t nn = (size_t)n * n, sz = 3 * nn;
   int rc;
+  int span = 1;  // leaves per segment at this level
   for (int kl = nleaf; kl > 1;) {
     const int per = kl / 2, kn = (kl + 1) / 2;
     const int nb = ninst * per;
+    // merge m covers leaves 2 m span ..; those starting below cut_leaf are short-range
+    const int cut = (cut_leaf + 2 * span - 1) / (2 * span);
     hipLaunchKernelGGL(mcp::k_tree_w, dim3(nblk256((size_t)nb * nn)), dim3(256), 0, c->stream, n, ninst, kl, per,
                        t.in, t.zb, tree_on_teams(c, n, nb) ? 1 : 0);
     HIPCHK(c, hipGetLastError());
-    if ((rc = tree_invert(c, n, nb, t.zb, t.kc, bk))) return rc;
+    if ((rc = tree_invert(c, n, nb, t.zb, t.kc, bk, per, cut))) return rc;
     hipLaunchKernelGGL(mcp::k_tree_count, dim3(nblk256(nb)), dim3(256), 0, c->stream, n, nb, per, t.kc, t.acc, t.bad);
     HIPCHK(c, hipGetLastError());
     const size_t si = (size_t)kl * sz, sm = 2 * sz;
@@ -576,6 +685,7 @@ int tree_levels(mcp_ctx* c, int n, int ninst, int nleaf, TreeBufs& t, int bk) {
     }
     std::swap(t.in, t.out);
     kl = kn;
+    span *= 2;
   }
   return MCP_OK;
 }
@@ -583,10 +693,10 @@ int tree_levels(mcp_ctx* c, int n, int ninst, int nleaf, TreeBufs& t, int bk) {
 // The whole map applied to the starting matrices: t.zb holds A0 + Y1 of each
 // instance's last segment (at t.in); res = Y4 - Y2^T (A0 + Y1)^-1 Y2, its
 // tiles on and below the diagonal.
-int tree_final(mcp_ctx* c, int n, int ninst, TreeBufs& t, int bk, double* res) {
+int tree_final(mcp_ctx* c, int n, int ninst, TreeBufs& t, int bk, double* res, bool fp64 = false) {
   const size_t nn = (size_t)n * n, sz = 3 * nn;
   int rc;
-  if ((rc = tree_invert(c, n, ninst, t.zb, t.kc, bk))) return rc;
+  if ((rc = tree_invert(c, n, ninst, t.zb, t.kc, bk, 1, fp64 ? 1 : 0))) return rc;
   hipLaunchKernelGGL(mcp::k_tree_count, dim3(nblk256(ninst)), dim3(256), 0, c->stream, n, ninst, 1, t.kc, t.acc,
                      t.bad);
   HIPCHK(c, hipGetLastError());
@@ -608,7 +718,8 @@ size_t tree_bytes(int n, int ninst, int nstep) {
 // else uc, uck the energy-independent sector matrices and their inertias.
 // Y in I.y is replaced by Y(RSTOP).
 int tree_mdprop(mcp_ctx* c, const mcp::BlockArgs& B, const mcp::GridArgs& G, const mcp::InstArgs& I, int ninst,
-                int nstep, double h, bool fused, const double* uc, const int* uck, const double* diag, int bk) {
+                int nstep, double h, bool fused, const double* uc, const int* uck, const double* diag, int bk,
+                int cut_leaf = 0) {
   const int n = B.n;
   const size_t nn = (size_t)n * n;
   TreeBufs t;
@@ -622,25 +733,25 @@ int tree_mdprop(mcp_ctx* c, const mcp::BlockArgs& B, const mcp::GridArgs& G, con
     hipLaunchKernelGGL(mcp::k_tree_zero, dim3(nblk256(nleaf)), dim3(256), 0, c->stream, epos, (int)nleaf);
     hipLaunchKernelGGL(mcp::k_tree_m, leaf_grid, dim3(256), 0, c->stream, B, G, I, ninst, nstep, h, diag, t.zb, epos,
                        tree_on_teams(c, n, (int)nleaf) ? 1 : 0);
-    if ((rc = tree_invert(c, n, (int)nleaf, t.zb, t.kc, bk))) return rc;
+    if ((rc = tree_invert(c, n, (int)nleaf, t.zb, t.kc, bk, nstep, cut_leaf))) return rc;
     hipLaunchKernelGGL(mcp::k_tree_count_leaf, dim3(nblk256(nleaf)), dim3(256), 0, c->stream, n, (int)nleaf, nstep,
                        t.kc, epos, t.acc, t.bad);
   } else {
     hipLaunchKernelGGL(mcp::k_tree_c1, dim3(nblk256(nleaf * nn)), dim3(256), 0, c->stream, n, ninst, nstep, uc, diag,
                        t.zb);
-    if ((rc = tree_invert(c, n, (int)nleaf, t.zb, t.kc, bk))) return rc;
+    if ((rc = tree_invert(c, n, (int)nleaf, t.zb, t.kc, bk, nstep, cut_leaf))) return rc;
     hipLaunchKernelGGL(mcp::k_tree_count, dim3(nblk256(nleaf)), dim3(256), 0, c->stream, n, (int)nleaf, nstep, t.kc,
                        t.acc, t.bad);
   }
   hipLaunchKernelGGL(mcp::k_tree_leaf, leaf_grid, dim3(256), 0, c->stream, B, G, I, ninst, nstep, h, diag, t.zb,
                      fused ? 1 : 0, t.in);
   HIPCHK(c, hipGetLastError());
-  if ((rc = tree_levels(c, n, ninst, nstep, t, bk))) return rc;
+  if ((rc = tree_levels(c, n, ninst, nstep, t, bk, cut_leaf))) return rc;
   // the starting matrix: Y = Y4 - Y2^T (A0 + Y1)^-1 Y2
   hipLaunchKernelGGL(mcp::k_tree_final_w, dim3(nblk256((size_t)ninst * nn)), dim3(256), 0, c->stream, B, G, I, ninst, h,
                      t.in, t.zb);
   HIPCHK(c, hipGetLastError());
-  if ((rc = tree_final(c, n, ninst, t, bk, t.out))) return rc;
+  if ((rc = tree_final(c, n, ninst, t, bk, t.out, cut_leaf > 0))) return rc;
   hipLaunchKernelGGL(mcp::k_tree_out, dim3(nblk256((size_t)ninst * nn)), dim3(256), 0, c->stream, n, ninst, h, t.out,
                      I.y);
   hipLaunchKernelGGL(mcp::k_tree_nodes, dim3(nblk256(ninst)), dim3(256), 0, c->stream, ninst, nstep,
@@ -727,8 +838,16 @@ int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const do
       if (c->large_mode == 0 && b.nham <= mcp::kLeafMaxHam) {
         size_t freeb = 0, totalb = 0;
         (void)hipMemGetInfo(&freeb, &totalb);
-        const size_t held = c->s_seg0.bytes + c->s_seg1.bytes + c->s_zb.bytes + c->s_pq.bytes + c->s_dg.bytes;
-        const size_t per = tree_bytes(n, 1, nstep) + (size_t)3 * nstep * n * sizeof(double);
+        const size_t held = c->s_seg0.bytes + c->s_seg1.bytes + c->s_zb.bytes + c->s_pq.bytes + c->s_dg.bytes +
+                            c->s_m32.bytes + c->s_mx.bytes + c->s_mtt.bytes + c->s_mx2.bytes;
+        size_t per = tree_bytes(n, 1, nstep) + (size_t)3 * nstep * n * sizeof(double);
+        // mixed precision: the FP32 copies, the Newton iterates and products
+        // of the widest level (the leaves)
+        if (c->mixed_steps > 0) per += (size_t)nstep * n * n * (3 * sizeof(double) + sizeof(float)) + nstep * sizeof(int);
+        // the short-range sectors (starting below mixed_r64) stay FP64
+        int cut_leaf = 0;
+        if (c->mixed_steps > 0)
+          while (cut_leaf < nstep && g.h_r[2 * cut_leaf] < c->mixed_r64) ++cut_leaf;
         const size_t room = freeb + held > (size_t(2) << 30) ? freeb + held - (size_t(2) << 30) : 0;
         // as many instances per tree as the device memory holds, in turn
         const int chunk = (int)std::min<size_t>((size_t)ninst, room / per);
@@ -758,7 +877,8 @@ int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const do
             HIPCHK(c, hipStreamSynchronize(c->stream));
             const bool fused = flag == 0;
             if (!fused && (rc = need_uc())) return rc;
-            if ((rc = tree_mdprop(c, B, G, Ic, m, nstep, h, fused, ucp, uck, c->s_dg.as<double>(), bk))) return rc;
+            if ((rc = tree_mdprop(c, B, G, Ic, m, nstep, h, fused, ucp, uck, c->s_dg.as<double>(), bk, cut_leaf)))
+              return rc;
           }
           HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
           return MCP_OK;
@@ -951,8 +1071,12 @@ int airy_prop_global(mcp_ctx* c, Block& b, Grid& g, int ninst, const double* d_e
     // in parallel over the steps where the device memory holds the tree
     size_t freeb = 0, totalb = 0;
     (void)hipMemGetInfo(&freeb, &totalb);
-    const size_t held = c->s_seg0.bytes + c->s_seg1.bytes + c->s_zb.bytes + c->s_pq.bytes;
-    if (freeb + held > tree_bytes(n, ninst, nstep) + (size_t(2) << 30)) {
+    const size_t held = c->s_seg0.bytes + c->s_seg1.bytes + c->s_zb.bytes + c->s_pq.bytes + c->s_m32.bytes +
+                        c->s_mx.bytes + c->s_mtt.bytes + c->s_mx2.bytes;
+    // mixed precision adds the FP32 copies and Newton buffers of the widest level
+    const size_t mixed_extra =
+        c->mixed_steps > 0 ? (size_t)ninst * nstep * nn * (3 * sizeof(double) + sizeof(float)) : 0;
+    if (freeb + held > tree_bytes(n, ninst, nstep) + mixed_extra + (size_t(2) << 30)) {
       TreeBufs t;
       if ((rc = tree_alloc(c, n, ninst, nstep, t))) return rc;
       const size_t nl = (size_t)ninst * nstep;
@@ -1937,6 +2061,23 @@ int mcp_set_large_mode(mcp_ctx* c, int32_t mode) {
     return c ? c->fail(MCP_ERR_ARG, "large mode %d: 0, 1 or 2, plus 4", mode) : MCP_ERR_ARG;
   c->large_mode = mode & 3;
   c->eig_global = (mode & 4) != 0;
+  return MCP_OK;
+}
+
+int mcp_set_precision(mcp_ctx* c, int32_t newton_steps, double r_fp64) {
+  if (!c || newton_steps < 0 || newton_steps > 4)
+    return c ? c->fail(MCP_ERR_ARG, "precision: 0 (FP64) to 4 Newton steps, not %d", newton_steps) : MCP_ERR_ARG;
+  c->mixed_steps = newton_steps;
+  c->mixed_r64 = r_fp64;
+  c->mixed_fp64 = 0;
+  c->mixed_total = 0;
+  return MCP_OK;
+}
+
+int mcp_precision_stats(const mcp_ctx* c, int64_t* mixed_matrices, int64_t* fp64_again) {
+  if (!c || !mixed_matrices || !fp64_again) return MCP_ERR_ARG;
+  *mixed_matrices = c->mixed_total;
+  *fp64_again = c->mixed_fp64;
   return MCP_OK;
 }
 

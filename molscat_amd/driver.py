@@ -215,7 +215,7 @@ class SweepResult:
 
 def sweep(eng, jlevels: Sequence[int], be: float, lambdas: Sequence[int], potl: S.PowerPotential, ured: float,
           energies: Sequence[float], jtots: Sequence[int], ibs: Sequence[int], ld_grid, airy_grid, rmax: float,
-          tuner=None, runit=1.0, progress=None, small_max: int = -1) -> SweepResult:
+          tuner=None, runit=1.0, progress=None, small_max: int = -1, mixed=None) -> SweepResult:
     """The partial-wave and energy sweep on one radial grid for every block:
     the reference's JTOT / IB / energy loops (source_code/mol.driver.f:
     1181-1856) for a fixed LDMD grid (ld_grid = (r, dr) of mcp_grid_create)
@@ -231,8 +231,10 @@ def sweep(eng, jlevels: Sequence[int], be: float, lambdas: Sequence[int], potl: 
     autotuner chose for their N (small_max >= 0 lowers the first group's
     N limit, e.g. 0 sends every block the second way).  Every energy of a
     block shares its U(C)
-    sector matrices and AIRY step bases.  A failed instance raises, as the
-    reference stops (source_code/mdprop.f:282-287)."""
+    sector matrices and AIRY step bases.  mixed = (newton_steps, r_fp64,
+    n_min) runs the large blocks with N >= n_min in mixed precision
+    (mcp_set_precision).  A failed instance raises, as the reference stops
+    (source_code/mdprop.f:282-287)."""
     import time as _time
 
     energies = np.asarray(energies, dtype=np.float64)
@@ -311,7 +313,14 @@ def sweep(eng, jlevels: Sequence[int], be: float, lambdas: Sequence[int], potl: 
         t0 = _time.perf_counter()
         h = eng.block(blk.vl, blk.eint, blk.cent, mxlam=blk.mxlam)
         grids, iprops = ([g, ga], [6, 9]) if ga is not None else ([g], [6])
-        out = eng.smatrix_segments(h, grids, iprops, ered, wv, nops, nb, rmax)
+        use_mixed = mixed is not None and blk.n >= mixed[2]
+        if use_mixed:
+            eng.set_precision(mixed[0], mixed[1])
+        try:
+            out = eng.smatrix_segments(h, grids, iprops, ered, wv, nops, nb, rmax)
+        finally:
+            if use_mixed:
+                eng.set_precision(0)
         eng.destroy_block(h)
         secs["large"] += _time.perf_counter() - t0
         if tuner is not None:

@@ -449,6 +449,16 @@ class Engine:
         """Batched large-N inversions: 1 (default) 32-pivot passes for N <= 448, 0 16-pivot passes."""
         self._check(self.lib.mcp_set_inverse_kernel(self.ctx, int(mode)), "mcp_set_inverse_kernel")
 
+    def set_precision(self, newton_steps: int = 0, r_fp64: float = 0.0):
+        """Mixed precision on the large-N tree's inversions: FP32 inverse +
+        newton_steps FP64 Newton-Schulz steps, FP64 below r_fp64; 0 = FP64."""
+        self._check(self.lib.mcp_set_precision(self.ctx, int(newton_steps), float(r_fp64)), "mcp_set_precision")
+
+    def precision_stats(self) -> dict:
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.lib.mcp_precision_stats(self.ctx, ctypes.byref(a), ctypes.byref(b)), "mcp_precision_stats")
+        return {"mixed_matrices": a.value, "fp64_again": b.value}
+
     def set_team_launch(self, cooperative: bool):
         """Team kernels through the cooperative launch API (default) or as
         ordinary launches of the same grids (for tracing tools)."""
