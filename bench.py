@@ -35,7 +35,7 @@ PEAK_FP64 = 78.6e12  # MI355X dense FP64 (vector and matrix), FLOP/s
 # stage's dispatches, each scaled by the counter's reading for a plain 256 MiB
 # copy), measured by the counter passes of profiles/rotor_N441_ld_stage_traffic.txt;
 # it changes only when the stage's kernels do
-LD_STAGE_TRAFFIC_BYTES = 567.264e9
+LD_STAGE_TRAFFIC_BYTES = 364.447e9
 
 
 def _cpu_baseline(blk, seg, ered, nsamp: int) -> dict:
@@ -570,7 +570,7 @@ def main() -> int:
                       "airy_step_bases": float(np.mean(t_abas)), "airy_propagation": float(np.mean(t_aprop))},
         "unitarity_err": unit_err,
         "roofline": {"bound": "mfma",
-                     "kernel": "LDMD propagation stage: k_mdprop_diag + tree (k_tree_m, k_syminv_large, "
+                     "kernel": "LDMD propagation stage: k_mdprop_diag + tree (k_tree_m, k_syminv_wide, "
                                "k_gemm_seg, k_tree_*)",
                      "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64, "traffic": LD_STAGE_TRAFFIC_BYTES,
