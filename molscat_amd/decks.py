@@ -15,7 +15,7 @@ would read, reduced to what the propagation path consumes.
 from __future__ import annotations
 
 from dataclasses import dataclass, field
-from typing import List
+from typing import List, Optional, Tuple
 
 import numpy as np
 
@@ -127,18 +127,19 @@ class FieldBlock:
     potl: S.PowerPotential
     mxlam: int = 3
     nconst: int = 1
+    scalam: float = 1.0                                    # SCAPOT's factor on the radial terms
+    expval: Optional[Tuple[int, int, float]] = None        # PERTRB's /EXPVAL/ (I, IPOW, DELTAN)
 
     @property
     def n(self) -> int:
         return len(self.eint)
 
     def ptab(self, r: np.ndarray) -> np.ndarray:
-        p = self.potl.table(np.asarray(r, dtype=np.float64) * self.rscale)
-        p = np.where(np.abs(p) > 1.0e60, np.sign(p) * 1.0e60, p)
-        return self.ep2ru * p
+        x = np.asarray(r, dtype=np.float64) * self.rscale
+        return S.hammat_p(self.potl.table(x), x, self.ep2ru, self.scalam, self.expval)
 
     def pconst(self, field_g: float) -> np.ndarray:
-        return np.array([field_g * self.cm2ru])
+        return S.pertrb_const(np.array([field_g * self.cm2ru]), self.mxlam, self.ep2ru, self.expval)
 
 
 def _pack_upper(m: np.ndarray) -> np.ndarray:

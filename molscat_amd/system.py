@@ -173,6 +173,47 @@ def lj_rotor_potential(cvals: Sequence[float], rm: float, epsil: float) -> Power
     return PowerPotential([[(c, -12), (-2.0 * c, -6)] for c in cvals], rm=rm, epsil=epsil)
 
 
+def hammat_p(p: np.ndarray, x: np.ndarray, ep2ru: float, scalam: float = 1.0,
+             expval: Optional[Tuple[int, int, float]] = None) -> np.ndarray:
+    """HAMMAT's P array (source_code/hammat.f:38-73) from POTENL's radial
+    coefficients p (npts, MXLAM) in EPSIL units at x = R*RSCALE: each |P|
+    clamped to 1e60, SCAPOT's scaling of the radial terms by SCALAM
+    (source_code/scapot-default.f), PERTRB's shift P(I) += DELTAN / x**IPOW
+    for /EXPVAL/ = (I, IPOW, DELTAN) with 1 <= I <= MXLAM (source_code/
+    pertrb.f; I outside the table changes nothing here, a constant term is
+    the block's pconst), then EP2RU (EPSIL units to reduced units)."""
+    p = np.where(np.abs(p) > 1.0e60, np.sign(p) * 1.0e60, p)
+    p = p * scalam
+    if expval is not None:
+        i, ipow, dn = int(expval[0]), int(expval[1]), float(expval[2])
+        if 1 <= i <= p.shape[1]:
+            p = p.copy()
+            p[:, i - 1] = p[:, i - 1] + dn / np.asarray(x, dtype=np.float64) ** ipow
+    return ep2ru * p
+
+
+def pertrb_const(pconst: np.ndarray, mxlam: int, ep2ru: float,
+                 expval: Optional[Tuple[int, int, float]] = None) -> np.ndarray:
+    """PERTRB on the NCONST constant terms (HAMMAT's P(MXLAM+1..), source_code/
+    hammat.f:55-67): P = VCONST / EP2CM in EPSIL units, shifted by DELTAN when
+    /EXPVAL/ names one of them, then EP2RU; pconst arrives as VCONST * CM2RU.
+    The device takes the constants once per instance, so a shift that depends
+    on R (IPOW != 0) is refused."""
+    pconst = np.asarray(pconst, dtype=np.float64)
+    if expval is None:
+        return pconst
+    i, ipow, dn = int(expval[0]), int(expval[1]), float(expval[2])
+    k = i - mxlam
+    if not 1 <= k <= len(pconst):
+        return pconst
+    if ipow != 0:
+        raise ValueError("PERTRB on a constant term with IPOW != 0 makes it depend on R; "
+                         "constants are taken once per instance")
+    out = pconst.copy()
+    out[k - 1] = ep2ru * (pconst[k - 1] / ep2ru + dn)
+    return out
+
+
 @dataclass
 class RotorBlock:
     """One (JTOT, IB) block of an atom + rigid rotor system in reduced units."""
@@ -191,6 +232,8 @@ class RotorBlock:
     potl: PowerPotential
     runit: float
     mxlam: int = field(init=False)
+    scalam: float = 1.0                                   # SCAPOT's factor (efvs SCALAM)
+    expval: Optional[Tuple[int, int, float]] = None       # PERTRB's /EXPVAL/ (I, IPOW, DELTAN)
 
     def __post_init__(self):
         self.mxlam = self.vl.shape[0]
@@ -200,10 +243,9 @@ class RotorBlock:
         return len(self.eint)
 
     def ptab(self, r: np.ndarray) -> np.ndarray:
-        """P(lambda, R) * EP2RU with HAMMAT's clamp (hammat.f:38-73)."""
-        p = self.potl.table(np.asarray(r, dtype=np.float64) * self.rscale)
-        p = np.where(np.abs(p) > 1.0e60, np.sign(p) * 1.0e60, p)
-        return self.ep2ru * p
+        """HAMMAT's P(lambda, R) in reduced units at the radii r (hammat_p)."""
+        x = np.asarray(r, dtype=np.float64) * self.rscale
+        return hammat_p(self.potl.table(x), x, self.ep2ru, self.scalam, self.expval)
 
 
 def rotor_block(jtot, ib, jlevels, be, lambdas, potl: PowerPotential, ured, runit=None) -> RotorBlock:

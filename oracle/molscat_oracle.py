@@ -264,19 +264,36 @@ class Block:
     vl_full: np.ndarray = field(default=None)  # (NHAM, N, N)
     nconst: int = 0  # constant terms: the last nconst VL blocks
     pconst: Optional[np.ndarray] = None  # their P values, VCONST * CM2RU (hammat.f:57-73)
+    scalam: float = 1.0  # SCAPOT's factor (source_code/scapot-default.f)
+    expval: Optional[Tuple[int, int, float]] = None  # PERTRB's /EXPVAL/ (I, IPOW, DELTAN), I 1-based
 
     def __post_init__(self):
         if self.vl_full is None:
             self.vl_full = np.stack([packed_to_full(self.vl[k], self.n) for k in range(self.vl.shape[0])])
 
     def pvec(self, r: float) -> np.ndarray:
-        """HAMMAT steps 1-6 (source_code/hammat.f:38-73): the radial terms,
-        then the NCONST constants."""
-        p = np.array(self.potl(r * self.rscale), dtype=np.float64)
+        """HAMMAT's P array (source_code/hammat.f:38-73), in its order: the
+        radial terms at R*RSCALE (EPSIL units) clamped to 1e60; the NCONST
+        constants appended as VCONST/EP2CM; SCAPOT multiplies the radial
+        terms by SCALAM; PERTRB adds DELTAN/(R*RSCALE)**IPOW to P(I) for
+        1 <= I <= MXLAM+NCONST (source_code/pertrb.f:19); then everything
+        times EP2RU."""
+        x = r * self.rscale
+        p = np.array(self.potl(x), dtype=np.float64)
         p = np.where(np.abs(p) > 1.0e60, np.sign(p) * 1.0e60, p)
+        mxlam = len(p)
+        p = p * self.scalam
+        i, ipow, dn = (0, 0, 0.0) if self.expval is None else (int(self.expval[0]), int(self.expval[1]),
+                                                                float(self.expval[2]))
+        if 0 < i <= mxlam:
+            p[i - 1] = p[i - 1] + dn / x ** ipow
         p = self.ep2ru * p
         if self.nconst > 0:
-            p = np.concatenate([p, np.asarray(self.pconst, dtype=np.float64)])
+            pc = np.array(self.pconst, dtype=np.float64)
+            if mxlam < i <= mxlam + self.nconst:
+                # the constant in EPSIL units (VCONST / EP2CM) takes the shift
+                pc[i - mxlam - 1] = self.ep2ru * (pc[i - mxlam - 1] / self.ep2ru + dn / x ** ipow)
+            p = np.concatenate([p, pc])
         return p
 
     def hammat(self, r: float, ered: float) -> Tuple[np.ndarray, np.ndarray]:

@@ -8,7 +8,11 @@ propagator in both directions).  Checks: node counts bit-exact against the
 oracle and the reference; EIGMIN against the oracle to 1e-9 of the largest
 eigenvalue modulus of M (5e-9 for MAPROP); the reference's converged level
 energies as zeros of the GPU's EIGMIN to 1e-10 cm-1; the level
-search reproducing them to 1e-9 cm-1; batch independence (bitwise)."""
+search reproducing them to 1e-9 cm-1; batch independence (bitwise).
+SCAPOT and PERTRB in the P table: with PERTRB adding DELTAN (IPOW = 0) to
+the isotropic term, whose coupling is the identity, every level moves by
+exactly DELTAN * EPSIL; with SCALAM and a 1/R^6 PERTRB on the anisotropic
+term, node counts bit-exact and EIGMIN against the oracle as above."""
 import json
 import os
 
@@ -39,9 +43,10 @@ def ref(golden_dir):
         return json.load(f)["decks"]
 
 
-def _setup(eng, iprop, jtot):
+def _setup(eng, iprop, jtot, scalam=1.0, expval=None):
     pot = S.PowerPotential(POT_TERMS, rm=3.5, epsil=50.0)
     blk = S.rotor_block(jtot, 2, [0, 2, 4, 6], 30.0, [0, 2], pot, 20.0)
+    blk.scalam, blk.expval = scalam, expval
     (r0, dro, no), (r1, dri, ni) = S.bound_segments(0.5, 1.0, 20.0, 0.001, 0.005 if iprop == 6 else None)
     ro = S.ld_radii(r0, dro, no)
     ri = S.ld_radii(r1, dri, ni)
@@ -50,7 +55,7 @@ def _setup(eng, iprop, jtot):
     gi = eng.grid(ri, blk.ptab(ri), dri)
     opot = O.PowerPotential(pot.terms, rm=pot.rm, epsil=pot.epsil)
     ob = O.Block(n=blk.n, vl=blk.vl, eint=blk.eint, cent=blk.cent, lvals=blk.lvals, levidx=blk.levidx, potl=opot,
-                 rscale=blk.rscale, ep2ru=blk.ep2ru, cm2ru=blk.cm2ru)
+                 rscale=blk.rscale, ep2ru=blk.ep2ru, cm2ru=blk.cm2ru, scalam=scalam, expval=expval)
     og = O.BoundGrid(0.5, 1.0, 20.0, no, dro, ni, dri)
     return blk, b, go, gi, ob, og
 
@@ -115,3 +120,30 @@ def test_batch_members_are_independent(eng):
         assert one["NODEC"][0] == all_["NODEC"][k]
         assert one["EIGMIN"][0] == all_["EIGMIN"][k]
         assert np.array_equal(one["PSIMCH"][0], all_["PSIMCH"][k])
+
+
+def test_pertrb_isotropic_shift_moves_every_level(eng, ref):
+    dn = 0.01  # EPSIL = 50 cm-1: every level 0.5 cm-1 higher
+    shift = dn * 50.0
+    blk, b, go, gi, _, _ = _setup(eng, 6, 0, expval=(1, 0, dn))
+    rb = _block(ref, 6, 0)
+    levels = BD.find_levels(eng, b, go, gi, blk.cm2ru, rb["emin"] + shift, rb["emax"] + shift, 6, 6)
+    assert [lv["STATE"] for lv in levels] == [st["state"] for st in rb["states"]]
+    for lv, st in zip(levels, rb["states"]):
+        assert abs(lv["ENERGY"] - (st["energy"] + shift)) < 1e-9, (lv, st["energy"])
+
+
+def test_scalam_and_pertrb_vs_oracle(eng):
+    blk, b, go, gi, ob, og = _setup(eng, 6, 1, scalam=0.97, expval=(2, 6, 0.3))
+    es = np.array([-9.0, -6.5, -3.1, -1.4])
+    out = eng.bound_match(b, go, gi, es * blk.cm2ru, 6, 6)
+    assert np.all(out["STATUS"] == 0)
+    for k, e in enumerate(es):
+        o = O.bound_match(ob, e * blk.cm2ru, og, 6, 6)
+        assert int(out["NODEC"][k]) == o["NODEC"], (e, out["NODEC"][k], o["NODEC"])
+        scale = max(1.0, float(np.abs(o["EVALS"]).max()))
+        assert abs(out["EIGMIN"][k] - o["EIGMIN"]) < 1e-9 * scale, (e, out["EIGMIN"][k], o["EIGMIN"])
+    # and the setting matters: the unperturbed deck gives other EIGMIN values
+    blk0, b0, go0, gi0, _, _ = _setup(eng, 6, 1)
+    out0 = eng.bound_match(b0, go0, gi0, es * blk.cm2ru, 6, 6)
+    assert np.abs(out0["EIGMIN"] - out["EIGMIN"]).max() > 1e-3
