@@ -75,20 +75,31 @@ def _copy(args) -> int:
 
 
 def _field(args) -> int:
-    """The field scan of the synthetic field-dependent deck (N = 48, one
-    constant term, the field B): 4096 fields uniform in [0, 1000] G in one
-    batched call, the collision energy 1e-4 cm-1 above each field's lowest
-    threshold; YTRANS's thresholds for all fields (mcp_ytrans), then YINIT
-    at 3.5 A, MDPROP to 25 A at DR = 0.02 A (1075 sectors), YTRANS and
-    YTOK/KTOS (mcp_smatrix_ld), timed from the first of the two device
-    calls to the last; CALCA's a(B) on the host afterwards.  Beside it the
-    oracle's rate for one field on one host thread."""
+    """The field scan of the N = 48 hyperfine deck (decks.hyperfine48: s-wave
+    channels, singlet/triplet projectors from a seeded random orthogonal
+    48 x 48 matrix, 48 Zeeman slopes ~ U[-2, 2] MHz/G, a dipolar coupling
+    of norm 1e-3 cm-1 times (RM/R)^3, one constant term, the field B):
+      * scattering: 4096 fields uniform in [0, 1000] G in one batched call,
+        the collision energy 1e-4 cm-1 above each field's lowest threshold;
+        YTRANS's thresholds for all fields (mcp_ytrans), then YINIT at
+        3.5 A, MDPROP to 25 A at DR = 0.02 A (1075 sectors), YTRANS and
+        YTOK/KTOS (mcp_smatrix_ld), timed from the first of the two device
+        calls to the last; CALCA's a(B) on the host afterwards;
+      * bound states: BOUND's node count and matching determinant at every
+        field in one batched call, 1 cm-1 below each field's lowest
+        threshold (outward 3.5 -> 6 A, inward 40 -> 6 A at DR = 0.02 A),
+        then the levels between 0.9 and 0.8 cm-1 below the lowest threshold
+        (a cluster of 12 split by the field) located at --bound-fields of
+        the fields >= 50 G (bound.levels_over_fields: every field's trial
+        energies in each batched call).
+    Beside them the oracle's rate for one field on one host thread."""
     os.environ.setdefault("OMP_NUM_THREADS", "1")
+    from molscat_amd import bound as BD
     from molscat_amd import decks
     from molscat_amd import system as S
     from molscat_amd.engine import Engine
 
-    fb = decks.field48()
+    fb = decks.hyperfine48()
     eng = Engine(0)
     nf = args.fields
     fields = np.linspace(0.0, 1000.0, nf)
@@ -116,10 +127,36 @@ def _field(args) -> int:
     for _ in range(args.warmup):
         solve()
     t0 = time.perf_counter()
+    ms_prop = []
     for _ in range(args.steps):
         out = solve()
+        ms_prop.append(eng.last_timing()["ms_prop"])
     dt = (time.perf_counter() - t0) / args.steps
     a = [S.scattering_length(out["SR"][m][0, 0], out["SI"][m][0, 0], wv[m][int(nb[m][0])], 0) for m in range(nf)]
+    # the MDPROP kernel's work: per sector-step of one instance two
+    # inversions, two W builds and the elementwise passes
+    n, nham = fb.n, fb.mxlam + fb.nconst
+    flops = nf * nstep * (2.0 * n ** 3 + nham * n * (n + 1) + 12.0 * n * n)
+    kern_s = float(np.mean(ms_prop)) * 1e-3
+
+    # BOUND: node counts and EIGMIN at every field in one call, then levels
+    rmin, rmatch, rmax, drb = 3.5, 6.0, 40.0, 0.02
+    (r0, dro, no), (r1, dri, ni) = S.bound_segments(rmin, rmatch, rmax, drb)
+    ro, ri = S.ld_radii(r0, dro, no), S.ld_radii(r1, dri, ni)
+    go, gi = eng.grid(ro, fb.ptab(ro), dro), eng.grid(ri, fb.ptab(ri), dri)
+    e_lo, e_hi = -0.9, -0.8
+    ebd = eint.min(axis=1) - 1.0 * fb.cm2ru
+    eng.bound_match(b, go, gi, ebd, pconst=pc)
+    t1 = time.perf_counter()
+    bm = eng.bound_match(b, go, gi, ebd, pconst=pc)
+    t_nodec = time.perf_counter() - t1
+    if np.any(bm["STATUS"] != 0):
+        raise RuntimeError("bound matching failed")
+    # a subset of the scan's fields away from B = 0 (where the clusters are degenerate)
+    sub = fields[fields >= 50.0][:: max(1, nf // max(1, args.bound_fields))]
+    t1 = time.perf_counter()
+    lev = BD.levels_over_fields(eng, fb, sub, e_lo, e_hi, rmin, rmatch, rmax, drb, width=1e-7)
+    t_lev = time.perf_counter() - t1
     cpu = None
     if args.cpu_sectors > 0:
         from oracle import molscat_oracle as O
@@ -138,9 +175,18 @@ def _field(args) -> int:
         "value": nf / dt, "unit": "S-matrices/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"field deck N=48, {nf} fields in [0, 1000] G, E = threshold + 1e-4 cm-1, "
+        "config": {"workload": f"hyperfine deck N=48, {nf} fields in [0, 1000] G, E = threshold + 1e-4 cm-1, "
                                f"MDPROP 3.5-25 A ({nstep} sectors), one batched call", "fields": nf},
         "a_range": [float(np.nanmin(np.real(a))), float(np.nanmax(np.real(a)))],
+        "roofline": {"bound": "mfma", "kernel": "MDPROP stage of the scan (k_mdprop_lds, one workgroup per field)",
+                     "achieved": flops / kern_s / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                     "frac": flops / kern_s / PEAK_FP64, "traffic": None, "flops_per_launch": flops,
+                     "ms_per_launch": kern_s * 1e3},
+        "bound": {"fields_one_call": nf, "ms_nodec_all_fields": t_nodec * 1e3,
+                  "nodec_range": [int(bm["NODEC"].min()), int(bm["NODEC"].max())],
+                  "sectors_per_field": no + ni, "nodec_at_cm": -1.0, "level_window_cm": [e_lo, e_hi], "level_fields": len(sub),
+                  "levels_found": int(len(lev["STATE"])), "level_search_s": t_lev,
+                  "levels_per_s": len(lev["STATE"]) / t_lev if t_lev > 0 else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
@@ -465,6 +511,8 @@ def main() -> int:
     ap.add_argument("--workload", choices=["rotor441", "allprop", "field", "copy", "sweep", "mixed-sweep"], default="rotor441",
                     help="rotor441: the N=441 deck (default); allprop: every propagator on the N=12 deck; "
                          "field: the N=48 field scan")
+    ap.add_argument("--bound-fields", type=int, default=128,
+                    help="--workload field: fields (an even subset of the scan) whose levels are located")
     ap.add_argument("--fields", type=int, default=4096, help="fields per call for --workload field")
     ap.add_argument("--energies", type=int, default=256, help="energies batched per call for --workload allprop")
     args = ap.parse_args()

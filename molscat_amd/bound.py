@@ -185,3 +185,101 @@ def field_levels(eng, fb, delta_cm: float, fields, rmin: float, rmatch: float, r
         eng.destroy_grid(go)
         eng.destroy_grid(gi)
     return out
+
+
+def levels_over_fields(eng, fb, fields, e_lo: float, e_hi: float, rmin: float, rmatch: float, rmax: float,
+                       dr: float, nsect: int = 3, width: float = 1e-5, etol: float = 1e-12,
+                       maxit: int = 60) -> Dict:
+    """BOUND at every field of `fields` for a block with one constant term
+    (the field): the levels between e_lo and e_hi cm-1 relative to each
+    field's lowest threshold (YTRANS's), located as BOUND does (source_code/
+    bd.driver.f:879-1215: NODEC(E) counts the states below E, EIGMIN(E)
+    vanishes at a level), every field's trial energies in each batched
+    ``mcp_bound_match`` call: the node counts at both ends of the window for
+    all fields, multisection on NODEC for every (field, state) pair until
+    its bracket is narrower than `width`, then Illinois regula falsi on
+    EIGMIN inside the brackets.
+
+    Returns arrays over the pairs: FIELD_INDEX, STATE (the 1-based count of
+    states below the level at that field), ENERGY (cm-1 relative to the
+    field's lowest threshold), EIGMIN there; and per field NODEC_LO,
+    NODEC_HI (the node counts at e_lo and e_hi), THRESHOLD (cm-1)."""
+    from . import system as S
+
+    fields = np.asarray(fields, dtype=np.float64)
+    nf = len(fields)
+    b = eng.block(fb.vl, fb.eint, fb.cent, mxlam=fb.mxlam, nconst=fb.nconst)
+    (r0, dro, no), (r1, dri, ni) = S.bound_segments(rmin, rmatch, rmax, dr)
+    ro = S.ld_radii(r0, dro, no)
+    ri = S.ld_radii(r1, dri, ni)
+    go = eng.grid(ro, fb.ptab(ro), dro)
+    gi = eng.grid(ri, fb.ptab(ri), dri)
+    try:
+        pc = np.stack([fb.pconst(x) for x in fields])
+        eint, _ = eng.ytrans(b, pc)
+        thr = eint.min(axis=1)
+
+        def evaluate(idx, erel):
+            r = eng.bound_match(b, go, gi, thr[idx] + np.asarray(erel) * fb.cm2ru, pconst=pc[idx])
+            if np.any(r["STATUS"] != 0):
+                raise RuntimeError(f"bound_match failed: {r['STATUS']}")
+            return r["NODEC"].astype(np.int64), r["EIGMIN"]
+
+        ends, _ = evaluate(np.concatenate([np.arange(nf), np.arange(nf)]),
+                           np.concatenate([np.full(nf, e_lo), np.full(nf, e_hi)]))
+        nlo, nhi = ends[:nf], ends[nf:]
+        cnt = np.maximum(nhi - nlo, 0)
+        pf = np.repeat(np.arange(nf), cnt)
+        pn = np.concatenate([np.arange(nlo[f] + 1, nhi[f] + 1) for f in range(nf)]) if pf.size else \
+            np.zeros(0, dtype=np.int64)
+        lo = np.full(pf.size, float(e_lo))
+        hi = np.full(pf.size, float(e_hi))
+        # multisection on the node count: state n lies where NODEC steps to n
+        frac = np.arange(1, nsect + 1) / (nsect + 1.0)
+        for _ in range(maxit):
+            act = np.where(hi - lo > width)[0]
+            if act.size == 0:
+                break
+            pts = lo[act, None] + (hi - lo)[act, None] * frac[None, :]
+            nod, _ = evaluate(np.repeat(pf[act], nsect), pts.ravel())
+            above = nod.reshape(act.size, nsect) >= pn[act, None]
+            # the first trial point at or above the state, and the last below it
+            hi[act] = np.where(above.any(axis=1), np.where(above, pts, np.inf).min(axis=1), hi[act])
+            lo[act] = np.where((~above).any(axis=1), np.where(~above, pts, -np.inf).max(axis=1), lo[act])
+        # Illinois regula falsi on EIGMIN inside each bracket
+        if pf.size:
+            _, f = evaluate(np.concatenate([pf, pf]), np.concatenate([lo, hi]))
+            fa, fbv = f[:pf.size].copy(), f[pf.size:].copy()
+        else:
+            fa = fbv = np.zeros(0)
+        side = np.zeros(pf.size, dtype=np.int64)
+        best_e = np.where(np.abs(fa) < np.abs(fbv), lo, hi)
+        best_f = np.where(np.abs(fa) < np.abs(fbv), fa, fbv)
+        for _ in range(maxit):
+            act = np.where((hi - lo > etol * np.maximum(1.0, np.abs(hi))) & (fa * fbv < 0.0))[0]
+            if act.size == 0:
+                break
+            trial = (fbv[act] * lo[act] - fa[act] * hi[act]) / (fbv[act] - fa[act])
+            _, ft = evaluate(pf[act], trial)
+            better = np.abs(ft) <= np.abs(best_f[act])
+            best_e[act] = np.where(better, trial, best_e[act])
+            best_f[act] = np.where(better, ft, best_f[act])
+            zero = ft == 0.0
+            right = (~zero) & (ft * fbv[act] > 0.0)
+            left = (~zero) & ~right
+            a = act[zero]
+            lo[a] = hi[a] = trial[zero]
+            a = act[right]
+            hi[a], fbv[a] = trial[right], ft[right]
+            fa[a] = np.where(side[a] == -1, 0.5 * fa[a], fa[a])
+            side[a] = -1
+            a = act[left]
+            lo[a], fa[a] = trial[left], ft[left]
+            fbv[a] = np.where(side[a] == 1, 0.5 * fbv[a], fbv[a])
+            side[a] = 1
+        return dict(FIELD_INDEX=pf, STATE=pn, ENERGY=best_e, EIGMIN=best_f, NODEC_LO=nlo, NODEC_HI=nhi,
+                    THRESHOLD=thr / fb.cm2ru)
+    finally:
+        eng.destroy_block(b)
+        eng.destroy_grid(go)
+        eng.destroy_grid(gi)

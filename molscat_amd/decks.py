@@ -175,3 +175,39 @@ def field48(seed: int = 4, zeeman=None) -> FieldBlock:
     lvals = np.tile(lv, na)
     return FieldBlock(vl=vl, eint=np.zeros(na * 4), cent=(lvals * (lvals + 1)).astype(np.float64), lvals=lvals,
                       levidx=np.repeat(np.arange(na), 4), cm2ru=c2r, ep2ru=1.0 * c2r, rscale=runit / rm, potl=potl)
+
+
+def hyperfine48(seed: int = 11, nsinglet: int = 12, dnorm: float = 1.0e-3) -> FieldBlock:
+    """The field-scan system of the batched BOUND/FIELD workload: N = 48
+    s-wave hyperfine channels (L = 0 throughout, so YTRANS has one set) with
+    H(R, B) = T + V_s(R) P_s + V_t(R) P_t + B Z + d(R) D:
+      * P_s = Q_s Q_s^T over the first nsinglet columns of a seeded random
+        orthogonal 48 x 48 matrix Q (QR of a Gaussian matrix, signs fixed by
+        R's diagonal), P_t = I - P_s;
+      * V_s, V_t LJ 12-6 curves of depth 4000 and 200 cm-1 at RM = 5 A
+        (POTENL case 1, EPSIL = 1 cm-1);
+      * Z = diag(s_i), Zeeman slopes s_i ~ U[-2, 2] MHz/G, one per channel,
+        a constant term whose value is the field B in G;
+      * D a seeded symmetric matrix of spectral norm dnorm cm-1 with
+        d(R) = (RM/R)^3.
+    URED = 40 amu, RUNIT = 1 A; EINT unused (NCONST = 1), the thresholds
+    are YTRANS's: s_i B."""
+    rng = np.random.default_rng(seed)
+    n = 48
+    g = rng.standard_normal((n, n))
+    q, r = np.linalg.qr(g)
+    q = q * np.sign(np.diag(r))
+    ps = q[:, :nsinglet] @ q[:, :nsinglet].T
+    pt = np.eye(n) - ps
+    slopes = rng.uniform(-2.0, 2.0, n)
+    d = rng.standard_normal((n, n))
+    d = 0.5 * (d + d.T)
+    d *= dnorm / np.abs(np.linalg.eigvalsh(d)).max()
+    mats = [ps, pt, d, np.diag(slopes * MHZ_IN_INV_CM)]
+    vl = np.stack([_pack_upper(m) for m in mats])
+    ured, runit, rm = 40.0, 1.0, 5.0
+    c2r = S.cm2ru(ured, runit)
+    potl = S.PowerPotential([[(4000.0, -12), (-8000.0, -6)], [(200.0, -12), (-400.0, -6)], [(1.0, -3)]],
+                            rm=rm, epsil=1.0)
+    return FieldBlock(vl=vl, eint=np.zeros(n), cent=np.zeros(n), lvals=np.zeros(n, dtype=np.int64),
+                      levidx=np.arange(n), cm2ru=c2r, ep2ru=1.0 * c2r, rscale=runit / rm, potl=potl)

@@ -1,9 +1,11 @@
-"""GPU field scan (molscat_amd.driver.field_scan) on the field-dependent
-deck (N = 48, one constant term, the field): a(B) at several fields in one
-batched call against the oracle's S-matrix of each field turned into CALCA's
-scattering length, and the batch of 4096 fields (the scan's full size)
-finishing with every instance's status 0 and a(B) identical to the small
-batch's at the shared fields."""
+"""GPU field scan (molscat_amd.driver.field_scan) on the two N = 48 field
+decks (one constant term, the field; decks.field48 with its L structure
+and decks.hyperfine48, the s-wave deck with random orthogonal singlet and
+triplet projectors): a(B) at several fields in one batched call against the
+oracle's S-matrix of each field turned into CALCA's scattering length, and
+the batch of 4096 fields (the scan's full size) finishing with every
+instance's status 0, a(B) identical to a small batch's at shared fields and
+equal to the oracle's at three fields inside the batch."""
 import numpy as np
 import pytest
 
@@ -26,9 +28,10 @@ def eng():
     e.close()
 
 
-def test_scattering_length_vs_oracle(eng):
-    fb = decks.field48()
-    fields = [0.0, 137.0, 512.0, 1000.0]
+@pytest.mark.parametrize("deck", ["field48", "hyperfine48"])
+def test_scattering_length_vs_oracle(eng, deck):
+    fb = getattr(decks, deck)()
+    fields = [0.0, 137.0, 512.0, 1000.0] if deck == "field48" else [0.25, 137.0, 512.0, 1000.0]
     print(f"field scan N={fb.n} at {fields} G", flush=True)
     fs = D.field_scan(eng, fb, fields, 3.5, 25.0, 0.02)
     assert np.all(fs.status == 0)
@@ -46,11 +49,22 @@ def test_scattering_length_vs_oracle(eng):
         assert d < 1e-8
 
 
-def test_full_scan_4096_fields(eng):
-    fb = decks.field48()
+@pytest.mark.parametrize("deck", ["field48", "hyperfine48"])
+def test_full_scan_4096_fields(eng, deck):
+    fb = getattr(decks, deck)()
     fields = np.linspace(0.0, 1000.0, 4096)
-    print("field scan: 4096 fields in one call", flush=True)
+    print(f"field scan ({deck}): 4096 fields in one call", flush=True)
     fs = D.field_scan(eng, fb, fields, 3.5, 25.0, 0.02)
     assert np.all(fs.status == 0) and np.all(np.isfinite(fs.a))
     sub = D.field_scan(eng, fb, fields[[0, 1000, 4095]], 3.5, 25.0, 0.02)
     assert np.array_equal(sub.a, fs.a[[0, 1000, 4095]])
+    # members of the full batch against the oracle
+    nstep, dr = S.drset_constant(3.5, 25.0, 0.02)
+    for m in (1, 2047, 4095):
+        ob = _oracle_block(fb, fields[m])
+        ref = O.smatrix_ld(ob, fs.ered[m], O.Grid(3.5, 25.0, nstep, dr), iprop=6)
+        e0 = int(ref["NB"][0])
+        aref = O.calca(ref["SR"][0, 0], ref["SI"][0, 0], ref["WVEC"][e0], 0)
+        d = abs(fs.a[m] - aref) / abs(aref)
+        print(f"  [{m}] B={fields[m]:.4f} G a={fs.a[m].real:.6f} (oracle {aref.real:.6f}) rel {d:.1e}", flush=True)
+        assert d < 1e-8
