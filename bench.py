@@ -101,6 +101,7 @@ def _field(args) -> int:
 
     fb = decks.hyperfine48()
     eng = Engine(0)
+    eng.set_md_kernel(args.md_kernel)
     nf = args.fields
     fields = np.linspace(0.0, 1000.0, nf)
     b = eng.block(fb.vl, fb.eint, fb.cent, mxlam=fb.mxlam, nconst=fb.nconst)
@@ -140,23 +141,31 @@ def _field(args) -> int:
     kern_s = float(np.mean(ms_prop)) * 1e-3
 
     # BOUND: node counts and EIGMIN at every field in one call, then levels
-    rmin, rmatch, rmax, drb = 3.5, 6.0, 40.0, 0.02
-    (r0, dro, no), (r1, dri, ni) = S.bound_segments(rmin, rmatch, rmax, drb)
-    ro, ri = S.ld_radii(r0, dro, no), S.ld_radii(r1, dri, ni)
-    go, gi = eng.grid(ro, fb.ptab(ro), dro), eng.grid(ri, fb.ptab(ri), dri)
-    e_lo, e_hi = -0.9, -0.8
-    ebd = eint.min(axis=1) - 1.0 * fb.cm2ru
-    eng.bound_match(b, go, gi, ebd, pconst=pc)
-    t1 = time.perf_counter()
-    bm = eng.bound_match(b, go, gi, ebd, pconst=pc)
-    t_nodec = time.perf_counter() - t1
-    if np.any(bm["STATUS"] != 0):
-        raise RuntimeError("bound matching failed")
-    # a subset of the scan's fields away from B = 0 (where the clusters are degenerate)
-    sub = fields[fields >= 50.0][:: max(1, nf // max(1, args.bound_fields))]
-    t1 = time.perf_counter()
-    lev = BD.levels_over_fields(eng, fb, sub, e_lo, e_hi, rmin, rmatch, rmax, drb, width=1e-7)
-    t_lev = time.perf_counter() - t1
+    # (skipped with --bound-fields 0)
+    bound_line = None
+    if args.bound_fields > 0:
+        rmin, rmatch, rmax, drb = 3.5, 6.0, 40.0, 0.02
+        (r0, dro, no), (r1, dri, ni) = S.bound_segments(rmin, rmatch, rmax, drb)
+        ro, ri = S.ld_radii(r0, dro, no), S.ld_radii(r1, dri, ni)
+        go, gi = eng.grid(ro, fb.ptab(ro), dro), eng.grid(ri, fb.ptab(ri), dri)
+        e_lo, e_hi = -0.9, -0.8
+        ebd = eint.min(axis=1) - 1.0 * fb.cm2ru
+        eng.bound_match(b, go, gi, ebd, pconst=pc)
+        t1 = time.perf_counter()
+        bm = eng.bound_match(b, go, gi, ebd, pconst=pc)
+        t_nodec = time.perf_counter() - t1
+        if np.any(bm["STATUS"] != 0):
+            raise RuntimeError("bound matching failed")
+        # a subset of the scan's fields away from B = 0 (where the clusters are degenerate)
+        sub = fields[fields >= 50.0][:: max(1, nf // max(1, args.bound_fields))]
+        t1 = time.perf_counter()
+        lev = BD.levels_over_fields(eng, fb, sub, e_lo, e_hi, rmin, rmatch, rmax, drb, width=1e-7)
+        t_lev = time.perf_counter() - t1
+        bound_line = {"fields_one_call": nf, "ms_nodec_all_fields": t_nodec * 1e3,
+                      "nodec_range": [int(bm["NODEC"].min()), int(bm["NODEC"].max())],
+                      "sectors_per_field": no + ni, "nodec_at_cm": -1.0, "level_window_cm": [e_lo, e_hi],
+                      "level_fields": len(sub), "levels_found": int(len(lev["STATE"])), "level_search_s": t_lev,
+                      "levels_per_s": len(lev["STATE"]) / t_lev if t_lev > 0 else None}
     cpu = None
     if args.cpu_sectors > 0:
         from oracle import molscat_oracle as O
@@ -178,15 +187,13 @@ def _field(args) -> int:
         "config": {"workload": f"hyperfine deck N=48, {nf} fields in [0, 1000] G, E = threshold + 1e-4 cm-1, "
                                f"MDPROP 3.5-25 A ({nstep} sectors), one batched call", "fields": nf},
         "a_range": [float(np.nanmin(np.real(a))), float(np.nanmax(np.real(a)))],
-        "roofline": {"bound": "mfma", "kernel": "MDPROP stage of the scan (k_mdprop_lds, one workgroup per field)",
+        "roofline": {"bound": "mfma", "kernel": ("MDPROP stage of the scan (k_mdprop_reg<48>, one wavefront per field)"
+                                                if args.md_kernel == 2 or (args.md_kernel == 0 and nf >= 512) else
+                                                "MDPROP stage of the scan (k_mdprop_lds, one workgroup per field)"),
                      "achieved": flops / kern_s / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": flops / kern_s / PEAK_FP64, "traffic": None, "flops_per_launch": flops,
                      "ms_per_launch": kern_s * 1e3},
-        "bound": {"fields_one_call": nf, "ms_nodec_all_fields": t_nodec * 1e3,
-                  "nodec_range": [int(bm["NODEC"].min()), int(bm["NODEC"].max())],
-                  "sectors_per_field": no + ni, "nodec_at_cm": -1.0, "level_window_cm": [e_lo, e_hi], "level_fields": len(sub),
-                  "levels_found": int(len(lev["STATE"])), "level_search_s": t_lev,
-                  "levels_per_s": len(lev["STATE"]) / t_lev if t_lev > 0 else None},
+        "bound": bound_line,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
@@ -514,6 +521,9 @@ def main() -> int:
     ap.add_argument("--bound-fields", type=int, default=128,
                     help="--workload field: fields (an even subset of the scan) whose levels are located")
     ap.add_argument("--fields", type=int, default=4096, help="fields per call for --workload field")
+    ap.add_argument("--md-kernel", type=int, default=0, choices=[0, 1, 2],
+                    help="MDPROP kernel for N <= 64 (--workload field): 0 automatic, 1 LDS-resident, "
+                         "2 register-resident")
     ap.add_argument("--energies", type=int, default=256, help="energies batched per call for --workload allprop")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -358,6 +358,15 @@ int mcp_precision_stats(const mcp_ctx *ctx, int64_t *mixed_matrices, int64_t *fp
  * counterpart: a tuning knob for the host program. */
 int mcp_set_lds_threads(mcp_ctx *ctx, int32_t nt);
 
+/* Which MDPROP kernel runs for N <= 64 (NHAM <= 4): 0 (default) chooses
+ * the register-resident kernel (one wavefront per instance, each lane one
+ * row of Y in registers) for batches of 512 instances or more and the
+ * LDS-resident one below; 1 always the LDS-resident kernel; 2 always the
+ * register-resident one.  Same recurrence and pivot decisions, results equal
+ * to rounding.  No reference counterpart: a tuning knob for the host
+ * program. */
+int mcp_set_md_kernel(mcp_ctx *ctx, int32_t mode);
+
 /* Wall time in milliseconds of the last mcp_prop_ld / mcp_smatrix_ld call's
  * device work, measured with HIP events on the library's stream, split into
  * the energy-independent sector matrices and the propagation loop. */

@@ -24,12 +24,20 @@
 #include "mcp_airytol.hpp"
 #include "mcp_mgprop.hpp"
 #include "mcp_maprop.hpp"
+#include "mcp_regprop.hpp"
 
 namespace {
 
 using mcp::kNT;
 // threads per workgroup of the batched one-workgroup-per-matrix inversions
 constexpr int kInvNT = 512;
+// Instances from which the register-resident MDPROP (one wavefront per
+// instance) is chosen over the LDS-resident one for N <= 64: enough to put
+// a few waves on every CU.
+constexpr int kRegAutoMinInst = 512;
+// Largest buffer of the register-resident MDPROP's midpoint terms (3 N
+// doubles per instance and sector); more instances go in chunks.
+constexpr size_t kRegMidBytes = size_t(8) << 30;
 
 struct DevBuf {
   void* p = nullptr;
@@ -262,6 +270,7 @@ struct mcp_ctx {
   bool eig_global = false;  // DIAGVC and AIRY on the global-memory eigen-solver for every N
   bool coop = true;         // team kernels through the cooperative launch API
   int lds_threads = 0;      // threads per instance of the LDS-resident MDPROP (0: 256)
+  int md_kernel = 0;        // small-N MDPROP: 0 automatic, 1 LDS-resident, 2 register-resident
   int inv_kernel = 1;       // batched large-N inversions: 1 32-pivot passes (N <= 448), 0 16-pivot passes
   unsigned char* scratch_dev = nullptr;  // this device's address of scratch (mcp_ytok_ktos_noalloc)
   // multi-block MDPROP (mcp_prop_ld_multi): sector matrices, inertias, block
@@ -284,6 +293,7 @@ struct mcp_ctx {
   DevBuf s_av, s_at, s_ae, s_ah;  // AIRY step bases, basis changes, eigenvalues, slopes
   DevBuf s_x;                     // column exchange of the register-resident MDPROP
   DevBuf s_dg;                    // its per-sector reference-potential terms
+  DevBuf s_rw, s_rm;              // register-resident MDPROP: W(B) off the diagonal, midpoint terms
   DevBuf s_seg0, s_seg1;          // segments of the parallel-in-R MDPROP, two tree levels
   DevBuf s_zb, s_pq;              // its merge pivots / inverses and products
   DevBuf s_kc, s_acc;             // its negative-pivot counts
@@ -344,7 +354,9 @@ int set_lds_limits(mcp_ctx* c) {
                        (const void*)mcp::k_syminv_mfma<1>, (const void*)mcp::k_match_lu,
                        (const void*)mcp::k_ytrans,         (const void*)mcp::k_vivs,           (const void*)mcp::k_airy_tolhi,
                        (const void*)mcp::k_mgprop,         (const void*)mcp::k_maprop_basis,
-                       (const void*)mcp::k_maprop_mats,    (const void*)mcp::k_maprop_prop};
+                       (const void*)mcp::k_maprop_mats,    (const void*)mcp::k_maprop_prop,
+                       (const void*)mcp::k_mdprop_reg<16>, (const void*)mcp::k_mdprop_reg<32>,
+                       (const void*)mcp::k_mdprop_reg<48>, (const void*)mcp::k_mdprop_reg<64>};
   const size_t total = c->lds_max;
   size_t lim = total;
   for (const void* f : fns) {
@@ -810,7 +822,54 @@ int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const do
       // four waves per instance by default; one wave (mcp_set_lds_threads)
       // measured slower at N = 12 and 48 even with 16 instances per CU
       const int nt = c->lds_threads ? c->lds_threads : kNT;
-      if (nt == 64)
+      const bool reg_ok = n <= mcp::kRegMaxN && b.nham <= mcp::kRegMaxHam && !b.const_offdiag &&
+                          mcp::reg_lds_bytes((n + 15) / 16 * 16) <= c->lds_max;
+      const bool use_reg = reg_ok && (c->md_kernel == 2 || (c->md_kernel == 0 && ninst >= kRegAutoMinInst));
+      if (use_reg) {
+        // W(B) off the diagonal for every sector, shared by the instances
+        HIPCHK(c, c->s_rw.ensure((size_t)nstep * n * n * sizeof(double)));
+        hipLaunchKernelGGL(mcp::k_mdprop_wb, dim3(nstep), dim3(256), 0, c->stream, B, G, nstep, c->s_rw.as<double>());
+        HIPCHK(c, hipGetLastError());
+        // the midpoint terms of every (instance, sector, channel), in chunks
+        // of instances that keep them within kRegMidBytes
+        const size_t per_inst = (size_t)nstep * 3 * n * sizeof(double);
+        int chunk = (int)std::min<size_t>((size_t)ninst, std::max<size_t>(1, kRegMidBytes / per_inst));
+        if (chunk < ninst) chunk = std::max(mcp::kRegWaves, chunk / mcp::kRegWaves * mcp::kRegWaves);
+        HIPCHK(c, c->s_rm.ensure((size_t)chunk * per_inst));
+        const int np = (n + 15) / 16 * 16;
+        for (int i0 = 0; i0 < ninst; i0 += chunk) {
+          const int m = std::min(chunk, ninst - i0);
+          mcp::InstArgs Ic = I;
+          Ic.ered = I.ered + i0;
+          Ic.pconst = I.pconst ? I.pconst + (size_t)i0 * b.nconst : nullptr;
+          Ic.y = I.y + (size_t)i0 * n * n;
+          Ic.nodes = I.nodes + i0;
+          Ic.status = I.status + i0;
+          const size_t tot = (size_t)m * nstep * n;
+          const int mblocks = (int)std::min<size_t>((tot + 255) / 256, (size_t)c->num_cu * 64);
+          hipLaunchKernelGGL(mcp::k_mdprop_mid, dim3(mblocks), dim3(256), 0, c->stream, B, G, Ic, m, nstep, h,
+                             c->s_rm.as<double>());
+          HIPCHK(c, hipGetLastError());
+          const dim3 grid((m + mcp::kRegWaves - 1) / mcp::kRegWaves), blk(64 * mcp::kRegWaves);
+          const double* ucp = c->s_e.as<double>();
+          const int* uck = c->s_f.as<int>();
+          const double* wbp = c->s_rw.as<double>();
+          const double* midp = c->s_rm.as<double>();
+          if (np == 16)
+            hipLaunchKernelGGL(mcp::k_mdprop_reg<16>, grid, blk, mcp::reg_lds_bytes(16), c->stream, B, G, Ic, m, nstep,
+                               h, ucp, uck, wbp, midp, bk);
+          else if (np == 32)
+            hipLaunchKernelGGL(mcp::k_mdprop_reg<32>, grid, blk, mcp::reg_lds_bytes(32), c->stream, B, G, Ic, m, nstep,
+                               h, ucp, uck, wbp, midp, bk);
+          else if (np == 48)
+            hipLaunchKernelGGL(mcp::k_mdprop_reg<48>, grid, blk, mcp::reg_lds_bytes(48), c->stream, B, G, Ic, m, nstep,
+                               h, ucp, uck, wbp, midp, bk);
+          else
+            hipLaunchKernelGGL(mcp::k_mdprop_reg<64>, grid, blk, mcp::reg_lds_bytes(64), c->stream, B, G, Ic, m, nstep,
+                               h, ucp, uck, wbp, midp, bk);
+          HIPCHK(c, hipGetLastError());
+        }
+      } else if (nt == 64)
         hipLaunchKernelGGL(mcp::k_mdprop_lds<64>, dim3(ninst), dim3(64), need, c->stream, B, G, I, nstep, h,
                            c->s_e.as<double>(), c->s_f.as<int>(), bk);
       else
@@ -1329,6 +1388,10 @@ int mcp_block_create(mcp_ctx* c, int32_t n, int32_t mxlam, int32_t nconst, int32
   b->conlen4inv = conlen > 0.0 ? 1.0 / (conlen * conlen * conlen * conlen) : 0.0;
   const size_t ntri = (size_t)n * (n + 1) / 2;
   int rc;
+  // kRegMaxHam zeros past the end: the register-resident MDPROP reads
+  // kRegMaxHam coefficients per element whatever NHAM (times P = 0 beyond it)
+  HIPCHK(c, b->vl.ensure((ntri * b->nham + mcp::kRegMaxHam) * sizeof(double)));
+  HIPCHK(c, hipMemsetAsync(b->vl.p, 0, (ntri * b->nham + mcp::kRegMaxHam) * sizeof(double), c->stream));
   if ((rc = upload(c, b->vl, vl, ntri * b->nham))) return rc;
   if ((rc = upload(c, b->eint, eint, (size_t)n))) return rc;
   if ((rc = upload(c, b->cent, cent, (size_t)n))) return rc;
@@ -2380,6 +2443,12 @@ int mcp_prop_mg(mcp_ctx* c, int32_t bid, int32_t gid, int32_t imgsel, int32_t ns
   if ((rc = download(c, nodes, c->s_vnd, (size_t)ninst))) return rc;
   if ((rc = download(c, status, c->s_vst, (size_t)ninst))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MCP_OK;
+}
+
+int mcp_set_md_kernel(mcp_ctx* c, int32_t mode) {
+  if (!c || mode < 0 || mode > 2) return c ? c->fail(MCP_ERR_ARG, "md_kernel: 0, 1 or 2") : MCP_ERR_ARG;
+  c->md_kernel = mode;
   return MCP_OK;
 }
 

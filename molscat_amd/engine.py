@@ -441,6 +441,10 @@ class Engine:
                                          dptr(pc), dptr(y), iptr(nodes), iptr(st)), "mcp_prop_mg")
         return _from_colmajor(y), nodes, st
 
+    def set_md_kernel(self, mode: int):
+        """MDPROP kernel for N <= 64: 0 automatic, 1 LDS-resident, 2 register-resident (one wave per instance)."""
+        self._check(self.lib.mcp_set_md_kernel(self.ctx, int(mode)), "mcp_set_md_kernel")
+
     def set_lds_threads(self, nt: int):
         """Threads per instance of the LDS-resident MDPROP: 64, 256 or 0 (256)."""
         self._check(self.lib.mcp_set_lds_threads(self.ctx, int(nt)), "mcp_set_lds_threads")

@@ -248,6 +248,11 @@ module mcprop
       type(c_ptr), value :: ctx
       integer(c_int64_t) :: mixed_matrices, fp64_again
     end function
+    integer(c_int) function mcp_set_md_kernel(ctx, mode) bind(C, name="mcp_set_md_kernel")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: mode
+    end function
     integer(c_int) function mcp_set_lds_threads(ctx, nt) bind(C, name="mcp_set_lds_threads")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
