@@ -13,8 +13,11 @@ roofline of the LDMD propagation stage (the energy-independent sector
 matrices and the recurrence, solved as a tree of merged sector maps),
 priced at the reference algorithm's work per sector-step and timed with HIP
 events on the library's stream; the AIRY segment's times beside it; and a
-CPU baseline (the oracle's MDPROP on a bounded sample of sectors on this
-host, one thread).
+CPU baseline: the oracle's FP64 solve in the reference's call structure
+with LAPACK inversions, on a bounded sample of LD sectors and AIRY steps,
+on this host's cores (one process with P BLAS threads for one instance; P
+single-threaded processes over instances for the sweep and field scans),
+timed before the GPU is touched.
 """
 from __future__ import annotations
 
@@ -38,25 +41,223 @@ PEAK_FP64 = 78.6e12  # MI355X dense FP64 (vector and matrix), FLOP/s
 LD_STAGE_TRAFFIC_BYTES = 364.447e9
 
 
-def _cpu_baseline(blk, seg, ered, nsamp: int) -> dict:
-    """Oracle MDPROP (C SYMINV + numpy) on the first nsamp sectors, one
-    thread, in S-matrices/s (sector rate / sectors per solve)."""
-    os.environ.setdefault("OMP_NUM_THREADS", "1")
+def _host_cpu() -> dict:
+    """The host's CPU model, logical and physical core counts, and P, the
+    worker count of the CPU baselines: this process's CPU affinity, at most
+    16 (the host share of one GPU)."""
+    model, cores, phys = "unknown", set(), "0"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                key, _, val = ln.partition(":")
+                key, val = key.strip(), val.strip()
+                if key == "model name" and model == "unknown":
+                    model = val
+                elif key == "physical id":
+                    phys = val
+                elif key == "core id":
+                    cores.add((phys, val))
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0))
+    return {"model": model, "logical_cpus": os.cpu_count(), "physical_cores": len(cores) or None,
+            "affinity_cpus": aff, "p": max(1, min(aff, 16))}
+
+
+def _oracle_block(O, blk, **kw):
+    return O.Block(n=blk.n, vl=blk.vl, eint=blk.eint, cent=blk.cent, lvals=blk.lvals, levidx=blk.levidx,
+                   potl=O.PowerPotential(blk.potl.terms, rm=blk.potl.rm, epsil=blk.potl.epsil), rscale=blk.rscale,
+                   ep2ru=blk.ep2ru, cm2ru=blk.cm2ru, **kw)
+
+
+def _cpu_solve_sample(ob, ered, ld, airy, rmatch, nld, nai, reps) -> dict:
+    """The oracle's FP64 solve of one (block, energy) instance in the
+    reference's call structure, its symmetric inversions through LAPACK
+    DSYTRF/DSYTRI for N > 30 as SYMINV hands them to SYMINL
+    (source_code/syminl-alloc.f:17-41): YINIT, the first nld sectors of the
+    LD grid ld = (rstart, nstep, dr), the first nai AIRY steps of airy =
+    (rstart, rstop, nstep, dr0, powr) (or none), YTOK + KTOS at rmatch.  Per
+    part the median of reps runs; est = the time of the whole solve, each
+    part scaled to its full step count.  AIRY's per-channel sector
+    coefficients (O(N) Airy-function values per step, which the oracle
+    evaluates with mpmath at 40 digits) are computed before the timed runs
+    and not counted."""
     from oracle import molscat_oracle as O
 
-    pot = O.PowerPotential(blk.potl.terms, rm=blk.potl.rm, epsil=blk.potl.epsil)
-    ob = O.Block(n=blk.n, vl=blk.vl, eint=blk.eint, cent=blk.cent, lvals=blk.lvals, levidx=blk.levidx, potl=pot,
-                 rscale=blk.rscale, ep2ru=blk.ep2ru, cm2ru=blk.cm2ru)
-    y0 = np.diag(np.sqrt(np.abs(np.diag(ob.hammat(seg.rstart, ered)[0]))))
-    t0 = time.perf_counter()
-    O.mdprop(y0, lambda r: ob.hammat(r, ered), seg.rstart, nsamp, seg.dr)
-    dt = time.perf_counter() - t0
-    rate = nsamp / dt
-    return {"value": rate / seg.nstep, "unit": "S-matrices/s", "cores": 1, "kind": "port",
-            "sector_steps_per_s": rate,
-            "sample": f"oracle MDPROP, first {nsamp} of {seg.nstep} sectors at N={blk.n}, single thread "
-                      f"({dt:.1f} s, {rate:.2f} sector-steps/s; YINIT, the AIRY segment and matching not "
-                      f"counted)"}
+    inv = O.syminv_lapack if ob.n > 30 else O.syminv
+    rstart, nstep, dr = ld
+    hm = lambda r: ob.hammat(r, ered)
+    wvec, nopen = O.gtwvec(ered, ob.eint)
+    nb = O.chnsrt(ob.eint)
+    cache = []
+    if airy is not None and nai > 0:
+        ra0, ra1, nair, dr0, powr = airy
+        for rn, dn in O.airy_steps(ra0, ra1, nair, dr0, powr)[:nai]:
+            eig, _, hp = O.airy_potent(ob, ered, rn, dn)
+            cache.append(O.airy_sector(dn, eig, hp))
+    parts = {"yinit": [], "ld": [], "airy": [], "match": []}
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        y = O.yinit(ob, ered, rstart)
+        t1 = time.perf_counter()
+        y, _ = O.mdprop(y, hm, rstart, nld, dr, inv=inv)
+        t2 = time.perf_counter()
+        if cache:
+            it = iter(cache)
+            y, _ = O.aiprop(y, ob, ered, ra0, ra1, nair, dr0, powr, inv=inv, sector=lambda *a: next(it),
+                            limit=len(cache))
+        t3 = time.perf_counter()
+        if nopen > 0:
+            O.ktos(O.ytok(nb, wvec, ob.n, nopen, y, rmatch, ob.cent))
+        t4 = time.perf_counter()
+        parts["yinit"].append(t1 - t0)
+        parts["ld"].append((t2 - t1) / nld)
+        parts["airy"].append((t3 - t2) / max(1, len(cache)))
+        parts["match"].append(t4 - t3)
+    med = {k: float(np.median(v)) for k, v in parts.items()}
+    nair_full = airy[2] if airy is not None else 0
+    med["est"] = med["yinit"] + nstep * med["ld"] + (nair_full * med["airy"] if cache else 0.0) + med["match"]
+    med["n"] = ob.n
+    return med
+
+
+_POOL_JOBS: list = []
+
+
+def _pool_run(i):
+    """One job of a CPU-baseline pool, single-threaded: ("sample", args of
+    _cpu_solve_sample), ("smat", (block, energy, grid)) for the oracle's
+    whole solve, ("bound", (block, energy, bound grid)) for BOUND's
+    matching.  The job list is set before the pool forks."""
+    from threadpoolctl import threadpool_limits
+
+    from oracle import molscat_oracle as O
+
+    kind, job = _POOL_JOBS[i]
+    with threadpool_limits(1):
+        if kind == "sample":
+            return _cpu_solve_sample(*job)
+        ob, ered, grid = job
+        if kind == "smat":
+            O.smatrix_ld(ob, ered, grid, iprop=6, inv=O.syminv_lapack if ob.n > 30 else None)
+        else:
+            O.bound_match(ob, ered, grid)
+    return None
+
+
+def _cpu_pool_samples(jobs, p):
+    """_cpu_solve_sample over jobs in p single-threaded processes at once
+    (forked before this process touches the GPU); each job's times are its
+    own, taken while all p workers run."""
+    import multiprocessing as mp
+
+    _POOL_JOBS[:] = [("sample", j) for j in jobs]
+    with mp.get_context("fork").Pool(p) as pool:
+        return pool.map(_pool_run, range(len(jobs)), chunksize=1)
+
+
+def _cpu_field(fb, fields, nstep, h, bound, reps=3) -> dict:
+    """P single-threaded processes over fields (forked before this process
+    touches the GPU): the oracle's whole solve of each field of an even
+    sample of the scan (YTRANS's thresholds, YINIT, MDPROP, YTRANS, YTOK +
+    KTOS), eight fields per worker, and BOUND's matching (both propagations,
+    NODEC and EIGMIN) at four fields per worker; wall clock of each pass,
+    the median of reps passes."""
+    import multiprocessing as mp
+
+    from oracle import molscat_oracle as O
+
+    host = _host_cpu()
+    p = host["p"]
+    sub = fields[np.linspace(0, len(fields) - 1, 8 * p).astype(int)]
+    obs = [_oracle_block(O, fb, nconst=1, pconst=fb.pconst(f)) for f in sub]
+    thr = [O.ytrans(None, ob)[1].min() for ob in obs]
+    grid = O.Grid(3.5, 25.0, nstep, h)
+    scat = [("smat", (ob, t + 1.0e-4 * fb.cm2ru, grid)) for ob, t in zip(obs, thr)]
+    bnd = []
+    if bound is not None:
+        rmin, rmatch, rmax, drb = bound
+        bg = O.bound_grid(rmin, rmatch, rmax, drb)
+        bnd = [("bound", (ob, t - 1.0 * fb.cm2ru, bg)) for ob, t in zip(obs[::2], thr[::2])]  # 4 per worker
+    _POOL_JOBS[:] = scat + bnd
+    out = {"host": host, "cores": p}
+    with mp.get_context("fork").Pool(p) as pool:
+        for key, idx in (("smat", range(len(scat))), ("bound", range(len(scat), len(scat) + len(bnd)))):
+            if len(idx) == 0:
+                continue
+            walls = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                pool.map(_pool_run, idx, chunksize=1)
+                walls.append(time.perf_counter() - t0)
+            w = float(np.median(walls))
+            out[key] = (len(idx) / w, w, len(idx))
+    return out
+
+
+def _cpu_sweep(deck, blks, ne, nld, nai, reps=3) -> dict:
+    """P single-threaded processes over (block, energy) instances (forked
+    before this process touches the GPU): P instances whose block sizes
+    are even quantiles of the blocks' distinct N (both ends included), each
+    timed by _cpu_solve_sample while all P run; the per-instance time at
+    each block's N by log-log interpolation between the sampled sizes; the
+    blocks' total over ne energies each, divided by P."""
+    from molscat_amd import decks
+    from molscat_amd import system as S
+    from oracle import molscat_oracle as O
+
+    host = _host_cpu()
+    p = host["p"]
+    seg = deck.ld
+    powr = 4.0 / 3.0
+    nair, dr0 = S.drset_airy(seg.rstop, deck.airy_stop, deck.airy_dr, powr)
+    by_n = {b.n: b for b in blks}
+    ns = sorted(by_n)
+    pick = [ns[int(round(q))] for q in np.linspace(0, len(ns) - 1, p)]
+    ev = decks.sweep_energies()
+    jobs = []
+    for k, n in enumerate(pick):
+        b = by_n[n]
+        e = float(ev[(k * ne) // p]) * b.cm2ru
+        jobs.append((_oracle_block(O, b), e, (seg.rstart, seg.nstep, seg.dr),
+                     (seg.rstop, deck.airy_stop, nair, dr0, powr), deck.airy_stop, nld, nai, reps))
+    res = _cpu_pool_samples(jobs, p)
+    tn = {}
+    for r in res:
+        tn.setdefault(r["n"], []).append(r["est"])
+    nn = np.array(sorted(tn), dtype=float)
+    tt = np.array([np.median(tn[n]) for n in sorted(tn)])
+    est = {b.n: float(np.exp(np.interp(np.log(b.n), np.log(nn), np.log(tt)))) for b in blks}
+    t_one = sum(est[b.n] for b in blks) * ne
+    return {"host": host, "cores": p, "s_one_process": t_one, "wall": t_one / p,
+            "per_instance_s": {str(int(n)): float(t) for n, t in zip(nn, tt)}}
+
+
+def _cpu_rotor441(blk, deck, ered, nld, nai, reps=3) -> dict:
+    """One process, BLAS threads = P.  The oracle's solve of the
+    N=441 deck: YINIT, nld of the 3600 LD sectors, nai of the AIRY steps,
+    YTOK + KTOS; the median of reps runs of each part, scaled to the whole
+    solve."""
+    from threadpoolctl import threadpool_limits
+
+    from molscat_amd import system as S
+    from oracle import molscat_oracle as O
+
+    host = _host_cpu()
+    seg = deck.ld
+    ob = _oracle_block(O, blk)
+    powr = 4.0 / 3.0
+    nair, dr0 = S.drset_airy(seg.rstop, deck.airy_stop, deck.airy_dr, powr)
+    with threadpool_limits(host["p"]):
+        m = _cpu_solve_sample(ob, ered, (seg.rstart, seg.nstep, seg.dr), (seg.rstop, deck.airy_stop, nair, dr0, powr),
+                              deck.airy_stop, nld, nai, reps)
+    return {"value": 1.0 / m["est"], "unit": "S-matrices/s", "cores": host["p"], "kind": "port",
+            "host": host, "s_per_solve": m["est"], "sector_steps_per_s": 1.0 / m["ld"],
+            "airy_steps_per_s": 1.0 / m["airy"], "ms_yinit": m["yinit"] * 1e3, "ms_match": m["match"] * 1e3,
+            "sample": f"the oracle (numpy + LAPACK DSYTRF/DSYTRI from numpy's OpenBLAS, {host['p']} BLAS threads, "
+                      f"one process) at N={blk.n}: YINIT, the first {nld} of {seg.nstep} LD sectors, the first {nai} "
+                      f"of {nair} AIRY steps, YTOK + KTOS; median of {reps} runs per part, scaled to the whole solve "
+                      f"({m['est']:.1f} s); not counted: AIRY's per-channel Airy-function values (O(N) per step)"}
 
 
 def _copy(args) -> int:
@@ -92,20 +293,23 @@ def _field(args) -> int:
         (a cluster of 12 split by the field) located at --bound-fields of
         the fields >= 50 G (bound.levels_over_fields: every field's trial
         energies in each batched call).
-    Beside them the oracle's rate for one field on one host thread."""
-    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    Beside them the oracle's rates on the host's cores (_cpu_field)."""
     from molscat_amd import bound as BD
     from molscat_amd import decks
     from molscat_amd import system as S
     from molscat_amd.engine import Engine
 
     fb = decks.hyperfine48()
-    eng = Engine(0)
-    eng.set_md_kernel(args.md_kernel)
     nf = args.fields
     fields = np.linspace(0.0, 1000.0, nf)
-    b = eng.block(fb.vl, fb.eint, fb.cent, mxlam=fb.mxlam, nconst=fb.nconst)
     nstep, h = S.drset_constant(3.5, 25.0, 0.02)
+    rmin, rmatch, rmax, drb = 3.5, 6.0, 40.0, 0.02
+    # the CPU baseline first: its worker processes fork before the GPU is touched
+    cpu_raw = (_cpu_field(fb, fields, nstep, h, (rmin, rmatch, rmax, drb) if args.bound_fields > 0 else None,
+                          args.cpu_reps) if args.cpu_sectors > 0 else None)
+    eng = Engine(0)
+    eng.set_md_kernel(args.md_kernel)
+    b = eng.block(fb.vl, fb.eint, fb.cent, mxlam=fb.mxlam, nconst=fb.nconst)
     r = S.ld_radii(3.5, h, nstep)
     g = eng.grid(r, fb.ptab(r), h)
     pc = np.stack([fb.pconst(f) for f in fields])
@@ -144,7 +348,6 @@ def _field(args) -> int:
     # (skipped with --bound-fields 0)
     bound_line = None
     if args.bound_fields > 0:
-        rmin, rmatch, rmax, drb = 3.5, 6.0, 40.0, 0.02
         (r0, dro, no), (r1, dri, ni) = S.bound_segments(rmin, rmatch, rmax, drb)
         ro, ri = S.ld_radii(r0, dro, no), S.ld_radii(r1, dri, ni)
         go, gi = eng.grid(ro, fb.ptab(ro), dro), eng.grid(ri, fb.ptab(ri), dri)
@@ -162,23 +365,26 @@ def _field(args) -> int:
         lev = BD.levels_over_fields(eng, fb, sub, e_lo, e_hi, rmin, rmatch, rmax, drb, width=1e-7)
         t_lev = time.perf_counter() - t1
         bound_line = {"fields_one_call": nf, "ms_nodec_all_fields": t_nodec * 1e3,
+                      "bound_evaluations_per_s": nf / t_nodec,
                       "nodec_range": [int(bm["NODEC"].min()), int(bm["NODEC"].max())],
                       "sectors_per_field": no + ni, "nodec_at_cm": -1.0, "level_window_cm": [e_lo, e_hi],
                       "level_fields": len(sub), "levels_found": int(len(lev["STATE"])), "level_search_s": t_lev,
                       "levels_per_s": len(lev["STATE"]) / t_lev if t_lev > 0 else None}
     cpu = None
-    if args.cpu_sectors > 0:
-        from oracle import molscat_oracle as O
-
-        ob = O.Block(n=fb.n, vl=fb.vl, eint=fb.eint, cent=fb.cent, lvals=fb.lvals, levidx=fb.levidx,
-                     potl=O.PowerPotential(fb.potl.terms, rm=fb.potl.rm, epsil=fb.potl.epsil), rscale=fb.rscale,
-                     ep2ru=fb.ep2ru, cm2ru=fb.cm2ru, nconst=1, pconst=fb.pconst(fields[1]))
-        t1 = time.perf_counter()
-        O.smatrix_ld(ob, ered[1], O.Grid(3.5, 25.0, nstep, h), iprop=6)
-        tc = time.perf_counter() - t1
-        cpu = {"value": 1.0 / tc, "unit": "S-matrices/s", "cores": 1, "kind": "port",
-               "sample": f"the oracle, one field ({fields[1]:.3f} G), YINIT + MDPROP + YTRANS + YTOK/KTOS, "
-                         f"single thread ({tc:.2f} s)"}
+    if cpu_raw is not None:
+        v, w, k = cpu_raw["smat"]
+        cpu = {"value": v, "unit": "S-matrices/s", "cores": cpu_raw["cores"], "kind": "port",
+               "host": cpu_raw["host"],
+               "sample": f"the oracle's whole solve (YTRANS thresholds, YINIT, MDPROP {nstep} sectors, YTRANS, "
+                         f"YTOK + KTOS; LAPACK DSYTRF/DSYTRI inversions) of {k} fields spread over the scan, "
+                         f"{cpu_raw['cores']} single-threaded processes; median of {args.cpu_reps} passes "
+                         f"({w:.2f} s)"}
+        if "bound" in cpu_raw:
+            vb, wb_, kb = cpu_raw["bound"]
+            cpu["bound_evaluations_per_s"] = vb
+            cpu["bound_sample"] = (f"BOUND's matching (outward {no} + inward {ni} sectors, NODEC + EIGMIN) at {kb} "
+                                   f"fields, 1 cm-1 below threshold, {cpu_raw['cores']} processes; median of "
+                                   f"{args.cpu_reps} passes ({wb_:.2f} s)")
     line = {
         "metric": "S-matrices/s over a field scan (N=48, 4096 fields, scattering length a(B))",
         "value": nf / dt, "unit": "S-matrices/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -230,6 +436,11 @@ def _sweep(args) -> int:
 
     sizes_all = [blk_of(jt, ib).n for jt in range(0, 121) for ib in (1, 2)]
     sizes_all = [n for n in sizes_all if n > 0]
+    # the CPU baseline first: its worker processes fork before the GPU is touched
+    cpu_raw = None
+    if args.cpu_sectors > 0:
+        subset = [b for b in (blk_of(jt, ib) for jt in jtots for ib in (1, 2)) if b.n > 0]
+        cpu_raw = _cpu_sweep(deck, subset, len(energies), args.cpu_sectors, args.cpu_airy, args.cpu_reps)
     eng = Engine(0)
     tuner = Autotuner(eng, path=args.tune_file) if args.tune_file else Autotuner(eng)
     t0 = time.perf_counter()
@@ -261,27 +472,16 @@ def _sweep(args) -> int:
     flops_ld = sum(nstep * ne * ((2.0 + 1.0 / ne) * n ** 3 + (2.0 / ne) * nham * n * (n + 1) + 12.0 * n * n)
                    for n in sizes)
     cpu = None
-    if args.cpu_sectors > 0:
-        os.environ.setdefault("OMP_NUM_THREADS", "1")
-        from oracle import molscat_oracle as O
-
-        nmax = max(sizes)
-        jt, ib = next((run.jtot, run.ib) for run in res.blocks if run.n == nmax)
-        b = blk_of(jt, ib)
-        ob = O.Block(n=b.n, vl=b.vl, eint=b.eint, cent=b.cent, lvals=b.lvals, levidx=b.levidx,
-                     potl=O.PowerPotential(b.potl.terms, rm=b.potl.rm, epsil=b.potl.epsil), rscale=b.rscale,
-                     ep2ru=b.ep2ru, cm2ru=b.cm2ru)
-        ered = energies[-1] * b.cm2ru
-        y0 = np.diag(np.sqrt(np.abs(np.diag(ob.hammat(seg.rstart, ered)[0]))))
-        tc0 = time.perf_counter()
-        O.mdprop(y0, lambda rr: ob.hammat(rr, ered), seg.rstart, args.cpu_sectors, seg.dr)
-        tc = time.perf_counter() - tc0
-        per_step_n3 = tc / args.cpu_sectors / float(nmax) ** 3
-        cpu_subset = per_step_n3 * nstep * ne * n3   # the LD stage of the subset, one thread
-        cpu = {"value": nsmat / cpu_subset, "unit": "S-matrices/s", "cores": 1, "kind": "port",
-               "sample": f"oracle MDPROP at N={nmax}, {args.cpu_sectors} sectors, one energy, single thread "
-                         f"({tc:.1f} s), scaled by N^3 x sectors x energies to the subset's LD stage "
-                         f"(AIRY and matching not counted)"}
+    if cpu_raw is not None:
+        cpu = {"value": nsmat / cpu_raw["wall"], "unit": "S-matrices/s", "cores": cpu_raw["cores"], "kind": "port",
+               "host": cpu_raw["host"], "per_instance_s": cpu_raw["per_instance_s"],
+               "sample": f"the oracle's solve (YINIT, MDPROP, AIRY, YTOK + KTOS; LAPACK DSYTRF/DSYTRI inversions "
+                         f"for N > 30) of {cpu_raw['cores']} (block, energy) instances at even quantiles of the "
+                         f"subset's block sizes, {cpu_raw['cores']} single-threaded processes at once, "
+                         f"{args.cpu_sectors} LD sectors + {args.cpu_airy} AIRY steps each (median of "
+                         f"{args.cpu_reps}), scaled to whole solves, interpolated in N over the subset's blocks "
+                         f"x {ne} energies and divided by the process count ({cpu_raw['wall']:.0f} s); not "
+                         f"counted: AIRY's per-channel Airy-function values"}
     line = {
         "metric": "S-matrices/s over the partial-wave x energy sweep (JTOT 0..120, both parities, 64 energies)",
         "value": nsmat / wall, "unit": "S-matrices/s", "n_gpus": 1, "steps": 1, "warmup": 0,
@@ -327,12 +527,15 @@ def _mixed_sweep(args) -> int:
     energies = decks.sweep_energies()[: args.energies_sweep]
     pot = deck.potential()
     jtots = [jt for jt in range(0, 121, args.jstep)]
+    big = [b for b in (S.rotor_block(jt, ib, deck.jlevels, decks.BE, deck.lambdas, pot, decks.URED, runit=1.0)
+                       for jt in jtots for ib in (1, 2)) if b.n >= 200]
+    blocks = [(b.jtot, b.ib) for b in big]
+    jt_use = sorted({jt for jt, _ in blocks})
+    # the CPU baseline first: its worker processes fork before the GPU is touched
+    cpu_raw = (_cpu_sweep(deck, big, len(energies), args.cpu_sectors, args.cpu_airy, args.cpu_reps)
+               if args.cpu_sectors > 0 else None)
     eng = Engine(0)
     tuner = Autotuner(eng, path=args.tune_file) if args.tune_file else Autotuner(eng)
-    info = eng.info()
-    blocks = [(jt, ib) for jt in jtots for ib in (1, 2)
-              if S.rotor_block(jt, ib, deck.jlevels, decks.BE, deck.lambdas, pot, decks.URED, runit=1.0).n >= 200]
-    jt_use = sorted({jt for jt, _ in blocks})
 
     def progress(jt, ib, n):
         print(f"mixed-sweep: JTOT={jt} IB={ib} N={n} done", file=sys.stderr, flush=True)
@@ -367,6 +570,15 @@ def _mixed_sweep(args) -> int:
                    "newton_steps": args.mixed_steps, "fp64_below_A": 3.0},
         "fp64_value": nsmat / w64, "ratio_mixed_to_fp64": w64 / wmx, "max_abs_dS": ds,
         "max_rel_dsigma": dsig, "nodes_equal": bool(nodes_equal), "s_matrices": nsmat,
+        "cpu_baseline": None if cpu_raw is None else {
+            "value": nsmat / cpu_raw["wall"], "unit": "S-matrices/s", "cores": cpu_raw["cores"], "kind": "port",
+            "host": cpu_raw["host"], "per_instance_s": cpu_raw["per_instance_s"],
+            "sample": f"the oracle's FP64 solve (YINIT, MDPROP, AIRY, YTOK + KTOS; LAPACK DSYTRF/DSYTRI) of "
+                      f"{cpu_raw['cores']} (block, energy) instances at even quantiles of the N >= 200 blocks' sizes, "
+                      f"{cpu_raw['cores']} single-threaded processes at once, {args.cpu_sectors} LD sectors + "
+                      f"{args.cpu_airy} AIRY steps each (median of {args.cpu_reps}), scaled to whole solves over "
+                      f"every block x {len(energies)} energies and divided by the process count "
+                      f"({cpu_raw['wall']:.0f} s); not counted: AIRY's per-channel Airy-function values"},
     }
     eng.close()
     print(json.dumps(line))
@@ -504,7 +716,10 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-sectors", type=int, default=600, help="sectors in the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-sectors", type=int, default=150,
+                    help="LD sectors per instance in the CPU baseline sample (0: skip the baseline)")
+    ap.add_argument("--cpu-airy", type=int, default=6, help="AIRY steps per instance in the CPU baseline sample")
+    ap.add_argument("--cpu-reps", type=int, default=3, help="runs of the CPU baseline sample (the median is kept)")
     ap.add_argument("--team", type=int, default=0, help="workgroups per instance on the large-N path (0: auto)")
     ap.add_argument("--plain-launch", action="store_true",
                     help="ordinary launches for the team kernels (for profilers that cannot trace cooperative ones)")
@@ -558,6 +773,8 @@ def main() -> int:
     ra, wa = deck.airy_grid()
     nairy = len(wa)
 
+    # the CPU baseline first, before this process touches the GPU
+    cpu = _cpu_rotor441(blk, deck, ered, args.cpu_sectors, args.cpu_airy, args.cpu_reps) if args.cpu_sectors > 0 else None
     eng = Engine(0)
     eng.set_inverse_kernel(args.inverse_kernel)
     eng.set_team_size(args.team)
@@ -661,9 +878,9 @@ def main() -> int:
             "inversions_fp64_again_per_solve": stats["fp64_again"] / args.steps,
             "precision": "tree inversions FP32 (matrix cores) + FP64 Newton-Schulz; everything else FP64",
         }
-    if args.cpu_sectors > 0:
-        line["cpu_baseline"] = _cpu_baseline(blk, seg, ered, args.cpu_sectors)
     eng.close()
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
     print(json.dumps(line))
     return 0
 

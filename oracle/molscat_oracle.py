@@ -94,6 +94,76 @@ def syminv(a: np.ndarray) -> Tuple[np.ndarray, int]:
     return full, int(k)
 
 
+@lru_cache(maxsize=1)
+def _openblas():
+    """numpy's bundled OpenBLAS (ILP64; the LAPACK names carry a scipy_
+    prefix and a 64_ suffix): the library numpy's own matrix products use, so
+    the process has one BLAS thread pool.  Used only by syminv_lapack, the
+    CPU baseline's inversion."""
+    import glob
+
+    d = os.path.join(os.path.dirname(os.path.dirname(np.__file__)), "numpy.libs")
+    cands = sorted(glob.glob(os.path.join(d, "libscipy_openblas64_*.so")))
+    if not cands:
+        raise RuntimeError("numpy's OpenBLAS not found")
+    lib = ctypes.CDLL(cands[0])
+    ip = ctypes.POINTER(ctypes.c_int64)
+    dp = ctypes.POINTER(ctypes.c_double)
+    cp = ctypes.c_char_p
+    lib.scipy_dsytrf_64_.restype = None
+    lib.scipy_dsytri_64_.restype = None
+    lib.scipy_dsytrf_64_.argtypes = [cp, ip, dp, ip, ip, dp, ip, ip, ctypes.c_size_t]
+    lib.scipy_dsytri_64_.argtypes = [cp, ip, dp, ip, ip, dp, ip, ctypes.c_size_t]
+    return lib
+
+
+def syminv_lapack(a: np.ndarray) -> Tuple[np.ndarray, int]:
+    """SYMINL as the reference runs it for N > 30 (source_code/syminl-alloc.f:
+    17-41): blocked DSYTRF('L') and DSYTRI('L') from an optimised LAPACK (here
+    numpy's OpenBLAS, threaded as its thread pool allows) with DSYNEG's
+    inertia count from the block-diagonal factor (source_code/dsyneg.f:47-73).
+    Same interface as syminv; the CPU baseline's inversion."""
+    lib = _openblas()
+    n = a.shape[0]
+    f = np.array(a, dtype=np.float64, order="F", copy=True)
+    ipiv = np.zeros(n, dtype=np.int64)
+    i64 = ctypes.c_int64
+    nn = i64(n)
+    info = i64(0)
+    dp = ctypes.POINTER(ctypes.c_double)
+    ipp = ctypes.POINTER(i64)
+    lwork = i64(-1)
+    wq = np.zeros(1)
+    lib.scipy_dsytrf_64_(b"L", ctypes.byref(nn), f.ctypes.data_as(dp), ctypes.byref(nn), ipiv.ctypes.data_as(ipp),
+                         wq.ctypes.data_as(dp), ctypes.byref(lwork), ctypes.byref(info), 1)
+    lw = max(1, int(wq[0]))
+    work = np.zeros(max(lw, n))
+    lwork = i64(len(work))
+    lib.scipy_dsytrf_64_(b"L", ctypes.byref(nn), f.ctypes.data_as(dp), ctypes.byref(nn), ipiv.ctypes.data_as(ipp),
+                         work.ctypes.data_as(dp), ctypes.byref(lwork), ctypes.byref(info), 1)
+    if info.value != 0:
+        return f, -abs(int(info.value))
+    # DSYNEG: 1x1 blocks by sign, 2x2 blocks (ipiv(k) = ipiv(k+1) < 0 in the
+    # lower factor) by the scaled determinant
+    kount = 0
+    k = 0
+    while k < n:
+        if ipiv[k] > 0:
+            if f[k, k] < 0.0:
+                kount += 1
+            k += 1
+        else:
+            aa, bb, cc = f[k, k], f[k + 1, k], f[k + 1, k + 1]
+            tt = abs(bb)
+            if (cc / tt) * aa - tt < 0.0:
+                kount += 1
+            k += 2
+    lib.scipy_dsytri_64_(b"L", ctypes.byref(nn), f.ctypes.data_as(dp), ctypes.byref(nn), ipiv.ctypes.data_as(ipp),
+                         work.ctypes.data_as(dp), ctypes.byref(info), 1)
+    lo = np.tril(f)
+    return lo + np.tril(f, -1).T, kount
+
+
 def syminv_raw(a: np.ndarray) -> Tuple[np.ndarray, int]:
     """SYMINV returning the raw column-major output (upper stale for N>30)."""
     n = a.shape[0]
@@ -556,12 +626,14 @@ def _sym_lower(m: np.ndarray) -> np.ndarray:
 
 
 def mdprop(y: np.ndarray, hammat: Callable[[float], Tuple[np.ndarray, np.ndarray]],
-           rstart: float, nstep: int, dr: float) -> Tuple[np.ndarray, int]:
+           rstart: float, nstep: int, dr: float, inv: Callable = None) -> Tuple[np.ndarray, int]:
     """MDPROP (diabatic modified log-derivative, IPROP=6): source_code/mdprop.f:64-279.
 
     ``hammat(r)`` returns (W, DIAG) at the propagation energy.  Returns the
-    final log-derivative matrix and NODES.
+    final log-derivative matrix and NODES.  ``inv`` replaces syminv (the CPU
+    baseline passes syminv_lapack).
     """
+    syminv = inv if inv is not None else globals()["syminv"]
     n = y.shape[0]
     h = dr / 2.0
     d1 = h * h / 3.0
@@ -1358,9 +1430,9 @@ def ytrans(y: Optional[np.ndarray], blk: Block) -> Tuple[Optional[np.ndarray], n
     return yt, eint, evec
 
 
-def smatrix_ld(blk: Block, ered: float, grid: Grid, iprop: int = 6, imgsel: int = 4):
+def smatrix_ld(blk: Block, ered: float, grid: Grid, iprop: int = 6, imgsel: int = 4, inv: Callable = None):
     """YINIT + MDPROP/LDPROP/ODPROP + (YTRANS when NCONST > 0) + YTOK + KTOS
-    for one energy.
+    for one energy.  ``inv`` replaces MDPROP's syminv.
 
     Returns dict with Y (final), K, SR, SI (NB-ordered open channels), NB,
     NOPEN, WVEC, NODES."""
@@ -1370,7 +1442,7 @@ def smatrix_ld(blk: Block, ered: float, grid: Grid, iprop: int = 6, imgsel: int 
     y = yinit(blk, ered, grid.rstart)
     hm = lambda r: blk.hammat(r, ered)
     if iprop == 6 and blk.n > 1:
-        y, nodes = mdprop(y, hm, grid.rstart, grid.nstep, grid.dr)
+        y, nodes = mdprop(y, hm, grid.rstart, grid.nstep, grid.dr, inv=inv)
     elif iprop == 6:
         yy, nodes = odprop(float(y[0, 0]), blk, ered, grid.rstart, grid.nstep, grid.dr)
         y = np.array([[yy]])
@@ -1542,21 +1614,28 @@ def airy_potent(blk: Block, ered: float, rnow: float, drnow: float):
 
 
 def aiprop(z: np.ndarray, blk: Block, ered: float, rstart: float, rstop: float, nstep: int, dr0: float,
-           powr: float) -> Tuple[np.ndarray, int]:
+           powr: float, inv: Callable = None, sector: Callable = None,
+           limit: Optional[int] = None) -> Tuple[np.ndarray, int]:
     """AIPROP for TOLHI = 0 (source_code/aiprop.f:86-327): per step the local
     basis of POTENT, Z <- T^T Z T with T = V_prev^T V_now, the Airy sector
     propagator, Z <- Y4 - Y2 (Z + Y1)^-1 Y2 with NODES += KOUNT (outward) or
-    NCH - KOUNT (inward); finally back to the primitive basis."""
+    NCH - KOUNT (inward); finally back to the primitive basis.  ``inv``
+    replaces syminv and ``sector`` airy_sector; ``limit`` stops after that
+    many steps (the CPU baseline's sample)."""
+    inv = inv if inv is not None else syminv
+    sector = sector if sector is not None else airy_sector
     n = blk.n
     nodes = 0
     vprev = None
-    for rnow, drnow in airy_steps(rstart, rstop, nstep, dr0, powr):
+    for kstep, (rnow, drnow) in enumerate(airy_steps(rstart, rstop, nstep, dr0, powr)):
+        if limit is not None and kstep >= limit:
+            break
         eig, v, hp = airy_potent(blk, ered, rnow, drnow)
         t = v if vprev is None else vprev.T @ v
         z = t.T @ z @ t
-        y1, y2, y4 = airy_sector(drnow, eig, hp)
+        y1, y2, y4 = sector(drnow, eig, hp)
         zz = z + np.diag(y1)
-        zi, kount = syminv(zz)
+        zi, kount = inv(zz)
         if kount > n:
             raise RuntimeError("AIPROP: instability in SYMINV")
         nodes += kount if rstop > rstart else n - kount

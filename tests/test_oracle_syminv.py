@@ -96,3 +96,19 @@ def test_bk_path_matches_inline_path_on_spd():
         inv2 = np.tril(f) + np.tril(f, -1).T
         assert k1 == k2 == 0
         assert np.abs(inv1 - inv2).max() < 1e-14 * np.abs(inv1).max() * n
+
+
+@pytest.mark.parametrize("n", [31, 48, 100, 441])
+def test_lapack_inversion_matches_restatement(n):
+    """The CPU baseline's inversion (OpenBLAS DSYTRF + DSYNEG + DSYTRI) gives
+    the restated SYMINL's inverse and inertia, including matrices with a zero
+    diagonal (2x2 pivots)."""
+    rng = np.random.default_rng(77 + n)
+    for t in range(3):
+        a = _rand_sym(n, rng)
+        if t == 2:
+            np.fill_diagonal(a, 0.0)
+        x, k = O.syminv_lapack(a)
+        y, k2 = O.syminv(a)
+        assert k == k2 == int(np.sum(np.linalg.eigvalsh(a) < 0.0))
+        assert np.abs(x - y).max() < 1e-10 * np.abs(y).max()
