@@ -393,8 +393,9 @@ def findrm(lowest, rmin: float, eint: np.ndarray, cent: np.ndarray, ered: float,
     """FINDRM (source_code/findrm.f, the MOLSCAT driver, not ITYPE=8): the
     inner classical turning point of the lowest adiabat by secant steps from
     the innermost centrifugal turning point of the open channels, then, for
-    IRMSET > 0, inwards in 3 + IRMSET/3 steps until the WKB phase integral
-    under the barrier reaches 2.303 IRMSET.  lowest(R) returns the lowest
+    IRMSET > 0, inwards in 3 + IRMSET/3 steps, from the secant iterate
+    before the converged turning point, until the WKB phase integral under
+    the barrier reaches 2.303 IRMSET.  lowest(R) returns the lowest
     eigenvalue of W(R) (DIAGVL).  Returns (RSTART, RTURN)."""
     rturn = 1.0e30
     for e, c in zip(eint, cent):
@@ -441,7 +442,9 @@ def findrm(lowest, rmin: float, eint: np.ndarray, cent: np.ndarray, ered: float,
     nstep = 3 + irmset // 3
     target = 2.303 * float(irmset)
     dr = (1.5 * target / math.sqrt(abs(dv1)) / float(nstep)) ** (2.0 / 3.0)
-    rstart = rturn
+    # the phase integral starts at rstart, the secant iterate before the
+    # converged rturn (findrm.f:109 sets RSTART=RTURN before the last step
+    # and :179 starts from RSTART), not at rturn itself
     while True:
         phase = 0.0
         xk_prev = 0.0

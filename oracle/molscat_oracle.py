@@ -548,7 +548,9 @@ def findrm(blk: Block, rmin: float, ered: float, irmset: int, cdrive: str = "M")
     nstep = 3 + irmset // 3
     target = 2.303 * float(irmset)
     dr = (1.5 * target / math.sqrt(abs(dv1)) / float(nstep)) ** (2.0 / 3.0)
-    rstart = rturn
+    # RSTART still holds the secant iterate before the converged RTURN
+    # (findrm.f:109 RSTART=RTURN, then RTURN=RTURN+DR and the test at :118);
+    # the phase integral starts there (:179 RNEXT=RSTART-DR)
     phase = 0.0
     while True:  # label 220
         restart = False
@@ -691,8 +693,15 @@ def mdprop(y: np.ndarray, hammat: Callable[[float], Tuple[np.ndarray, np.ndarray
     return y, nodes - nsave
 
 
-def ldprop(z: np.ndarray, hammat, rstart: float, nstep: int, dr: float) -> Tuple[np.ndarray, int]:
-    """LDPROP (Johnson log-derivative, IPROP=5): source_code/ldprop.f:24-147."""
+def ldprop(z: np.ndarray, hammat, rstart: float, nstep: int, dr: float,
+           eshift: Optional[float] = None) -> Tuple[np.ndarray, int]:
+    """LDPROP (Johnson log-derivative, IPROP=5): source_code/ldprop.f:24-147.
+
+    With ``eshift`` the matrices are replayed as the reference does for the
+    second and later energies of a run with ISCRU (IREAD): ``hammat``
+    evaluates W at the first energy, the stored scaled matrix is formed and
+    its diagonal shifted afterwards by -ESHIFT, -D4*ESHIFT or -D2*ESHIFT
+    (ldprop.f:34-38, 58-63, 107-112), ESHIFT = ERED - EFIRST."""
     n = z.shape[0]
     h = dr / 2.0
     d1 = h * h / 3.0
@@ -702,6 +711,9 @@ def ldprop(z: np.ndarray, hammat, rstart: float, nstep: int, dr: float) -> Tuple
     nodes = 0
     idx = np.arange(n)
     u, _ = hammat(r)
+    if eshift is not None:
+        u = u.copy()
+        u[idx, idx] -= eshift
     z = h * z + d1 * u
     z[idx, idx] += 1.0
     for istep in range(1, nstep + 1):
@@ -709,6 +721,8 @@ def ldprop(z: np.ndarray, hammat, rstart: float, nstep: int, dr: float) -> Tuple
         u, _ = hammat(r)
         u = d4 * u
         u[idx, idx] += 0.125
+        if eshift is not None:
+            u[idx, idx] += -d4 * eshift
         u, ncu = syminv(u)
         if ncu > n or ncu < 0:
             raise RuntimeError("LDPROP: ERROR IN SYMINV")
@@ -728,6 +742,8 @@ def ldprop(z: np.ndarray, hammat, rstart: float, nstep: int, dr: float) -> Tuple
         u, _ = hammat(r)
         u = d2 * u
         u[idx, idx] += 2.0
+        if eshift is not None:
+            u[idx, idx] += -d2 * eshift
         z = u - z
     hi = 1.0 / h
     z = _sym_lower(hi * z)
@@ -1430,9 +1446,11 @@ def ytrans(y: Optional[np.ndarray], blk: Block) -> Tuple[Optional[np.ndarray], n
     return yt, eint, evec
 
 
-def smatrix_ld(blk: Block, ered: float, grid: Grid, iprop: int = 6, imgsel: int = 4, inv: Callable = None):
+def smatrix_ld(blk: Block, ered: float, grid: Grid, iprop: int = 6, imgsel: int = 4, inv: Callable = None,
+               efirst: Optional[float] = None):
     """YINIT + MDPROP/LDPROP/ODPROP + (YTRANS when NCONST > 0) + YTOK + KTOS
-    for one energy.  ``inv`` replaces MDPROP's syminv.
+    for one energy.  ``inv`` replaces MDPROP's syminv; ``efirst`` (LDPROP)
+    replays the first energy's matrices as ldprop.f does with ISCRU.
 
     Returns dict with Y (final), K, SR, SI (NB-ordered open channels), NB,
     NOPEN, WVEC, NODES."""
@@ -1446,6 +1464,9 @@ def smatrix_ld(blk: Block, ered: float, grid: Grid, iprop: int = 6, imgsel: int 
     elif iprop == 6:
         yy, nodes = odprop(float(y[0, 0]), blk, ered, grid.rstart, grid.nstep, grid.dr)
         y = np.array([[yy]])
+    elif iprop == 5 and efirst is not None:
+        y, nodes = ldprop(y, lambda r: blk.hammat(r, efirst), grid.rstart, grid.nstep, grid.dr,
+                          eshift=ered - efirst)
     elif iprop == 5:
         y, nodes = ldprop(y, hm, grid.rstart, grid.nstep, grid.dr)
     elif iprop == 7:

@@ -83,7 +83,7 @@ def test_vivs_fixture_smatrices(eng, deck):
         d = max(np.abs(sr[0] - np.array(sm["sr"])).max(), np.abs(si[0] - np.array(sm["si"])).max())
         print(f"  E={[1000, 900][m]} cm-1 nopen={nopen} max|dS| vs printed {d:.2e}", flush=True)
         assert sst[0] == 0 and nopen == sm["nopen"]
-        assert d < 1e-7
+        assert d < 1e-11  # measured 9.9e-13 and 1.6e-12
     # R(RMAX) against the oracle's VVPROP from the same R(RMID)
     ro, steps = O.vvprop(rmat[0], blk, ered[0], ld.rstop, 20.0, ld.dr, tldiag, toff)
     ro2, _ = O.vvprop(rmat[1], blk, ered[1], ld.rstop, 20.0, ld.dr, tldiag, toff, replay=steps)

@@ -52,15 +52,13 @@ def test_open_channel_order_and_wavevectors(system, ref):
 
 
 @pytest.mark.parametrize("iprop,label,tol", [
-    (6, "model system: ITYPE=1; IPROPS=6", 2e-11),
-    # LDPROP: the oracle agrees with an extended-precision evaluation of the
-    # same recurrence to 2e-12; the printed reference values sit 1.5e-8 away.
-    (5, "model system: ITYPE=1; IPROPS=5", 3e-8),
-    # MGPROP with IMGSEL = 5 (MA5); measured 4.7e-12 and 2.1e-11, while CS4
+    # measured 1.2e-12 and 2.7e-12
+    (6, "model system: ITYPE=1; IPROPS=6", 5e-12),
+    # MGPROP with IMGSEL = 5 (MA5); measured 3.1e-11 and 3.5e-12, while CS4
     # lands 1e-3 away
     (8, "model system: ITYPE=1; IPROPS=8", 5e-11),
-    # MAPROP (LDMA): measured 3.8e-12 and 5.4e-12
-    (7, "model system: ITYPE=1; IPROPS=7", 2e-11),
+    # MAPROP (LDMA): measured 4.5e-12 and 4.9e-12
+    (7, "model system: ITYPE=1; IPROPS=7", 1e-11),
 ])
 @pytest.mark.parametrize("ie,energy", [(0, 1000.0), (1, 900.0)])
 def test_smatrix_matches_reference(system, ref, iprop, label, tol, ie, energy):
@@ -72,6 +70,35 @@ def test_smatrix_matches_reference(system, ref, iprop, label, tol, ie, energy):
     s = out["SR"] + 1j * out["SI"]
     assert np.abs(s @ s.conj().T - np.eye(s.shape[0])).max() < 1e-12
     assert np.abs(s - s.T).max() < 1e-13
+
+
+def _sdiff(out, sm):
+    return max(np.abs(out["SR"] - np.array(sm["sr"])).max(), np.abs(out["SI"] - np.array(sm["si"])).max())
+
+
+def test_ldprop_matches_reference(system, ref):
+    """LDPROP's S is a thousand times more sensitive to the grid's start
+    than MDPROP's (1e-5 A of RSTART moves S by 1e-8 against 3e-12), so it
+    pins FINDRM: the phase integral starts at the secant iterate before the
+    converged turning point (findrm.f:109,179); started at the turning
+    point itself, RSTART moves by 1.4e-5 A and S by 1.5e-8.  The second
+    energy replays the first energy's matrices with the diagonal shifted
+    (ISCRU, ldprop.f:34-38,58-63,107-112); evaluated afresh it lands 6.5e-11
+    away, the replay's order of rounding amplified by the recurrence."""
+    blk, runit, grid, _ = system
+    label = "model system: ITYPE=1; IPROPS=5"
+    e1, e2 = 1000.0 * blk.cm2ru, 900.0 * blk.cm2ru
+    first = O.smatrix_ld(blk, e1, grid, iprop=5)
+    assert _sdiff(first, ref[label]["smat"][0]) < 5e-12  # measured 1.0e-12
+    replay = O.smatrix_ld(blk, e2, grid, iprop=5, efirst=e1)
+    assert _sdiff(replay, ref[label]["smat"][1]) < 5e-12  # measured 1.1e-12
+    afresh = O.smatrix_ld(blk, e2, grid, iprop=5)
+    assert _sdiff(afresh, ref[label]["smat"][1]) < 1e-10  # measured 6.5e-11
+    assert int(replay["NODES"]) == int(afresh["NODES"])
+    # the turning-point start of the phase integral is what the printed
+    # values rule out
+    g0 = O.Grid(grid.rstart + 1.4129e-5, grid.rstop, grid.nstep, (grid.rstop - grid.rstart - 1.4129e-5) / grid.nstep)
+    assert _sdiff(O.smatrix_ld(blk, e1, g0, iprop=5), ref[label]["smat"][0]) > 1e-8
 
 
 @pytest.mark.parametrize("imgsel,order,nsteps", [(4, 4, (20, 40, 80)), (5, 5, (20, 40))])

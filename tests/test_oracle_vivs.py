@@ -191,13 +191,11 @@ def test_vivs_smatrices_match_reference(fixture_run):
     assert props[0][0] == "LDPROP" and props[0][3] == grid.nstep
     assert abs(props[0][2] - grid.rstop) < 5e-4
     assert props[1][0] == "VVPROP" and props[1][3] == len(first["STEPS"]) == len(second["STEPS"])
-    # 1e-7: the printed LDPROP-only run of the same deck sits 1.5e-8 from
-    # the oracle's LDPROP (which an extended-precision evaluation confirms
-    # to 2e-12); measured here 4.9e-8 and 3.1e-8
+    # measured 1.0e-12 and 1.5e-12
     for out, sm in ((first, sec["smat"][0]), (second, sec["smat"][1])):
         assert out["NOPEN"] == sm["nopen"]
-        assert np.abs(out["SR"] - np.array(sm["sr"])).max() < 1e-7
-        assert np.abs(out["SI"] - np.array(sm["si"])).max() < 1e-7
+        assert np.abs(out["SR"] - np.array(sm["sr"])).max() < 5e-12
+        assert np.abs(out["SI"] - np.array(sm["si"])).max() < 5e-12
         s = out["SR"] + 1j * out["SI"]
         assert np.abs(s @ s.conj().T - np.eye(s.shape[0])).max() < 1e-12
 
