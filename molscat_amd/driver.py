@@ -89,7 +89,8 @@ def cross_sections(eng, jlevels: Sequence[int], be: float, lambdas: Sequence[int
                    ured: float, energies: Sequence[float], jtotl: int, jtotu: int, jstep: int, rmin: float,
                    rmax: float, dr: float = None, iprop: int = 6, irmset: int = 9, ibs: Sequence[int] = (1, 2),
                    runit=None, steps: float = 0.0, iprop_long: int = 0, rvfac: float = 0.0, rmid: float = 0.0,
-                   tolhi: float = 1.0e-4, powr: float = 3.0) -> CrossSections:
+                   tolhi: float = 1.0e-4, powr: float = 3.0, isaveu: str = None, irstrt: int = 0,
+                   label: str = "") -> CrossSections:
     """Integral cross sections SIG(F, I) over the open levels at each
     energy, summed over JTOT (step JSTEP) and the parity blocks ibs; rmin,
     rmax and dr in RUNIT (RM by default), energies in cm-1.  The LD step is
@@ -97,7 +98,15 @@ def cross_sections(eng, jlevels: Sequence[int], be: float, lambdas: Sequence[int
     41-64; CAY the largest open wavevector at the largest energy).  With
     iprop_long = 9 (AIRY, TOLHI > 0) or 4 (VIVS) the LD segment ends at RMID
     (rmid, or rvfac x FINDRM's turning point, source_code/mol.driver.f:
-    1504-1506) and the long-range propagator carries Y on to RMAX."""
+    1504-1506) and the long-range propagator carries Y on to RMAX.
+
+    isaveu names an S-matrix file (molscat_amd/smatfile.py) that every
+    S-matrix is written to as SPROC does (source_code/sproc.f:254-281).
+    irstrt = 1 or 2 restarts from that file as RESTRT does
+    (source_code/restrt.f:19-27): the cross sections of the blocks it
+    holds complete (every JTOT set, or every symmetry block, up to the last
+    complete one) are recreated from the saved S-matrices, those blocks are
+    skipped, and the file continues after them."""
     if iprop not in (5, 6, 7):
         raise ValueError("cross_sections: the LD segment's IPROPS must be 5, 6 or 7")
     if iprop_long not in (0, 4, 9):
@@ -107,8 +116,11 @@ def cross_sections(eng, jlevels: Sequence[int], be: float, lambdas: Sequence[int
     nsig = int(np.sum(elev < energies.max()))
     acc = np.zeros((len(energies), nsig, nsig))
     runs: List[BlockRun] = []
+    save, done = _open_save(isaveu, irstrt, label, jlevels, be, ured, energies, max(ibs), acc)
     for jtot in range(jtotl, jtotu + 1, jstep):
         for ib in ibs:
+            if (jtot, ib) in done:
+                continue
             blk = S.rotor_block(jtot, ib, jlevels, be, lambdas, potl, ured, runit=runit)
             if blk.n == 0:
                 continue
@@ -153,9 +165,43 @@ def cross_sections(eng, jlevels: Sequence[int], be: float, lambdas: Sequence[int
                     continue
                 acc[m] += S.partial_sigma(out["SR"][m], out["SI"][m], nb, nops[m], blk.levidx, jlevels, wv[m],
                                           jtot, blk.runit, nsig, wght=1.0)
+                if save is not None:
+                    op = nb[:nops[m]]
+                    save.write(jtot, m + 1, ib, float(energies[m]), blk.levidx[op] + 1, blk.lvals[op],
+                               np.asarray(wv[m])[op] / blk.runit, out["SR"][m], out["SI"][m])
             runs.append(BlockRun(jtot, ib, blk.n, rstart, nstep, h, list(nops), [int(x) for x in out["NODES"]],
                                  [int(x) for x in out["STATUS"]], sr=list(out["SR"]), si=list(out["SI"])))
+    if save is not None:
+        save.close()
     return CrossSections(energies=energies, sigma=acc * float(jstep), blocks=runs)
+
+
+def _open_save(path, irstrt, label, jlevels, be, ured, energies, mxpar, acc):
+    """The S-matrix file of cross_sections: a new one with SPROC's header
+    (ITYPE = 1: JSTATE = (j, level index), NQN = 2; source_code/sproc.f:
+    797-807), or, for irstrt = 1 or 2, the existing one cut after its last
+    complete unit with that unit's cross sections added to acc.  Returns
+    (writer or None, set of (JTOT, IB) blocks already done)."""
+    from . import smatfile as F
+
+    if path is None:
+        if irstrt:
+            raise ValueError("cross_sections: a restart needs the S-matrix file (isaveu)")
+        return None, set()
+    nlev = len(jlevels)
+    hdr = F.Header(label=label, itype=1, jstate=np.stack([np.asarray(jlevels), np.arange(1, nlev + 1)], axis=1),
+                   ured=float(ured), elevel=np.array([be * float(j * (j + 1)) for j in jlevels]),
+                   energies=np.asarray(energies, dtype=np.float64))
+    if not irstrt:
+        return F.SMatrixWriter(path, hdr), set()
+    old, keep, offset, done = F.restart_point(path, mxpar, irstrt)
+    if len(old.energies) != len(energies) or np.abs(old.energies - energies).max() > 1e-8:
+        raise ValueError("cross_sections: the S-matrix file was written for other energies")
+    nsig = acc.shape[1]
+    for e in keep:
+        acc[e.inrg - 1] += S.partial_sigma(e.sr, e.si, np.arange(e.nopen), e.nopen, e.level - 1, jlevels, e.wvec,
+                                           e.jtot, 1.0, nsig, wght=e.wght)
+    return F.SMatrixWriter(path, hdr, append_at=offset), done
 
 
 @dataclass

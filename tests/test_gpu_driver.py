@@ -101,3 +101,42 @@ def test_two_segment_runs_match_reference(eng, golden_dir, iprops, ipropl, label
         d = max(np.abs(br.sr[m] - np.array(sm["sr"])).max(), np.abs(br.si[m] - np.array(sm["si"])).max())
         print(f"  E={[1000, 900][m]} cm-1 max|dS| vs printed {d:.2e}", flush=True)
         assert d < tol
+
+
+def test_saved_smatrices_and_restart(eng, tmp_path):
+    """cross_sections writes every S-matrix to the saved S-matrix file as
+    SPROC does; a run stopped inside its third symmetry block and restarted
+    with IRSTRT = 2 recomputes only the blocks after the last complete one,
+    recreates the earlier cross sections from the file and leaves the same
+    file behind."""
+    from molscat_amd import smatfile as F
+
+    energies = [1250.0, 1100.0, 800.0]
+    args = (eng, JLEV, 30.0, [0, 2], POT, 20.0, energies, 4, 6, 1, 0.5, 20.0, 0.004)
+    full = str(tmp_path / "full.sav")
+    print("driver: JTOT 4..6 both parities, 3 energies, saved S-matrices", flush=True)
+    a = D.cross_sections(*args, isaveu=full, label="basic1 deck, JTOT 4-6")
+    h, ents = F.read(full)
+    assert h.label == "basic1 deck, JTOT 4-6" and np.array_equal(h.energies, energies)
+    want = [(br, m) for br in a.blocks for m in range(3) if br.nopen[m] > 0]
+    assert len(ents) == len(want) == 18
+    low = lambda s: np.tril(s) + np.tril(s, -1).T   # the lower triangle is what SPROC writes
+    for e, (br, m) in zip(ents, want):
+        assert (e.jtot, e.ib, e.inrg, e.nopen) == (br.jtot, br.ib, m + 1, br.nopen[m])
+        assert np.array_equal(e.sr, low(br.sr[m])) and np.array_equal(e.si, low(br.si[m]))
+    # stopped after the first energy of the third block (JTOT 5, IB 1)
+    _, _, ends = F.scan(full)
+    cut = str(tmp_path / "cut.sav")
+    with open(full, "rb") as f:
+        data = f.read()
+    with open(cut, "wb") as f:
+        f.write(data[:ends[7] + 20])
+    print("driver: restart after the second block", flush=True)
+    b = D.cross_sections(*args, isaveu=cut, irstrt=2, label="basic1 deck, JTOT 4-6")
+    assert [(br.jtot, br.ib) for br in b.blocks] == [(5, 1), (5, 2), (6, 1), (6, 2)]
+    assert np.abs(b.sigma - a.sigma).max() <= 1e-13 * np.abs(a.sigma).max()
+    _, ents2 = F.read(cut)
+    assert len(ents2) == len(ents)
+    for e, f2 in zip(ents, ents2):
+        assert (e.jtot, e.ib, e.inrg) == (f2.jtot, f2.ib, f2.inrg)
+        assert np.abs(e.sr - f2.sr).max() <= 1e-13 and np.abs(e.si - f2.si).max() <= 1e-13
