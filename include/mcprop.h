@@ -70,6 +70,23 @@ int mcp_block_create(mcp_ctx *ctx, int32_t n, int32_t mxlam, int32_t nconst, int
                      const double *eint, const double *cent, double conlen, int32_t *block_id);
 int mcp_block_destroy(mcp_ctx *ctx, int32_t block_id);
 
+/* COUPLE for ITYPE = 1 (atom + linear rigid rotor, source_code/couple.f:
+ * 51-143) on the device: the Percival-Seaton coefficients of the n channels
+ * (rotor j = jch[i], orbital L = lch[i]) of total J = jtot for the nlam
+ * Legendre terms lambdas[], written to vl in the reference's layout
+ * VL(NLAM, N(N+1)/2) (lambda fastest, the upper triangle packed column by
+ * column, source_code/couple.f:120-127).  Accurate to a few ulp of the
+ * exact coefficients. */
+int mcp_couple_rotor(mcp_ctx *ctx, int32_t n, const int32_t *jch, const int32_t *lch, int32_t jtot, int32_t nlam,
+                     const int32_t *lambdas, double *vl);
+/* mcp_block_create with VL made on the device by the same coefficients
+ * (MXLAM = nlam, NCONST = NRSQ = 0): the (JTOT, IB) block of an ITYPE = 1
+ * run without VL crossing the host bus (source_code/mol.driver.f:1235-1244
+ * allocates VL, couple.f:51-143 fills it). */
+int mcp_block_create_rotor(mcp_ctx *ctx, int32_t n, const int32_t *jch, const int32_t *lch, int32_t jtot,
+                           int32_t nlam, const int32_t *lambdas, const double *eint, const double *cent,
+                           double conlen, int32_t *block_id);
+
 /* Upload a radial grid: npts radii and P(lambda, R) = EP2RU * POTENL(R*RSCALE)
  * (clamped at 1e60, after SCAPOT/PERTRB), ptab[k*mxlam + l].  For the LD
  * propagators the grid is R0 = RSTART, R_{2k-1} = midpoint, R_{2k} = end of

@@ -100,7 +100,7 @@ class Autotuner:
                 return got
         eng = self.eng
         r = S.ld_radii(rstart, dr, nsec)
-        b = eng.block(blk.vl, blk.eint, blk.cent, mxlam=blk.mxlam)
+        b = eng.block_rotor(blk.jch, blk.lvals, blk.jtot, blk.lambdas, blk.eint, blk.cent)
         g = eng.grid(r, blk.ptab(r), dr)
         best = None
         try:

@@ -25,6 +25,7 @@
 #include "mcp_mgprop.hpp"
 #include "mcp_maprop.hpp"
 #include "mcp_regprop.hpp"
+#include "mcp_couple.hpp"
 
 namespace {
 
@@ -298,6 +299,8 @@ struct mcp_ctx {
   DevBuf s_zb, s_pq;              // its merge pivots / inverses and products
   DevBuf s_kc, s_acc;             // its negative-pivot counts
   DevBuf s_yv;                    // large-N YINIT: scales, flags, inertias
+  DevBuf lnf_hi, lnf_lo;          // ln(k!) as hi + lo for the coupling coefficients
+  int lnf_kmax = -1;
   DevBuf s_flag;                  // a device-side decision flag
   DevBuf s_aa, s_ad, s_az;        // large-N AIRY: step matrices / reflectors, slopes' matrices, Z of T / D V
   DevBuf s_ed, s_ew;              // global-memory eigen-solver: d, e, tau; inverse-iteration factors
@@ -1413,6 +1416,93 @@ int mcp_block_destroy(mcp_ctx* c, int32_t id) {
   if (!c) return MCP_ERR_ARG;
   (void)hipStreamSynchronize(c->stream);
   return c->blocks.erase(id) ? MCP_OK : c->fail(MCP_ERR_ARG, "unknown block id %d", id);
+}
+
+// ---------------------------------------------------------------------------
+// COUPLE for ITYPE = 1 on the device (source_code/couple.f:51-143)
+// ---------------------------------------------------------------------------
+static int couple_rotor_device(mcp_ctx* c, int32_t n, const int32_t* jch, const int32_t* lch, int32_t jtot,
+                               int32_t nlam, const int32_t* lambdas, double* d_vl) {
+  int jmax = 0, lmax = 0, lmx = 0;
+  for (int i = 0; i < n; ++i) {
+    if (jch[i] < 0 || lch[i] < 0) return c->fail(MCP_ERR_ARG, "couple_rotor: negative j or L at channel %d", i);
+    jmax = std::max(jmax, (int)jch[i]);
+    lmax = std::max(lmax, (int)lch[i]);
+  }
+  for (int i = 0; i < nlam; ++i) {
+    if (lambdas[i] < 0) return c->fail(MCP_ERR_ARG, "couple_rotor: negative lambda");
+    lmx = std::max(lmx, (int)lambdas[i]);
+  }
+  const int need = mcp::couple_lnf_need(jmax, lmax, jtot, lmx);
+  if (need > (1 << 20)) return c->fail(MCP_ERR_ARG, "couple_rotor: quantum numbers too large (ln k! up to %d)", need);
+  if (need > c->lnf_kmax) {
+    const int kmax = std::max(need, 2047);
+    std::vector<double> hi(kmax + 1), lo(kmax + 1);
+    mcp::build_lnfact(kmax, hi.data(), lo.data());
+    int rc;
+    if ((rc = upload(c, c->lnf_hi, hi.data(), hi.size()))) return rc;
+    if ((rc = upload(c, c->lnf_lo, lo.data(), lo.size()))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->lnf_kmax = kmax;
+  }
+  DevBuf dj, dl, dlam;
+  int rc;
+  if ((rc = upload(c, dj, jch, (size_t)n))) return rc;
+  if ((rc = upload(c, dl, lch, (size_t)n))) return rc;
+  if ((rc = upload(c, dlam, lambdas, (size_t)nlam))) return rc;
+  const long long ntri = (long long)n * (n + 1) / 2;
+  const long long tot = ntri * nlam;
+  mcp::LnFact f{(const double*)c->lnf_hi.p, (const double*)c->lnf_lo.p, c->lnf_kmax};
+  if (tot > 0) {
+    hipLaunchKernelGGL(mcp::k_couple_rotor, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream,
+                       (const int*)dj.p, (const int*)dl.p, ntri, (int)jtot, (const int*)dlam.p, (int)nlam, f, d_vl);
+    HIPCHK(c, hipGetLastError());
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));  // the small uploads above are freed on return
+  return MCP_OK;
+}
+
+int mcp_couple_rotor(mcp_ctx* c, int32_t n, const int32_t* jch, const int32_t* lch, int32_t jtot, int32_t nlam,
+                     const int32_t* lambdas, double* vl) {
+  if (!c || n <= 0 || nlam <= 0 || jtot < 0 || !jch || !lch || !lambdas || !vl)
+    return c ? c->fail(MCP_ERR_ARG, "couple_rotor: bad args") : MCP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  DevBuf d;
+  const size_t cnt = (size_t)n * (n + 1) / 2 * nlam;
+  HIPCHK(c, d.ensure(cnt * sizeof(double)));
+  int rc;
+  if ((rc = couple_rotor_device(c, n, jch, lch, jtot, nlam, lambdas, (double*)d.p))) return rc;
+  if ((rc = download(c, vl, d, cnt))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MCP_OK;
+}
+
+int mcp_block_create_rotor(mcp_ctx* c, int32_t n, const int32_t* jch, const int32_t* lch, int32_t jtot,
+                           int32_t nlam, const int32_t* lambdas, const double* eint, const double* cent, double conlen,
+                           int32_t* block_id) {
+  if (!c || n <= 0 || nlam <= 0 || jtot < 0 || !jch || !lch || !lambdas || !eint || !cent || !block_id)
+    return c ? c->fail(MCP_ERR_ARG, "block_create_rotor: bad args") : MCP_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  auto b = std::make_unique<Block>();
+  b->n = n;
+  b->mxlam = nlam;
+  b->nconst = 0;
+  b->nrsq = 0;
+  b->nham = nlam;
+  b->conlen4inv = conlen > 0.0 ? 1.0 / (conlen * conlen * conlen * conlen) : 0.0;
+  const size_t ntri = (size_t)n * (n + 1) / 2;
+  HIPCHK(c, b->vl.ensure((ntri * b->nham + mcp::kRegMaxHam) * sizeof(double)));
+  HIPCHK(c, hipMemsetAsync(b->vl.p, 0, (ntri * b->nham + mcp::kRegMaxHam) * sizeof(double), c->stream));
+  int rc;
+  if ((rc = couple_rotor_device(c, n, jch, lch, jtot, nlam, lambdas, (double*)b->vl.p))) return rc;
+  if ((rc = upload(c, b->eint, eint, (size_t)n))) return rc;
+  if ((rc = upload(c, b->cent, cent, (size_t)n))) return rc;
+  b->h_cent.assign(cent, cent + n);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const int id = c->next_id++;
+  c->blocks[id] = std::move(b);
+  *block_id = id;
+  return MCP_OK;
 }
 
 int mcp_grid_create(mcp_ctx* c, int32_t mxlam, int32_t npts, const double* r, const double* ptab, double dr,

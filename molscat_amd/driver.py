@@ -56,6 +56,12 @@ class CrossSections:
     blocks: List[BlockRun] = field(default_factory=list)
 
 
+def device_block(eng, blk):
+    """An ITYPE = 1 block on the device with its VL made there by COUPLE's
+    coefficients (mcp_block_create_rotor), so the host never forms it."""
+    return eng.block_rotor(blk.jch, blk.lvals, blk.jtot, blk.lambdas, blk.eint, blk.cent)
+
+
 def _lowest_on_device(eng, blk, ered):
     def lowest(r: float) -> float:
         w = S.hammat_host(blk, r, ered)
@@ -127,6 +133,8 @@ def cross_sections(eng, jlevels: Sequence[int], be: float, lambdas: Sequence[int
             ered = energies * blk.cm2ru
             eredmx = float(ered.max())
             if irmset > 0:
+                # FINDRM's W on the host from the device's coupling coefficients
+                blk.set_vl(eng.couple_rotor(blk.jch, blk.lvals, blk.jtot, blk.lambdas))
                 rstart, rturn = S.findrm(_lowest_on_device(eng, blk, eredmx), rmin, blk.eint, blk.cent, eredmx,
                                          irmset)
             else:
@@ -137,7 +145,7 @@ def cross_sections(eng, jlevels: Sequence[int], be: float, lambdas: Sequence[int
                 rend = rmid if rmid > 0.0 else rvfac * rturn
             nstep, h = S.drset_constant(rstart, rend, drfit)
             r = S.ld_radii(rstart, h, nstep)
-            b = eng.block(blk.vl, blk.eint, blk.cent, mxlam=blk.mxlam)
+            b = device_block(eng, blk)
             g = eng.grid(r, blk.ptab(r), h)
             wv, nops = zip(*[S.gtwvec(e, blk.eint) for e in ered])
             nb = S.chnsrt(blk.eint)
@@ -328,7 +336,7 @@ def sweep(eng, jlevels: Sequence[int], be: float, lambdas: Sequence[int], potl: 
     # the small blocks: one persistent MDPROP launch for all of them
     if small:
         t0 = _time.perf_counter()
-        hs = [eng.block(b.vl, b.eint, b.cent, mxlam=b.mxlam) for _, _, b in small]
+        hs = [device_block(eng, b) for _, _, b in small]
         chans = [channels(b) for _, _, b in small]
         y0s = [eng.yinit(h, g, c[0]) for h, c in zip(hs, chans)]
         outs = eng.prop_ld_multi(hs, g, [c[0] for c in chans], y0s)
@@ -357,7 +365,7 @@ def sweep(eng, jlevels: Sequence[int], be: float, lambdas: Sequence[int], potl: 
             tuned[blk.n] = ch.__dict__.copy()
             ch.apply(eng)
         t0 = _time.perf_counter()
-        h = eng.block(blk.vl, blk.eint, blk.cent, mxlam=blk.mxlam)
+        h = device_block(eng, blk)
         grids, iprops = ([g, ga], [6, 9]) if ga is not None else ([g], [6])
         use_mixed = mixed is not None and blk.n >= mixed[2]
         if use_mixed:

@@ -127,6 +127,33 @@ class Engine:
                                               ctypes.byref(bid)), "mcp_block_create")
         return BlockHandle(bid.value, n, mxlam, nconst, nrsq, np.array(cent, dtype=np.float64))
 
+    def couple_rotor(self, jch, lch, jtot: int, lambdas) -> np.ndarray:
+        """COUPLE for ITYPE = 1 on the device (mcp_couple_rotor): VL of the
+        channels (j, L) = (jch[i], lch[i]) at total J, shape (NLAM, N(N+1)/2)
+        as RotorBlock.vl."""
+        jch = np.ascontiguousarray(jch, dtype=np.int32)
+        lch = np.ascontiguousarray(lch, dtype=np.int32)
+        lam = np.ascontiguousarray(lambdas, dtype=np.int32)
+        n = len(jch)
+        out = np.empty((n * (n + 1) // 2, len(lam)))
+        self._check(self.lib.mcp_couple_rotor(self.ctx, n, iptr(jch), iptr(lch), int(jtot), len(lam), iptr(lam),
+                                              dptr(out)), "mcp_couple_rotor")
+        return np.ascontiguousarray(out.T)
+
+    def block_rotor(self, jch, lch, jtot: int, lambdas, eint, cent, conlen: float = 0.0) -> BlockHandle:
+        """mcp_block_create_rotor: an ITYPE = 1 block whose VL is made on the
+        device from its channels (j, L) = (jch[i], lch[i])."""
+        jch = np.ascontiguousarray(jch, dtype=np.int32)
+        lch = np.ascontiguousarray(lch, dtype=np.int32)
+        lam = np.ascontiguousarray(lambdas, dtype=np.int32)
+        n = len(jch)
+        bid = ctypes.c_int32()
+        self._check(self.lib.mcp_block_create_rotor(self.ctx, n, iptr(jch), iptr(lch), int(jtot), len(lam),
+                                                    iptr(lam), dptr(np.ascontiguousarray(eint, dtype=np.float64)),
+                                                    dptr(np.ascontiguousarray(cent, dtype=np.float64)),
+                                                    float(conlen), ctypes.byref(bid)), "mcp_block_create_rotor")
+        return BlockHandle(bid.value, n, len(lam), 0, 0, np.array(cent, dtype=np.float64))
+
     def grid(self, r: np.ndarray, ptab: np.ndarray, dr: float = 0.0) -> GridHandle:
         """Upload radii r (npts,), the P(lambda, R) table ptab (npts, MXLAM) and,
         for an LD grid, its step DR."""

@@ -36,6 +36,8 @@ SIGNATURES = {
     "mcp_eigh_batched": (ctypes.c_int, [_vp, _i32, _i32, _pd, _pd, _pd]),
     "mcp_block_create": (ctypes.c_int, [_vp, _i32, _i32, _i32, _i32, _pd, _pd, _pd, _dbl, _pi]),
     "mcp_block_destroy": (ctypes.c_int, [_vp, _i32]),
+    "mcp_couple_rotor": (ctypes.c_int, [_vp, _i32, _pi, _pi, _i32, _i32, _pi, _pd]),
+    "mcp_block_create_rotor": (ctypes.c_int, [_vp, _i32, _pi, _pi, _i32, _i32, _pi, _pd, _pd, _dbl, _pi]),
     "mcp_grid_create": (ctypes.c_int, [_vp, _i32, _i32, _pd, _pd, _dbl, _pi]),
     "mcp_grid_destroy": (ctypes.c_int, [_vp, _i32]),
     "mcp_hammat": (ctypes.c_int, [_vp, _i32, _i32, _i32, _pi, _pd, _dbl, _pd]),

@@ -221,7 +221,6 @@ class RotorBlock:
     jtot: int
     ib: int
     chans: list
-    vl: np.ndarray
     eint: np.ndarray
     cent: np.ndarray
     lvals: np.ndarray
@@ -231,12 +230,32 @@ class RotorBlock:
     rscale: float
     potl: PowerPotential
     runit: float
-    mxlam: int = field(init=False)
+    jch: np.ndarray                                       # rotor j of each channel
+    lambdas: Tuple[int, ...]                              # the Legendre terms of VL
+    jlevels: Tuple[int, ...]
     scalam: float = 1.0                                   # SCAPOT's factor (efvs SCALAM)
     expval: Optional[Tuple[int, int, float]] = None       # PERTRB's /EXPVAL/ (I, IPOW, DELTAN)
+    _vl: Optional[np.ndarray] = field(default=None, repr=False)
 
-    def __post_init__(self):
-        self.mxlam = self.vl.shape[0]
+    @property
+    def vl(self) -> np.ndarray:
+        """VL(MXLAM, N(N+1)/2) by COUPLE's exact coefficients, made on first
+        use (the device makes the same coefficients itself through
+        mcp_block_create_rotor without it)."""
+        if self._vl is None:
+            self._vl = rotor_coupling(self.jtot, self.chans, list(self.jlevels), self.lambdas)
+        return self._vl
+
+    def set_vl(self, vl: np.ndarray) -> None:
+        """Use vl (e.g. the device's coefficients, mcp_couple_rotor) as VL."""
+        vl = np.asarray(vl, dtype=np.float64)
+        if vl.shape != (self.mxlam, self.n * (self.n + 1) // 2):
+            raise ValueError(f"VL shape {vl.shape}")
+        self._vl = vl
+
+    @property
+    def mxlam(self) -> int:
+        return len(self.lambdas)
 
     @property
     def n(self) -> int:
@@ -259,10 +278,11 @@ def rotor_block(jtot, ib, jlevels, be, lambdas, potl: PowerPotential, ured, runi
     eint = np.array([c2r * elev[c[0]] for c in chans])
     lv = np.array([c[1] for c in chans], dtype=np.int64)
     cent = np.array([float(l * (l + 1)) for l in lv])
-    vl = rotor_coupling(jtot, chans, jlevels, lambdas)
-    return RotorBlock(jtot=jtot, ib=ib, chans=chans, vl=vl, eint=eint, cent=cent, lvals=lv,
+    return RotorBlock(jtot=jtot, ib=ib, chans=chans, eint=eint, cent=cent, lvals=lv,
                       levidx=np.array([c[0] for c in chans], dtype=np.int64), cm2ru=c2r,
-                      ep2ru=potl.epsil * c2r, rscale=runit / potl.rm, potl=potl, runit=runit)
+                      ep2ru=potl.epsil * c2r, rscale=runit / potl.rm, potl=potl, runit=runit,
+                      jch=np.array([jlevels[c[0]] for c in chans], dtype=np.int64), lambdas=tuple(lambdas),
+                      jlevels=tuple(jlevels))
 
 
 # ---------------------------------------------------------------------------
