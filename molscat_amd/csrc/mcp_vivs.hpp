@@ -1,5 +1,6 @@
-// VIVS (IPROP = 4): the R-matrix propagator of source_code/vvprop.f with the
-// first-order perturbation integrals of source_code/pert2.f and the interval
+// VIVS (IPROP = 4): the R-matrix propagator of source_code/vvprop.f with
+// first-order perturbation integrals over each step (the quantities of
+// source_code/pert2.f, evaluated as in mcp_stepint.hpp) and the interval
 // control of source_code/delrd.f, for the reference's default options
 // (source_code/mol.driver.f:277-349): IV only (the constant off-diagonal
 // perturbation; no DERMAT), IPERT, ISYM, IALPHA = 6 steps per interval in
@@ -12,7 +13,7 @@
 // POTENL's inverse-power / exponential terms at each step centre and each
 // interval's diagonalisation point.  Per step: W at the step centre turned
 // into the interval's local basis (two N^3 products), the reference
-// solutions per channel, PERT2's integrals per channel pair accumulated into
+// solutions per channel, the step integrals per channel pair accumulated into
 // EYE11/EYE12/EYE22, and the estimates of the corrections (block maxima).
 // Per interval: the Jacobi eigen-solver on W at RMIDI, the R-matrix turned
 // into the new basis, and at the interval's end the new R-matrix from
@@ -32,6 +33,7 @@
 #include "mcp_airyprop.hpp"
 #include "mcp_eig.hpp"
 #include "mcp_kernels.hpp"
+#include "mcp_stepint.hpp"
 
 namespace mcp {
 
@@ -145,116 +147,36 @@ __device__ __forceinline__ bool vivs_delrd(double& dr, double cdiag, double coff
   return !(dr < 1.0e-06 * drmax);
 }
 
-// PERT2 (source_code/pert2.f:36-222) for one channel pair i <= j with V1 =
-// V2 = 0: R1V0 and R2V0 by PERT2's branch for the pair, then the step
-// integrals CC, CS, SS (written to both triangles of V0/V1/V2) and their
-// accumulation into the EYE matrices.  cx = COSX, sx = SINE, xsq = DIF*H^2,
-// x = sqrt|DIF|*H; a1/a1p regular and a2/a2p irregular step-start values.
-__device__ __forceinline__ void vivs_pert2_pair(int n, int i, int j, const double* cx, const double* sx,
-                                                const double* xsq, const double* x, double h, double* V0, double* V1,
-                                                double* V2, double* E11, double* E12, double* E22, const double* a1,
-                                                const double* a2, const double* a1p, const double* a2p) {
-  const double hsq = h * h;
-  const double hinv = 1.0 / h;
-  const double hsqinv = 1.0 / hsq;
-  const double hf = h * h * h;
-  const double xi = x[i], sxi = sx[i], cxi = cx[i], xsqi = xsq[i];
-  const double xsxi = xi * sxi;
+// One channel pair i <= j of a step (the role of PERT2 with V1 = V2 = 0,
+// source_code/pert2.f:36-222): the pair's constant perturbation V0_ij times
+// the step integrals of the reference solutions (mcp_stepint.hpp) into
+// V0 (c_i c_j), V1 (c_i s_j; its transposed entry c_j s_i) and V2 (s_i s_j),
+// and the integrals of the step-start solutions g = a' s + a c against the
+// perturbation accumulated into E11 (g1 g1), E12 (g1 g2) and E22 (g2 g2).
+// z = DIF h^2 per channel; a1/a1p regular, a2/a2p irregular step-start
+// values.
+__device__ __forceinline__ void vivs_pair_integrals(int n, int i, int j, const double* z, double h, double* V0,
+                                                    double* V1, double* V2, double* E11, double* E12, double* E22,
+                                                    const double* a1, const double* a2, const double* a1p,
+                                                    const double* a2p) {
   const size_t ij = (size_t)i + (size_t)j * n, ji = (size_t)j + (size_t)i * n;
-  double r1, r2, f6, f7, cxj, xsxj;
-  if (i == j) {
-    const double g1 = xsxi / xsqi;
-    f6 = xsxi * xsxi;
-    cxj = cxi;
-    xsxj = xsxi;
-    f7 = cxi * cxi;
-    const double coef = hf / (2.0 * f6) * V0[ij];
-    r1 = coef * (1.0 + cxi * g1);
-    r2 = coef * (g1 + cxi);
-  } else {
-    const double xj = x[j], sxj = sx[j], xsqj = xsq[j];
-    cxj = cx[j];
-    xsxj = xj * sxj;
-    const double f5 = xsqi * xsqj;
-    f6 = xsxi * xsxj;
-    f7 = cxi * cxj;
-    const double sd = xsqi - xsqj;
-    const double total = xsqi + xsqj;
-    const double coef = hf / (2.0 * f6) * V0[ij];
-    const double d = xi - xj;
-    constexpr double tol = 1.0e-3;
-    int br;  // 130 closed form, 140 series about XI = XJ, 150 series about XSQ = 0
-    if (fabs(d) > tol)
-      br = 130;
-    else if (fabs(total) < tol)
-      br = 150;
-    else if ((xsqi < 0.0 && xsqj < 0.0) || !(f5 < 0.0))
-      br = 140;
-    else
-      br = 130;
-    if (br == 130) {
-      const double f25 = 1.0 / sd;
-      const double f8 = xsxj * cxi;
-      const double f9 = xsxi * cxj;
-      const double f12 = 2.0 * coef * f25;
-      r1 = f12 * (f9 - f8);
-      r2 = f12 * (xsxi - xsxj);
-    } else if (br == 140) {
-      const double s = xi + xj;
-      double ssq = s * s, dsq = d * d;
-      const double shalf = s * 0.5;
-      double c2, s2;
-      if (xsqi < 0.0 && xsqj < 0.0) {
-        c2 = cosh(shalf);
-        s2 = -sinh(shalf);
-        ssq = -ssq;
-        dsq = -dsq;
-      } else {
-        c2 = cos(shalf);
-        s2 = sin(shalf);
-      }
-      const double ss2 = s * s2;
-      const double f14 = ss2 / ssq;
-      const double f16 = dsq * (1.0 / 24.0);
-      const double f17 = dsq / 80.0;
-      const double f15 = ss2 * c2;
-      const double f36 = dsq * (1.0 / 168.0);
-      const double f39 = 1.0 - dsq * (1.0 / 8.0) * (1.0 - dsq * (1.0 / 48.0) * (1.0 - dsq * (1.0 / 120.0)));
-      const double f40 = 1.0 / ssq;
-      r1 = coef * (1.0 - dsq * (1.0 / 6.0) * (1.0 - dsq / 20.0 * (1.0 - dsq * (1.0 / 42.0))) + 2.0 * f15 * f40);
-      r2 = coef * (f14 * f39 + 0.5 * c2 * (1.0 - f16 * (1.0 - f17 * (1.0 - f36)))) * 2.0;
-    } else {
-      const double g3 = xsqi * xsqi;
-      const double f20 = total * total;
-      const double f22 = 4.0 * xsqi * xsqj;
-      const double f23 = xsqj * xsqj;
-      const double f29 = total * (1.0 / 6.0);
-      const double f28 = 14.0 * f5;
-      const double f30 = coef * 2.0;
-      r1 = f30 * (1.0 - f29 + (f20 + f22) * (1.0 / 120.0) - total * (g3 + f28 + f23) * (1.0 / 5040.0));
-      r2 = f30 * (1.0 - f29 + (xsqi * total + f23) * (1.0 / 120.0) - (g3 + f23) * total * (1.0 / 5040.0));
-    }
-  }
-  // label 170
-  const double term1 = -cxj * r1;
-  const double ccij = f6 * r1 * hsqinv;
-  const double csij = xsxi * (r2 + term1) * hinv;
-  const double ssij = r1 - cxj * r2 - cxi * r2 + f7 * r1;
-  const double csji = xsxj * (r2 - cxi * r1) * hinv;
-  V0[ij] = ccij;
-  V0[ji] = ccij;
-  V1[ij] = csij;
-  V1[ji] = csji;
-  V2[ij] = ssij;
-  V2[ji] = ssij;
-  const double t1 = a1p[j] * ssij + a1[j] * csji;
-  const double t2 = a1p[j] * csij + a1[j] * ccij;
-  const double t3 = a2p[j] * ssij + a2[j] * csji;
-  const double t4 = a2p[j] * csij + a2[j] * ccij;
-  const double e11 = a1p[i] * t1 + a1[i] * t2 + E11[ij];
-  const double e22 = a2p[i] * t3 + a2[i] * t4 + E22[ij];
-  const double e12 = a1p[i] * t3 + a1[i] * t4 + E12[ij];
-  if (i != j) E12[ji] = a2p[i] * t1 + a2[i] * t2 + E12[ji];
+  const double v = V0[ij];
+  const StepIntegrals q = step_integrals(z[i], z[j], h);
+  const double cc = v * q.cc, cs = v * q.cs, sc = v * q.sc, ss = v * q.ss;
+  V0[ij] = cc;
+  V0[ji] = cc;
+  V1[ij] = cs;
+  V1[ji] = sc;
+  V2[ij] = ss;
+  V2[ji] = ss;
+  // int (p_i s_i + u_i c_i) V (q_j s_j + w_j c_j) dr
+  auto gvg = [&](double p_i, double u_i, double q_j, double w_j) {
+    return p_i * q_j * ss + p_i * w_j * sc + u_i * q_j * cs + u_i * w_j * cc;
+  };
+  const double e11 = E11[ij] + gvg(a1p[i], a1[i], a1p[j], a1[j]);
+  const double e22 = E22[ij] + gvg(a2p[i], a2[i], a2p[j], a2[j]);
+  const double e12 = E12[ij] + gvg(a1p[i], a1[i], a2p[j], a2[j]);
+  if (i != j) E12[ji] += a1p[j] * a2p[i] * ss + a1p[j] * a2[i] * cs + a1[j] * a2p[i] * sc + a1[j] * a2[i] * cc;
   E11[ij] = e11;
   E11[ji] = e11;
   E22[ij] = e22;
@@ -490,7 +412,6 @@ __global__ void __launch_bounds__(kNT) k_vivs(BlockArgs B, PotlArgs P, VivsArgs 
         cx = cos(x);
       }
       sinx[i] = sx;
-      sine[i] = dif < 0.0 ? -(sx * xl) : sx * xl;
       cosx[i] = cx;
       xk[i] = x;
       const double a = g1p[i], b = g1[i], cc = g2p[i], d = g2[i];
@@ -504,14 +425,14 @@ __global__ void __launch_bounds__(kNT) k_vivs(BlockArgs B, PotlArgs P, VivsArgs 
       g2p[i] = cc * cx - dif * d * sx;
     }
     __syncthreads();
-    // PERT2 over the pairs i <= j
+    // the step integrals of every pair i <= j
     const int ntri = n * (n + 1) / 2;
     for (int e = tid; e < ntri; e += kNT) {
       int jj = (int)((sqrt(8.0 * (double)e + 1.0) - 1.0) * 0.5);
       while ((jj + 1) * (jj + 2) / 2 <= e) ++jj;
       while (jj * (jj + 1) / 2 > e) --jj;
       const int ii = e - jj * (jj + 1) / 2;
-      vivs_pert2_pair(n, ii, jj, cosx, sine, xsq, xk, drnow, W0, W1, W2, E11, E12, E22, a1, b1, a1p, b1p);
+      vivs_pair_integrals(n, ii, jj, xsq, drnow, W0, W1, W2, E11, E12, E22, a1, b1, a1p, b1p);
     }
     __syncthreads();
     if (own) {
