@@ -188,14 +188,15 @@ int mcp_block_set_potl(mcp_ctx *ctx, int32_t block_id, int32_t nterm, const int3
                        const double *a, const double *e, double rscale, double ep2ru);
 
 /* VVPROP (source_code/vvprop.f:1-4; VIVS, IPROP = 4) for ninst energies of
- * one block (NCONST = NRSQ = 0, potential from mcp_block_set_potl) with the
- * reference's default options (IV, IPERT, ISYM, IALPHA = 6, IALFP off,
- * source_code/mol.driver.f:277-349): rmat (ninst N x N) holds the R-matrix
+ * one block (NCONST = NRSQ = 0, potential from mcp_block_set_potl) under the
+ * options of mcp_set_vivs_options (by default the reference's: IV, IPERT,
+ * ISYM, IALPHA = 6, IALFP off, source_code/mol.driver.f:277-349): rmat (ninst N x N) holds the R-matrix
  * (the inverse of Y, source_code/scctrl.f:150-166) at rstart on entry and at
  * rstop on exit; drnow = the first step, drmax = DRMAX, tldiag / toff = TLDIAG
  * / TOFF (scctrl.f:157-158: 0.064 sqrt(TOLHI / 0.001) for both).  The step
  * record steps (max_steps x 4: RNOW, DRNOW, RMIDI, flags with bit 0 = the
- * step opens an interval, bit 1 = the last step) stands for the reference's
+ * step opens an interval, bit 1 = the last step, bit 2 = the step ends an
+ * interval) stands for the reference's
  * scratch file: replay 0 = every instance its own step control (steps, if
  * not NULL, receives instance 0's); replay 1 = every instance follows steps;
  * replay 2 = instance 0 chooses the steps, written to steps when not NULL,
@@ -207,6 +208,27 @@ int mcp_block_set_potl(mcp_ctx *ctx, int32_t block_id, int32_t nterm, const int3
 int mcp_prop_vivs(mcp_ctx *ctx, int32_t block_id, int32_t ninst, const double *ered, double *rmat, double rstart,
                   double rstop, double drnow, double drmax, double tldiag, double toff, int32_t replay,
                   int32_t max_steps, double *steps, int32_t *nstep, int32_t *status);
+
+/* The /LDVVCM/ controls of VVPROP (source_code/vvprop.f:57-82, set in
+ * source_code/mol.driver.f:277-349 and mol.prpini.f:75-83) for this
+ * context's later mcp_prop_vivs calls; nonzero = .TRUE.  iv, ivp, ivpp: the
+ * perturbation corrections from the constant part of W, its first and its
+ * second derivative (DERMAT, source_code/dermat.f; PERT1 when ivp without
+ * ivpp, else PERT2); ishift: the reference potential moved by h^2/24 times
+ * the second derivative's diagonal; idiag: every diagonal correction (the
+ * IVD/IVPD/IVPPD of vvprop.f:137-139; the driver's IDIAG default of also
+ * setting iv, ivp, ivpp, ishift and ipert is the caller's, as MOL.PRPINI
+ * does it); ipert: apply the corrections (else the steps only); isym:
+ * symmetrise R; ialpha > 0: steps per interval in geometric progression with
+ * ALPHA from alpha1 to alpha2 across the range (ialfp: predicted from the
+ * diagonal corrections' trend), ialpha <= 0: each step from DELRD, an
+ * interval ended when the corrections' estimates grow; xsqmax: closed
+ * channels deeper than this (reduced units) drop out of the estimates.  The
+ * defaults are 1, 0, 0, 0, 0, 1, 1, 6, 0, 1.0, 1.5, 1e4.  Derivatives are
+ * taken with respect to R * RSCALE, as DERMAT takes them. */
+int mcp_set_vivs_options(mcp_ctx *ctx, int32_t iv, int32_t ivp, int32_t ivpp, int32_t ishift, int32_t idiag,
+                         int32_t ipert, int32_t isym, int32_t ialpha, int32_t ialfp, double alpha1, double alpha2,
+                         double xsqmax);
 
 /* AIPROP with TOLHI > 0 (source_code/aiprop.f:1-4, the steps chosen as the
  * propagation goes, aiprop.f:288-316) for ninst energies of one block

@@ -253,6 +253,8 @@ constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 struct mcp_ctx {
+  mcp::VivsOpts vivs_opt = mcp::vivs_default_opts();
+  int64_t coop_refused = 0;  // cooperative launches refused and taken as ordinary ones
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -461,11 +463,17 @@ unsigned nblk256(size_t cnt) { return (unsigned)((cnt + 255) / 256); }
 // go through hipLaunchCooperativeKernel; with c->coop off they are ordinary
 // launches of the same grids after the same residency check (every team is
 // sized to fit the device, and a team barrier that waits too long sets the
-// abort flag instead of hanging).
+// abort flag instead of hanging).  A cooperative launch the runtime refuses
+// in its present state (hipErrorIllegalState, seen while another process
+// held the device) takes the ordinary launch instead: same kernel, same
+// results.
 int launch_team(mcp_ctx* c, const void* kern, dim3 grid, dim3 block, void** args, size_t lds) {
   if (c->coop) {
-    HIPCHK(c, hipLaunchCooperativeKernel(kern, grid, block, args, lds, c->stream));
-    return MCP_OK;
+    const hipError_t e = hipLaunchCooperativeKernel(kern, grid, block, args, lds, c->stream);
+    if (e == hipSuccess) return MCP_OK;
+    if (e != hipErrorIllegalState) HIPCHK(c, e);
+    (void)hipGetLastError();
+    ++c->coop_refused;
   }
   int per_cu = 0;
   HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, (int)(block.x * block.y * block.z), lds));
@@ -2217,6 +2225,30 @@ int mcp_set_large_mode(mcp_ctx* c, int32_t mode) {
   return MCP_OK;
 }
 
+int mcp_set_vivs_options(mcp_ctx* c, int32_t iv, int32_t ivp, int32_t ivpp, int32_t ishift, int32_t idiag,
+                         int32_t ipert, int32_t isym, int32_t ialpha, int32_t ialfp, double alpha1, double alpha2,
+                         double xsqmax) {
+  if (!c) return MCP_ERR_ARG;
+  if (ialpha > 0 && !(alpha1 > 0.0 && alpha2 > 0.0))
+    return c->fail(MCP_ERR_ARG, "vivs_options: IALPHA > 0 needs ALPHA1, ALPHA2 > 0");
+  if (!(xsqmax > 0.0)) return c->fail(MCP_ERR_ARG, "vivs_options: XSQMAX must be positive");
+  mcp::VivsOpts o;
+  o.iv = iv != 0;
+  o.ivp = ivp != 0;
+  o.ivpp = ivpp != 0;
+  o.ishift = ishift != 0;
+  o.idiag = idiag != 0;
+  o.ipert = ipert != 0;
+  o.isym = isym != 0;
+  o.ialpha = ialpha;
+  o.ialfp = ialfp != 0;
+  o.alpha1 = alpha1;
+  o.alpha2 = alpha2;
+  o.xsqmax = xsqmax;
+  c->vivs_opt = o;
+  return MCP_OK;
+}
+
 int mcp_set_precision(mcp_ctx* c, int32_t newton_steps, double r_fp64) {
   if (!c || newton_steps < 0 || newton_steps > 4)
     return c ? c->fail(MCP_ERR_ARG, "precision: 0 (FP64) to 4 Newton steps, not %d", newton_steps) : MCP_ERR_ARG;
@@ -2322,6 +2354,7 @@ int mcp_prop_vivs(mcp_ctx* c, int32_t bid, int32_t ninst, const double* ered, do
   P.rscale = b->potl_rscale;
   P.ep2ru = b->potl_ep2ru;
   mcp::VivsArgs A;
+  A.opt = c->vivs_opt;
   A.rstart = rstart;
   A.rstop = rstop;
   A.drnow = drnow;

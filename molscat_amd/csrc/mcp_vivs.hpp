@@ -1,11 +1,17 @@
 // VIVS (IPROP = 4): the R-matrix propagator of source_code/vvprop.f with
 // first-order perturbation integrals over each step (the quantities of
-// source_code/pert2.f, evaluated as in mcp_stepint.hpp) and the interval
-// control of source_code/delrd.f, for the reference's default options
-// (source_code/mol.driver.f:277-349): IV only (the constant off-diagonal
-// perturbation; no DERMAT), IPERT, ISYM, IALPHA = 6 steps per interval in
-// geometric progression with ALPHA from ALPHA1 = 1 to ALPHA2 = 1.5 across the
-// range, IALFP off, XSQMAX = 1e4.
+// source_code/pert2.f and pert1.f, evaluated as in mcp_stepint.hpp) and the
+// interval control of source_code/delrd.f, under the /LDVVCM/ controls
+// (VivsOpts; the reference's defaults, source_code/mol.driver.f:277-349, are
+// IV, IPERT, ISYM, IALPHA = 6 steps per interval in geometric progression
+// with ALPHA from ALPHA1 = 1 to ALPHA2 = 1.5 across the range, IALFP off,
+// XSQMAX = 1e4).  IVP / IVPP (or IDIAG) add DERMAT's first / second
+// derivatives of W at the step centre (source_code/dermat.f) as the
+// perturbations V1 (r - h/2) and V2 (r - h/2)^2 / 2; ISHIFT moves the
+// reference potential by h^2/24 times the second derivative's diagonal;
+// IALPHA <= 0 takes each step from DELRD and ends an interval when the
+// corrections' estimates grow (vvprop.f:480-500); IALFP predicts ALPHA from
+// the diagonal corrections' trend over the interval (vvprop.f:630-660).
 //
 // One workgroup per instance runs the whole propagation: the radii are
 // chosen as it goes (the step control reads the perturbation estimates of
@@ -23,7 +29,8 @@
 // pointers.
 //
 // Steps are recorded as (RNOW, DRNOW, RMIDI, flags) per step: flags bit 0 =
-// the step opens an interval, bit 1 = the last step.  A replaying instance
+// the step opens an interval, bit 1 = the last step, bit 2 = the step ends
+// an interval.  A replaying instance
 // takes its steps from such a record instead of its own control, as the
 // reference's energies after the first do from the scratch file
 // (vvprop.f:155-160, 350-352, 410, 590).
@@ -62,7 +69,31 @@ struct PotlArgs {
   double rscale, ep2ru;
 };
 
+// the /LDVVCM/ controls (source_code/vvprop.f:57-82)
+struct VivsOpts {
+  int iv, ivp, ivpp, ishift, idiag, ipert, isym, ialpha, ialfp;
+  double alpha1, alpha2, xsqmax;
+};
+
+__host__ __device__ inline VivsOpts vivs_default_opts() {
+  VivsOpts o;
+  o.iv = 1;
+  o.ivp = 0;
+  o.ivpp = 0;
+  o.ishift = 0;
+  o.idiag = 0;
+  o.ipert = 1;
+  o.isym = 1;
+  o.ialpha = kVivsIalpha;
+  o.ialfp = 0;
+  o.alpha1 = kVivsAlpha1;
+  o.alpha2 = kVivsAlpha2;
+  o.xsqmax = kVivsXsqmax;
+  return o;
+}
+
 struct VivsArgs {
+  VivsOpts opt;
   double rstart, rstop, drnow, drmax, tldiag, toff;
   int replay;            // 0: own step control; 1: steps from the record
   int record;            // own control: write the steps to the record (one instance)
@@ -80,8 +111,9 @@ struct VivsArgs {
 
 struct VivsState {
   double rnow, drnow, drint, diagi, rmid, rmidi, alpha, balpha;
-  double coff, cdiag, soff, sdiag, soff1, sdiag1;
-  int istep, icrmat, last, k, stop, status, piv;
+  double coff, cdiag, soff, sdiag, soff1, sdiag1, coffl;
+  double xbar, xsbar, ebar, exbar, alfnew;
+  int istep, icrmat, last, k, stop, status, piv, endint;
 };
 
 __device__ __forceinline__ void vivs_pvec(const BlockArgs& B, const PotlArgs& P, double r, double* pv) {
@@ -99,6 +131,51 @@ __device__ __forceinline__ void vivs_pvec(const BlockArgs& B, const PotlArgs& P,
     }
     if (fabs(s) > 1.0e60) s = copysign(1.0e60, s);
     pv[l] = P.ep2ru * s;
+  }
+}
+
+// POTENL with ICNTRL = ider (source_code/potenl.f:177-185): the radial terms'
+// derivatives with respect to x = R * RSCALE, times EP2RU, unclamped, as
+// DERMAT takes them (source_code/dermat.f:69-80)
+__device__ __forceinline__ void vivs_pvec_der(const BlockArgs& B, const PotlArgs& P, double r, int ider, double* pv) {
+  for (int l = threadIdx.x; l < B.mxlam; l += blockDim.x) {
+    const double x = r * P.rscale;
+    double s = 0.0;
+    for (int t = 0; t < P.nterm; ++t) {
+      if (P.lam[t] != l) continue;
+      const int np = P.npow[t];
+      double term;
+      if (np != 0) {
+        const double rr = (np < 0 && fabs(x) < 1.0e-8) ? 1.0e-8 : x;
+        term = pow(rr, (double)np);
+        term = ider == 1 ? (double)np * term / rr : (double)(np * (np - 1)) * term / (rr * rr);
+      } else {
+        const double e = P.e[t];
+        term = exp(e * x) * (ider == 1 ? e : e * e);
+      }
+      s += P.a[t] * term;
+    }
+    pv[l] = P.ep2ru * s;
+  }
+}
+
+// DERMAT (source_code/dermat.f:38-126; NCONST = NRSQ = 0): the ider'th
+// derivative of W at R into the full column-major M.  The diagonal takes the
+// centrifugal term's derivative, -2/R^3 or 6/R^4 (clamped at 1e16) times
+// L(L+1) (the reference sets RSQ in its NRSQ > 0 branch only, dermat.f:
+// 99-110, and adds RSQ*CENT for NRSQ = 0 with the variable unset; its
+// meaning is this derivative), and CONLEN's confinement 2R/CONLEN^4 or
+// 2/CONLEN^4.
+__device__ __forceinline__ void vivs_dermat(const BlockArgs& B, const double* pv, double r, int ider, double* M) {
+  const int n = B.n;
+  double rsq = ider == 1 ? -2.0 / (r * r * r) : 6.0 / (r * r * r * r);
+  rsq = fmin(rsq, 1.0e16);
+  const double conf = ider == 1 ? 2.0 * r * B.conlen4inv : 2.0 * B.conlen4inv;
+  for (int e = threadIdx.x; e < n * n; e += kNT) {
+    const int j = e / n, i = e - j * n;
+    double s = sumlam(B, i, j, pv);
+    if (i == j) s = s + rsq * B.cent[i] + conf;
+    M[e] = s;
   }
 }
 
@@ -147,9 +224,11 @@ __device__ __forceinline__ bool vivs_delrd(double& dr, double cdiag, double coff
   return !(dr < 1.0e-06 * drmax);
 }
 
-// One channel pair i <= j of a step (the role of PERT2 with V1 = V2 = 0,
-// source_code/pert2.f:36-222): the pair's constant perturbation V0_ij times
-// the step integrals of the reference solutions (mcp_stepint.hpp) into
+// One channel pair i <= j of a step (the role of PERT2, source_code/
+// pert2.f:36-222): the pair's perturbation V0_ij + V1_ij (r - h/2) + V2_ij
+// (r - h/2)^2 / 2 (use1 / use2: V1 / V2 present; pert1: PERT1, V2 on the
+// diagonal only, source_code/pert1.f:160-187) integrated against the
+// products of the reference solutions (mcp_stepint.hpp) into
 // V0 (c_i c_j), V1 (c_i s_j; its transposed entry c_j s_i) and V2 (s_i s_j),
 // and the integrals of the step-start solutions g = a' s + a c against the
 // perturbation accumulated into E11 (g1 g1), E12 (g1 g2) and E22 (g2 g2).
@@ -158,11 +237,20 @@ __device__ __forceinline__ bool vivs_delrd(double& dr, double cdiag, double coff
 __device__ __forceinline__ void vivs_pair_integrals(int n, int i, int j, const double* z, double h, double* V0,
                                                     double* V1, double* V2, double* E11, double* E12, double* E22,
                                                     const double* a1, const double* a2, const double* a1p,
-                                                    const double* a2p) {
+                                                    const double* a2p, bool use1, bool use2, bool pert1) {
   const size_t ij = (size_t)i + (size_t)j * n, ji = (size_t)j + (size_t)i * n;
   const double v = V0[ij];
   const StepIntegrals q = step_integrals(z[i], z[j], h);
-  const double cc = v * q.cc, cs = v * q.cs, sc = v * q.sc, ss = v * q.ss;
+  double cc = v * q.cc, cs = v * q.cs, sc = v * q.sc, ss = v * q.ss;
+  if (use1 || use2) {
+    const double v1 = use1 ? V1[ij] : 0.0;
+    const double v2 = (use2 && !(pert1 && i != j)) ? V2[ij] : 0.0;
+    const StepIntegralsW qw = step_integrals_w(z[i], z[j], h);
+    cc += v1 * qw.w1.cc + v2 * qw.w2.cc;
+    cs += v1 * qw.w1.cs + v2 * qw.w2.cs;
+    sc += v1 * qw.w1.sc + v2 * qw.w2.sc;
+    ss += v1 * qw.w1.ss + v2 * qw.w2.ss;
+  }
   V0[ij] = cc;
   V0[ji] = cc;
   V1[ij] = cs;
@@ -303,6 +391,11 @@ __global__ void __launch_bounds__(kNT) k_vivs(BlockArgs B, PotlArgs P, VivsArgs 
   double* TS = VO + nn;
   const double ered = A.ered[inst];
   const bool own = A.replay == 0;
+  const VivsOpts& o = A.opt;
+  const bool ivd = o.iv || o.idiag, ivpd = o.ivp || o.idiag, ivppd = o.ivpp || o.idiag;
+  const bool need_w2 = ivppd || o.ishift;
+  const bool do_pert = o.iv || o.ivp || o.ivpp;
+  const bool pert1 = o.ivp && !o.ivpp;
   double* rg = A.rmat + (size_t)inst * nn;
   for (size_t e = tid; e < nn; e += kNT) {
     const int j = (int)(e / n), i = (int)(e - (size_t)j * n);
@@ -324,11 +417,21 @@ __global__ void __launch_bounds__(kNT) k_vivs(BlockArgs B, PotlArgs P, VivsArgs 
     st.diagi = A.rstart + 0.5 * st.drint;
     st.rmid = A.rstart;
     st.rmidi = st.diagi;
-    st.balpha = (kVivsAlpha2 - kVivsAlpha1) / (A.rstop - A.rstart);
-    st.alpha = kVivsAlpha1 + st.balpha * (st.diagi - A.rstart);
-    st.drnow = st.alpha == 1.0 ? st.drint / kVivsIalpha
-                               : st.drint * (st.alpha - 1.0) / (pow(st.alpha, (double)kVivsIalpha) - 1.0);
+    st.drnow = A.drnow;
+    st.alpha = 1.0;
+    st.balpha = 0.0;
+    st.alfnew = 1.0;
+    if (o.ialpha > 0) {
+      st.balpha = (o.alpha2 - o.alpha1) / (A.rstop - A.rstart);
+      st.alpha = o.alpha1 + st.balpha * (st.diagi - A.rstart);
+      if (o.ialfp) st.alpha = o.alpha1;
+      st.drnow = st.alpha == 1.0 ? st.drint / o.ialpha
+                                 : st.drint * (st.alpha - 1.0) / (pow(st.alpha, (double)o.ialpha) - 1.0);
+    }
     st.rnow = A.rstart + st.drnow;
+    st.coffl = 0.0;
+    st.xbar = st.xsbar = st.ebar = st.exbar = 0.0;
+    st.endint = 0;
     st.icrmat = 1;
     st.last = 0;
     st.k = 0;
@@ -348,6 +451,7 @@ __global__ void __launch_bounds__(kNT) k_vivs(BlockArgs B, PotlArgs P, VivsArgs 
         st.drnow = rec[1];
         if (st.icrmat) st.rmidi = rec[2];
         st.last = (((int)rec[3]) & 2) != 0;
+        st.endint = (((int)rec[3]) & 4) != 0;
         ++st.k;
       }
     }
@@ -361,6 +465,19 @@ __global__ void __launch_bounds__(kNT) k_vivs(BlockArgs B, PotlArgs P, VivsArgs 
     vivs_hammat(B, pv, rcent, ered, W0);
     if (icrmat && own && tid == 0) st.rmidi = st.diagi;
     __syncthreads();
+    // DERMAT's derivatives at the step centre (vvprop.f:236-247)
+    if (need_w2) {
+      vivs_pvec_der(B, P, rcent, 2, pv);
+      __syncthreads();
+      vivs_dermat(B, pv, rcent, 2, W2);
+      __syncthreads();
+    }
+    if (ivpd) {
+      vivs_pvec_der(B, P, rcent, 1, pv);
+      __syncthreads();
+      vivs_dermat(B, pv, rcent, 1, W1);
+      __syncthreads();
+    }
     if (icrmat) {
       const double rmidi = st.rmidi;
       if (rmidi == rcent) {
@@ -384,6 +501,22 @@ __global__ void __launch_bounds__(kNT) k_vivs(BlockArgs B, PotlArgs P, VivsArgs 
     __syncthreads();
     vivs_symmetrise(W0, n);
     __syncthreads();
+    if (ivpd) {
+      lds_gemm(W1, false, VN, false, TS, n);
+      __syncthreads();
+      lds_gemm(VN, true, TS, false, W1, n);
+      __syncthreads();
+      vivs_symmetrise(W1, n);
+      __syncthreads();
+    }
+    if (need_w2) {
+      lds_gemm(W2, false, VN, false, TS, n);
+      __syncthreads();
+      lds_gemm(VN, true, TS, false, W2, n);
+      __syncthreads();
+      vivs_symmetrise(W2, n);
+      __syncthreads();
+    }
     if (icrmat) {
       // R into the new basis (vvprop.f:292-300)
       lds_gemm(VO, true, VN, false, TS, n);
@@ -392,13 +525,23 @@ __global__ void __launch_bounds__(kNT) k_vivs(BlockArgs B, PotlArgs P, VivsArgs 
       __syncthreads();
       lds_gemm(TS, true, WM, false, R, n);
       __syncthreads();
-      vivs_symmetrise(R, n);
+      if (o.isym) vivs_symmetrise(R, n);
       if (tid == 0) st.icrmat = 0;
     }
     // reference potential and zeroth-order solutions (vvprop.f:305-395)
+    const double factor = o.ishift ? drnow * drnow / 24.0 : 0.0;
     for (int i = tid; i < n; i += kNT) {
-      const double dif = -W0[i + (size_t)i * n];
-      W0[i + (size_t)i * n] = 0.0;
+      const size_t ii = i + (size_t)i * n;
+      double dif, w0d = 0.0;
+      if (o.ishift) {
+        // ISHIFT: the reference potential moved by h^2/24 W2_ii (vvprop.f:305-312)
+        const double d2 = W2[ii];
+        dif = -W0[ii] - factor * d2;
+        w0d = -factor * d2;
+      } else {
+        dif = -W0[ii];
+      }
+      W0[ii] = w0d;
       dg[i] = dif;
       xsq[i] = dif * drnow * drnow;
       const double xl = sqrt(fabs(dif));
@@ -425,14 +568,30 @@ __global__ void __launch_bounds__(kNT) k_vivs(BlockArgs B, PotlArgs P, VivsArgs 
       g2p[i] = cc * cx - dif * d * sx;
     }
     __syncthreads();
+    // contributions not wanted (vvprop.f:318-332): without IV, IVP or IVPP
+    // no step integrals are taken, and the estimates read the potential
+    // matrices kept (all of them under IDIAG, none otherwise)
+    if (!ivd) {
+      for (size_t e = tid; e < nn; e += kNT) W0[e] = 0.0;
+    }
+    if (!do_pert) {
+      for (size_t e = tid; e < nn; e += kNT) {
+        if (!ivpd) W1[e] = 0.0;
+        if (!ivppd) W2[e] = 0.0;
+      }
+    }
+    __syncthreads();
     // the step integrals of every pair i <= j
     const int ntri = n * (n + 1) / 2;
-    for (int e = tid; e < ntri; e += kNT) {
-      int jj = (int)((sqrt(8.0 * (double)e + 1.0) - 1.0) * 0.5);
-      while ((jj + 1) * (jj + 2) / 2 <= e) ++jj;
-      while (jj * (jj + 1) / 2 > e) --jj;
-      const int ii = e - jj * (jj + 1) / 2;
-      vivs_pair_integrals(n, ii, jj, xsq, drnow, W0, W1, W2, E11, E12, E22, a1, b1, a1p, b1p);
+    if (do_pert) {
+      for (int e = tid; e < ntri; e += kNT) {
+        int jj = (int)((sqrt(8.0 * (double)e + 1.0) - 1.0) * 0.5);
+        while ((jj + 1) * (jj + 2) / 2 <= e) ++jj;
+        while (jj * (jj + 1) / 2 > e) --jj;
+        const int ii = e - jj * (jj + 1) / 2;
+        vivs_pair_integrals(n, ii, jj, xsq, drnow, W0, W1, W2, E11, E12, E22, a1, b1, a1p, b1p, ivpd, ivppd,
+                            pert1);
+      }
     }
     __syncthreads();
     if (own) {
@@ -447,7 +606,7 @@ __global__ void __launch_bounds__(kNT) k_vivs(BlockArgs B, PotlArgs P, VivsArgs 
         double w = 1.0;
         if (!(dg[i] > 0.0)) {
           w = exp(-xk[i] * drint / drnow);
-          if (dg[i] < -kVivsXsqmax) w = 0.0;
+          if (dg[i] < -o.xsqmax) w = 0.0;
           an *= w;
           bn *= w;
           anp *= w;
@@ -500,6 +659,32 @@ __global__ void __launch_bounds__(kNT) k_vivs(BlockArgs B, PotlArgs P, VivsArgs 
         st.cdiag = cdiag;
         st.soff = soff == 0.0 ? 1.0e-30 : soff;
         st.sdiag = sdiag;
+        // does the interval end here (vvprop.f:480-500)
+        int endint;
+        if (st.last) {
+          endint = 1;
+        } else if (o.ialpha > 0) {
+          endint = st.istep % o.ialpha == 0;
+        } else {
+          // IALPHA <= 0: when the irregular solution's slope would grow past
+          // 1e4, or the corrections' estimates grow towards their tolerances
+          endint = -1;
+          const double g2pmax = g2p[n - 1] * cosx[n - 1];
+          if (fabs(g2pmax) > 1.0e4) {
+            endint = 1;
+          } else if (st.coffl != 0.0) {
+            const double facc = fmin(st.coff / st.coffl, 2.0);
+            const double facs = fmin(st.soff / st.soff1, 2.0);
+            if (facc * st.coff > 0.8 * A.toff || facs * st.soff > 0.8 * A.tldiag) endint = 1;
+          }
+          if (endint < 0) {
+            st.coffl = st.coff;
+            st.soff1 = st.soff;
+            st.sdiag1 = st.sdiag;
+            endint = !(st.coff < 0.8 * A.toff && st.cdiag < 0.8 * A.toff);
+          }
+        }
+        st.endint = endint;
         // every run ends: at its last step, or here after max_steps
         if (st.istep > A.max_steps) {
           st.status = kVivsTooManySteps;
@@ -509,28 +694,31 @@ __global__ void __launch_bounds__(kNT) k_vivs(BlockArgs B, PotlArgs P, VivsArgs 
           rec[0] = st.rnow;
           rec[1] = st.drnow;
           rec[2] = st.rmidi;
-          rec[3] = (double)((icrmat ? 1 : 0) + (st.last ? 2 : 0));
+          rec[3] = (double)((icrmat ? 1 : 0) + (st.last ? 2 : 0) + (endint ? 4 : 0));
         }
       }
     }
     __syncthreads();
     if (st.stop) break;
-    const bool endint = st.last || (st.istep % kVivsIalpha == 0);
+    const bool endint = st.endint != 0;
     if (endint) {
-      // G + D for both solutions and their derivatives (vvprop.f:440-452, 510-519)
+      // G + D for both solutions and their derivatives (vvprop.f:440-452,
+      // 510-519); D = 0 without IPERT (vvprop.f:460-466)
+      const bool pert = o.ipert != 0;
       for (size_t e = tid; e < nn; e += kNT) {
         const int j = (int)(e / n), i = (int)(e - (size_t)j * n);
         const size_t ji = (size_t)j + (size_t)i * n;
-        const double p1 = g1[j] * E12[e] - g2[j] * E11[e];
-        const double p2 = g1[i] * E22[e] - g2[i] * E12[e];
-        const double p1p = g1p[j] * E12[e] - g2p[j] * E11[e];
-        const double p2p = g1p[i] * E22[e] - g2p[i] * E12[e];
+        const double p1 = pert ? g1[j] * E12[e] - g2[j] * E11[e] : 0.0;
+        const double p2 = pert ? g1[i] * E22[e] - g2[i] * E12[e] : 0.0;
+        const double p1p = pert ? g1p[j] * E12[e] - g2p[j] * E11[e] : 0.0;
+        const double p2p = pert ? g1p[i] * E22[e] - g2p[i] * E12[e] : 0.0;
         W0[e] = i == j ? p1p + g1p[i] : p1p;
         W2[e] = i == j ? p1 + g1[i] : p1;
         WM[ji] = i == j ? p2 + g2[i] : p2;
         W1[ji] = i == j ? p2p + g2p[i] : p2p;
       }
       if (own && tid == 0) {
+        st.coffl = 0.0;
         st.soff1 = st.soff;
         st.sdiag1 = st.sdiag;
       }
@@ -552,7 +740,7 @@ __global__ void __launch_bounds__(kNT) k_vivs(BlockArgs B, PotlArgs P, VivsArgs 
       if (!ok && tid == 0 && st.status == kVivsOk) st.status = kVivsSingular;
       for (size_t e = tid; e < nn; e += kNT) {
         const int j = (int)(e / n), i = (int)(e - (size_t)j * n);
-        R[e] = 0.5 * (VO[e] + VO[(size_t)j + (size_t)i * n]);
+        R[e] = o.isym ? 0.5 * (VO[e] + VO[(size_t)j + (size_t)i * n]) : VO[(size_t)j + (size_t)i * n];
         E11[e] = 0.0;
         E12[e] = 0.0;
         E22[e] = 0.0;
@@ -574,10 +762,50 @@ __global__ void __launch_bounds__(kNT) k_vivs(BlockArgs B, PotlArgs P, VivsArgs 
         // next step and interval (vvprop.f:600-665)
         const double rstop = A.rstop;
         double rnow = st.rnow;
-        double dr = st.drnow * st.alpha;
+        st.xbar += rnow;
+        st.xsbar += rnow * rnow;
+        st.ebar += st.sdiag;
+        st.exbar += rnow * st.sdiag;
+        double dr;
+        if (o.ialpha > 0) {
+          dr = st.drnow * st.alpha;
+        } else {
+          // IALPHA <= 0: the step from DELRD on the step corrections
+          double soff = st.soff;
+          const double tmxx0 = fmax(0.5 * A.toff, A.tldiag);
+          if (st.drint != st.drnow) soff = 0.8 * A.tldiag * pow(soff / (0.8 * tmxx0), 1.5);
+          dr = st.drnow;
+          if (!vivs_delrd(dr, st.sdiag, soff, A.tldiag, A.drmax, dg[0], dg[n - 1], rnow, rstop)) {
+            st.status = kVivsStepTooSmall;
+            st.stop = 1;
+          }
+        }
         if (st.icrmat) {
           double drint = rnow - st.rmid;
           double drint1 = drint;
+          if (o.ialpha > 0) {
+            // IALFP's prediction of ALPHA from the trend of the diagonal
+            // corrections over the interval (vvprop.f:630-660)
+            const double ia = (double)o.ialpha;
+            const double xbar = st.xbar / ia, xsbar = st.xsbar / ia, ebar = st.ebar / ia, exbar = st.exbar / ia;
+            const double slope = o.ialpha == 1 ? 0.0 : (exbar - xbar * ebar) / (xsbar - xbar * xbar);
+            const double bint = ebar - xbar * slope;
+            double emax = bint + slope * rnow;
+            const double emin = bint + slope * (rnow - drint);
+            st.alfnew = st.alpha;
+            if (o.ialpha > 1) {
+              if (emax == 0.0) emax = 1.0e-30;
+              double fac = emin / emax;
+              if (fac <= 0.0) {
+                fac = emin <= 0.0 ? 0.9 : 1.1;
+              } else {
+                fac = pow(fac, 1.0 / (double)(3 * o.ialpha - 3));
+                fac = fmax(fmin(fac, 1.1), 0.9);
+              }
+              st.alfnew = st.alpha * fac;
+            }
+          }
+          st.xbar = st.xsbar = st.ebar = st.exbar = 0.0;
           const double tmxx = fmax(A.toff, A.tldiag);
           double coff = st.coff;
           if (drint != dr) coff = 0.8 * tmxx * pow(coff / (0.8 * tmxx), 1.5);
@@ -596,9 +824,12 @@ __global__ void __launch_bounds__(kNT) k_vivs(BlockArgs B, PotlArgs P, VivsArgs 
           st.drint = drint;
           st.rmid = rnow;
           st.diagi = rnow + 0.5 * drint;
-          st.alpha = kVivsAlpha1 + st.balpha * (st.diagi - A.rstart);
-          dr = st.alpha == 1.0 ? drint / kVivsIalpha
-                               : drint * (st.alpha - 1.0) / (pow(st.alpha, (double)kVivsIalpha) - 1.0);
+          if (o.ialpha > 0) {
+            st.alpha = o.alpha1 + st.balpha * (st.diagi - A.rstart);
+            if (o.ialfp) st.alpha = st.alfnew;
+            dr = st.alpha == 1.0 ? drint / o.ialpha
+                                 : drint * (st.alpha - 1.0) / (pow(st.alpha, (double)o.ialpha) - 1.0);
+          }
         }
         if (fabs(rstop - rnow - dr) < fabs(0.01 * dr)) dr = rstop - rnow;
         if ((rstop - rnow - dr) * dr < 0.0) dr = rstop - rnow;
@@ -617,7 +848,7 @@ __global__ void __launch_bounds__(kNT) k_vivs(BlockArgs B, PotlArgs P, VivsArgs 
   __syncthreads();
   lds_gemm(WM, false, VN, true, R, n);
   __syncthreads();
-  vivs_symmetrise(R, n);
+  if (o.isym) vivs_symmetrise(R, n);
   __syncthreads();
   for (size_t e = tid; e < nn; e += kNT) rg[e] = R[e];
   if (tid == 0) {

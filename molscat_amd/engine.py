@@ -480,6 +480,16 @@ class Engine:
         """Batched large-N inversions: 1 (default) 32-pivot passes for N <= 448, 0 16-pivot passes."""
         self._check(self.lib.mcp_set_inverse_kernel(self.ctx, int(mode)), "mcp_set_inverse_kernel")
 
+    def set_vivs_options(self, iv: bool = True, ivp: bool = False, ivpp: bool = False, ishift: bool = False,
+                         idiag: bool = False, ipert: bool = True, isym: bool = True, ialpha: int = 6,
+                         ialfp: bool = False, alpha1: float = 1.0, alpha2: float = 1.5, xsqmax: float = 1.0e4):
+        """The /LDVVCM/ controls of VVPROP (source_code/vvprop.f:57-82) for
+        this context's later prop_vivs calls; the defaults are the
+        reference's (source_code/mol.driver.f:277-349)."""
+        self._check(self.lib.mcp_set_vivs_options(self.ctx, int(iv), int(ivp), int(ivpp), int(ishift), int(idiag),
+                                                  int(ipert), int(isym), int(ialpha), int(ialfp), float(alpha1),
+                                                  float(alpha2), float(xsqmax)), "mcp_set_vivs_options")
+
     def set_precision(self, newton_steps: int = 0, r_fp64: float = 0.0):
         """Mixed precision on the large-N tree's inversions: FP32 inverse +
         newton_steps FP64 Newton-Schulz steps, FP64 below r_fp64; 0 = FP64."""

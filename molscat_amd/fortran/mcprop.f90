@@ -237,6 +237,13 @@ module mcprop
       type(c_ptr), value :: ctx
       integer(c_int), value :: mode
     end function
+    integer(c_int) function mcp_set_vivs_options(ctx, iv, ivp, ivpp, ishift, idiag, ipert, isym, ialpha, ialfp, &
+        alpha1, alpha2, xsqmax) bind(C, name="mcp_set_vivs_options")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: iv, ivp, ivpp, ishift, idiag, ipert, isym, ialpha, ialfp
+      real(c_double), value :: alpha1, alpha2, xsqmax
+    end function
     integer(c_int) function mcp_set_precision(ctx, newton_steps, r_fp64) bind(C, name="mcp_set_precision")
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: ctx
@@ -306,6 +313,12 @@ module mcprop
   integer, save :: mcpf_potl_nterm = -1
   integer(c_int), allocatable, save :: mcpf_potl_lam(:), mcpf_potl_npower(:)
   real(c_double), allocatable, save :: mcpf_potl_a(:), mcpf_potl_e(:)
+
+  ! VVPROP's steps as the energy that chose them recorded them (the part of
+  ! the reference's scratch file ISCRU that the later energies read back):
+  ! 4 values per step, mcpf_vivs_nrec steps.
+  real(c_double), allocatable, save :: mcpf_vivs_rec(:)
+  integer, save :: mcpf_vivs_nrec = 0
 
 contains
 

@@ -279,6 +279,84 @@ subroutine AIPROP(NCH, MXLAM, NHAM, Z, VL, IV, EINT, CENT, RSTART, RSTOP, NSTEP,
   NODES = nd(1)
 end subroutine
 
+subroutine VVPROP(N, MXLAM, NHAM, RMAT, VL, IVL, EINT, CENT, RSTART, RSTOP, NSTEP, DRNOW, DRMAX, TLDIAG, TOFF, &
+                  ERED, EP2RU, CM2RU, RSCALE, IPRINT)
+  ! source_code/vvprop.f:1-4 (VIVS, IPROP = 4): RMAT, the R-matrix, from
+  ! RSTART to RSTOP under the /LDVVCM/ controls (vvprop.f:57-82), the
+  ! potential from the &POTL terms registered with mcpf_set_potl (the device
+  ! evaluates them, and DERMAT's derivatives, where the step control lands).
+  ! /PRPSCR/'s IWRITE keeps this call's steps, IREAD replays the kept steps
+  ! (the scratch file ISCRU of the energies after the first, vvprop.f:
+  ! 150-160, 350-352, 590-593).  NSTEP returns the steps taken.
+  use, intrinsic :: iso_c_binding
+  use mcprop
+  use potential, only: NCONST, NRSQ, CONLEN
+  implicit none
+  integer :: N, MXLAM, NHAM, NSTEP, IPRINT, IVL(*)
+  double precision :: RMAT(N, N), VL(*), EINT(N), CENT(N), RSTART, RSTOP, DRNOW, DRMAX, TLDIAG, TOFF, ERED, &
+                      EP2RU, CM2RU, RSCALE
+  double precision :: XSQMAX, ALPHA1, ALPHA2, ESHIFT
+  integer :: IALPHA, ISCRU, ISCRUR
+  logical :: IALFP, IV, IVP, IVPP, ISHIFT, IDIAG, IPERT, ISYM, IREAD, IWRITE, IREADR, IWRITR
+  common /LDVVCM/ XSQMAX, ALPHA1, ALPHA2, IALPHA, IALFP, IV, IVP, IVPP, ISHIFT, IDIAG, IPERT, ISYM
+  common /PRPSCR/ ESHIFT, ISCRU, ISCRUR, IREAD, IWRITE, IREADR, IWRITR
+  real(c_double), allocatable :: vlc(:), steps(:)
+  real(c_double) :: e1(1)
+  integer(c_int) :: ib, st(1), ns(1), replay, mxstep
+  type(c_ptr) :: ctx
+  if (NCONST /= 0 .or. NRSQ /= 0) then
+    write (6, '(A)') ' *** VVPROP: mcprop TAKES BLOCKS WITH NCONST = NRSQ = 0 ONLY.  RUN HALTED.'
+    stop
+  end if
+  if (mcpf_potl_nterm < 0) then
+    write (6, '(A)') ' *** VVPROP: VIVS NEEDS THE &POTL TERMS (mcpf_set_potl).  RUN HALTED.'
+    stop
+  end if
+  if (IREAD .and. mcpf_vivs_nrec <= 0) then
+    write (6, '(A)') ' *** VVPROP: IREAD WITHOUT STEPS KEPT BY AN EARLIER IWRITE CALL.  RUN HALTED.'
+    stop
+  end if
+  ctx = mcpf_context()
+  allocate (vlc(NHAM*(N*(N + 1)/2)))
+  call mcpf_compact_vl(N, NHAM, VL, IVL, vlc)
+  ib = mcpf_block(N, MXLAM, NCONST, NRSQ, vlc, EINT, CENT, CONLEN)
+  call mcpf_check(mcp_block_set_potl(ctx, ib, int(mcpf_potl_nterm, c_int), mcpf_potl_lam, mcpf_potl_npower, &
+                                     mcpf_potl_a, mcpf_potl_e, RSCALE, EP2RU), 'VVPROP')
+  call mcpf_check(mcp_set_vivs_options(ctx, l2i(IV), l2i(IVP), l2i(IVPP), l2i(ISHIFT), l2i(IDIAG), l2i(IPERT), &
+                                       l2i(ISYM), int(IALPHA, c_int), l2i(IALFP), ALPHA1, ALPHA2, XSQMAX), 'VVPROP')
+  e1(1) = ERED
+  if (IREAD) then
+    mxstep = int(mcpf_vivs_nrec, c_int)
+    replay = 1
+    allocate (steps(4*mxstep))
+    steps = mcpf_vivs_rec(1:4*mxstep)
+  else
+    mxstep = 200000
+    replay = 0
+    allocate (steps(4*mxstep))
+  end if
+  call mcpf_check(mcp_prop_vivs(ctx, ib, 1_c_int, e1, RMAT, RSTART, RSTOP, DRNOW, DRMAX, TLDIAG, TOFF, replay, &
+                                mxstep, steps, ns, st), 'VVPROP')
+  if (st(1) == 1) then
+    write (6, '(A)') ' *** VVPROP: DELRD STEP SIZE TOO SMALL.  RUN HALTED.'
+    stop
+  else if (st(1) /= 0) then
+    write (6, '(A,I3,A)') ' *** VVPROP: PROPAGATION FAILED (STATUS', st(1), ').  RUN HALTED.'
+    stop
+  end if
+  NSTEP = ns(1)
+  if (IWRITE .and. .not. IREAD) then
+    mcpf_vivs_rec = steps(1:4*NSTEP)
+    mcpf_vivs_nrec = NSTEP
+  end if
+contains
+  integer(c_int) function l2i(b)
+    logical, intent(in) :: b
+    l2i = 0
+    if (b) l2i = 1
+  end function
+end subroutine
+
 subroutine SYMINV(A, IA, N, IFAIL)
   ! source_code/syminv.f:1 (with SYMINL for N > 30): the lower triangle of
   ! A(1:N,1:N) in, the full symmetric inverse out; IFAIL = the number of

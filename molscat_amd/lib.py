@@ -58,6 +58,8 @@ SIGNATURES = {
     "mcp_set_large_mode": (ctypes.c_int, [_vp, _i32]),
     "mcp_set_inverse_kernel": (ctypes.c_int, [_vp, _i32]),
     "mcp_set_precision": (ctypes.c_int, [_vp, _i32, _dbl]),
+    "mcp_set_vivs_options": (ctypes.c_int, [_vp, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _dbl, _dbl,
+                                             _dbl]),
     "mcp_set_md_kernel": (ctypes.c_int, [_vp, _i32]),
     "mcp_precision_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     "mcp_prop_ld_multi": (ctypes.c_int, [_vp, _i32, _pi, _i32, _pi, _pd, _pd, _pi, _pi]),

@@ -309,6 +309,30 @@ class PowerPotential:
             p[i] = tot
         return p
 
+    def deriv(self, r: float, ider: int) -> np.ndarray:
+        """POTENL with ICNTRL = IDER (source_code/potenl.f:177-185): the
+        IDER'th derivative of each radial term with respect to its argument
+        (R_prog * RSCALE), N R^(N-1) and N (N-1) R^(N-2) for powers, E^IDER
+        exp(E R) for exponentials; no clamp."""
+        p = np.zeros(len(self.terms))
+        for i, tl in enumerate(self.terms):
+            tot = 0.0
+            for t in tl:
+                if t[1] != 0:
+                    rr = r
+                    if t[1] < 0 and abs(r) < 1.0e-8:
+                        rr = 1.0e-8
+                    term = rr ** t[1]
+                    if ider == 1:
+                        term = float(t[1]) * term / rr
+                    elif ider == 2:
+                        term = float(t[1] * (t[1] - 1)) * term / (rr * rr)
+                else:
+                    term = math.exp(t[2] * r) * t[2] ** ider
+                tot += t[0] * term
+            p[i] = tot
+        return p
+
 
 def lj_rotor_potential(cvals: Sequence[float], rm: float, epsil: float) -> PowerPotential:
     """V_lam = c_lam * eps * [(Rm/R)^12 - 2 (Rm/R)^6] as &POTL NTERM=2,
@@ -365,6 +389,28 @@ class Block:
                 pc[i - mxlam - 1] = self.ep2ru * (pc[i - mxlam - 1] / self.ep2ru + dn / x ** ipow)
             p = np.concatenate([p, pc])
         return p
+
+    def dermat(self, r: float, ider: int) -> np.ndarray:
+        """DERMAT (source_code/dermat.f:38-126, NUMDER off, NRSQ = 0): the
+        IDER'th derivative of W at R: POTENL's derivatives at R*RSCALE
+        (derivatives with respect to that argument, as DERMAT takes them)
+        times EP2RU, zero for the NCONST constant terms, SCAPOT's SCALAM, no
+        PERTRB (dermat.f:93-95); on the diagonal the centrifugal term's
+        derivative, -2/R^3 or 6/R^4 (clamped at 1e16) times L(L+1).  For
+        NRSQ = 0 the reference adds RSQ*CENT with RSQ set only in its NRSQ > 0
+        branch (dermat.f:99-110), so its value there is whatever the variable
+        held; the derivative of the centrifugal term is what the code means."""
+        x = r * self.rscale
+        p = self.potl.deriv(x, ider)
+        p = self.ep2ru * p
+        p = p * self.scalam
+        if self.nconst > 0:
+            p = np.concatenate([p, np.zeros(self.nconst)])
+        w = np.tensordot(p, self.vl_full, axes=(0, 0))
+        rsq = -2.0 / r ** 3 if ider == 1 else 6.0 / r ** 4
+        rsq = min(rsq, 1.0e16)
+        w[np.diag_indices(self.n)] += rsq * self.cent
+        return w
 
     def hammat(self, r: float, ered: float) -> Tuple[np.ndarray, np.ndarray]:
         """HAMMAT: source_code/hammat.f:1-116 with SUMLAM source_code/sumlam.f:29-36
@@ -1785,6 +1831,35 @@ VIVS_XSQMAX = 1.0e4
 VIVS_DRMAX = 5.0
 
 
+@dataclass
+class VivsOptions:
+    """The /LDVVCM/ controls of VVPROP (source_code/vvprop.f:57-82), with
+    the reference's defaults (source_code/mol.driver.f:277-349).  IDIAG as
+    MOL.PRPINI leaves it: IDIAG itself only adds the diagonal terms (IVD,
+    IVPD, IVPPD); the driver's IDIAG default also turns on IV, IVP, IVPP,
+    ISHIFT and IPERT (source_code/mol.prpini.f:75-83, vivs_idiag_defaults)."""
+    iv: bool = True
+    ivp: bool = False
+    ivpp: bool = False
+    ishift: bool = False
+    idiag: bool = False
+    ipert: bool = True
+    isym: bool = True
+    ialpha: int = VIVS_IALPHA
+    ialfp: bool = False
+    alpha1: float = VIVS_ALPHA1
+    alpha2: float = VIVS_ALPHA2
+    xsqmax: float = VIVS_XSQMAX
+
+
+def vivs_idiag_defaults(o: VivsOptions) -> VivsOptions:
+    """MOL.PRPINI for IPROPL = 4 (source_code/mol.prpini.f:75-83): IDIAG
+    switches on IV, IVP, IVPP, ISHIFT and IPERT."""
+    if o.idiag:
+        o = VivsOptions(**{**o.__dict__, "iv": True, "ivp": True, "ivpp": True, "ishift": True, "ipert": True})
+    return o
+
+
 def vivs_tolerances(tolhi: float) -> Tuple[float, float]:
     """TLDIAG and TOFF from the segment's TOLHI (source_code/scctrl.f:157-158)."""
     tldiag = 0.064 * math.sqrt(tolhi / 0.001)
@@ -1814,17 +1889,29 @@ def delrd(dr: float, cdiag: float, coff: float, tol: float, drmax: float, e1: fl
     return dr
 
 
-def pert2_v0(cx, sx, xsq, x, h, v0, a1, a2, a1p, a2p, eye11, eye12, eye22):
-    """PERT2 (source_code/pert2.f:1-230) with V1 = V2 = 0 (IV only): the step
-    integrals CC, CS, SS of the constant off-diagonal perturbation V0 between
-    the step's reference solutions, and their accumulation into EYE11,
-    EYE12, EYE22 with the step-start values A1, A1P (regular) and A2, A2P
-    (irregular).  The V1 and V2 terms of PERT2 are finite multiples of V1
-    and V2 and vanish exactly.  Returns (CC, CS, SS); the EYE are updated in
-    place.  Vectorised over the pairs I <= J; each pair takes PERT2's branch:
+def pert2(cx, sx, xsq, x, h, v0, a1, a2, a1p, a2p, eye11, eye12, eye22, v1=None, v2=None, pert1=False,
+          ref_sign_150=False):
+    """PERT2 (source_code/pert2.f:1-230), or PERT1 (source_code/pert1.f:1-224)
+    with pert1=True: the step integrals CC, CS, SS of the perturbation
+    V0 + V1 (r - h/2) + V2 (r - h/2)^2 / 2 (r measured from the step start;
+    V1, V2 the first and second derivatives of W at the step centre, DERMAT)
+    between the step's reference solutions, and their accumulation into
+    EYE11, EYE12, EYE22 with the step-start values A1, A1P (regular) and A2,
+    A2P (irregular).  PERT1 keeps V2 on the diagonal only (pert1.f:160-187).
+    v1 = v2 = None: the V1 and V2 terms are exact zeros, as PERT2 gives them
+    for V1 = V2 = 0.  Returns (CC, CS, SS); the EYE are updated in place.
+    Vectorised over the pairs I <= J; each pair takes PERT2's branch:
     |XI - XJ| > TOL (1e-3) or one open and one closed channel: the closed
     form; else |XSQI + XSQJ| < TOL: the series about XSQ = 0; else the
-    series about XI = XJ (both open or both closed)."""
+    series about XI = XJ (both open or both closed).
+
+    The series about XSQ = 0 gives R2V1 as HC (1/12 + ...) (XSQI - XSQJ)
+    (pert2.f:175, and pert1.f alike); its leading term makes the V1 parts
+    of CS and CS' come out as -h^3/12 where the integrals of c_i s_j (r -
+    h/2) tend to +h^3/12 (both solutions near threshold), i.e. the
+    difference enters with the wrong sign.  This restatement takes the sign
+    that matches the integrals (checked against quadrature); ref_sign_150 =
+    True keeps the reference's expression as written."""
     n = len(x)
     hsq = h * h
     hinv = 1.0 / h
@@ -1843,7 +1930,13 @@ def pert2_v0(cx, sx, xsq, x, h, v0, a1, a2, a1p, a2p, eye11, eye12, eye22):
     f7 = cxi * cxj
     sd = xsqi - xsqj
     total = xsqi + xsqj
-    coef = hf / (2.0 * f6) * v0[iu, ju]
+    coef0 = hf / (2.0 * f6)
+    coef = coef0 * v0[iu, ju]
+    use1 = v1 is not None
+    use2 = v2 is not None
+    hc = h * coef0 * v1[iu, ju] if use1 else None
+    hhc = hsq * coef0 * v2[iu, ju] if use2 else None
+    zero = np.zeros(len(iu))
     d = xi - xj
     tol = 1.0e-3
     with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
@@ -1859,6 +1952,19 @@ def pert2_v0(cx, sx, xsq, x, h, v0, a1, a2, a1p, a2p, eye11, eye12, eye22):
         f12 = 2.0 * coef * f25
         r1_130 = f12 * (f9 - f8)
         r2_130 = f12 * (xsxi - xsxj)
+        f10 = sd * sd
+        f18 = 2.0 * total
+        if use1:
+            f13 = hc * f25
+            r1v1_130 = f13 * ((f18 * (f7 - 1.0) + 4.0 * f6) * f25 + f9 - f8)
+            r2v1_130 = f13 * (f18 * (cxi - cxj) * f25 + xsxi + xsxj)
+        if use2:
+            f19 = 0.25 * f25
+            f24 = -8.0 * (total + 2.0 * xsqj)
+            f26 = -8.0 * (total + 2.0 * xsqi)
+            f31 = hhc / f10
+            r1v2_130 = f31 * (((f10 + f24) * f9 - (f10 + f26) * f8) * f19 + total * (f7 + 1.0) + 2.0 * f6)
+            r2v2_130 = f31 * (((f10 + f24) * xsxi - (f10 + f26) * xsxj) * f19 + total * (cxi + cxj))
         # series about XI = XJ (labels 110/120/140)
         s = xi + xj
         ssq = s * s
@@ -1874,10 +1980,22 @@ def pert2_v0(cx, sx, xsq, x, h, v0, a1, a2, a1p, a2p, eye11, eye12, eye22):
         f17 = dsq / 80.0
         f15 = ss2 * c2
         f36 = dsq * (1.0 / 168.0)
+        f37 = dsq * (1.0 / 112.0)
+        f38 = dsq * (1.0 / 40.0)
         f39 = 1.0 - dsq * (1.0 / 8.0) * (1.0 - dsq * (1.0 / 48.0) * (1.0 - dsq * (1.0 / 120.0)))
         f40 = 1.0 / ssq
         r1_140 = coef * (1.0 - dsq * (1.0 / 6.0) * (1.0 - dsq / 20.0 * (1.0 - dsq * (1.0 / 42.0))) + 2.0 * f15 * f40)
         r2_140 = coef * (f14 * f39 + 0.5 * c2 * (1.0 - f16 * (1.0 - f17 * (1.0 - f36)))) * 2.0
+        if use1:
+            r1v1_140 = hc * ((-2.0 * ss2 * f14 + f15) * f40 - f16 * (1.0 - dsq * (1.0 / 15.0) * (1.0 - 3.0 * f37)))
+            r2v1_140 = hc * (-s2 * (1.0 / 12.0) * (1.0 - f38 * (1.0 - f37))
+                             + (0.5 * c2 - f14) / s * (1.0 - f16 * (1.0 - f17))) * d
+        if use2:
+            f41 = (ssq - 8.0) * 0.5 * f14
+            r1v2_140 = hhc * (c2 * (2.0 * c2 + f41) * f40
+                              + (1.0 - dsq * 0.2 * (1.0 - 11.0 * f36 * (1.0 - dsq * (1.0 / 36.0)))) * (1.0 / 12.0)) * 0.5
+            r2v2_140 = hhc * (f39 * (c2 + f41 * 0.5) * f40
+                              + (1.0 - f38 * (3.0 - f37 * (5.0 - 7.0 * dsq * (1.0 / 216.0)))) * c2 * (1.0 / 24.0))
         # series about XSQ = 0 (label 150)
         g3 = xsqi * xsqi
         f20 = total * total
@@ -1888,18 +2006,52 @@ def pert2_v0(cx, sx, xsq, x, h, v0, a1, a2, a1p, a2p, eye11, eye12, eye22):
         f30 = coef * 2.0
         r1_150 = f30 * (1.0 - f29 + (f20 + f22) * (1.0 / 120.0) - total * (g3 + f28 + f23) * (1.0 / 5040.0))
         r2_150 = f30 * (1.0 - f29 + (xsqi * total + f23) * (1.0 / 120.0) - (g3 + f23) * total * (1.0 / 5040.0))
+        if use1:
+            r1v1_150 = -hc * (1.0 / 12.0) * (total - (f20 + f22) * (1.0 / 15.0)
+                                             + total * (g3 + f28 + f23) * (1.0 / 560.0))
+            r2v1_150 = hc * (1.0 / 12.0 + total * (1.0 / 180.0)
+                             - (3.0 * f20 - 2.0 * xsqi * xsqj) * (1.0 / 20160.0)) * (sd if ref_sign_150 else -sd)
+        if use2:
+            f21 = total * 0.2
+            g7 = g3 * xsqi
+            r1v2_150 = hhc * (1.0 - f21 + 11.0 * (f20 + f22) * (1.0 / 840.0)
+                              - 11.0 * total * (g3 + f28 + f23) * (1.0 / 30240.0)) * (1.0 / 12.0)
+            r2v2_150 = hhc * (1.0 - f21 + (11.0 * f20 - 19.0 * f5) * (1.0 / 840.0)
+                              - (11.0 * (g7 + f23 * xsqj) + 3.0 * f5 * total) * (1.0 / 30240.0)) * (1.0 / 12.0)
         # diagonal (label 160)
         g1 = xsxi / xsqi
         r1_d = coef * (1.0 + cxi * g1)
         r2_d = coef * (g1 + cxi)
-    r1 = np.where(diag, r1_d, np.where(b130, r1_130, np.where(b140, r1_140, r1_150)))
-    r2 = np.where(diag, r2_d, np.where(b130, r2_130, np.where(b140, r2_140, r2_150)))
-    # label 170
-    term1 = -cxj * r1
-    ccij = f6 * r1 * hsqinv
-    csij = xsxi * (r2 + term1) * hinv
-    ssij = r1 - cxj * r2 - cxi * r2 + f7 * r1
-    csji = xsxj0 * (r2 - cxi * r1) * hinv
+        if use1:
+            r1v1_d = hc * (cxi - g1) * (0.5 * g1)
+        if use2:
+            r1v2_d = hhc * (cxi * (cxi + (xsqi - 2.0) * (0.5 * g1)) / xsqi * 0.25 + 1.0 / 24.0)
+            r2v2_d = hhc * ((6.0 + xsqi) * cxi * (1.0 / 3.0) + (xsqi - 2.0) * g1) / (8.0 * xsqi)
+
+    def pick(dv, v130, v140, v150):
+        return np.where(diag, dv, np.where(b130, v130, np.where(b140, v140, v150)))
+
+    r1 = pick(r1_d, r1_130, r1_140, r1_150)
+    r2 = pick(r2_d, r2_130, r2_140, r2_150)
+    r1v1 = pick(r1v1_d, r1v1_130, r1v1_140, r1v1_150) if use1 else zero
+    r2v1 = pick(zero, r2v1_130, r2v1_140, r2v1_150) if use1 else zero
+    if use2:
+        r1v2 = pick(r1v2_d, r1v2_130, r1v2_140, r1v2_150)
+        r2v2 = pick(r2v2_d, r2v2_130, r2v2_140, r2v2_150)
+        if pert1:
+            # PERT1: the second derivative on the diagonal only
+            r1v2 = np.where(diag, r1v2, 0.0)
+            r2v2 = np.where(diag, r2v2, 0.0)
+    else:
+        r1v2 = r2v2 = zero
+    # label 170 (pert2.f:199-211; pert1.f:180-200 with R1V2 = R2V2 = 0 off the diagonal)
+    r1s = r1 + r1v1 + r1v2
+    r2s = r2 + r2v1 + r2v2
+    term1 = -cxj * r1s
+    ccij = f6 * r1s * hsqinv
+    csij = xsxi * (r2s + term1) * hinv
+    ssij = r1 - r1v1 + r1v2 - cxj * (r2 - r2v1 + r2v2) - cxi * r2s + f7 * r1s
+    csji = xsxj0 * (r2 - r2v1 + r2v2 - cxi * r1s) * hinv
     cc = np.zeros((n, n))
     cs = np.zeros((n, n))
     ss = np.zeros((n, n))
@@ -1927,6 +2079,12 @@ def pert2_v0(cx, sx, xsq, x, h, v0, a1, a2, a1p, a2p, eye11, eye12, eye22):
     return cc, cs, ss
 
 
+def pert2_v0(cx, sx, xsq, x, h, v0, a1, a2, a1p, a2p, eye11, eye12, eye22):
+    """PERT2 with V1 = V2 = 0 (the default IV option): pert2 without its V1
+    and V2 terms."""
+    return pert2(cx, sx, xsq, x, h, v0, a1, a2, a1p, a2p, eye11, eye12, eye22)
+
+
 @dataclass
 class VivsStep:
     """One VVPROP step as the reference's scratch file records it for the
@@ -1938,29 +2096,47 @@ class VivsStep:
     newint: bool
     rmidi: float
     last: bool
+    endint: bool = False  # the step ends an interval (NEWINT on the scratch file, vvprop.f:590-593)
 
 
 def vvprop(rmat: np.ndarray, blk: Block, ered: float, rstart: float, rstop: float, drnow: float,
            tldiag: float, toff: float, drmax: float = VIVS_DRMAX,
-           replay: Optional[List[VivsStep]] = None) -> Tuple[np.ndarray, List[VivsStep]]:
-    """VVPROP (source_code/vvprop.f:1-750) for the default options (IV,
-    IPERT, ISYM, IALPHA = 6, IALFP = .FALSE.): propagates the R-matrix (the
-    inverse of Y) from RSTART to RSTOP.  Per step: W at the step centre in
-    the interval's local basis (the eigenvectors of W at the interval's
-    centre RMIDI, ascending, DIAGVC), its diagonal the reference potential
-    and its off-diagonal part the perturbation; sine/cosine (sinh/cosh)
-    reference solutions G1 (regular) and G2 (irregular) of each channel;
-    PERT2's step integrals accumulated over the interval.  At each interval
-    end the R-matrix R <- (G1' + D1' + R (G2' + D2'))^-1 (G1 + D1 + R (G2 +
-    D2)), symmetrised, D the first-order corrections (DSYMM + DGESV,
-    vvprop.f:510-618), and the next interval size from DELRD on the
-    corrections' estimates.  With replay (the steps of an earlier energy),
-    the steps are the recorded ones, as with the reference's scratch file
-    (the energies after the first, vvprop.f:155-160, 410, 590).
-    Returns (R at RSTOP in the primitive basis, the steps taken)."""
+           replay: Optional[List[VivsStep]] = None,
+           opts: Optional[VivsOptions] = None) -> Tuple[np.ndarray, List[VivsStep]]:
+    """VVPROP (source_code/vvprop.f:1-750) with the /LDVVCM/ controls of
+    opts (the reference's defaults when None: IV, IPERT, ISYM, IALPHA = 6,
+    IALFP = .FALSE.): propagates the R-matrix (the inverse of Y) from RSTART
+    to RSTOP.  Per step: W at the step centre in the interval's local basis
+    (the eigenvectors of W at the interval's centre RMIDI, ascending,
+    DIAGVC), its diagonal the reference potential and its off-diagonal part
+    the perturbation; with IVP / IVPP (or IDIAG) DERMAT's first / second
+    derivatives of W at the step centre, in the same basis, add the
+    perturbations V1 (r - h/2) and V2 (r - h/2)^2/2; ISHIFT moves the
+    reference potential by h^2/24 times the second derivative's diagonal
+    (vvprop.f:238-330).  Sine/cosine (sinh/cosh) reference solutions G1
+    (regular) and G2 (irregular) of each channel; PERT2's (or PERT1's) step
+    integrals accumulated over the interval.  At each interval end the
+    R-matrix R <- (G1' + D1' + R (G2' + D2'))^-1 (G1 + D1 + R (G2 + D2)),
+    symmetrised under ISYM, D the first-order corrections (zero without
+    IPERT; DSYMM + DGESV, vvprop.f:510-618).  Intervals of IALPHA steps in
+    geometric progression (ALPHA predicted from the diagonal corrections'
+    trend under IALFP), or for IALPHA <= 0 steps from DELRD with an interval
+    ended when the corrections' estimates grow (vvprop.f:480-500); the next
+    interval size from DELRD.  With replay (the steps of an earlier energy),
+    the steps and interval ends are the recorded ones, as with the
+    reference's scratch file (the energies after the first, vvprop.f:155-160,
+    410, 590).  Returns (R at RSTOP in the primitive basis, the steps
+    taken)."""
+    o = VivsOptions() if opts is None else opts
     n = blk.n
     rmat = np.array(rmat, dtype=np.float64)
-    ialpha = VIVS_IALPHA
+    ialpha = o.ialpha
+    ivd = o.iv or o.idiag
+    ivpd = o.ivp or o.idiag
+    ivppd = o.ivpp or o.idiag
+    need_w2 = ivppd or o.ishift
+    do_pert = o.ivpp or o.ivp or o.iv
+    pert1 = o.ivp and not o.ivpp
     g1 = np.zeros(n)
     g1p = np.ones(n)
     g2 = np.ones(n)
@@ -1978,21 +2154,32 @@ def vvprop(rmat: np.ndarray, blk: Block, ered: float, rstart: float, rstop: floa
     drint = drnow
     diagi = rstart + 0.5 * drint
     rmid = rstart
-    balpha = (VIVS_ALPHA2 - VIVS_ALPHA1) / (rstop - rstart)
-    alpha = VIVS_ALPHA1 + balpha * (diagi - rstart)
-    drnow = drint / ialpha if alpha == 1.0 else drint * (alpha - 1.0) / (alpha ** ialpha - 1.0)
+    xbar = xsbar = ebar = exbar = 0.0
+    alpha = alfnew = 1.0
+    balpha = 0.0
+    if ialpha > 0:
+        balpha = (o.alpha2 - o.alpha1) / (rstop - rstart)
+        alpha = o.alpha1 + balpha * (diagi - rstart)
+        if o.ialfp:
+            alpha = o.alpha1
+        drnow = drint / ialpha if alpha == 1.0 else drint * (alpha - 1.0) / (alpha ** ialpha - 1.0)
     rnow = rstart + drnow
     rmidi = diagi
     k = 0
     while True:
+        rec_end = False
         if replay is not None:
             st = replay[k]
             rnow, drnow, last = st.rnow, st.drnow, st.last
             if st.newint:
                 rmidi = st.rmidi
+            rec_end = st.endint
             k += 1
         rcent = rnow - 0.5 * drnow
         w0, _ = blk.hammat(rcent, ered)
+        w1 = blk.dermat(rcent, 1) if ivpd else None
+        w2 = blk.dermat(rcent, 2) if need_w2 else None
+        factor = drnow * drnow / 24.0 if o.ishift else 0.0
         newint = icrmat
         if icrmat:
             if replay is None:
@@ -2002,13 +2189,32 @@ def vvprop(rmat: np.ndarray, blk: Block, ered: float, rstart: float, rstop: floa
             _, vecnew = np.linalg.eigh(wm)
         w0 = vecnew.T @ w0 @ vecnew
         w0 = 0.5 * (w0 + w0.T)
+        if w1 is not None:
+            w1 = vecnew.T @ w1 @ vecnew
+            w1 = 0.5 * (w1 + w1.T)
+        if w2 is not None:
+            w2 = vecnew.T @ w2 @ vecnew
+            w2 = 0.5 * (w2 + w2.T)
         if icrmat:
             tstore = vecold.T @ vecnew
             rmat = tstore.T @ rmat @ tstore
-            rmat = 0.5 * (rmat + rmat.T)
+            if o.isym:
+                rmat = 0.5 * (rmat + rmat.T)
         icrmat = False
-        diag = -np.diag(w0).copy()
-        np.fill_diagonal(w0, 0.0)
+        if o.ishift:
+            d2 = np.diag(w2).copy()
+            diag = -np.diag(w0) - factor * d2
+            np.fill_diagonal(w0, -factor * d2)
+        else:
+            diag = -np.diag(w0).copy()
+            np.fill_diagonal(w0, 0.0)
+        # contributions not wanted (vvprop.f:318-332)
+        if not ivd:
+            w0 = w0 * 0.0
+        if w1 is not None and not ivpd:
+            w1 = None
+        if w2 is not None and not ivppd:
+            w2 = None
         # zeroth-order solutions over the step (vvprop.f:360-395)
         dif = diag
         xsq = dif * drnow * drnow
@@ -2026,7 +2232,15 @@ def vvprop(rmat: np.ndarray, blk: Block, ered: float, rstart: float, rstop: floa
         g1p = a1p * cx - dif * a1 * sx
         g2 = b1p * sx + b1 * cx
         g2p = b1p * cx - dif * b1 * sx
-        cc, cs, ss = pert2_v0(cosx, sine, xsq, xk, drnow, w0, a1, b1, a1p, b1p, eye11, eye12, eye22)
+        g2pmax = g2p[n - 1] * cx[n - 1]
+        if do_pert:
+            cc, cs, ss = pert2(cosx, sine, xsq, xk, drnow, w0, a1, b1, a1p, b1p, eye11, eye12, eye22,
+                               v1=w1, v2=w2, pert1=pert1)
+        else:
+            # no PERT call: the estimates read the potential matrices themselves
+            cc = w0
+            cs = w1 if w1 is not None else np.zeros((n, n))
+            ss = w2 if w2 is not None else np.zeros((n, n))
         # first-order corrections and their size estimates (vvprop.f:410-470)
         prt1 = g1[None, :] * eye12 - g2[None, :] * eye11
         prt2 = g1[:, None] * eye22 - g2[:, None] * eye12
@@ -2038,7 +2252,7 @@ def vvprop(rmat: np.ndarray, blk: Block, ered: float, rstart: float, rstop: floa
             anp = drnow * an / xk
             bnp = drnow * bn / xk
             expx = np.where(diag > 0.0, 1.0, np.exp(-xk * drint / drnow))
-            expx = np.where((diag <= 0.0) & (diag < -VIVS_XSQMAX), 0.0, expx)
+            expx = np.where((diag <= 0.0) & (diag < -o.xsqmax), 0.0, expx)
             wgt = expx
             an, bn, anp, bnp = an * expx, bn * expx, anp * expx, bnp * expx
             e = np.maximum.reduce([np.abs(prt1) * an[None, :], np.abs(prt2) * bn[:, None],
@@ -2074,8 +2288,30 @@ def vvprop(rmat: np.ndarray, blk: Block, ered: float, rstart: float, rstop: floa
             sdiag = float(np.diag(es).max())
             if soff == 0.0:
                 soff = 1.0e-30
-        steps.append(VivsStep(rnow, drnow, newint, rmidi, last))
-        endint = last or (istep % ialpha == 0)
+        if not o.ipert:
+            prt1 = prt2 = prt1p = prt2p = np.zeros((n, n))
+        # is this the interval's last step (vvprop.f:480-500)
+        if last:
+            endint = True
+        elif replay is not None:
+            endint = rec_end
+        elif ialpha > 0:
+            endint = istep % ialpha == 0
+        else:
+            endint = None
+            if abs(g2pmax) > 1.0e4:
+                endint = True
+            elif coffl != 0.0:
+                facc = min(coff / coffl, 2.0)
+                facs = min(soff / soff1, 2.0)
+                if facc * coff > 0.8 * toff or facs * soff > 0.8 * tldiag:
+                    endint = True
+            if endint is None:
+                coffl = coff
+                soff1 = soff
+                sdiag1 = sdiag
+                endint = not (coff < 0.8 * toff and cdiag < 0.8 * toff)
+        steps.append(VivsStep(rnow, drnow, newint, rmidi, last, endint))
         if endint:
             if replay is None:
                 coffl = 0.0
@@ -2090,7 +2326,7 @@ def vvprop(rmat: np.ndarray, blk: Block, ered: float, rstart: float, rstop: floa
             ts = w0m + rmat @ w1m
             vo = w2m + rmat @ wm
             vo = np.linalg.solve(ts, vo)
-            rmat = 0.5 * (vo + vo.T)
+            rmat = 0.5 * (vo + vo.T) if o.isym else vo.T.copy()
             g1 = np.zeros(n)
             g2 = np.ones(n)
             g1p = np.ones(n)
@@ -2105,10 +2341,40 @@ def vvprop(rmat: np.ndarray, blk: Block, ered: float, rstart: float, rstop: floa
         if replay is not None:
             continue
         # next step and interval (vvprop.f:600-665)
-        drnow = drnow * alpha
+        xbar += rnow
+        xsbar += rnow * rnow
+        ebar += sdiag
+        exbar += rnow * sdiag
+        tmxx = max(0.5 * toff, tldiag)
+        if drint != drnow:
+            soff = 0.8 * tldiag * (soff / (0.8 * tmxx)) ** 1.5
+        if ialpha > 0:
+            drnow = drnow * alpha
+        else:
+            drnow = delrd(drnow, sdiag, soff, tldiag, drmax, diag[0], diag[n - 1], rnow, rstop)
         if icrmat:
             drint = rnow - rmid
             drint1 = drint
+            if ialpha > 0:
+                xbar /= ialpha
+                xsbar /= ialpha
+                ebar /= ialpha
+                exbar /= ialpha
+                slope = 0.0 if ialpha == 1 else (exbar - xbar * ebar) / (xsbar - xbar * xbar)
+                bint = ebar - xbar * slope
+                emax = bint + slope * rnow
+                emin = bint + slope * (rnow - drint)
+                alfnew = alpha
+                if ialpha > 1:
+                    if emax == 0.0:
+                        emax = 1.0e-30
+                    fac = emin / emax
+                    if fac <= 0.0:
+                        fac = 0.9 if emin <= 0.0 else 1.1
+                    else:
+                        fac = min(max(fac ** (1.0 / float(3 * ialpha - 3)), 0.9), 1.1)
+                    alfnew = alpha * fac
+            xbar = xsbar = ebar = exbar = 0.0
             tmxx = max(toff, tldiag)
             if drint != drnow:
                 coff = 0.8 * tmxx * (coff / (0.8 * tmxx)) ** 1.5
@@ -2126,8 +2392,11 @@ def vvprop(rmat: np.ndarray, blk: Block, ered: float, rstart: float, rstop: floa
                 drint = rstop - rnow
             rmid = rnow
             diagi = rnow + 0.5 * drint
-            alpha = VIVS_ALPHA1 + balpha * (diagi - rstart)
-            drnow = drint / ialpha if alpha == 1.0 else drint * (alpha - 1.0) / (alpha ** ialpha - 1.0)
+            if ialpha > 0:
+                alpha = o.alpha1 + balpha * (diagi - rstart)
+                if o.ialfp:
+                    alpha = alfnew
+                drnow = drint / ialpha if alpha == 1.0 else drint * (alpha - 1.0) / (alpha ** ialpha - 1.0)
         if abs(rstop - rnow - drnow) < abs(0.01 * drnow):
             drnow = rstop - rnow
         if (rstop - rnow - drnow) * drnow < 0.0:
@@ -2137,16 +2406,17 @@ def vvprop(rmat: np.ndarray, blk: Block, ered: float, rstart: float, rstop: floa
             last = True
     # back to the primitive basis (vvprop.f:670-682)
     rmat = vecnew @ rmat @ vecnew.T
-    return 0.5 * (rmat + rmat.T), steps
+    return (0.5 * (rmat + rmat.T) if o.isym else rmat), steps
 
 
 def smatrix_ld_vivs(blk: Block, ered: float, ld: Grid, rmax: float, tolhi: float = 1.0e-4,
-                    replay: Optional[List[VivsStep]] = None):
+                    replay: Optional[List[VivsStep]] = None, opts: Optional[VivsOptions] = None):
     """The two-segment run IPROPS = 5, IPROPL = 4 (source_code/scctrl.f:
     99-208): YINIT and LDPROP on the constant-step grid RSTART -> RMID, Y
     inverted to R, VVPROP RMID -> RMAX with the LD segment's step as its
     first step (DRSET returns at once for IPROP = 4, source_code/drset.f:
-    37-39) and TLDIAG = TOFF from TOLHI, R inverted to Y, YTOK + KTOS.
+    37-39) and TLDIAG = TOFF from TOLHI, R inverted to Y, YTOK + KTOS;
+    opts = the /LDVVCM/ controls (VivsOptions).
     Returns the smatrix_ld dict plus STEPS (the VVPROP steps; pass them as
     replay for a later energy, as the reference does with its scratch
     file)."""
@@ -2156,7 +2426,7 @@ def smatrix_ld_vivs(blk: Block, ered: float, ld: Grid, rmax: float, tolhi: float
     y, nodes = ldprop(y, lambda r: blk.hammat(r, ered), ld.rstart, ld.nstep, ld.dr)
     rm, _ = syminv(y)
     tldiag, toff = vivs_tolerances(tolhi)
-    rm, steps = vvprop(rm, blk, ered, ld.rstop, rmax, ld.dr, tldiag, toff, replay=replay)
+    rm, steps = vvprop(rm, blk, ered, ld.rstop, rmax, ld.dr, tldiag, toff, replay=replay, opts=opts)
     y, _ = syminv(rm)
     out = dict(Y=y, NB=nb, NOPEN=nopen, WVEC=wvec, NODES=nodes, STEPS=steps)
     if nopen > 0:

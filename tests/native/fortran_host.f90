@@ -2,8 +2,8 @@
 ! the part of the reference's host program around the propagators, with
 ! its own module `potential`, POTENL (a sum of power / exponential terms per
 ! radial coefficient, the &POTL form), SCAPOT, PERTRB and DRCALC, and drives
-! MDPROP / LDPROP / MAPROP / AIPROP / YTOK / SYMINV through their reference
-! argument lists.
+! MDPROP / LDPROP / MAPROP / AIPROP / YTOK / SYMINV / VVPROP through their
+! reference argument lists.
 !
 !   fortran_host cpu                 checks that need no GPU
 !   fortran_host run <in> <out>      one deck through the drop-ins
@@ -121,7 +121,14 @@ contains
     use host_potl
     character(len=512) :: fin, fout
     integer :: n, mxlam, nham, nstep, nstepa, nopen, nodes, nodes2, nodes_ld, nodes_ma, nodes_a, ifail, i, j, t
-    integer :: nb_c, ng_c, nup_b, nup_g, nup_b1, nup_g1, nsteps_used
+    integer :: nb_c, ng_c, nup_b, nup_g, nup_b1, nup_g1, nsteps_used, nstep2
+    ! the /LDVVCM/ and /PRPSCR/ common blocks (local names prefixed with c:
+    ! IV is also the host's coupling index array here)
+    double precision :: cxsqmax, calpha1, calpha2, ceshift
+    integer :: cialpha, ciscru, ciscrur
+    logical :: cialfp, civ, civp, civpp, cishift, cidiag, cipert, cisym, ciread, ciwrite, cireadr, ciwritr
+    common /LDVVCM/ cxsqmax, calpha1, calpha2, cialpha, cialfp, civ, civp, civpp, cishift, cidiag, cipert, cisym
+    common /PRPSCR/ ceshift, ciscru, ciscrur, ciread, ciwrite, cireadr, ciwritr
     double precision :: ep2ru, cm2ru, rscale, ered, rstart, rmid, dr, drnow, powr, rmax
     double precision, allocatable :: vl(:), eint(:), cent(:), y0(:, :), y(:, :), z(:, :), y2(:, :), wvec(:), &
                                      kmat(:), a(:, :)
@@ -199,6 +206,34 @@ contains
     call SYMINV(a, n, n, ifail)
     write (u, '(A,I6)') 'SYMINV', ifail
     write (u, '(ES25.16)') a
+    ! VVPROP from RMID to RMAX on R = Y^-1 under IDIAG with DELRD steps
+    ! (IALPHA = 0): once choosing and keeping its steps (IWRITE), once
+    ! replaying them (IREAD)
+    call mcpf_set_potl(nterm, tlam, tpow, ta, te)
+    cxsqmax = 1.0d4
+    calpha1 = 1.0d0
+    calpha2 = 1.5d0
+    cialpha = 0
+    cialfp = .false.
+    civ = .true.
+    civp = .true.
+    civpp = .true.
+    cishift = .true.
+    cidiag = .true.
+    cipert = .true.
+    cisym = .true.
+    ciread = .false.
+    ciwrite = .true.
+    y = a
+    call VVPROP(n, mxlam, nham, y, vl, iv, eint, cent, rmid, rmax, nsteps_used, 0.002d0, 5.0d0, 0.064d0, 0.064d0, &
+                ered, ep2ru, cm2ru, rscale, 0)
+    ciread = .true.
+    ciwrite = .false.
+    y2 = a
+    call VVPROP(n, mxlam, nham, y2, vl, iv, eint, cent, rmid, rmax, nstep2, 0.002d0, 5.0d0, 0.064d0, 0.064d0, &
+                ered, ep2ru, cm2ru, rscale, 0)
+    write (u, '(A,I10,I10,ES25.16)') 'VVPROP', nsteps_used, nstep2, maxval(abs(y - y2))
+    write (u, '(ES25.16)') y
     close (u)
     call mcpf_release()
     write (6, '(A)') 'RUN_DONE'

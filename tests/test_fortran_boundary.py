@@ -1,5 +1,5 @@
 """The Fortran side of the boundary: the ISO_C_BINDING module and the drop-in
-MDPROP / LDPROP / MAPROP / AIPROP / SYMINV / YTOK (molscat_amd/fortran),
+MDPROP / LDPROP / MAPROP / AIPROP / SYMINV / YTOK / VVPROP (molscat_amd/fortran),
 compiled with amdflang and linked into a test host program
 (tests/native/fortran_host.f90) that calls them with the reference's
 argument lists (source_code/mdprop.f:1-4, ldprop.f:1-4, maprop.f:1-4,
@@ -166,3 +166,16 @@ def test_fortran_dropins_vs_oracle(tmp_path):
     err = np.abs(ainv - inv).max() / np.abs(inv).max()
     print(f"  SYMINV IFAIL={int(head[0])} rel|dA|={err:.2e}", flush=True)
     assert err < 1e-11
+
+    # VVPROP under IDIAG with DELRD steps (IALPHA = 0), RMID -> RMAX, from
+    # the inverse just checked; its IREAD replay of the kept steps is bitwise
+    # the IWRITE call
+    opts = O.vivs_idiag_defaults(O.VivsOptions(idiag=True, ialpha=0))
+    rv, steps = O.vvprop(ainv, ob, ered, rmid, rmax, 0.002, 0.064, 0.064, drmax=5.0, opts=opts)
+    head, r = out["VVPROP"]
+    r = r.reshape(n, n, order="F")
+    err = np.abs(r - rv).max() / np.abs(rv).max()
+    print(f"  VVPROP steps={int(head[0])} (oracle {len(steps)}) rel|dR|={err:.2e}", flush=True)
+    assert int(head[0]) == int(head[1]) == len(steps)
+    assert float(head[2]) == 0.0
+    assert err < 1e-10

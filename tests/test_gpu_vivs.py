@@ -1,5 +1,5 @@
-"""GPU parity of the VIVS propagator (mcp_prop_vivs, VVPROP with the
-default options) through the C ABI:
+"""GPU parity of the VIVS propagator (mcp_prop_vivs, VVPROP) through the
+C ABI:
 - the reference's 'IPROPS=5, IPROPL=4' run (examples/output/molscat-all_
   propagators.output; ITYPE=1 rotor, N = 12): YINIT + LDPROP on the device
   to RMID, R = Y^-1, VIVS to RMAX = 20 for 1000 cm-1 choosing its steps and
@@ -10,7 +10,10 @@ default options) through the C ABI:
 - the replay modes: the first instance's run and the record it writes are
   the same alone and in a batch, and a replay of the record is bitwise the
   batch's replaying instance;
-- N = 56 (work matrices beyond LDS) against the oracle."""
+- N = 56 (work matrices beyond LDS) against the oracle;
+- the /LDVVCM/ options (derivative perturbations, ISHIFT, IDIAG, IALPHA,
+  IALFP, ISYM, IPERT) against the oracle with the same options, and the
+  replay of interval ends decided by the corrections' estimates."""
 import json
 import math
 import os
@@ -191,3 +194,79 @@ def test_vivs_replay_after_failed_recording(eng, deck):
     assert s_bad[0] == 2
     assert list(s_bad[1:]) == [4, 4]
     assert list(n_bad[1:]) == [1, 1]
+
+
+VIVS_OPTION_SETS = {
+    "ivp": dict(ivp=True),
+    "ivpp": dict(ivpp=True),
+    "ishift": dict(ishift=True),
+    "idiag": dict(idiag=True, iv=True, ivp=True, ivpp=True, ishift=True, ipert=True),
+    "idiag-only": dict(idiag=True),
+    "ialpha0+idiag": dict(idiag=True, iv=True, ivp=True, ivpp=True, ishift=True, ipert=True, ialpha=0),
+    "ialfp+idiag": dict(idiag=True, iv=True, ivp=True, ivpp=True, ishift=True, ipert=True, ialfp=True),
+    "ialpha3": dict(ialpha=3, alpha1=1.1, alpha2=1.3),
+    "nosym": dict(isym=False),
+    "nopert": dict(ipert=False),
+}
+
+
+@pytest.mark.parametrize("name", list(VIVS_OPTION_SETS))
+def test_vivs_options_vs_oracle(eng, deck, name):
+    """The /LDVVCM/ options (mcp_set_vivs_options) against the oracle's
+    VVPROP with the same options, each instance choosing its own steps: the
+    rotor deck (N = 12) from 1.1 to 12 RM at 1000 cm-1, from a seeded R.
+    The device takes the step integrals of the derivative terms from its own
+    closed forms (mcp_stepint.hpp), the oracle from PERT2's."""
+    sec, blk, ld = deck
+    b = _upload(eng, blk)
+    kw = VIVS_OPTION_SETS[name]
+    rng = np.random.default_rng(7)
+    a = rng.standard_normal((blk.n, blk.n))
+    rmat = 0.05 * (a + a.T)
+    ered = 1000.0 * blk.cm2ru
+    tl, to = O.vivs_tolerances(1e-4)
+    print(f"VIVS options {name}", flush=True)
+    eng.set_vivs_options(**kw)
+    try:
+        rout, nstep, st, rec = eng.prop_vivs(b, [ered], rmat[None], 1.1, 12.0, 0.003, tl, to, replay=0)
+    finally:
+        eng.set_vivs_options()
+    ro, steps = O.vvprop(rmat, blk, ered, 1.1, 12.0, 0.003, tl, to, opts=O.VivsOptions(**kw))
+    d = np.abs(rout[0] - ro).max() / np.abs(ro).max()
+    print(f"  {nstep[0]} steps (oracle {len(steps)}), rel |dR| {d:.2e}", flush=True)
+    assert st[0] == 0 and nstep[0] == len(steps)
+    # the record's interval ends are the oracle's
+    assert [bool(int(f) & 4) for f in rec[:, 3]] == [s.endint for s in steps]
+    # measured 5.9e-13 to 4.3e-12 over the option sets
+    assert d < 2e-11
+
+
+def test_vivs_options_replay(eng, deck):
+    """IALPHA = 0 with IDIAG: the interval ends come from the corrections'
+    estimates, so a replaying instance must take them from the record (bit
+    2): the batch's second energy equals the oracle replaying the first
+    energy's steps, and a replay=1 run of the record is bitwise the same."""
+    sec, blk, ld = deck
+    b = _upload(eng, blk)
+    kw = VIVS_OPTION_SETS["ialpha0+idiag"]
+    rng = np.random.default_rng(8)
+    a = rng.standard_normal((2, blk.n, blk.n))
+    rmat = 0.05 * (a + np.transpose(a, (0, 2, 1)))
+    ered = np.array([1000.0, 900.0]) * blk.cm2ru
+    tl, to = O.vivs_tolerances(1e-4)
+    print("VIVS options replay", flush=True)
+    eng.set_vivs_options(**kw)
+    try:
+        r2, n2, s2, rec = eng.prop_vivs(b, ered, rmat, 1.1, 12.0, 0.003, tl, to, replay=2)
+        r1, n1, s1, _ = eng.prop_vivs(b, ered[1:], rmat[1:], 1.1, 12.0, 0.003, tl, to, replay=1, steps=rec)
+    finally:
+        eng.set_vivs_options()
+    assert np.all(s2 == 0) and s1[0] == 0 and n1[0] == n2[1] == n2[0]
+    assert np.array_equal(r1[0], r2[1])
+    opts = O.VivsOptions(**kw)
+    _, steps = O.vvprop(rmat[0], blk, ered[0], 1.1, 12.0, 0.003, tl, to, opts=opts)
+    ro, _ = O.vvprop(rmat[1], blk, ered[1], 1.1, 12.0, 0.003, tl, to, replay=steps, opts=opts)
+    d = np.abs(r2[1] - ro).max() / np.abs(ro).max()
+    print(f"  {n2[1]} steps, rel |dR| {d:.2e}", flush=True)
+    # measured 4.5e-11
+    assert d < 2e-10

@@ -7,7 +7,9 @@ c_i s_j, s_i c_j and s_i s_j, for open, closed and mixed pairs in each of
 the three evaluations (both arguments small; apart; same sign and close),
 at their boundaries, and for equal and nearly equal k.  Tolerance 4e-15
 relative to the largest of the four integrals' scales (h, h^2, h^2, h^3
-times the product of the two solutions' sizes)."""
+times the product of the two solutions' sizes).  The same for the integrals
+against the weights r - h/2 and (r - h/2)^2/2 of the derivative
+perturbations (IVP, IVPP; scales one and two powers of h higher)."""
 import ctypes
 import os
 import shutil
@@ -31,14 +33,16 @@ def slib(tmp_path_factory):
     lib = ctypes.CDLL(out)
     lib.step_integrals_h.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                      ctypes.POINTER(ctypes.c_double)]
+    lib.step_integrals_w_h.argtypes = lib.step_integrals_h.argtypes
     return lib
 
 
-def _exact(zi, zj, h):
+def _exact(zi, zj, h, p=0):
     import mpmath as mp
 
     mp.mp.dps = 40
     H = mp.mpf(h)
+    w = [lambda r: 1, lambda r: r - H / 2, lambda r: (r - H / 2) ** 2 / 2][p]
 
     def fns(z):
         k2 = mp.mpf(z) / (H * H)
@@ -50,7 +54,7 @@ def _exact(zi, zj, h):
 
     ci, si = fns(zi)
     cj, sj = fns(zj)
-    q = lambda f: mp.quad(f, [0, H / 2, H])
+    q = lambda f: mp.quad(lambda r: f(r) * w(r), [0, H / 2, H])
     return [float(q(lambda r: ci(r) * cj(r))), float(q(lambda r: ci(r) * sj(r))),
             float(q(lambda r: si(r) * cj(r))), float(q(lambda r: si(r) * sj(r)))]
 
@@ -95,3 +99,22 @@ def test_symmetry_under_exchange(slib):
         sc = max(1.0, abs(a[0]))
         assert abs(a[0] - b[0]) <= 1e-14 * sc and abs(a[3] - b[3]) <= 1e-14 * max(1.0, abs(a[3]))
         assert abs(a[1] - b[2]) <= 1e-14 * max(1.0, abs(a[1])) and abs(a[2] - b[1]) <= 1e-14 * max(1.0, abs(a[2]))
+
+
+@pytest.mark.parametrize("zi,zj", CASES + [(1e-3, 1.5e-3), (0.0135, 0.0135), (-2.0, 1e-4), (60.0, 58.0)])
+def test_weighted_step_integrals_against_quadrature(slib, zi, zj):
+    h = 0.37
+    out = (ctypes.c_double * 8)()
+    slib.step_integrals_w_h(zi, zj, h, out)
+
+    def size(z):
+        x = np.sqrt(abs(z))
+        return np.cosh(x) if z < 0 else 1.0
+
+    scale = size(zi) * size(zj)
+    for p in (1, 2):
+        want = _exact(zi, zj, h, p)
+        got = out[4 * (p - 1):4 * p]
+        scales = [h ** (1 + p) * scale, h ** (2 + p) * scale, h ** (2 + p) * scale, h ** (3 + p) * scale]
+        for name, g, ref, sc in zip(("cc", "cs", "sc", "ss"), got, want, scales):
+            assert abs(g - ref) <= 4e-15 * sc, (zi, zj, p, name, g, ref, abs(g - ref) / sc)
