@@ -340,6 +340,25 @@ int mcp_bound_match(mcp_ctx *ctx, int32_t block_id, int32_t grid_out, int32_t gr
                     int32_t iprop_in, int32_t ninst, const double *ered, const double *pconst, int32_t *nodec,
                     double *eigmin, int32_t *imin, double *psimch, double *evals, int32_t *status);
 
+/* BDCTRL's wavefunction (source_code/bdctrl.f:266-360, for MDPROP in both
+ * directions, BOUND's IWAVE with IPROPS = IPROPL = 6) at one energy ered
+ * (reduced units) of block_id on the outward grid gid_out (RMIN -> RMATCH,
+ * DR > 0) and the inward grid gid_in (RMAX -> RMATCH, DR < 0): the matching
+ * of mcp_bound_match (nodec = NODEC, eigmin = EIGMIN, psimch = PSIMCH, the
+ * eigenvector of the matching matrix for EIGMIN, largest component
+ * positive), with both propagations writing MDPROP's WAVE records
+ * (source_code/mdprop.f:170-176,212-245: per step W3 = W W2, kept on the
+ * device), then WVBKSB's back-substitution (source_code/wvbksb.f:64-114)
+ * of PSIMCH through each segment's records, last to first: psi_out
+ * ((NSTEP_out + 1) x N, row k = psi after k steps from RMATCH towards RMIN)
+ * and psi_in ((NSTEP_in + 1) x N, towards RMAX), both unnormalised (row 0 =
+ * PSIMCH).  The normalisation (WVBKSB's extended Simpson rule) and the
+ * file are the host's.  status as for mcp_bound_match.  N within the
+ * LDS-resident MDPROP limit. */
+int mcp_bound_wave(mcp_ctx *ctx, int32_t block_id, int32_t gid_out, int32_t gid_in, double ered,
+                   const double *pconst, int32_t *nodec, double *eigmin, double *psimch, double *psi_out,
+                   double *psi_in, int32_t *status);
+
 /* Workgroups that cooperate on one instance on the sequential large-N
  * kernels (N beyond the LDS-resident ones): g >= 1 asks for g (the
  * register-resident MDPROP takes ceil(N/16) by default, and between

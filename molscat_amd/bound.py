@@ -283,3 +283,41 @@ def levels_over_fields(eng, fb, fields, e_lo: float, e_hi: float, rmin: float, r
         eng.destroy_block(b)
         eng.destroy_grid(go)
         eng.destroy_grid(gi)
+
+
+def simpson_weights(nstep: int) -> np.ndarray:
+    """Quadrature weights (in units of the step) over nstep + 1 equally
+    spaced points as WVBKSB applies them (source_code/wvbksb.f:139-170):
+    the alternative extended Simpson rule, 17/48, 59/48, 43/48, 49/48 from
+    each end and 1 inside, each end's weights laid from both ends towards
+    the middle so the central point is counted once."""
+    ends = (17.0 / 48.0, 59.0 / 48.0, 43.0 / 48.0, 49.0 / 48.0)
+    w = np.zeros(nstep + 1)
+    for k in range((nstep + 2) // 2):
+        a = ends[k] if k < 4 else 1.0
+        w[k] += a
+        if nstep - k != k:
+            w[nstep - k] += a
+    return w
+
+
+def wavefunction(eng, b, g_out, g_in, ered: float, pconst=None) -> Dict:
+    """BOUND's wavefunction at one energy (reduced units) of a level found
+    on the grids g_out (RMIN -> RMATCH) and g_in (RMAX -> RMATCH), MDPROP in
+    both directions (source_code/bdctrl.f:266-360): the device's matching
+    and back-substitution (``mcp_bound_wave``), then each channel's
+    normalisation integral SUMPSI from both segments and the wavefunction
+    from RMIN to RMAX divided by sqrt(sum SUMPSI).  Returns R, PSI (npts,
+    N), SUMPSI (N), NODEC, EIGMIN, PSIMCH."""
+    w = eng.bound_wave(b, g_out, g_in, ered, pconst)
+    if w["STATUS"] != 0:
+        raise RuntimeError(f"bound_wave failed: status {w['STATUS']}")
+    po, pi = w["PSI_OUT"], w["PSI_IN"]
+    nout, nin = po.shape[0] - 1, pi.shape[0] - 1
+    dro = abs(float(g_out.r[2] - g_out.r[0]))
+    dri = abs(float(g_in.r[2] - g_in.r[0]))
+    sumpsi = (simpson_weights(nout)[:, None] * po * po).sum(0) * dro + \
+        (simpson_weights(nin)[:, None] * pi * pi).sum(0) * dri
+    r = np.concatenate([np.asarray(g_out.r)[::2], np.asarray(g_in.r)[::2][::-1][1:]])
+    psi = np.concatenate([po[::-1], pi[1:]]) / np.sqrt(sumpsi.sum())
+    return dict(R=r, PSI=psi, SUMPSI=sumpsi, NODEC=w["NODEC"], EIGMIN=w["EIGMIN"], PSIMCH=w["PSIMCH"])

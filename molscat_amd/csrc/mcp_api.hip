@@ -2217,6 +2217,100 @@ int mcp_bound_match(mcp_ctx* c, int32_t bid, int32_t gid_out, int32_t gid_in, in
   return MCP_OK;
 }
 
+int mcp_bound_wave(mcp_ctx* c, int32_t bid, int32_t gid_out, int32_t gid_in, double ered, const double* pconst,
+                   int32_t* nodec, double* eigmin, double* psimch, double* psi_out, double* psi_in,
+                   int32_t* status) {
+  if (!c || !nodec || !eigmin || !psimch || !psi_out || !psi_in || !status)
+    return c ? c->fail(MCP_ERR_ARG, "bound_wave: bad args") : MCP_ERR_ARG;
+  Block* b;
+  Grid* go;
+  Grid* gi;
+  int rc;
+  if ((rc = lookup(c, bid, gid_out, &b, &go))) return rc;
+  if ((rc = lookup(c, bid, gid_in, &b, &gi))) return rc;
+  const int n = b->n;
+  const size_t nn = (size_t)n * n;
+  if (lds_eig(n) > c->lds_max || lds_md(n, b->nham) > c->lds_max || lds_uc(n, b->nham) > c->lds_max)
+    return c->fail(MCP_ERR_UNSUPPORTED, "bound_wave: N=%d exceeds the LDS-resident MDPROP limit", n);
+  if (!(go->dr > 0.0) || !(gi->dr < 0.0))
+    return c->fail(MCP_ERR_ARG, "bound_wave: the outward grid needs DR > 0 and the inward grid DR < 0");
+  if ((go->npts - 1) % 2 != 0 || (gi->npts - 1) % 2 != 0 || go->npts < 3 || gi->npts < 3)
+    return c->fail(MCP_ERR_ARG, "bound_wave: LD grids need 2*NSTEP+1 points");
+  const int nout = (go->npts - 1) / 2, nin = (gi->npts - 1) / 2;
+  HIPCHK(c, hipSetDevice(c->device));
+  if ((rc = upload_inst(c, *b, 1, &ered, pconst))) return rc;
+  HIPCHK(c, c->s_i.ensure(nn * sizeof(double)));
+  HIPCHK(c, c->s_k.ensure(nn * sizeof(double)));
+  HIPCHK(c, c->s_j.ensure(9 * sizeof(int)));
+  HIPCHK(c, c->s_d.ensure((2 + 2 * (size_t)n) * sizeof(double)));
+  HIPCHK(c, c->s_e.ensure((size_t)std::max(nout, nin) * nn * sizeof(double)));
+  HIPCHK(c, c->s_f.ensure((size_t)std::max(nout, nin) * sizeof(int)));
+  // the records of both propagations, the first inverse of a step, psi
+  HIPCHK(c, c->s_a.ensure(((size_t)nout + nin + 1) * nn * sizeof(double)));
+  HIPCHK(c, c->s_b.ensure(((size_t)nout + nin + 2) * n * sizeof(double)));
+  double* rec_out = c->s_a.as<double>();
+  double* rec_in = rec_out + (size_t)nout * nn;
+  double* wtmp = rec_in + (size_t)nin * nn;
+  double* d_pout = c->s_b.as<double>();
+  double* d_pin = d_pout + ((size_t)nout + 1) * n;
+  mcp::InstArgs I;
+  I.ered = c->s_g.as<double>();
+  I.pconst = b->nconst > 0 ? c->s_h.as<double>() : nullptr;
+  int* dn = c->s_j.as<int>();  // [nodes_out, status_out, yinit_out, nodes_in, status_in, yinit_in, nodec, imin]
+  double* yo = c->s_i.as<double>();
+  double* yi = c->s_k.as<double>();
+  const mcp::BlockArgs B = block_args(*b);
+  // one direction: YINIT, U(C) of every step, MDPROP writing its records
+  auto run = [&](Grid& g, int outward, double* y, int* d_nodes, int* d_status, int* d_yst, double* recs) -> int {
+    mcp::InstArgs Ii = I;
+    Ii.y = y;
+    Ii.nodes = d_nodes;
+    Ii.status = d_status;
+    int r2 = yinit_device(c, *b, g, Ii, 1, 0, outward, 0.0, d_yst);
+    if (r2) return r2;
+    const int nstep = (g.npts - 1) / 2;
+    const double h = g.dr / 2.0;
+    const int bk = n > 30 ? 1 : 0;
+    const mcp::GridArgs G = grid_args(g);
+    hipLaunchKernelGGL(mcp::k_mdprop_uc, dim3(nstep), dim3(kNT), lds_uc(n, b->nham), c->stream, B, G, Ii.pconst, 1,
+                       nstep, h, bk, c->s_e.as<double>(), c->s_f.as<int>());
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(mcp::k_mdprop_lds_wave, dim3(1), dim3(kNT), lds_md(n, b->nham), c->stream, B, G, Ii, nstep, h,
+                       c->s_e.as<double>(), c->s_f.as<int>(), bk, recs, wtmp);
+    HIPCHK(c, hipGetLastError());
+    return MCP_OK;
+  };
+  if ((rc = run(*go, 1, yo, dn, dn + 1, dn + 2, rec_out))) return rc;
+  if ((rc = run(*gi, 0, yi, dn + 3, dn + 4, dn + 5, rec_in))) return rc;
+  double* de = c->s_d.as<double>();  // [eigmin | psimch | evals]
+  hipLaunchKernelGGL(mcp::k_bound_match, dim3(1), dim3(kNT), lds_eig(n), c->stream, n, yo, yi, dn, dn + 3, dn + 6, de,
+                     dn + 7, de + 1, de + 1 + n);
+  HIPCHK(c, hipGetLastError());
+  mcp::WaveSeg s0, s1;
+  s0.recs = rec_out;
+  s0.out = d_pout;
+  s0.nstep = nout;
+  s1.recs = rec_in;
+  s1.out = d_pin;
+  s1.nstep = nin;
+  hipLaunchKernelGGL(mcp::k_wvbksb, dim3(2), dim3(kNT), 2 * (size_t)n * sizeof(double), c->stream, n, de + 1, s0, s1);
+  HIPCHK(c, hipGetLastError());
+  int tmp[8];
+  HIPCHK(c, hipMemcpyAsync(tmp, dn, 8 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(eigmin, de, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(psimch, de + 1, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(psi_out, d_pout, ((size_t)nout + 1) * n * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(psi_in, d_pin, ((size_t)nin + 1) * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *nodec = tmp[6];
+  *status = 0;
+  if (tmp[1] != 0) *status = -tmp[1];
+  if (tmp[4] != 0) *status = -(100000 + tmp[4]);
+  if (tmp[2] != 0 || tmp[5] != 0) *status = -1000000;
+  return MCP_OK;
+}
+
 int mcp_set_large_mode(mcp_ctx* c, int32_t mode) {
   if (!c || mode < 0 || mode > 7 || (mode & 3) > 2)
     return c ? c->fail(MCP_ERR_ARG, "large mode %d: 0, 1 or 2, plus 4", mode) : MCP_ERR_ARG;

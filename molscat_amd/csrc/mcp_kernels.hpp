@@ -224,11 +224,17 @@ struct InstArgs {
 // One instance's MDPROP with Y in LDS (the body of k_mdprop_lds and of the
 // persistent multi-block launch k_mdprop_lds_queue): yg is Y(RSTART) on
 // entry and Y(RSTOP) on exit, *nodes_out and *status_out the results.
+// With wrec (BOUND's WAVE option, source_code/mdprop.f:170-176,212-245) step
+// ks also writes W3 = W W2 to wrec + (ks-1) N^2 (column-major): W the first
+// inverse [H(Y(A) + Y1(A,C))]^-1 (kept in wtmp, N^2), W2 = Y23 [H(Y(C) +
+// Y1(C,B))]^-1 Y23 the second; W3 carries the wavefunction from the step's
+// end back to its start.
 template <int NT>
 __device__ __forceinline__ void mdprop_lds_run(const BlockArgs& B, const GridArgs& G, double ered,
                                                const double* pconst, double* yg, int* nodes_out, int* status_out,
                                                int nstep, double h, const double* __restrict__ uc,
-                                               const int* __restrict__ uc_kount, int bk, double* smem) {
+                                               const int* __restrict__ uc_kount, int bk, double* smem,
+                                               double* wrec = nullptr, double* wtmp = nullptr) {
   const int n = B.n, lda = n;
   LdsCarve c(smem);
   double* Y = c.take<double>((size_t)n * lda);
@@ -313,6 +319,16 @@ __device__ __forceinline__ void mdprop_lds_run(const BlockArgs& B, const GridArg
       break;
     }
     nodes += kc;
+    if (wrec) {
+      for (int j = t2.tj; j < n; j += Tile2D<NT>::kWaves)
+        for (int i = t2.ti; i < n; i += 64) {
+          if (i < j) continue;
+          const double v = Y[i + j * lda];
+          wtmp[i + (size_t)j * n] = v;
+          wtmp[j + (size_t)i * n] = v;
+        }
+      __syncthreads();
+    }
     // Y <- U(C) + 2 Y14 - Y23 Y Y23   (mdprop.f:181-185)
     {
       const double* U = uc + (size_t)(ks - 1) * n * n;
@@ -333,6 +349,20 @@ __device__ __forceinline__ void mdprop_lds_run(const BlockArgs& B, const GridArg
       break;
     }
     nodes += kc;
+    if (wrec) {
+      // W3(i,j) = sum_m W(i,m) y23[m] Y(m,j) y23[j], Y's lower triangle
+      double* dst = wrec + (size_t)(ks - 1) * n * n;
+      for (int e = threadIdx.x; e < n * n; e += NT) {
+        const int j = e / n, i = e - j * n;
+        double acc = 0.0;
+        for (int m = 0; m < n; ++m) {
+          const double ymj = m >= j ? Y[m + j * lda] : Y[j + m * lda];
+          acc += wtmp[i + (size_t)m * n] * (y23[m] * ymj);
+        }
+        dst[e] = acc * y23[j];
+      }
+      __syncthreads();
+    }
     // Y <- D2*W(B) - Y23 Y Y23 + Y14   (mdprop.f:197-230)
     const double dd2 = (ks == nstep) ? d1 : d2;
     load_pvec(B, G, 2 * ks, pconst, pv);
@@ -370,6 +400,53 @@ __global__ void __launch_bounds__(NT, NT == 256 ? 4 : 1) k_mdprop_lds(BlockArgs 
   const double* pconst = I.pconst ? I.pconst + (size_t)inst * B.nconst : nullptr;
   mdprop_lds_run<NT>(B, G, I.ered[inst], pconst, I.y + (size_t)inst * B.n * B.n, I.nodes + inst, I.status + inst,
                      nstep, h, uc, uc_kount, bk, smem);
+}
+
+// MDPROP of one instance writing BOUND's WAVE records (wrec: nstep N x N
+// matrices; wtmp: N^2 scratch).
+__global__ void __launch_bounds__(kNT) k_mdprop_lds_wave(BlockArgs B, GridArgs G, InstArgs I, int nstep, double h,
+                                                         const double* __restrict__ uc,
+                                                         const int* __restrict__ uc_kount, int bk, double* wrec,
+                                                         double* wtmp) {
+  extern __shared__ double smem[];
+  mdprop_lds_run<kNT>(B, G, I.ered[0], I.pconst, I.y, I.nodes, I.status, nstep, h, uc, uc_kount, bk, smem, wrec,
+                      wtmp);
+}
+
+// WVBKSB's back-substitution (source_code/wvbksb.f:64-114) of one segment
+// per workgroup: from psi0 at the matching point, psi(A) = W3 psi(B) with
+// the segment's records taken last to first; out[k] (N values) is psi after
+// k steps back from the matching point, k = 0 .. nstep.  Segment s has
+// nstep[s] records at recs[s] and writes out[s].
+struct WaveSeg {
+  const double* recs;
+  double* out;
+  int nstep;
+};
+
+__global__ void __launch_bounds__(kNT) k_wvbksb(int n, const double* __restrict__ psi0, WaveSeg s0, WaveSeg s1) {
+  extern __shared__ double smem[];
+  const WaveSeg S = blockIdx.x == 0 ? s0 : s1;
+  double* pa = smem;
+  double* pb = smem + n;
+  for (int i = threadIdx.x; i < n; i += kNT) {
+    pa[i] = psi0[i];
+    S.out[i] = psi0[i];
+  }
+  __syncthreads();
+  for (int k = 0; k < S.nstep; ++k) {
+    const double* w3 = S.recs + (size_t)(S.nstep - 1 - k) * n * n;
+    for (int i = threadIdx.x; i < n; i += kNT) {
+      double acc = 0.0;
+      for (int m = 0; m < n; ++m) acc += w3[i + (size_t)m * n] * pa[m];
+      pb[i] = acc;
+      S.out[(size_t)(k + 1) * n + i] = acc;
+    }
+    __syncthreads();
+    double* t = pa;
+    pa = pb;
+    pb = t;
+  }
 }
 
 // One block of a multi-block MDPROP launch: its arguments, its

@@ -346,6 +346,25 @@ class Engine:
                                              iptr(st)), "mcp_bound_match")
         return dict(NODEC=nodec, EIGMIN=eigmin, IMIN=imin, PSIMCH=psi, EVALS=ev, STATUS=st)
 
+    def bound_wave(self, b: BlockHandle, g_out: GridHandle, g_in: GridHandle, ered: float, pconst=None):
+        """BOUND's wavefunction at one energy (MDPROP both ways, WAVE records
+        and WVBKSB on the device): dict with NODEC, EIGMIN, PSIMCH, PSI_OUT
+        ((NSTEP_out + 1, N), row k k steps from RMATCH towards RMIN), PSI_IN
+        (towards RMAX), STATUS; unnormalised (row 0 = PSIMCH)."""
+        nout = (len(g_out.r) - 1) // 2
+        nin = (len(g_in.r) - 1) // 2
+        nodec = np.empty(1, dtype=np.int32)
+        eigmin = np.empty(1)
+        psi = np.empty(b.n)
+        po = np.empty((nout + 1, b.n))
+        pi = np.empty((nin + 1, b.n))
+        st = np.empty(1, dtype=np.int32)
+        pc = None if pconst is None else np.ascontiguousarray(pconst, dtype=np.float64)
+        self._check(self.lib.mcp_bound_wave(self.ctx, b.id, g_out.id, g_in.id, float(ered), dptr(pc), iptr(nodec),
+                                            dptr(eigmin), dptr(psi), dptr(po), dptr(pi), iptr(st)), "mcp_bound_wave")
+        return dict(NODEC=int(nodec[0]), EIGMIN=float(eigmin[0]), PSIMCH=psi, PSI_OUT=po, PSI_IN=pi,
+                    STATUS=int(st[0]))
+
     def set_team_size(self, g: int):
         """Workgroups per instance on the large-N path (0 = automatic)."""
         self._check(self.lib.mcp_set_team_size(self.ctx, int(g)), "mcp_set_team_size")

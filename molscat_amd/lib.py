@@ -54,6 +54,7 @@ SIGNATURES = {
                                              _pd, _pi, _pi]),
     "mcp_bound_match": (ctypes.c_int, [_vp, _i32, _i32, _i32, _i32, _i32, _i32, _pd, _pd, _pi, _pd, _pi, _pd, _pd,
                                         _pi]),
+    "mcp_bound_wave": (ctypes.c_int, [_vp, _i32, _i32, _i32, _dbl, _pd, _pi, _pd, _pd, _pd, _pd, _pi]),
     "mcp_set_team_size": (ctypes.c_int, [_vp, _i32]),
     "mcp_set_large_mode": (ctypes.c_int, [_vp, _i32]),
     "mcp_set_inverse_kernel": (ctypes.c_int, [_vp, _i32]),

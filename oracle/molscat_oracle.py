@@ -674,12 +674,17 @@ def _sym_lower(m: np.ndarray) -> np.ndarray:
 
 
 def mdprop(y: np.ndarray, hammat: Callable[[float], Tuple[np.ndarray, np.ndarray]],
-           rstart: float, nstep: int, dr: float, inv: Callable = None) -> Tuple[np.ndarray, int]:
+           rstart: float, nstep: int, dr: float, inv: Callable = None,
+           wave: Optional[list] = None) -> Tuple[np.ndarray, int]:
     """MDPROP (diabatic modified log-derivative, IPROP=6): source_code/mdprop.f:64-279.
 
     ``hammat(r)`` returns (W, DIAG) at the propagation energy.  Returns the
     final log-derivative matrix and NODES.  ``inv`` replaces syminv (the CPU
-    baseline passes syminv_lapack).
+    baseline passes syminv_lapack).  With ``wave`` a list, BOUND's WAVE
+    records are appended to it, one per step (mdprop.f:170-176, 212-245):
+    W3 = W W2 with W = [H(Y(A) + Y1(A,C))]^-1 after the first inversion and
+    W2 = Y23 [H(Y(C) + Y1(C,B))]^-1 Y23 after the second, the matrix that
+    carries the wavefunction back from the step's end B to its start A.
     """
     syminv = inv if inv is not None else globals()["syminv"]
     n = y.shape[0]
@@ -723,11 +728,14 @@ def mdprop(y: np.ndarray, hammat: Callable[[float], Tuple[np.ndarray, np.ndarray
         if kount > n or kount < 0:
             raise RuntimeError(f"MDPROP: MATRIX INVERSION ERROR AT STEP K = {kstep}")
         nodes += kount
+        w_first = _sym_lower(y) if wave is not None else None
         y = u - y23[:, None] * y * y23[None, :]
         y, kount = syminv(y)
         if kount > n or kount < 0:
             raise RuntimeError(f"MDPROP: MATRIX INVERSION ERROR AT STEP K = {kstep}")
         nodes += kount
+        if wave is not None:
+            wave.append(w_first @ _sym_lower(y23[:, None] * y * y23[None, :]))
         if kstep == nstep:
             d2 = d1
         r = r + h
@@ -1589,6 +1597,67 @@ def bound_match(blk: Block, ered: float, g: BoundGrid, iprop_out: int = 6, iprop
     psi *= math.copysign(1.0, big)
     return dict(NODEC=n_out + n_in + negcnt, NODES_OUT=n_out, NODES_IN=n_in, NEGCNT=negcnt, EIGMIN=eigmin,
                 IMIN=imin, PSIMCH=psi, EVALS=ev, M=m)
+
+
+def wvbksb(psi_match: np.ndarray, recs: List[np.ndarray]) -> np.ndarray:
+    """WVBKSB's back-substitution (source_code/wvbksb.f:64-114): from the
+    wavefunction at the matching point, psi at the start of each step from
+    psi at its end, psi(A) = W3 psi(B), the records read last to first.
+    Returns psi at the segment's NSTEP+1 grid points, [0] at the matching
+    point and [NSTEP] at the segment's far end."""
+    out = np.empty((len(recs) + 1, len(psi_match)))
+    p = np.array(psi_match, dtype=np.float64)
+    out[0] = p
+    for k, w3 in enumerate(reversed(recs)):
+        p = w3 @ p
+        out[k + 1] = p
+    return out
+
+
+def simpson_alt_weights(nstep: int) -> np.ndarray:
+    """WVBKSB's quadrature weights over NSTEP+1 equally spaced points (in
+    units of the step): the alternative extended Simpson rule (Numerical
+    Recipes eq. 4.1.14, source_code/wvbksb.f:76-82,139-170): 17/48, 59/48,
+    43/48, 49/48 at both ends, 1 inside, filled from both ends towards the
+    middle with the central point counted once."""
+    w = np.zeros(nstep + 1)
+    ends = [17.0 / 48.0, 59.0 / 48.0, 43.0 / 48.0, 49.0 / 48.0]
+    for istep in range(1, (nstep + 2) // 2 + 1):
+        right, left = istep - 1, nstep + 1 - istep
+        a = ends[istep - 1] if istep <= 4 else 1.0
+        w[right] += a
+        if left != right:
+            w[left] += a
+    return w
+
+
+def bound_wavefunction(blk: Block, ered: float, g: BoundGrid) -> dict:
+    """BDCTRL's wavefunction (source_code/bdctrl.f:266-360) for MDPROP in
+    both directions: the WAVE records of the outward and inward
+    propagations, PSIMCH (bound_match) carried back to RMIN through the
+    outward records and out to RMAX through the inward ones (wvbksb), the
+    normalisation integral SUMPSI of each channel from both segments
+    (simpson_alt_weights times |DR|), and the wavefunction from RMIN to RMAX
+    divided by sqrt(sum SUMPSI).  Returns R, PSI (normalised), SUMPSI and
+    the bound_match dict."""
+    hm = lambda r: blk.hammat(r, ered)
+    rec_out: list = []
+    rec_in: list = []
+    y_out = yinit(blk, ered, g.rmin, zout=True)
+    y_out, n_out = mdprop(y_out, hm, g.rmin, g.nout, g.drout, wave=rec_out)
+    y_in = yinit(blk, ered, g.rmax, zout=False)
+    y_in, n_in = mdprop(y_in, hm, g.rmax, g.nin, g.drin, wave=rec_in)
+    m = bound_match(blk, ered, g)
+    psi0 = m["PSIMCH"]
+    p_out = wvbksb(psi0, rec_out)  # [0] at RMATCH ... [nout] at RMIN
+    p_in = wvbksb(psi0, rec_in)  # [0] at RMATCH ... [nin] at RMAX
+    sumpsi = (simpson_alt_weights(g.nout)[:, None] * p_out ** 2).sum(0) * abs(g.drout) + \
+        (simpson_alt_weights(g.nin)[:, None] * p_in ** 2).sum(0) * abs(g.drin)
+    r_out = g.rmatch - np.arange(g.nout + 1) * g.drout
+    r_in = g.rmatch - np.arange(g.nin + 1) * g.drin
+    r = np.concatenate([r_out[::-1], r_in[1:]])
+    psi = np.concatenate([p_out[::-1], p_in[1:]]) / math.sqrt(float(sumpsi.sum()))
+    return dict(R=r, PSI=psi, SUMPSI=sumpsi, MATCH=m)
 
 
 # ---------------------------------------------------------------------------
