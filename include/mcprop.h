@@ -265,6 +265,27 @@ int mcp_prop_airy_tolhi(mcp_ctx *ctx, int32_t block_id, int32_t ninst, const dou
  * the block has NCONST > 0.  N up to the LDS-resident limit (≈80). */
 int mcp_ytrans(mcp_ctx *ctx, int32_t block_id, int32_t ninst, const double *pconst, double *y, double *eint);
 
+/* YTRANS (source_code/ytrans.f90:1-485) as mcp_ytrans, for blocks with
+ * NCONST > 0 and/or the centrifugal operator among the terms (NRSQ > 0),
+ * with cent (ninst x N, may be NULL) receiving the CENT that YTOK takes in
+ * the asymptotic basis.  NRSQ > 0: no grouping by L; MULTOP's resolution
+ * (source_code/multop.f90:55-180) of the constant operator C and L^2 (the
+ * NRSQ VL terms with P = 1): C diagonalised, L^2 diagonalised within each
+ * set of C eigenvalues closer than DEGTOL (mcp_set_degtol) to their
+ * neighbour, channels in ascending C then ascending L^2, eigenvectors with
+ * the largest component positive, EINT = C's and CENT = L^2's eigenvalues
+ * (ytrans.f90:264-275,353-361,411-430); without constant terms L^2 alone
+ * (pconst may then be NULL) and EINT unchanged.  NRSQ = 0: CENT is the
+ * block's.  mcp_smatrix_ld and mcp_smatrix_segments apply the same
+ * transformation and match with these CENT.  Extra operators (NEXTRA) are
+ * not taken. */
+int mcp_ytrans_cent(mcp_ctx *ctx, int32_t block_id, int32_t ninst, const double *pconst, double *y, double *eint,
+                    double *cent);
+
+/* DEGTOL (source_code/mol.driver.f:286, 1e-10 by default): eigenvalues of
+ * YTRANS's constant operator closer than this are one degenerate set. */
+int mcp_set_degtol(mcp_ctx *ctx, double degtol);
+
 /* YTOK + KTOS (source_code/ytok.f:1, ytokg.f:1, ktos.f:1) for ninst
  * instances: y (N x N), wvec (N, GTWVEC's signed wave vectors), nopen, nb
  * (N, 0-based CHNSRT order), cent (N), rval = RSTOP.  Writes kmat, sr, si as

@@ -151,6 +151,16 @@ size_t lds_yinit(int n, int nham) {
       .b;
 }
 
+size_t lds_ytrans_rsq(int n) {
+  // same pieces, in the same order, as k_ytrans_rsq carves them
+  const size_t nn = (size_t)n * n;
+  const size_t half = (size_t)(n + 1) / 2 + 1;
+  LdsSize z;
+  z.add<double>(nn).add<double>(nn).add<double>(nn).add<double>(nn);
+  z.add<double>(half).add<double>(half).add<int>(half).add<int>(half);
+  z.add<double>(kNT / 64).add<double>(n).add<double>(n).add<double>(n).add<int>(n).add<double>(n).add<int>(n).add<int>(n);
+  return z.b;
+}
 size_t lds_ytrans(int n) {
   // same pieces, in the same order, as k_ytrans carves them
   const size_t nn = (size_t)n * n;
@@ -255,6 +265,7 @@ static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 struct mcp_ctx {
   mcp::VivsOpts vivs_opt = mcp::vivs_default_opts();
   int64_t coop_refused = 0;  // cooperative launches refused and taken as ordinary ones
+  double degtol = 1.0e-10;   // DEGTOL of YTRANS's degenerate sets (source_code/mol.driver.f:286)
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -308,6 +319,7 @@ struct mcp_ctx {
   DevBuf s_ed, s_ew;              // global-memory eigen-solver: d, e, tau; inverse-iteration factors
   DevBuf s_ay, s_ax, s_am, s_akc; // large-N AIRY per instance: Y1 Y2 Y4, products, pivots, inertias
   DevBuf s_ye;                    // asymptotic thresholds of YTRANS
+  DevBuf s_yc;                    // YTRANS with NRSQ > 0: each instance's CENT
   DevBuf s_vr, s_vw, s_vs, s_ve, s_vn, s_vst;  // VIVS: R, work matrices, step record, ERED, NSTEP, status
   DevBuf s_vnd;                                // AIRY with TOLHI > 0: NODES
   DevBuf s_mt, s_mc, s_mp, s_me, s_mq;         // MAPROP: bases, corrections, basis changes, eigenvalues, end term
@@ -1069,8 +1081,17 @@ int eig_global(mcp_ctx* c, int n, int nmat, double* a, double* evals, double* ev
 // instances (d_pconst: ninst x nconst) into d_eint, and their Y (dy, may be
 // null) turned into the asymptotic basis.
 int ytrans_device(mcp_ctx* c, Block& b, int ninst, const double* d_pconst, double* dy, double* d_eint) {
+  if (b.nrsq > 0) {
+    // the centrifugal operator among the terms: CENT per instance into s_yc
+    const size_t lr = lds_ytrans_rsq(b.n);
+    if (lr > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "ytrans: N=%d exceeds the LDS-resident limit", b.n);
+    HIPCHK(c, c->s_yc.ensure((size_t)ninst * b.n * sizeof(double)));
+    hipLaunchKernelGGL(mcp::k_ytrans_rsq, dim3(ninst), dim3(kNT), lr, c->stream, block_args(b), d_pconst, dy, d_eint,
+                       c->s_yc.as<double>(), c->degtol);
+    HIPCHK(c, hipGetLastError());
+    return MCP_OK;
+  }
   if (b.nconst <= 0) return c->fail(MCP_ERR_ARG, "ytrans: the block has no constant terms (NCONST = 0)");
-  if (b.nrsq != 0) return c->fail(MCP_ERR_UNSUPPORTED, "ytrans: NRSQ > 0 (the centrifugal operator) is not supported");
   const size_t lds = lds_ytrans(b.n);
   if (lds > c->lds_max) return c->fail(MCP_ERR_UNSUPPORTED, "ytrans: N=%d exceeds the LDS-resident limit", b.n);
   hipLaunchKernelGGL(mcp::k_ytrans, dim3(ninst), dim3(kNT), lds, c->stream, block_args(b), d_pconst, dy, d_eint);
@@ -1929,7 +1950,7 @@ static int match_run(mcp_ctx* c, int n, int ninst, const mcp::MatchArgs& M, unsi
 }
 
 static int match_device(mcp_ctx* c, int n, int ninst, const double* d_y, const double* wvec, const int32_t* nopen,
-                        const int32_t* nb, const double* d_cent, double rval) {
+                        const int32_t* nb, const double* d_cent, double rval, int cent_stride = 0) {
   int rc;
   if ((rc = upload(c, c->s_a, wvec, (size_t)ninst * n))) return rc;
   if ((rc = upload(c, c->s_c, nopen, (size_t)ninst))) return rc;
@@ -1944,6 +1965,7 @@ static int match_device(mcp_ctx* c, int n, int ninst, const double* d_y, const d
   M.nopen = c->s_c.as<int>();
   M.nb = c->s_d.as<int>();
   M.cent = d_cent;
+  M.cent_stride = cent_stride;
   M.rval = rval;
   M.work = c->s_b.as<double>();
   M.kmat = c->s_k.as<double>();
@@ -2022,6 +2044,7 @@ int mcp_ytok_ktos_noalloc(mcp_ctx* c, int32_t n, int32_t ninst, const double* d_
   M.nopen = d_nopen;
   M.nb = d_nb;
   M.cent = d_cent;
+  M.cent_stride = 0;
   M.rval = rval;
   M.work = (double*)(c->scratch_dev + ctr);
   M.kmat = d_kmat;
@@ -2065,12 +2088,14 @@ int mcp_smatrix_ld(mcp_ctx* c, int32_t bid, int32_t gid, int32_t iprop, int32_t 
   const int ip = (iprop == 6 && n == 1) ? MCP_PROP_ODPROP : iprop;
   if ((rc = prop_ld_device(c, *b, *g, ip, ninst, I.ered, I.pconst, I.y, dn, dn + ninst))) return rc;
   if (!(rval > 0.0)) return c->fail(MCP_ERR_ARG, "smatrix_ld: matching radius %g must be positive", rval);
-  if (b->nconst > 0) {
+  if (b->nconst > 0 || b->nrsq > 0) {
     // YTRANS before YTOK (source_code/scctrl.f:274-287)
     HIPCHK(c, c->s_ye.ensure((size_t)ninst * n * sizeof(double)));
     if ((rc = ytrans_device(c, *b, ninst, I.pconst, I.y, c->s_ye.as<double>()))) return rc;
   }
-  if ((rc = match_device(c, n, ninst, I.y, wvec, nopen, nb, b->cent.as<double>(), rval))) return rc;
+  // with the centrifugal operator, YTOK's CENT are YTRANS's, per instance
+  const double* dcent = b->nrsq > 0 ? c->s_yc.as<double>() : b->cent.as<double>();
+  if ((rc = match_device(c, n, ninst, I.y, wvec, nopen, nb, dcent, rval, b->nrsq > 0 ? n : 0))) return rc;
   std::vector<int> tmp(3 * (size_t)ninst);
   HIPCHK(c, hipMemcpyAsync(tmp.data(), dn, 3 * ninst * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   if ((rc = fetch_match(c, n, ninst, nopen, kmat, sr, si, status))) return rc;
@@ -2128,11 +2153,12 @@ int mcp_smatrix_segments(mcp_ctx* c, int32_t bid, int32_t nseg, const int32_t* g
     c->seg_ms[k][1] = c->t_prop;
     c->seg_timed = k + 1;
   }
-  if (b->nconst > 0) {
+  if (b->nconst > 0 || b->nrsq > 0) {
     HIPCHK(c, c->s_ye.ensure((size_t)ninst * n * sizeof(double)));
     if ((rc = ytrans_device(c, *b, ninst, I.pconst, I.y, c->s_ye.as<double>()))) return rc;
   }
-  if ((rc = match_device(c, n, ninst, I.y, wvec, nopen, nb, b->cent.as<double>(), rval))) return rc;
+  const double* dcent = b->nrsq > 0 ? c->s_yc.as<double>() : b->cent.as<double>();
+  if ((rc = match_device(c, n, ninst, I.y, wvec, nopen, nb, dcent, rval, b->nrsq > 0 ? n : 0))) return rc;
   std::vector<int> tmp((size_t)ninst * (2 * nseg + 1));
   HIPCHK(c, hipMemcpyAsync(tmp.data(), dn, tmp.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   if ((rc = fetch_match(c, n, ninst, nopen, kmat, sr, si, status))) return rc;
@@ -2367,24 +2393,48 @@ int mcp_set_inverse_kernel(mcp_ctx* c, int32_t mode) {
 }
 
 int mcp_ytrans(mcp_ctx* c, int32_t bid, int32_t ninst, const double* pconst, double* y, double* eint) {
-  if (!c || ninst < 0 || !pconst || !eint) return c ? c->fail(MCP_ERR_ARG, "ytrans: bad args") : MCP_ERR_ARG;
+  return mcp_ytrans_cent(c, bid, ninst, pconst, y, eint, nullptr);
+}
+
+int mcp_set_degtol(mcp_ctx* c, double degtol) {
+  if (!c || !(degtol >= 0.0)) return c ? c->fail(MCP_ERR_ARG, "degtol: must be >= 0") : MCP_ERR_ARG;
+  c->degtol = degtol;
+  return MCP_OK;
+}
+
+int mcp_ytrans_cent(mcp_ctx* c, int32_t bid, int32_t ninst, const double* pconst, double* y, double* eint,
+                    double* cent) {
   Block* b;
   int rc;
+  if (!c || ninst < 0 || !eint) return c ? c->fail(MCP_ERR_ARG, "ytrans: bad args") : MCP_ERR_ARG;
   if ((rc = lookup(c, bid, 0, &b, nullptr))) return rc;
   if (ninst == 0) return MCP_OK;
   const int n = b->n;
-  if (b->nconst <= 0) return c->fail(MCP_ERR_ARG, "ytrans: the block has no constant terms (NCONST = 0)");
+  if (b->nconst <= 0 && b->nrsq <= 0)
+    return c->fail(MCP_ERR_ARG, "ytrans: the block has neither constant nor centrifugal terms");
+  if (b->nconst > 0 && !pconst) return c->fail(MCP_ERR_ARG, "ytrans: NCONST > 0 needs pconst");
   HIPCHK(c, hipSetDevice(c->device));
-  if ((rc = upload(c, c->s_h, pconst, (size_t)ninst * b->nconst))) return rc;
+  if (b->nconst > 0 && (rc = upload(c, c->s_h, pconst, (size_t)ninst * b->nconst))) return rc;
   HIPCHK(c, c->s_ye.ensure((size_t)ninst * n * sizeof(double)));
   double* dy = nullptr;
   if (y) {
     if ((rc = upload(c, c->s_i, y, (size_t)ninst * n * n))) return rc;
     dy = c->s_i.as<double>();
   }
-  if ((rc = ytrans_device(c, *b, ninst, c->s_h.as<double>(), dy, c->s_ye.as<double>()))) return rc;
+  if ((rc = ytrans_device(c, *b, ninst, b->nconst > 0 ? c->s_h.as<double>() : nullptr, dy, c->s_ye.as<double>())))
+    return rc;
   if ((rc = download(c, eint, c->s_ye, (size_t)ninst * n))) return rc;
   if (y && (rc = download(c, y, c->s_i, (size_t)ninst * n * n))) return rc;
+  if (cent) {
+    if (b->nrsq > 0) {
+      if ((rc = download(c, cent, c->s_yc, (size_t)ninst * n))) return rc;
+    } else {
+      std::vector<double> c0(n);
+      HIPCHK(c, hipMemcpyAsync(c0.data(), b->cent.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      for (int i = 0; i < ninst; ++i) std::copy(c0.begin(), c0.end(), cent + (size_t)i * n);
+    }
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return MCP_OK;
 }

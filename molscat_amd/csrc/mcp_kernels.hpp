@@ -870,7 +870,8 @@ struct MatchArgs {
   const double* wvec;  // [ninst][n]  sign(E-EINT) sqrt|E-EINT|
   const int* nopen;    // [ninst]
   const int* nb;       // [ninst][n]
-  const double* cent;  // [n]
+  const double* cent;  // [n] shared, or [ninst][n] with cent_stride = n
+  int cent_stride;
   double rval;
   double* work;        // [ninst][n*n + n*n + 4n]
   double* kmat;        // [ninst][n*n] (nopen x nopen used)
@@ -929,7 +930,7 @@ __global__ void __launch_bounds__(kNT) k_match(MatchArgs M, int phase) {
   // Riccati-Bessel functions per channel (ytok.f:24-45)
   for (int ii = threadIdx.x; ii < n; ii += kNT) {
     const int nx = nb[ii];
-    const double ell = sqrt(M.cent[nx] + 0.25) - 0.5;
+    const double ell = sqrt(M.cent[(size_t)inst * M.cent_stride + nx] + 0.25) - 0.5;
     if (ii < nop) {
       const double dw = wv[nx];
       double uj, ujp, un, unp;

@@ -268,6 +268,23 @@ class Engine:
         self._check(self.lib.mcp_ytrans(self.ctx, b.id, m, dptr(pc), dptr(buf), dptr(eint)), "mcp_ytrans")
         return eint, (None if buf is None else _from_colmajor(buf))
 
+    def ytrans_cent(self, b: BlockHandle, pconst=None, y0: Optional[np.ndarray] = None, ninst: int = 1):
+        """YTRANS for a block with NCONST > 0 and/or NRSQ > 0: (eint (m, N),
+        cent (m, N), y or None) in the asymptotic basis; pconst (m, nconst)
+        or None when NCONST = 0 (then m = ninst)."""
+        pc = None if pconst is None else np.ascontiguousarray(np.atleast_2d(pconst), dtype=np.float64)
+        m = pc.shape[0] if pc is not None else int(ninst)
+        eint = np.empty((m, b.n))
+        cent = np.empty((m, b.n))
+        buf = None if y0 is None else _colmajor(np.asarray(y0, dtype=np.float64).reshape(m, b.n, b.n))
+        self._check(self.lib.mcp_ytrans_cent(self.ctx, b.id, m, dptr(pc), dptr(buf), dptr(eint), dptr(cent)),
+                    "mcp_ytrans_cent")
+        return eint, cent, (None if buf is None else _from_colmajor(buf))
+
+    def set_degtol(self, degtol: float):
+        """YTRANS's DEGTOL (1e-10 by default)."""
+        self._check(self.lib.mcp_set_degtol(self.ctx, float(degtol)), "mcp_set_degtol")
+
     def smatrix_segments(self, b: BlockHandle, grids, iprops, ered, wvec, nopen, nb, rval: float, pconst=None):
         """SCCTRL over several segments (grids with their propagator codes)."""
         ered = np.ascontiguousarray(np.atleast_1d(ered), dtype=np.float64)

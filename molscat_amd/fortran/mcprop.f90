@@ -237,6 +237,27 @@ module mcprop
       type(c_ptr), value :: ctx
       integer(c_int), value :: mode
     end function
+    integer(c_int) function mcp_bound_wave(ctx, block_id, gid_out, gid_in, ered, pconst, nodec, eigmin, psimch, &
+        psi_out, psi_in, status) bind(C, name="mcp_bound_wave")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: block_id, gid_out, gid_in
+      real(c_double), value :: ered
+      real(c_double) :: pconst(*), eigmin, psimch(*), psi_out(*), psi_in(*)
+      integer(c_int) :: nodec, status
+    end function
+    integer(c_int) function mcp_ytrans_cent(ctx, block_id, ninst, pconst, y, eint, cent) &
+        bind(C, name="mcp_ytrans_cent")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: block_id, ninst
+      real(c_double) :: pconst(*), y(*), eint(*), cent(*)
+    end function
+    integer(c_int) function mcp_set_degtol(ctx, degtol) bind(C, name="mcp_set_degtol")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), value :: degtol
+    end function
     integer(c_int) function mcp_set_vivs_options(ctx, iv, ivp, ivpp, ishift, idiag, ipert, isym, ialpha, ialfp, &
         alpha1, alpha2, xsqmax) bind(C, name="mcp_set_vivs_options")
       import :: c_int, c_double, c_ptr

@@ -68,6 +68,8 @@ SIGNATURES = {
     "mcp_set_team_launch": (ctypes.c_int, [_vp, _i32]),
     "mcp_set_lds_threads": (ctypes.c_int, [_vp, _i32]),
     "mcp_ytrans": (ctypes.c_int, [_vp, _i32, _i32, _pd, _pd, _pd]),
+    "mcp_ytrans_cent": (ctypes.c_int, [_vp, _i32, _i32, _pd, _pd, _pd, _pd]),
+    "mcp_set_degtol": (ctypes.c_int, [_vp, _dbl]),
     "mcp_last_timing": (ctypes.c_int, [_vp, _pd, _pd, _pd]),
     "mcp_last_segment_timing": (ctypes.c_int, [_vp, _i32, _pd, _pd]),
     "mcp_block_set_potl": (ctypes.c_int, [_vp, _i32, _i32, _pi, _pi, _pd, _pd, _dbl, _dbl]),

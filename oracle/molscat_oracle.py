@@ -356,7 +356,8 @@ class Block:
     ep2ru: float
     cm2ru: float
     vl_full: np.ndarray = field(default=None)  # (NHAM, N, N)
-    nconst: int = 0  # constant terms: the last nconst VL blocks
+    nconst: int = 0  # constant terms: the VL blocks after the radial ones
+    nrsq: int = 0  # terms multiplying 1/R^2 (the centrifugal operator): the last nrsq VL blocks
     pconst: Optional[np.ndarray] = None  # their P values, VCONST * CM2RU (hammat.f:57-73)
     scalam: float = 1.0  # SCAPOT's factor (source_code/scapot-default.f)
     expval: Optional[Tuple[int, int, float]] = None  # PERTRB's /EXPVAL/ (I, IPOW, DELTAN), I 1-based
@@ -420,9 +421,13 @@ class Block:
         """
         rsq = min(1.0 / (r * r), 1.0e16)
         p = self.pvec(r)
+        if self.nrsq > 0:
+            # the centrifugal operator's terms take P = 1/R^2 (hammat.f:74-83)
+            p = np.concatenate([p, np.full(self.nrsq, rsq)])
         w = np.tensordot(p, self.vl_full, axes=(0, 0))
         d = np.diag(w).copy()
-        d = d + rsq * self.cent
+        if self.nrsq == 0:
+            d = d + rsq * self.cent
         if self.nconst == 0:
             d = d + self.eint
         d = d - ered
@@ -1483,6 +1488,8 @@ def ytrans(y: Optional[np.ndarray], blk: Block) -> Tuple[Optional[np.ndarray], n
     EINT its eigenvalue (:330-350), Y <- EVEC^T Y EVEC (TRNSFM,
     source_code/trnsfm.f:20-32).  Returns (Y or None, EINT, EVEC)."""
     n = blk.n
+    if blk.nrsq > 0:
+        return ytrans_rsq(y, blk)[:3]
     mx = blk.vl.shape[0] - blk.nconst
     c = np.tensordot(np.asarray(blk.pconst, dtype=np.float64), blk.vl_full[mx:], axes=(0, 0))
     evec = np.zeros((n, n))
@@ -1500,15 +1507,63 @@ def ytrans(y: Optional[np.ndarray], blk: Block) -> Tuple[Optional[np.ndarray], n
     return yt, eint, evec
 
 
+def ytrans_rsq(y: Optional[np.ndarray], blk: Block, degtol: float = 1.0e-10):
+    """YTRANS with the centrifugal operator among the terms (NRSQ > 0,
+    source_code/ytrans.f90:79-430, MULTOP source_code/multop.f90:55-230):
+    no grouping by L; the operators are C = sum_c VL(MXLAM+c) P_c (when
+    NCONST > 0) and L^2 = sum_r VL(MXLAM+NCONST+r) (P = 1).  With both: C
+    diagonalised (ascending), L^2 turned into its eigenbasis, within each
+    set of C eigenvalues differing by at most DEGTOL from their neighbour
+    L^2's block diagonalised (ascending; a set of one keeps its diagonal
+    element), the total eigenvectors V_C V_L (EVEORD with the identity
+    order), EINT = C's eigenvalues; with L^2 alone its eigenvectors and EINT
+    unchanged.  Every eigenvector with its largest component (IDAMAX)
+    positive; CENT = L^2's eigenvalues (ytrans.f90:353-361,411-430);
+    Y <- EVEC^T Y EVEC.  Returns (Y or None, EINT, EVEC, CENT)."""
+    n = blk.n
+    mx = blk.vl.shape[0] - blk.nconst - blk.nrsq
+    l2 = blk.vl_full[mx + blk.nconst:].sum(axis=0)
+    if blk.nconst > 0:
+        c = np.tensordot(np.asarray(blk.pconst, dtype=np.float64), blk.vl_full[mx:mx + blk.nconst], axes=(0, 0))
+        e1, v1 = np.linalg.eigh(c)
+        w = v1.T @ l2 @ v1
+        e2 = np.empty(n)
+        v2 = np.zeros((n, n))
+        beg = 0
+        while beg < n:
+            end = beg + 1
+            while end < n and abs(e1[end] - e1[end - 1]) <= degtol:
+                end += 1
+            ev, vv = np.linalg.eigh(w[beg:end, beg:end])
+            e2[beg:end] = ev
+            v2[beg:end, beg:end] = vv
+            beg = end
+        evec = v1 @ v2
+        eint = e1
+        cent = e2
+    else:
+        cent, evec = np.linalg.eigh(l2)
+        eint = np.array(blk.eint, dtype=np.float64)
+    for j in range(n):
+        col = evec[:, j]
+        if col[int(np.argmax(np.abs(col)))] < 0.0:
+            evec[:, j] = -col
+    yt = None if y is None else evec.T @ y @ evec
+    return yt, eint, evec, cent
+
+
 def smatrix_ld(blk: Block, ered: float, grid: Grid, iprop: int = 6, imgsel: int = 4, inv: Callable = None,
                efirst: Optional[float] = None):
-    """YINIT + MDPROP/LDPROP/ODPROP + (YTRANS when NCONST > 0) + YTOK + KTOS
+    """YINIT + MDPROP/LDPROP/ODPROP + (YTRANS when NCONST > 0 or NRSQ > 0, whose
+    CENT then go to YTOK) + YTOK + KTOS
     for one energy.  ``inv`` replaces MDPROP's syminv; ``efirst`` (LDPROP)
     replays the first energy's matrices as ldprop.f does with ISCRU.
 
     Returns dict with Y (final), K, SR, SI (NB-ordered open channels), NB,
     NOPEN, WVEC, NODES."""
-    eint = ytrans(None, blk)[1] if blk.nconst > 0 else blk.eint
+    tr = blk.nconst > 0 or blk.nrsq > 0
+    eint = ytrans(None, blk)[1] if tr else blk.eint
+    cent = ytrans_rsq(None, blk)[3] if blk.nrsq > 0 else blk.cent
     wvec, nopen = gtwvec(ered, eint)
     nb = chnsrt(eint)
     y = yinit(blk, ered, grid.rstart)
@@ -1529,11 +1584,11 @@ def smatrix_ld(blk: Block, ered: float, grid: Grid, iprop: int = 6, imgsel: int 
         y, nodes = mgprop(y, hm, grid.rstart, grid.nstep, grid.dr, imgsel)
     else:
         raise ValueError(iprop)
-    if blk.nconst > 0:
+    if tr:
         y = ytrans(y, blk)[0]
     out = dict(Y=y, NB=nb, NOPEN=nopen, WVEC=wvec, NODES=nodes)
     if nopen > 0:
-        k = ytok(nb, wvec, blk.n, nopen, y, grid.rstop, blk.cent)
+        k = ytok(nb, wvec, blk.n, nopen, y, grid.rstop, cent)
         sr, si = ktos(k)
         out.update(K=k, SR=sr, SI=si)
     return out

@@ -8,7 +8,11 @@ basis is a real rotation), through the C ABI:
 - whole S-matrix solves (YINIT, MDPROP with the field as the instance
   constant, YTRANS, YTOK, KTOS) at three fields in one call against the
   oracle's, with the node count, for a diagonal and a rotated Zeeman
-  operator."""
+  operator;
+- NRSQ > 0 (the centrifugal operator among the terms, MULTOP's resolution
+  of C and L^2): the rotor deck turned by a random rotation, YTRANS against
+  the oracle and the plain deck's K-matrix recovered; the whole solve
+  against the oracle's with node counts exact."""
 import numpy as np
 import pytest
 
@@ -132,3 +136,76 @@ def test_field_bound_match_vs_oracle(eng):
               f"|dEIGMIN|={de:.1e} |dEVALS|={dv:.1e}", flush=True)
         assert int(out["NODEC"][m]) == int(ref["NODEC"])
         assert de < 1e-9 and dv < 1e-9
+
+
+# ---------------------------------------------------------------------------
+# NRSQ > 0: the centrifugal operator among the terms (mcp_ytrans_cent,
+# k_ytrans_rsq; MULTOP's resolution of C and L^2)
+# ---------------------------------------------------------------------------
+def _turned(eng):
+    from test_oracle_ytrans import turned_rotor_blocks
+
+    blk, tb, q = turned_rotor_blocks()
+    b = eng.block(tb.vl, tb.eint, tb.cent, mxlam=tb.vl.shape[0] - 2, nconst=1, nrsq=1)
+    return blk, tb, q, b
+
+
+def test_ytrans_rsq_vs_oracle(eng):
+    """The turned rotor deck (N = 12): thresholds, CENT and Y in the
+    asymptotic basis against the oracle's MULTOP restatement, for a Y turned
+    from the plain deck's Y at 20 RM; the K-matrix from the device's YTOK
+    with those CENT equals the plain deck's channel for channel."""
+    from test_oracle_ytrans import _label_map
+
+    blk, tb, q, b = _turned(eng)
+    ered = 1000.0 * blk.cm2ru
+    rstart, _ = O.findrm(blk, 0.5, ered, 9)
+    g = O.Grid(rstart, 20.0, 2000, (20.0 - rstart) / 2000)
+    a = O.smatrix_ld(blk, ered, g)
+    yt = q.T @ a["Y"] @ q
+    print(f"ytrans NRSQ=1 N={blk.n}", flush=True)
+    eint, cent, y2 = eng.ytrans_cent(b, np.array([[1.0]]), yt[None])
+    yr, er, _, cr = O.ytrans_rsq(yt, tb)
+    de = np.abs(eint[0] - er).max() / np.abs(er).max()
+    dc = np.abs(cent[0] - cr).max()
+    dy = np.abs(y2[0] - yr).max() / np.abs(yr).max()
+    print(f"  EINT {de:.2e} CENT {dc:.2e} Y {dy:.2e}", flush=True)
+    # measured 3.8e-15, 4.8e-13 (of L(L+1) up to 240) and 1.2e-15
+    assert de < 1e-13 and dc < 1e-10 and dy < 1e-12
+    wvec, nopen = O.gtwvec(ered, eint[0])
+    nb = O.chnsrt(eint[0])
+    k, sr, si, st = eng.ytok_ktos(y2, wvec[None], [nopen], nb, cent[0], 20.0)
+    assert st[0] == 0
+    pi = _label_map(blk, eint[0], cent[0], a["NB"], nb, nopen)
+    kd = k[0]
+    kp = a["K"][np.ix_(pi, pi)]
+    assert np.abs(np.abs(kd) - np.abs(kp)).max() < 1e-10 * max(1.0, np.abs(kp).max())
+    assert np.abs(np.diag(kd) - np.diag(kp)).max() < 1e-10 * max(1.0, np.abs(kp).max())
+
+
+def test_smatrix_rsq_vs_oracle(eng):
+    """The whole solve on the turned deck (YINIT, MDPROP, YTRANS with C and
+    L^2, YTOK with YTRANS's CENT per instance, KTOS) at two energies in one
+    call against the oracle's on the same grid, node counts exact."""
+    blk, tb, q, b = _turned(eng)
+    energies = np.array([1000.0, 900.0]) * blk.cm2ru
+    rstart, _ = O.findrm(blk, 0.5, energies[0], 9)
+    ns = 3000
+    dr = (20.0 - rstart) / ns
+    r = S.ld_radii(rstart, dr, ns)
+    ptab = np.stack([tb.ep2ru * np.clip(tb.potl(x * tb.rscale), -1e60, 1e60) for x in r])
+    gr = eng.grid(r, ptab, dr)
+    eint, _, _ = eng.ytrans_cent(b, np.array([[1.0]]))
+    print(f"smatrix NRSQ=1 N={blk.n} {ns} sectors", flush=True)
+    for m, e in enumerate(energies):
+        wvec, nopen = O.gtwvec(e, eint[0])
+        nb = O.chnsrt(eint[0])
+        out = eng.smatrix_ld(b, gr, 6, [e], wvec[None], [nopen], nb, 20.0, pconst=np.array([[1.0]]))
+        ref = O.smatrix_ld(tb, e, O.Grid(rstart, 20.0, ns, dr))
+        assert out["STATUS"][0] == 0
+        nodes = out["NODES"]
+        d = max(np.abs(out["SR"][0] - ref["SR"]).max(), np.abs(out["SI"][0] - ref["SI"]).max())
+        print(f"  E={e / blk.cm2ru:.0f} cm-1 nodes {int(nodes[0])} (oracle {ref['NODES']}) max|dS| {d:.2e}", flush=True)
+        assert int(nodes[0]) == ref["NODES"]
+        # measured 1.2e-11 and 5.0e-12
+        assert d < 1e-10

@@ -121,3 +121,99 @@ def test_free_particles_give_unit_s_matrix():
     # without YTRANS the primitive basis is not the asymptotic one: S != I
     kbad = O.ytok(nb, wvec, fb.n, nopen, yprim, r, fb.cent)
     assert np.abs(kbad).max() > 1e-3
+
+
+# ---------------------------------------------------------------------------
+# NRSQ > 0: the centrifugal operator among the terms, resolved with the
+# thresholds by MULTOP
+# ---------------------------------------------------------------------------
+def turned_rotor_blocks(seed=5):
+    """The all-propagators rotor deck (JTOT = 10, IB = 1, j = 0..6, N = 12)
+    and the same physics in a basis turned by a random orthogonal Q: every
+    radial term Q^T VL Q, the thresholds as a constant term Q^T EINT Q (P = 1,
+    EINT = 0) and the centrifugal term as the operator Q^T L(L+1) Q (NRSQ =
+    1).  YTRANS must undo the turn: channels by ascending threshold, then
+    ascending L within a level."""
+    pot = O.PowerPotential([[(1.0, -12), (-2.0, -6)], [(-0.2, -6)]], rm=3.5, epsil=50.0)
+    blk, _ = O.make_itype1_block(10, 1, [0, 2, 4, 6], 30.0, [0, 2], pot, 20.0)
+    n = blk.n
+    q, _ = np.linalg.qr(np.random.default_rng(seed).standard_normal((n, n)))
+    pack = lambda m: np.array([m[i, j] for j in range(n) for i in range(j + 1)])
+    vls = [pack(q.T @ blk.vl_full[k] @ q) for k in range(blk.vl.shape[0])]
+    vls.append(pack(q.T @ np.diag(blk.eint) @ q))
+    vls.append(pack(q.T @ np.diag(blk.cent) @ q))
+    tb = O.Block(n=n, vl=np.array(vls), eint=np.zeros(n), cent=np.zeros(n), lvals=np.zeros(n, dtype=int),
+                 levidx=np.zeros(n, dtype=int), potl=blk.potl, rscale=blk.rscale, ep2ru=blk.ep2ru, cm2ru=blk.cm2ru,
+                 nconst=1, pconst=np.array([1.0]), nrsq=1)
+    return blk, tb, q
+
+
+def test_ytrans_rsq_recovers_thresholds_and_l():
+    blk, tb, q = turned_rotor_blocks()
+    _, eint, evec, cent = O.ytrans_rsq(None, tb)
+    order = sorted(range(blk.n), key=lambda i: (blk.eint[i], blk.lvals[i]))
+    assert np.allclose(eint, blk.eint[order], rtol=0, atol=1e-12 * np.abs(blk.eint).max())
+    assert np.allclose(cent, blk.cent[order], rtol=0, atol=1e-9)
+    # the columns are the turned unit vectors, largest component positive
+    for m, i in enumerate(order):
+        t = q.T[:, i] * np.sign(q.T[np.argmax(np.abs(q.T[:, i])), i])
+        assert np.abs(evec[:, m] - t).max() < 1e-9
+
+
+def _label_map(blk, eint_t, cent_t, nb_o, nb_t, nopen):
+    """Open channel a of the turned problem (NB order) -> the position of
+    the original open channel with the same threshold and L(L+1)."""
+    out = []
+    for a in range(nopen):
+        k = nb_t[a]
+        i = next(i for i in range(blk.n) if abs(blk.eint[i] - eint_t[k]) < 1e-9 * abs(blk.eint).max()
+                 and abs(blk.cent[i] - cent_t[k]) < 1e-6)
+        out.append(int(np.where(nb_o[:nopen] == i)[0][0]))
+    return out
+
+
+def test_turned_ytrans_then_ytok_reproduces_k():
+    """Y at 20 RM of the plain deck (MDPROP), turned by Q: YTRANS (C and
+    L^2 resolved by MULTOP) then YTOK with YTRANS's CENT gives the plain
+    deck's K-matrix, channel for channel (matched by threshold and L) up to
+    the sign of each asymptotic channel: K''_ab = d_a d_b K_pi(a)pi(b), to
+    1e-11 of K's scale."""
+    blk, tb, q = turned_rotor_blocks()
+    ered = 1000.0 * blk.cm2ru
+    rstart, _ = O.findrm(blk, 0.5, ered, 9)
+    g = O.Grid(rstart, 20.0, 2000, (20.0 - rstart) / 2000)
+    a = O.smatrix_ld(blk, ered, g)
+    y = a["Y"]
+    y2, eint2, evec2, cent2 = O.ytrans_rsq(q.T @ y @ q, tb)
+    wvec2, nopen2 = O.gtwvec(ered, eint2)
+    nb2 = O.chnsrt(eint2)
+    assert nopen2 == a["NOPEN"]
+    k2 = O.ytok(nb2, wvec2, blk.n, nopen2, y2, 20.0, cent2)
+    pi = _label_map(blk, eint2, cent2, a["NB"], nb2, nopen2)
+    # the sign of each asymptotic channel: its EVEC column against the turned unit vector
+    d = np.array([np.sign(evec2[:, nb2[m]] @ q.T[:, a["NB"][pi[m]]]) for m in range(nopen2)])
+    kp = a["K"][np.ix_(pi, pi)] * d[:, None] * d[None, :]
+    assert np.abs(k2 - kp).max() < 1e-11 * max(1.0, np.abs(kp).max())
+
+
+def test_turned_basis_smatrix_converges_to_the_same():
+    """Through the whole path (YINIT, MDPROP, YTRANS, YTOK, KTOS) the turned
+    deck's S-matrix approaches the plain deck's as the step shrinks: MDPROP
+    takes W's diagonal as its reference potential, so its discretisation
+    error depends on the basis, and the difference falls at fourth order
+    (4000 -> 8000 sectors: at least 8 times smaller)."""
+    blk, tb, _ = turned_rotor_blocks()
+    ered = 1000.0 * blk.cm2ru
+    rstart, _ = O.findrm(blk, 0.5, ered, 9)
+    diffs = []
+    for ns in (4000, 8000):
+        g = O.Grid(rstart, 20.0, ns, (20.0 - rstart) / ns)
+        a = O.smatrix_ld(blk, ered, g)
+        b = O.smatrix_ld(tb, ered, g)
+        assert a["NOPEN"] == b["NOPEN"] and a["NODES"] == b["NODES"]
+        _, eint2, _, cent2 = O.ytrans_rsq(None, tb)
+        pi = _label_map(blk, eint2, cent2, a["NB"], b["NB"], a["NOPEN"])
+        sa = (a["SR"] + 1j * a["SI"])[np.ix_(pi, pi)]
+        sb = b["SR"] + 1j * b["SI"]
+        diffs.append(np.abs(np.diag(sa) - np.diag(sb)).max())
+    assert diffs[0] > 8.0 * diffs[1]
