@@ -5,6 +5,10 @@ log-derivative propagation and matching path.
   * ``engine.Engine``: ctypes handle on the C ABI.
   * ``system``: host-side set-up (units, rotor basis and coupling, POTENL
     terms, grids, channel bookkeeping).
-  * ``scctrl``: SCCTRL-style control of one block (segments, matching).
+  * ``driver``: the MOLSCAT partial-wave / energy loop (SCCTRL's segments
+    and matching per block), the batched sweep and the field scan.
+  * ``bound``: BOUND's level search, FIELD's field search and the
+    wavefunction.
+  * ``smatfile``: the saved S-matrix file (ISAVEU) and restart.
 """
 __all__ = ["engine", "system", "build"]

@@ -1003,8 +1003,18 @@ int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const do
     }
   } else if (iprop == 7) {
     // MAPROP: the energy-independent bases, corrections and basis changes of
-    // every sector, then one workgroup per instance
-    if (b.nconst != 0) return c->fail(MCP_ERR_UNSUPPORTED, "MAPROP: blocks with NCONST terms are not supported");
+    // every sector, then one workgroup per instance.  Constant terms enter
+    // W, so with NCONST > 0 the bases are particular to each instance's
+    // constants: the instances one at a time.
+    if (b.nconst > 0 && ninst > 1) {
+      const size_t nn1 = (size_t)n * n;
+      for (int i = 0; i < ninst; ++i) {
+        int r2 = prop_ld_device(c, b, g, iprop, 1, d_ered + i, d_pconst + (size_t)i * b.nconst, dy + i * nn1,
+                                d_nodes + i, d_status + i);
+        if (r2) return r2;
+      }
+      return MCP_OK;
+    }
     const size_t lb = lds_ma_basis(n, b.nham), lm = lds_ma_mats(n, b.nham), lp = lds_ma_prop(n);
     if (std::max(lb, std::max(lm, lp)) > c->lds_max)
       return c->fail(MCP_ERR_UNSUPPORTED, "MAPROP: N=%d exceeds the LDS-resident limit", n);
@@ -1015,10 +1025,11 @@ int prop_ld_device(mcp_ctx* c, Block& b, Grid& g, int iprop, int ninst, const do
     HIPCHK(c, c->s_me.ensure((size_t)nstep * n * sizeof(double)));
     HIPCHK(c, c->s_mq.ensure(nn * sizeof(double)));
     double* mt = c->s_mt.as<double>();
-    hipLaunchKernelGGL(mcp::k_maprop_basis, dim3(nstep), dim3(kNT), lb, c->stream, B, G, mt, c->s_me.as<double>());
+    hipLaunchKernelGGL(mcp::k_maprop_basis, dim3(nstep), dim3(kNT), lb, c->stream, B, G, mt, c->s_me.as<double>(),
+                       d_pconst);
     HIPCHK(c, hipGetLastError());
     hipLaunchKernelGGL(mcp::k_maprop_mats, dim3(nstep), dim3(kNT), lm, c->stream, B, G, nstep, g.dr / 6.0, mt,
-                       c->s_mc.as<double>(), c->s_mp.as<double>(), c->s_mq.as<double>());
+                       c->s_mc.as<double>(), c->s_mp.as<double>(), c->s_mq.as<double>(), d_pconst);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     hipLaunchKernelGGL(mcp::k_maprop_prop, dim3(ninst), dim3(kNT), lp, c->stream, n, nstep, g.dr,

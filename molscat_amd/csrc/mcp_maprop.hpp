@@ -14,6 +14,9 @@
 // adiabat (tan / tanh of sqrt|EIVAL - ERED| DR/2), the inversion with its
 // inertia (not counted when some adiabat lies above 24/DR^2, NCHECK), the
 // Y update, and at the end Y <- T Y T^T + (W_end - W_mid) DR/6.
+// Constant terms (NCONST > 0) enter W with the instance's constants
+// (pconst), so with them the bases belong to one instance and the caller
+// runs the instances one at a time.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -33,7 +36,7 @@ __device__ __forceinline__ double ma_w(const BlockArgs& B, const double* pv, dou
 
 // one workgroup per sector: T_k and the eigenvalues of W at its midpoint
 __global__ void __launch_bounds__(kNT) k_maprop_basis(BlockArgs B, GridArgs G, double* __restrict__ tout,
-                                                      double* __restrict__ eout) {
+                                                      double* __restrict__ eout, const double* __restrict__ pconst) {
   extern __shared__ double smem[];
   const int n = B.n;
   const size_t nn = (size_t)n * n;
@@ -52,7 +55,7 @@ __global__ void __launch_bounds__(kNT) k_maprop_basis(BlockArgs B, GridArgs G, d
   double* tmp = c.take<double>(n);
   double* pv = c.take<double>(B.nham > 0 ? B.nham : 1);
   const int km = 2 * k + 1;
-  load_pvec(B, G, km, nullptr, pv);
+  load_pvec(B, G, km, pconst, pv);
   __syncthreads();
   const double r = G.r[km], rsq = rsq_of(r);
   for (size_t e = threadIdx.x; e < nn; e += kNT) {
@@ -73,7 +76,8 @@ __global__ void __launch_bounds__(kNT) k_maprop_basis(BlockArgs B, GridArgs G, d
 // correction qend); cr = DR/6
 __global__ void __launch_bounds__(kNT) k_maprop_mats(BlockArgs B, GridArgs G, int nstep, double cr,
                                                      const double* __restrict__ tout, double* __restrict__ cout,
-                                                     double* __restrict__ pout, double* __restrict__ qend) {
+                                                     double* __restrict__ pout, double* __restrict__ qend,
+                                                     const double* __restrict__ pconst) {
   extern __shared__ double smem[];
   const int n = B.n;
   const size_t nn = (size_t)n * n;
@@ -86,9 +90,9 @@ __global__ void __launch_bounds__(kNT) k_maprop_mats(BlockArgs B, GridArgs G, in
   double* pm = c.take<double>(B.nham > 0 ? B.nham : 1);
   double* pp = c.take<double>(B.nham > 0 ? B.nham : 1);
   const int ks = 2 * k, km = 2 * k + 1;
-  load_pvec(B, G, ks, nullptr, ps);
-  load_pvec(B, G, km, nullptr, pm);
-  if (k > 0) load_pvec(B, G, ks - 1, nullptr, pp);
+  load_pvec(B, G, ks, pconst, ps);
+  load_pvec(B, G, km, pconst, pm);
+  if (k > 0) load_pvec(B, G, ks - 1, pconst, pp);
   __syncthreads();
   const double rs = G.r[ks], rm = G.r[km];
   const double rss = rsq_of(rs), rsm = rsq_of(rm);
@@ -120,7 +124,7 @@ __global__ void __launch_bounds__(kNT) k_maprop_mats(BlockArgs B, GridArgs G, in
   if (k == nstep - 1) {
     // (W_end - W_mid) DR/6 of the last sector (maprop.f:171-177)
     __syncthreads();
-    load_pvec(B, G, ks + 2, nullptr, pp);
+    load_pvec(B, G, ks + 2, pconst, pp);
     __syncthreads();
     const double re = G.r[ks + 2], rse = rsq_of(re);
     for (size_t e = threadIdx.x; e < nn; e += kNT) {

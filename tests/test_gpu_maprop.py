@@ -91,3 +91,36 @@ def test_maprop_constant_coupling_exact(eng):
     yex = u @ np.diag(yc) @ u.T
     assert np.abs(y[0] - yex).max() < 1e-9 * np.abs(yex).max()
     assert int(nodes[0]) == zeros
+
+
+def test_maprop_with_constant_terms_vs_oracle(eng):
+    """MAPROP for a block with NCONST > 0 (the synthetic field deck, N = 48,
+    a rotated Zeeman operator): the adiabatic bases include each instance's
+    constants, so the instances run one at a time; three fields in one call
+    against the oracle's MAPROP from the same Y, node counts exact."""
+    from molscat_amd import decks
+    from test_oracle_ytrans import _oracle_block, _random_zeeman
+
+    fb = decks.field48(zeeman=_random_zeeman(5))
+    b = eng.block(fb.vl, fb.eint, fb.cent, mxlam=fb.mxlam, nconst=fb.nconst)
+    fields = [25.0, 300.0, 900.0]
+    pc = np.stack([fb.pconst(f) for f in fields])
+    ns, r0, r1 = 240, 4.0, 10.0
+    dr = (r1 - r0) / ns
+    r = S.ld_radii(r0, dr, ns)
+    g = eng.grid(r, fb.ptab(r), dr)
+    rng = np.random.default_rng(4)
+    a = rng.standard_normal((len(fields), fb.n, fb.n))
+    y0 = a + np.transpose(a, (0, 2, 1))
+    ered = np.full(len(fields), -2.0 * fb.cm2ru)
+    print(f"MAPROP with NCONST=1 N={fb.n} {ns} sectors, fields {fields}", flush=True)
+    y, nodes, st = eng.prop_ld(b, g, 7, ered, y0, pconst=pc)
+    assert np.all(st == 0)
+    for m, f in enumerate(fields):
+        ob = _oracle_block(fb, f)
+        yo, no = O.maprop(y0[m].copy(), ob, ered[m], r0, ns, dr)
+        d = np.abs(y[m] - yo).max() / np.abs(yo).max()
+        print(f"  B={f} G nodes {int(nodes[m])} (oracle {no}) rel|dY| {d:.2e}", flush=True)
+        assert int(nodes[m]) == no
+        # measured up to 2.0e-11
+        assert d < 2e-10

@@ -10,7 +10,9 @@ diagonal and columns, and their Frobenius norms to 1e-12 relative.  The
 default (tree over the radial grid) and the sequential recurrence
 (mcp_set_large_mode(1)) must give the same node count over all 3600
 sectors: the Haynsworth inertia argument for the tree's merge pivots at
-full size."""
+full size.  The workgroup-team kernels issued as ordinary launches (the
+mode the kernel profiles are taken in) give the cooperative launches'
+S-matrix bit for bit at this size."""
 import json
 import os
 
@@ -113,3 +115,19 @@ def test_tree_and_sequential_node_counts_agree(eng, fx, setup):
     assert int(ns[0]) == int(nt[0]) == fx["nodes_ld"]
     rel = np.abs(ys[0] - yt[0]).max() / np.abs(yt[0]).max()
     assert rel < 1e-10, rel
+
+
+def test_plain_team_launch_bitwise_full_size(eng, setup):
+    deck, blk, b, g, ga, ered = setup
+    wv, nopen = S.gtwvec(ered, blk.eint)
+    nb = S.chnsrt(blk.eint)
+    print("rotor441: cooperative and ordinary team launches", flush=True)
+    a = eng.smatrix_segments(b, [g, ga], [6, 9], [ered], wv, [nopen], nb, deck.airy_stop)
+    eng.set_team_launch(False)
+    try:
+        p = eng.smatrix_segments(b, [g, ga], [6, 9], [ered], wv, [nopen], nb, deck.airy_stop)
+    finally:
+        eng.set_team_launch(True)
+    assert a["STATUS"][0] == 0 and p["STATUS"][0] == 0
+    assert np.array_equal(a["SR"][0], p["SR"][0]) and np.array_equal(a["SI"][0], p["SI"][0])
+    assert np.array_equal(a["NODES"], p["NODES"])
