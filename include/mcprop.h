@@ -246,7 +246,9 @@ int mcp_set_vivs_options(mcp_ctx *ctx, int32_t iv, int32_t ivp, int32_t ivpp, in
  * SYMINV instability ("INSTABILITY IN SYMINV IN AIPROP", aiprop.f:211-215),
  * -1 (not finished in max_steps steps: Z at the last point reached, as
  * aiprop.f:331-333) or -2 (a replay past the end of its record).  N up to
- * the LDS-resident limit (about 56 on gfx950), MCP_ERR_UNSUPPORTED beyond. */
+ * 448: the six N x N work matrices in LDS up to about N = 56 on gfx950, in
+ * a per-instance global workspace beyond (slower: the eigen-solver and the
+ * products then run on global memory). */
 int mcp_prop_airy_tolhi(mcp_ctx *ctx, int32_t block_id, int32_t ninst, const double *ered, double *z, double rstart,
                         double rstop, double dr0, double powr, double tolhi, int32_t replay, int32_t max_steps,
                         double *steps, int32_t *nrec, int32_t *nodes, int32_t *nstep, int32_t *status);

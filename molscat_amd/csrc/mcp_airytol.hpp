@@ -43,6 +43,8 @@ struct AiryTolArgs {
   const double* ered;  // [ninst]
   double* z;           // [ninst][n*n] Z(RSTART) in, Z(RSTOP) out (primitive basis)
   int inst0;
+  double* work;  // [ninst][kAiryTolMats*n*n] when the matrices are not in LDS
+  int lds_mats;
   int* nodes;   // [ninst]
   int* nstep;   // [ninst]
   int* status;  // [ninst]; positive: 1-based step of a SYMINV failure (aiprop.f:211-215)
@@ -99,7 +101,7 @@ __global__ void __launch_bounds__(kNT) k_airy_tolhi(BlockArgs B, PotlArgs P, Air
   double* pa = c.take<double>(B.nham > 0 ? B.nham : 1);
   double* pb = c.take<double>(B.nham > 0 ? B.nham : 1);
   SweepScratch sw = carve_sweep(c, n);
-  double* mats = c.take<double>(kAiryTolMats * nn);
+  double* mats = A.lds_mats ? c.take<double>(kAiryTolMats * nn) : A.work + (size_t)inst * kAiryTolMats * nn;
   double* Z = mats;
   double* M = Z + nn;   // the matrix to diagonalise, then T
   double* V = M + nn;   // this step's basis

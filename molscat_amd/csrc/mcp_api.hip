@@ -2581,10 +2581,14 @@ int mcp_prop_airy_tolhi(mcp_ctx* c, int32_t bid, int32_t ninst, const double* er
   const int n = b->n;
   const size_t nn = (size_t)n * n;
   HIPCHK(c, hipSetDevice(c->device));
-  // the matrices in LDS only (N up to about 56 on gfx950)
-  const size_t lds = lds_airytol(n, b->nham, true);
+  // the six N x N matrices in LDS when they fit (N up to about 56 on
+  // gfx950), else in a per-instance global workspace behind the same pointers
+  if (n > mcp::kEMaxN) return c->fail(MCP_ERR_UNSUPPORTED, "prop_airy_tolhi: N=%d above %d", n, mcp::kEMaxN);
+  const bool lds_mats = lds_airytol(n, b->nham, true) <= c->lds_max;
+  const size_t lds = lds_airytol(n, b->nham, lds_mats);
   if (lds > c->lds_max)
-    return c->fail(MCP_ERR_UNSUPPORTED, "prop_airy_tolhi: N=%d exceeds the LDS-resident limit", n);
+    return c->fail(MCP_ERR_UNSUPPORTED, "prop_airy_tolhi: N=%d exceeds the LDS limit", n);
+  if (!lds_mats) HIPCHK(c, c->s_vw.ensure((size_t)ninst * mcp::kAiryTolMats * nn * sizeof(double)));
   if ((rc = upload(c, c->s_ve, ered, (size_t)ninst))) return rc;
   if ((rc = upload(c, c->s_vr, z, (size_t)ninst * nn))) return rc;
   HIPCHK(c, c->s_vs.ensure((size_t)max_steps * 2 * sizeof(double)));
@@ -2614,6 +2618,8 @@ int mcp_prop_airy_tolhi(mcp_ctx* c, int32_t bid, int32_t ninst, const double* er
   A.nodes = c->s_vnd.as<int>();
   A.nstep = c->s_vn.as<int>();
   A.status = c->s_vst.as<int>();
+  A.work = lds_mats ? nullptr : c->s_vw.as<double>();
+  A.lds_mats = lds_mats ? 1 : 0;
   const mcp::BlockArgs B = block_args(*b);
   // replay 2: instance 0 chooses and records the steps, the others follow
   const bool record = replay == 2 || (replay == 0 && steps);

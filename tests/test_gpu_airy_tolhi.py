@@ -138,3 +138,25 @@ def test_airy_tolhi_inward_vs_oracle(eng, deck):
     dz = np.abs(z[0] - zo).max() / np.abs(zo).max()
     print(f"  Z vs oracle rel {dz:.2e}", flush=True)
     assert dz < 1e-9
+
+
+def test_airy_tolhi_beyond_lds_vs_oracle(eng):
+    """N = 72 (JTOT = 30, even j = 0..16): the six N x N matrices no longer
+    fit in LDS and live in a per-instance global workspace behind the same
+    pointers.  Inward from 20 to 8 RM below every threshold against the
+    oracle's AIPROP on the same start: steps, node count and Z."""
+    pot = O.PowerPotential([[(1.0, -12), (-2.0, -6)], [(-0.2, -6)]], rm=3.5, epsil=50.0)
+    blk, _ = O.make_itype1_block(30, 1, list(range(0, 17, 2)), 30.0, [0, 2], pot, 20.0)
+    assert blk.n == 72
+    b = _upload(eng, blk)
+    ered = -3.0 * blk.cm2ru
+    z0 = O.yinit(blk, ered, 20.0, zout=False)
+    print(f"AIRY TOLHI=1e-4 inward 20 -> 8 RM, N={blk.n} (matrices in global memory)", flush=True)
+    z, nodes, nstep, st, rec = eng.prop_airy_tolhi(b, [ered], z0[None], 20.0, 8.0, -0.05, 3.0, 1e-4, replay=0)
+    zo, no, steps = O.aiprop_tolhi(z0, blk, ered, 20.0, 8.0, -0.05, 3.0, 1e-4)
+    print(f"  steps {nstep[0]} (oracle {len(steps)}) nodes {nodes[0]} (oracle {no})", flush=True)
+    assert st[0] == 0 and nstep[0] == len(steps) and nodes[0] == no
+    dz = np.abs(z[0] - zo).max() / np.abs(zo).max()
+    print(f"  Z vs oracle rel {dz:.2e}", flush=True)
+    # measured 2.3e-15
+    assert dz < 1e-11
