@@ -357,6 +357,83 @@ contains
   end function
 end subroutine
 
+subroutine BDCTRL(N, MXLAM, NHAM, PSIMCH, VL, IV, EINT, CENT, NODEC, ERED, EP2RU, CM2RU, RSCALE, EIGMIN, WAVE, &
+                  IMIN, IPRINT)
+  ! source_code/bdctrl.f:1-4: BOUND's matching at one energy for the
+  ! two-segment layout with RMID = RMATCH (source_code/bd.prpini.f:24-62,
+  ! bdpset.f:26-42): the outward segment RMIN -> RMATCH with DRS and the
+  ! inward one RMAX -> RMATCH with DRL, each at DRSET's constant step
+  ! (STEP = -10, drset.f:62-64), IPROPS outward and IPROPL inward (5, 6 or
+  ! 7), read from /RADIAL/; the matching matrix's eigenvalues, NODEC (both
+  ! propagations' node counts plus the negative eigenvalues), EIGMIN, IMIN
+  ! and PSIMCH from mcp_bound_match.  The wavefunction (WAVE) is
+  ! mcp_bound_wave's, not this drop-in's.
+  use, intrinsic :: iso_c_binding
+  use mcprop
+  use potential, only: NCONST, NRSQ, CONLEN
+  implicit none
+  integer :: N, MXLAM, NHAM, NODEC, IMIN, IPRINT, IV(*)
+  double precision, target :: PSIMCH(N)
+  double precision :: VL(*), EINT(N), CENT(N), ERED, EP2RU, CM2RU, RSCALE, EIGMIN
+  logical :: WAVE
+  double precision :: RMNINT, RMXINT, RMID, RMATCH, DRS, DRL, STEPS, STEPL, POWRS, POWRL, TOLHIS, TOLHIL, &
+                      CAYS, CAYL, UNSET
+  integer :: IPROPS, IPROPL, NSEG
+  common /RADIAL/ RMNINT, RMXINT, RMID, RMATCH, DRS, DRL, STEPS, STEPL, POWRS, POWRL, TOLHIS, TOLHIL, CAYS, CAYL, &
+                  UNSET, IPROPS, IPROPL, NSEG
+  real(c_double), allocatable :: vlc(:), ro(:), ri(:), pto(:), pti(:), dummy(:)
+  real(c_double), allocatable, target :: pconst(:), evals(:)
+  real(c_double) :: e1(1), em(1), dro, dri
+  integer(c_int) :: ib, igo, igi, nd(1), im(1), st(1)
+  type(c_ptr) :: ctx, pc
+  integer :: nout, nin, k
+  double precision :: drl_use
+  if (WAVE) then
+    write (6, '(A)') ' *** BDCTRL: THE WAVEFUNCTION IS mcp_bound_wave''S, NOT THIS DROP-IN''S.  RUN HALTED.'
+    stop
+  end if
+  if (IPROPS < 5 .or. IPROPS > 7 .or. IPROPL < 5 .or. IPROPL > 7) then
+    write (6, '(A,2I3,A)') ' *** BDCTRL: IPROPS, IPROPL =', IPROPS, IPROPL, ' (5, 6 OR 7 HERE).  RUN HALTED.'
+    stop
+  end if
+  drl_use = DRL
+  if (DRL == UNSET) drl_use = DRS
+  ! DRSET's constant step: NSTEP = max(nint(range / DR), 1), DR = range / NSTEP
+  nout = max(int(floor(abs(RMATCH - RMNINT)/DRS + 0.5d0)), 1)
+  dro = abs(RMATCH - RMNINT)/dble(nout)
+  nin = max(int(floor(abs(RMXINT - RMATCH)/drl_use + 0.5d0)), 1)
+  dri = -abs(RMXINT - RMATCH)/dble(nin)
+  allocate (ro(2*nout + 1), ri(2*nin + 1), pto(MXLAM*(2*nout + 1)), pti(MXLAM*(2*nin + 1)), dummy(0), &
+            vlc(NHAM*(N*(N + 1)/2)), pconst(max(NCONST, 1)), evals(N))
+  ro(1) = RMNINT
+  do k = 2, 2*nout + 1
+    ro(k) = ro(k - 1) + 0.5d0*dro
+  end do
+  ri(1) = RMXINT
+  do k = 2, 2*nin + 1
+    ri(k) = ri(k - 1) + 0.5d0*dri
+  end do
+  call mcpf_ptable(2*nout + 1, ro, MXLAM, EP2RU, CM2RU, RSCALE, IPRINT, pto, pconst)
+  call mcpf_ptable(2*nin + 1, ri, MXLAM, EP2RU, CM2RU, RSCALE, IPRINT, pti, pconst)
+  ctx = mcpf_context()
+  call mcpf_compact_vl(N, NHAM, VL, IV, vlc)
+  ib = mcpf_block(N, MXLAM, NCONST, NRSQ, vlc, EINT, CENT, CONLEN)
+  igo = mcpf_grid(1, MXLAM, 2*nout + 1, ro, pto, dro, dummy)
+  igi = mcpf_grid(1, MXLAM, 2*nin + 1, ri, pti, dri, dummy)
+  pc = c_null_ptr
+  if (NCONST > 0) pc = c_loc(pconst)
+  e1(1) = ERED
+  call mcpf_check(mcp_bound_match(ctx, ib, igo, igi, int(IPROPS, c_int), int(IPROPL, c_int), 1_c_int, e1, pc, &
+                                  nd, em, im, c_loc(PSIMCH), c_loc(evals), st), 'BDCTRL')
+  if (st(1) /= 0) then
+    write (6, '(A,I9,A)') ' *** BDCTRL: PROPAGATION FAILED (STATUS', st(1), ').  RUN HALTED.'
+    stop
+  end if
+  NODEC = nd(1)
+  EIGMIN = em(1)
+  IMIN = im(1) + 1
+end subroutine
+
 subroutine SYMINV(A, IA, N, IFAIL)
   ! source_code/syminv.f:1 (with SYMINL for N > 30): the lower triangle of
   ! A(1:N,1:N) in, the full symmetric inverse out; IFAIL = the number of

@@ -2,7 +2,7 @@
 ! the part of the reference's host program around the propagators, with
 ! its own module `potential`, POTENL (a sum of power / exponential terms per
 ! radial coefficient, the &POTL form), SCAPOT, PERTRB and DRCALC, and drives
-! MDPROP / LDPROP / MAPROP / AIPROP / YTOK / SYMINV / VVPROP through their
+! MDPROP / LDPROP / MAPROP / AIPROP / YTOK / SYMINV / VVPROP / BDCTRL through their
 ! reference argument lists.
 !
 !   fortran_host cpu                 checks that need no GPU
@@ -129,6 +129,13 @@ contains
     logical :: cialfp, civ, civp, civpp, cishift, cidiag, cipert, cisym, ciread, ciwrite, cireadr, ciwritr
     common /LDVVCM/ cxsqmax, calpha1, calpha2, cialpha, cialfp, civ, civp, civpp, cishift, cidiag, cipert, cisym
     common /PRPSCR/ ceshift, ciscru, ciscrur, ciread, ciwrite, cireadr, ciwritr
+    double precision :: crmnint, crmxint, crmid, crmatch, cdrs, cdrl, csteps, cstepl, cpowrs, cpowrl, ctolhis, &
+                        ctolhil, ccays, ccayl, cunset
+    integer :: ciprops, ciprorl, cnseg, nodec_b, imin_b
+    double precision :: eigmin_b
+    double precision, allocatable :: psi_b(:)
+    common /RADIAL/ crmnint, crmxint, crmid, crmatch, cdrs, cdrl, csteps, cstepl, cpowrs, cpowrl, ctolhis, ctolhil, &
+                    ccays, ccayl, cunset, ciprops, ciprorl, cnseg
     double precision :: ep2ru, cm2ru, rscale, ered, rstart, rmid, dr, drnow, powr, rmax
     double precision, allocatable :: vl(:), eint(:), cent(:), y0(:, :), y(:, :), z(:, :), y2(:, :), wvec(:), &
                                      kmat(:), a(:, :)
@@ -140,7 +147,7 @@ contains
     read (u, *) n, mxlam, nham
     NVLBLK = nham
     allocate (vl(nham*n*(n + 1)/2), eint(n), cent(n), y0(n, n), y(n, n), z(n, n), y2(n, n), wvec(n), &
-              kmat(n*n), nb(n), lv(n), iv(1), a(n, n))
+              kmat(n*n), nb(n), lv(n), iv(1), a(n, n), psi_b(n))
     read (u, *) vl
     read (u, *) eint
     read (u, *) cent
@@ -234,6 +241,21 @@ contains
                 ered, ep2ru, cm2ru, rscale, 0)
     write (u, '(A,I10,I10,ES25.16)') 'VVPROP', nsteps_used, nstep2, maxval(abs(y - y2))
     write (u, '(ES25.16)') y
+    ! BDCTRL: MDPROP outward RSTART -> RMID and inward RMAX -> RMID at ERED
+    crmnint = rstart
+    crmxint = rmax
+    crmid = rmid
+    crmatch = rmid
+    cdrs = dr
+    cdrl = 2.0d0*dr
+    cunset = -1.0d30
+    ciprops = 6
+    ciprorl = 6
+    cnseg = 2
+    call BDCTRL(n, mxlam, nham, psi_b, vl, iv, eint, cent, nodec_b, ered, ep2ru, cm2ru, rscale, eigmin_b, &
+                .false., imin_b, 0)
+    write (u, '(A,I10,I6,ES25.16)') 'BDCTRL', nodec_b, imin_b, eigmin_b
+    write (u, '(ES25.16)') psi_b
     close (u)
     call mcpf_release()
     write (6, '(A)') 'RUN_DONE'

@@ -1,5 +1,5 @@
 """The Fortran side of the boundary: the ISO_C_BINDING module and the drop-in
-MDPROP / LDPROP / MAPROP / AIPROP / SYMINV / YTOK / VVPROP (molscat_amd/fortran),
+MDPROP / LDPROP / MAPROP / AIPROP / SYMINV / YTOK / VVPROP / BDCTRL (molscat_amd/fortran),
 compiled with amdflang and linked into a test host program
 (tests/native/fortran_host.f90) that calls them with the reference's
 argument lists (source_code/mdprop.f:1-4, ldprop.f:1-4, maprop.f:1-4,
@@ -179,3 +179,15 @@ def test_fortran_dropins_vs_oracle(tmp_path):
     assert int(head[0]) == int(head[1]) == len(steps)
     assert float(head[2]) == 0.0
     assert err < 1e-10
+
+    # BDCTRL: MDPROP out to RMID with DRS = DR and in from RMAX with DRL = 2 DR
+    (_, dro, no), (_, dri, ni) = S.bound_segments(rstart, rmid, rmax, dr, 2.0 * dr)
+    og = O.BoundGrid(rstart, rmid, rmax, no, dro, ni, dri)
+    m = O.bound_match(ob, ered, og, 6, 6)
+    head, psi = out["BDCTRL"]
+    print(f"  BDCTRL NODEC={int(head[0])} (oracle {m['NODEC']}) EIGMIN={float(head[2]):.6e} "
+          f"(oracle {m['EIGMIN']:.6e})", flush=True)
+    assert int(head[0]) == m["NODEC"] and int(head[1]) == m["IMIN"] + 1
+    scale = max(1.0, float(np.abs(m["EVALS"]).max()))
+    assert abs(float(head[2]) - m["EIGMIN"]) < 1e-9 * scale
+    assert np.abs(psi - m["PSIMCH"]).max() < 1e-8
