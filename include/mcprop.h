@@ -183,7 +183,9 @@ int mcp_prop_airy(mcp_ctx *ctx, int32_t block_id, int32_t grid_id, int32_t ninst
  * they go: term t adds a[t] * x^npower[t] (npower != 0; x below 1e-8 raised
  * to 1e-8 for negative powers) or a[t] * exp(e[t] * x) (npower == 0) to
  * radial term lam[t] (0-based, < mxlam) at x = R * rscale; P(lam, R) = ep2ru
- * * the sum, clamped at 1e60 first (HAMMAT, source_code/hammat.f:38-56). */
+ * * the sum, clamped at 1e60 first (HAMMAT, source_code/hammat.f:38-56).  A
+ * caller with SCAPOT's SCALAM != 1 (source_code/scapot-default.f:19, applied
+ * to the clamped terms before EP2RU) passes ep2ru * SCALAM. */
 int mcp_block_set_potl(mcp_ctx *ctx, int32_t block_id, int32_t nterm, const int32_t *lam, const int32_t *npower,
                        const double *a, const double *e, double rscale, double ep2ru);
 

@@ -152,7 +152,10 @@ def cross_sections(eng, jlevels: Sequence[int], be: float, lambdas: Sequence[int
             if not iprop_long:
                 out = eng.smatrix_ld(b, g, iprop, ered, np.array(wv), np.array(nops), nb, rmax)
             else:
-                eng.set_potl(b, blk.potl, blk.rscale, blk.ep2ru)
+                if getattr(blk, "expval", None) is not None:
+                    raise NotImplementedError("PERTRB (/EXPVAL/) on the potential that VIVS and AIRY with "
+                                              "TOLHI > 0 evaluate as they go is not supported")
+                eng.set_potl(b, blk.potl, blk.rscale, blk.ep2ru, getattr(blk, "scalam", 1.0))
                 y0 = eng.yinit(b, g, ered)
                 y, nodes_ld, st = eng.prop_ld(b, g, iprop, ered, y0)
                 y, st2 = _long_range(eng, b, blk, ered, y, iprop_long, rend, rmax, h, drfit, tolhi, powr)

@@ -399,10 +399,15 @@ class Engine:
     def destroy_grid(self, g: GridHandle):
         self._check(self.lib.mcp_grid_destroy(self.ctx, g.id), "mcp_grid_destroy")
 
-    def set_potl(self, b: BlockHandle, potl, rscale: float, ep2ru: float):
+    def set_potl(self, b: BlockHandle, potl, rscale: float, ep2ru: float, scalam: float = 1.0):
         """POTENL's radial terms for the propagators that choose their radii
-        as they go (VIVS): potl.terms[lam] = [(A, n) | (A, 0, E), ...] (A x^n
-        or A exp(E x) at x = R * rscale), P = ep2ru * the sum."""
+        as they go (VIVS, AIRY with TOLHI > 0): potl.terms[lam] = [(A, n) |
+        (A, 0, E), ...] (A x^n or A exp(E x) at x = R * rscale), P = ep2ru *
+        scalam * the sum clamped at 1e60.  SCAPOT's SCALAM (source_code/
+        scapot-default.f:19) multiplies the clamped radial terms before EP2RU
+        (source_code/hammat.f:55-73) and DERMAT's derivatives alike
+        (source_code/dermat.f:86), so it travels folded into the library's
+        EP2RU factor."""
         lam, npw, a, e = [], [], [], []
         for l, tl in enumerate(potl.terms):
             for t in tl:
@@ -415,7 +420,7 @@ class Engine:
         a = np.ascontiguousarray(a, dtype=np.float64)
         e = np.ascontiguousarray(e, dtype=np.float64)
         self._check(self.lib.mcp_block_set_potl(self.ctx, b.id, len(lam), iptr(lam), iptr(npw), dptr(a), dptr(e),
-                                                float(rscale), float(ep2ru)), "mcp_block_set_potl")
+                                                float(rscale), float(ep2ru) * float(scalam)), "mcp_block_set_potl")
 
     def prop_vivs(self, b: BlockHandle, ered, rmat0: np.ndarray, rstart: float, rstop: float, drnow: float,
                   tldiag: float, toff: float, drmax: float = 5.0, replay: int = 2, steps=None,

@@ -160,3 +160,31 @@ def test_airy_tolhi_beyond_lds_vs_oracle(eng):
     print(f"  Z vs oracle rel {dz:.2e}", flush=True)
     # measured 2.3e-15
     assert dz < 1e-11
+
+
+def test_airy_tolhi_scalam_vs_oracle(eng, deck):
+    """SCAPOT's SCALAM (source_code/scapot-default.f:19) on the potential
+    AIRY evaluates where its step control lands: the inward propagation
+    above with SCALAM = 0.93 (folded into the library's EP2RU factor by
+    Engine.set_potl) against the oracle's block with the same SCALAM; steps,
+    node count and Z, and the unscaled oracle run differs."""
+    import copy
+
+    sec, blk0, ld, dr2 = deck
+    blk = copy.copy(blk0)
+    blk.scalam = 0.93
+    b = eng.block(blk.vl, blk.eint, blk.cent, mxlam=blk.vl.shape[0])
+    eng.set_potl(b, blk.potl, blk.rscale, blk.ep2ru, blk.scalam)
+    ered = -3.0 * blk.cm2ru
+    z0 = O.yinit(blk, ered, 20.0, zout=False)
+    print(f"AIRY TOLHI=1e-4 inward with SCALAM=0.93, N={blk.n}", flush=True)
+    z, nodes, nstep, st, rec = eng.prop_airy_tolhi(b, [ered], z0[None], 20.0, 1.2, -0.05, 3.0, 1e-4, replay=0)
+    zo, no, steps = O.aiprop_tolhi(z0, blk, ered, 20.0, 1.2, -0.05, 3.0, 1e-4)
+    z1, _, _ = O.aiprop_tolhi(z0, blk0, ered, 20.0, 1.2, -0.05, 3.0, 1e-4)
+    print(f"  steps {nstep[0]} (oracle {len(steps)}) nodes {nodes[0]} (oracle {no})", flush=True)
+    assert st[0] == 0 and nstep[0] == len(steps) and nodes[0] == no
+    dz = np.abs(z[0] - zo).max() / np.abs(zo).max()
+    d1 = np.abs(z1 - zo).max() / np.abs(zo).max()
+    print(f"  Z vs oracle rel {dz:.2e} (unscaled differs by {d1:.2e})", flush=True)
+    assert d1 > 1e-3
+    assert dz < 1e-9

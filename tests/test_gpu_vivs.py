@@ -270,3 +270,39 @@ def test_vivs_options_replay(eng, deck):
     print(f"  {n2[1]} steps, rel |dR| {d:.2e}", flush=True)
     # measured 4.5e-11
     assert d < 2e-10
+
+
+def test_vivs_scalam_vs_oracle(eng, deck):
+    """SCAPOT's SCALAM (source_code/scapot-default.f:19) on the potential
+    VIVS evaluates as it goes, and on DERMAT's derivatives with IVP and
+    IVPP: the device (SCALAM folded into the library's EP2RU factor by
+    Engine.set_potl) against the oracle's block with SCALAM = 0.93, whose
+    HAMMAT and DERMAT scale the clamped radial terms; the unscaled run
+    differs, so the factor is not lost on either side."""
+    import copy
+
+    sec, blk0, ld = deck
+    blk = copy.copy(blk0)
+    blk.scalam = 0.93
+    b = eng.block(blk.vl, blk.eint, blk.cent, mxlam=blk.vl.shape[0])
+    eng.set_potl(b, blk.potl, blk.rscale, blk.ep2ru, blk.scalam)
+    rng = np.random.default_rng(11)
+    a = rng.standard_normal((blk.n, blk.n))
+    rmat = 0.05 * (a + a.T)
+    ered = 1000.0 * blk.cm2ru
+    tl, to = O.vivs_tolerances(1e-4)
+    kw = dict(ivp=True, ivpp=True)
+    print("VIVS SCALAM=0.93 with IVP, IVPP", flush=True)
+    eng.set_vivs_options(**kw)
+    try:
+        rout, nstep, st, _ = eng.prop_vivs(b, [ered], rmat[None], 1.1, 12.0, 0.003, tl, to, replay=0)
+    finally:
+        eng.set_vivs_options()
+    ro, steps = O.vvprop(rmat, blk, ered, 1.1, 12.0, 0.003, tl, to, opts=O.VivsOptions(**kw))
+    r1, _ = O.vvprop(rmat, blk0, ered, 1.1, 12.0, 0.003, tl, to, opts=O.VivsOptions(**kw))
+    d = np.abs(rout[0] - ro).max() / np.abs(ro).max()
+    d1 = np.abs(r1 - ro).max() / np.abs(ro).max()
+    print(f"  {nstep[0]} steps (oracle {len(steps)}), rel |dR| {d:.2e} (unscaled differs by {d1:.2e})", flush=True)
+    assert st[0] == 0 and nstep[0] == len(steps)
+    assert d1 > 1e-3
+    assert d < 2e-11
