@@ -334,6 +334,9 @@ module mcprop
   integer, save :: mcpf_potl_nterm = -1
   integer(c_int), allocatable, save :: mcpf_potl_lam(:), mcpf_potl_npower(:)
   real(c_double), allocatable, save :: mcpf_potl_a(:), mcpf_potl_e(:)
+  ! SCAPOT's SCALAM (module efvs in the reference, source_code/scapot-default.f:
+  ! 2-19): the factor on those radial terms, given with mcpf_set_scalam.
+  real(c_double), save :: mcpf_scalam = 1.0d0
 
   ! VVPROP's steps as the energy that chose them recorded them (the part of
   ! the reference's scratch file ISCRU that the later energies read back):
@@ -528,6 +531,12 @@ contains
     mcpf_potl_npower = int(npower, c_int)
     mcpf_potl_a = a
     mcpf_potl_e = e
+  end subroutine
+
+  ! SCALAM for the &POTL terms above (the caller's efvs SCALAM).
+  subroutine mcpf_set_scalam(scalam)
+    real(c_double), intent(in) :: scalam
+    mcpf_scalam = scalam
   end subroutine
 
   ! Destroy every cached device object and the context.

@@ -260,7 +260,7 @@ subroutine AIPROP(NCH, MXLAM, NHAM, Z, VL, IV, EINT, CENT, RSTART, RSTOP, NSTEP,
       stop
     end if
     call mcpf_check(mcp_block_set_potl(ctx, ib, int(mcpf_potl_nterm, c_int), mcpf_potl_lam, mcpf_potl_npower, &
-                                       mcpf_potl_a, mcpf_potl_e, RSCALE, EP2RU), 'AIPROP')
+                                       mcpf_potl_a, mcpf_potl_e, RSCALE, EP2RU*mcpf_scalam), 'AIPROP')
     mxstep = 100000
     allocate (steps(2*mxstep))
     nrec = 0
@@ -321,7 +321,7 @@ subroutine VVPROP(N, MXLAM, NHAM, RMAT, VL, IVL, EINT, CENT, RSTART, RSTOP, NSTE
   call mcpf_compact_vl(N, NHAM, VL, IVL, vlc)
   ib = mcpf_block(N, MXLAM, NCONST, NRSQ, vlc, EINT, CENT, CONLEN)
   call mcpf_check(mcp_block_set_potl(ctx, ib, int(mcpf_potl_nterm, c_int), mcpf_potl_lam, mcpf_potl_npower, &
-                                     mcpf_potl_a, mcpf_potl_e, RSCALE, EP2RU), 'VVPROP')
+                                     mcpf_potl_a, mcpf_potl_e, RSCALE, EP2RU*mcpf_scalam), 'VVPROP')
   call mcpf_check(mcp_set_vivs_options(ctx, l2i(IV), l2i(IVP), l2i(IVPP), l2i(ISHIFT), l2i(IDIAG), l2i(IPERT), &
                                        l2i(ISYM), int(IALPHA, c_int), l2i(IALFP), ALPHA1, ALPHA2, XSQMAX), 'VVPROP')
   e1(1) = ERED
